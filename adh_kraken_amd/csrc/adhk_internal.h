@@ -1,0 +1,119 @@
+/*
+ * adhk_internal.h -- device context shared by the C host driver (newton.c)
+ * and the gfx950 kernels (*.hip).  Plain C so both sides see one layout.
+ *
+ * HBM layout (N nodes, E triangles, S boundary segments):
+ *   node coordinates      x, y, z                   SoA double [N]
+ *   triangles             tri                       SoA int [3][E]
+ *   geometry              djac [E], grad [6][E]     (gx0,gx1,gx2,gy0,gy1,gy2)
+ *   state                 sol, sol_old, sol_older   AoS double [3N] (h,u,v per node)
+ *   matrix                block CSR of 3x3 node blocks: brow [N+1], bcol [nnzb],
+ *                         bvals [nnzb][9] (row-major block), diag_blk [N]
+ *   assembly lists        cptr [nnzb+1], contrib [npairs]: contributions to each
+ *                         block in element order (2-D first, then 1-D)
+ *   elemental buffers     erhs [E*9 + S*6], emat [E*81 + S*36] stored [elem][col][row]
+ */
+#ifndef ADHK_INTERNAL_H
+#define ADHK_INTERNAL_H
+
+#include <hip/hip_runtime_api.h>
+
+#include "../../include/adhk.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* device-resident scalars of the Krylov solver and the Newton norms */
+typedef struct {
+    double rho, rhop, alpha, omega, gamma, beta;
+    double rnorm, bnorm, tol;
+    int it, max_it, done, pad0;
+    double rmax, rl2sum, incmax;
+    int nanflag, pad1;
+} adhk_scalars;
+
+/* contribution encoding: tri: (e*3 + a)*3 + b  (a,b local nodes, row a, col b)
+ *                        seg: CONTRIB_SEG_BASE + (s*2 + a)*2 + b                */
+#define ADHK_NPARTIAL_MAX 8192
+
+/* per-kernel device timers (hipEvent pairs around launches on the context stream) */
+#define ADHK_KT_RING 256
+typedef struct {
+    int on, n;
+    int kind[ADHK_KT_RING];
+    hipEvent_t a[ADHK_KT_RING], b[ADHK_KT_RING];
+    double ms[ADHK_KT_NKINDS];
+    long long cnt[ADHK_KT_NKINDS];
+} adhk_ktimer;
+
+struct adhk_sw2 {
+    int N, E, S, ndofs, nnzb, npairs;
+    adhk_params p;
+    hipStream_t stream;
+    /* mesh */
+    double *x, *y, *z;
+    int *tri;
+    double *djac, *grad, *ks;
+    int *seg, *seg_kind;
+    double *seg_val, *sdjac, *snrml;
+    /* state */
+    double *sol, *sol_old, *sol_older, *dirichlet, *dsol, *residual, *scale;
+    int *bc_mask, *wd_flag, *wd_tmp;
+    /* block CSR pattern and assembly lists */
+    int *brow, *bcol, *diag_blk, *cptr, *contrib;
+    /* matrix values and preconditioner */
+    double *bvals, *minv;
+    float *bvals32, *minv32;
+    /* elemental buffers */
+    double *erhs, *emat;
+    /* Krylov vectors (FP64 and FP32 sets) */
+    double *r, *pv, *Ap, *s, *As, *q, *x0;
+    float *r32, *p32, *Ap32, *s32, *As32, *q32, *xk32;
+    /* reductions */
+    double *partials;
+    unsigned int *counter;
+    adhk_scalars *sc;
+    int *flags; /* [0] = partially dry element seen, [1] = NaN seen */
+    /* timing */
+    adhk_timing timing;
+    hipEvent_t ev[8];
+    adhk_ktimer kt;
+};
+typedef struct adhk_sw2 adhk_ctx;
+
+/* ---- launchers (sw2_kernels.hip) ---- */
+int adhk_launch_wdflag(adhk_ctx *c);
+int adhk_launch_dry_bc(adhk_ctx *c);
+int adhk_launch_elem_resid(adhk_ctx *c);
+int adhk_launch_resid_gather(adhk_ctx *c);
+int adhk_launch_elem_jac(adhk_ctx *c);
+int adhk_launch_jac_gather(adhk_ctx *c);
+
+/* ---- launchers (pattern.hip) ---- */
+int adhk_build_pattern(adhk_ctx *c);
+int adhk_launch_csr_pattern(adhk_ctx *c, int *d_indptr, int *d_cols);
+int adhk_launch_csr_vals(adhk_ctx *c, double *d_vals);
+
+/* ---- launchers (la_kernels.hip) ---- */
+int adhk_launch_norms(adhk_ctx *c);
+int adhk_launch_check_diag(adhk_ctx *c);
+int adhk_launch_dirichlet(adhk_ctx *c, int do_scale);
+int adhk_launch_scale_prec(adhk_ctx *c, int mixed);
+int adhk_launch_krylov_init(adhk_ctx *c, int mixed, double tol, int max_it);
+int adhk_launch_krylov_iter(adhk_ctx *c, int mixed);
+int adhk_launch_krylov_finish(adhk_ctx *c, int mixed);
+int adhk_launch_increment(adhk_ctx *c);
+int adhk_launch_linesearch(adhk_ctx *c);
+int adhk_launch_init_system(adhk_ctx *c);
+int adhk_launch_rotate(adhk_ctx *c);
+
+/* error reporting and kernel timers (newton.c) */
+void adhk_set_error(const char *fmt, ...);
+void adhk_kt_begin(adhk_ctx *c, int kind);
+void adhk_kt_end(adhk_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,701 @@
+/*
+ * la_kernels.hip -- linear algebra of the Newton step on the node-block CSR
+ * matrix (gfx950): check_diag, Dirichlet, row scaling, 3x3 block-Jacobi
+ * preconditioner and a device-resident, left-preconditioned BiCGStab.
+ *
+ * Reference: src/la/matrix_utilities.c (check_diag :120, apply_Dirichlet_BC :42),
+ * src/la/scale_linear_system.c (:31, :137), src/la/bcgstab_solver.c (:65-335).
+ * The reference preconditions with a full UMFPACK LU of the matrix
+ * (bcgstab_solver.c:371-424); this path uses the 3x3 nodal block-Jacobi
+ * inverse instead.
+ *
+ * Every BiCGStab iteration is three HBM-bound kernels:
+ *   k_spmv_p : Ap = M^-1 A p                           + q.Ap        -> alpha
+ *   k_spmv_s : s = r - alpha Ap, As = M^-1 A s          + As.s, As.As, As.q -> omega, rho, beta
+ *   k_update : x += alpha p + omega s, r = s - omega As, p = r + beta (p - omega Ap) + max|r| -> done
+ * Scalars live in device memory; every kernel exits at once when the solver
+ * has converged, so the host launches iterations in chunks and reads the
+ * "done" flag between chunks.  Reductions are deterministic: per-workgroup
+ * partials, and the last workgroup to finish sums them in index order.
+ * The block matrix and vectors are templated on the storage type (double for
+ * the FP64 path, float for the mixed-precision path); reductions and scalars
+ * are always double.
+ */
+#include <hip/hip_runtime.h>
+
+#include "adhk_internal.h"
+
+#define TPB 256
+#define MAX_GRID 2048
+#define D_SMALL 2.220446049250313080847e-16
+
+#define CHECK_LAUNCH()                                                                                            \
+    do {                                                                                                          \
+        hipError_t e_ = hipGetLastError();                                                                        \
+        if (e_ != hipSuccess) {                                                                                   \
+            adhk_set_error("%s: %s", __func__, hipGetErrorString(e_));                                            \
+            return -1;                                                                                            \
+        }                                                                                                         \
+    } while (0)
+
+static inline int grid_for(long n)
+{
+    long g = (n + TPB - 1) / TPB;
+    if (g > MAX_GRID) g = MAX_GRID;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+/* ---- deterministic block reductions ---- */
+template <int K, bool MAX> __device__ __forceinline__ void block_reduce(double (&v)[K], double *sh)
+{
+    /* wave (64 lanes) butterfly, then across the 4 waves of the workgroup through LDS */
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double o = __shfl_xor(v[k], off, 64);
+            v[k] = MAX ? fmax(v[k], o) : v[k] + o;
+        }
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) sh[wv * K + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double a = sh[k];
+            for (int w = 1; w < TPB / 64; w++) a = MAX ? fmax(a, sh[w * K + k]) : a + sh[w * K + k];
+            v[k] = a;
+        }
+    }
+}
+
+/* writes this workgroup's partials; returns true in the last workgroup to finish, whose
+ * thread 0 then holds the totals (summed over workgroups in index order) in v */
+template <int K, bool MAX>
+__device__ __forceinline__ bool grid_reduce(double (&v)[K], double *partials, unsigned int *counter)
+{
+    __shared__ double sh[(TPB / 64) * K];
+    __shared__ int amlast;
+    block_reduce<K, MAX>(v, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) partials[k * MAX_GRID + blockIdx.x] = v[k];
+        __threadfence();
+        unsigned t = atomicAdd(counter, 1u);
+        amlast = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!amlast) return false;
+    __threadfence();
+    /* last workgroup: strided partial sums in a fixed order, then the same tree */
+    double w[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) w[k] = MAX ? 0.0 : 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += TPB)
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double pv = __builtin_nontemporal_load(&partials[k * MAX_GRID + b]);
+            w[k] = MAX ? fmax(w[k], pv) : w[k] + pv;
+        }
+    __syncthreads();
+    block_reduce<K, MAX>(w, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = w[k];
+        *counter = 0u;
+    }
+    return true;
+}
+
+/* ---- per-node block row product y = A_n x (x given by a functor for the column values) ---- */
+template <typename T> struct ColPlain {
+    const T *x;
+    __device__ __forceinline__ void get(int m, double &a, double &b, double &c) const
+    {
+        a = (double)x[3 * m];
+        b = (double)x[3 * m + 1];
+        c = (double)x[3 * m + 2];
+    }
+};
+/* s = r + (-alpha) Ap evaluated on the fly (explicit fma so the stored and the recomputed value agree) */
+template <typename T> struct ColS {
+    const T *r, *Ap;
+    double alpha;
+    __device__ __forceinline__ T one(int i) const
+    {
+        return (T)fma(-alpha, (double)Ap[i], (double)r[i]);
+    }
+    __device__ __forceinline__ void get(int m, double &a, double &b, double &c) const
+    {
+        a = (double)one(3 * m);
+        b = (double)one(3 * m + 1);
+        c = (double)one(3 * m + 2);
+    }
+};
+
+template <typename T, typename Col>
+__device__ __forceinline__ void block_row(int n, const int *__restrict__ brow, const int *__restrict__ bcol,
+                                          const T *__restrict__ bv, const Col &col, double &y0, double &y1,
+                                          double &y2)
+{
+    y0 = 0.0;
+    y1 = 0.0;
+    y2 = 0.0;
+    int j0 = brow[n], j1 = brow[n + 1];
+    for (int j = j0; j < j1; j++) {
+        int m = bcol[j];
+        double x0, x1, x2;
+        col.get(m, x0, x1, x2);
+        const T *a = bv + 9 * (size_t)j;
+        y0 += (double)a[0] * x0;
+        y0 += (double)a[1] * x1;
+        y0 += (double)a[2] * x2;
+        y1 += (double)a[3] * x0;
+        y1 += (double)a[4] * x1;
+        y1 += (double)a[5] * x2;
+        y2 += (double)a[6] * x0;
+        y2 += (double)a[7] * x1;
+        y2 += (double)a[8] * x2;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void apply_minv(const T *__restrict__ mi, int n, double y0, double y1, double y2,
+                                           double &z0, double &z1, double &z2)
+{
+    const T *m = mi + 9 * (size_t)n;
+    z0 = (double)m[0] * y0 + (double)m[1] * y1 + (double)m[2] * y2;
+    z1 = (double)m[3] * y0 + (double)m[4] * y1 + (double)m[5] * y2;
+    z2 = (double)m[6] * y0 + (double)m[7] * y1 + (double)m[8] * y2;
+}
+
+/* ---------------- Newton-level kernels ---------------- */
+/* get_residual_norms, newton_tools.c:303-373: max|r|, sum r^2, max|dsol| over non-Dirichlet dofs */
+__global__ __launch_bounds__(TPB) void k_norms(int ndofs, const double *__restrict__ res,
+                                               const double *__restrict__ dsol, const int *__restrict__ bc,
+                                               double *partials, unsigned int *counter, adhk_scalars *sc,
+                                               int *flags)
+{
+    double vmax[3] = {0.0, 0.0, 0.0}; /* max|r|, max|dsol|, nan indicator */
+    double vsum[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ndofs; i += gridDim.x * blockDim.x) {
+        double r = res[i];
+        if (isnan(r)) vmax[2] = 1.0;
+        if (bc[i] != 1) {
+            vmax[0] = fmax(vmax[0], fabs(r));
+            vmax[1] = fmax(vmax[1], fabs(dsol[i]));
+            vsum[0] += r * r;
+        }
+    }
+    if (grid_reduce<3, true>(vmax, partials, counter + 0)) {
+        if (threadIdx.x == 0) {
+            sc->rmax = vmax[0];
+            sc->incmax = vmax[1];
+            if (vmax[2] != 0.0) flags[1] = 1;
+        }
+    }
+    if (grid_reduce<1, false>(vsum, partials + 4 * MAX_GRID, counter + 1)) {
+        if (threadIdx.x == 0) sc->rl2sum = vsum[0];
+    }
+}
+
+extern "C" int adhk_launch_norms(adhk_ctx *c)
+{
+    hipLaunchKernelGGL(k_norms, dim3(grid_for(c->ndofs)), dim3(TPB), 0, c->stream, c->ndofs, c->residual, c->dsol,
+                       c->bc_mask, c->partials, c->counter, c->sc, c->flags);
+    CHECK_LAUNCH();
+    return 0;
+}
+
+/* check_diag, matrix_utilities.c:120-152 */
+__global__ void k_check_diag(int N, const int *diag_blk, const double *bv, int *bc)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    int d = diag_blk[n];
+    for (int r = 0; r < 3; r++) {
+        double a = (d >= 0) ? bv[9 * (size_t)d + 4 * r] : 0.0;
+        if (fabs(a) < D_SMALL) bc[3 * n + r] = 1;
+    }
+}
+
+extern "C" int adhk_launch_check_diag(adhk_ctx *c)
+{
+    adhk_kt_begin(c, ADHK_KT_PREP);
+    hipLaunchKernelGGL(k_check_diag, dim3((c->N + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->N, c->diag_blk,
+                       c->bvals, c->bc_mask);
+    CHECK_LAUNCH();
+    adhk_kt_end(c);
+    return 0;
+}
+
+/* apply_Dirichlet_BC (matrix_utilities.c:42-105): Dirichlet columns -> 0, Dirichlet rows -> identity,
+ * residual -> 0; then the row maxima of scale_linear_system (scale_linear_system.c:31-121):
+ * s = 1/sqrt(max|a|) (1/sqrt(eps) for a vanishing row), b *= s, x = dsol = 0 */
+__global__ __launch_bounds__(TPB) void k_dirichlet_scale(int N, const int *__restrict__ brow,
+                                                         const int *__restrict__ bcol, double *__restrict__ bv,
+                                                         const int *__restrict__ bc, double *__restrict__ res,
+                                                         double *__restrict__ dsol, double *__restrict__ scale,
+                                                         int do_scale)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    int brc[3] = {bc[3 * n], bc[3 * n + 1], bc[3 * n + 2]};
+    double mx[3] = {0., 0., 0.};
+    for (int j = brow[n]; j < brow[n + 1]; j++) {
+        int m = bcol[j];
+        int bcc[3] = {bc[3 * m], bc[3 * m + 1], bc[3 * m + 2]};
+        double *a = bv + 9 * (size_t)j;
+        for (int r = 0; r < 3; r++)
+            for (int cc = 0; cc < 3; cc++) {
+                double v = a[3 * r + cc];
+                if (brc[r] == 1)
+                    v = (m == n && cc == r) ? 1.0 : 0.0;
+                else if (bcc[cc] == 1)
+                    v = 0.0;
+                a[3 * r + cc] = v;
+                double t = fabs(v);
+                if (t > mx[r]) mx[r] = t;
+            }
+    }
+    if (!do_scale) {
+        for (int r = 0; r < 3; r++)
+            if (brc[r] == 1) res[3 * n + r] = 0.0;
+        return;
+    }
+    const double small_factor = 1.0 / sqrt(D_SMALL);
+    for (int r = 0; r < 3; r++) {
+        int i = 3 * n + r;
+        double s = (mx[r] < D_SMALL) ? small_factor : 1.0 / sqrt(fabs(mx[r]));
+        scale[i] = s;
+        double b = (brc[r] == 1) ? 0.0 : res[i];
+        res[i] = b * s;
+        dsol[i] = 0.0;
+    }
+}
+
+extern "C" int adhk_launch_dirichlet(adhk_ctx *c, int do_scale)
+{
+    adhk_kt_begin(c, ADHK_KT_PREP);
+    hipLaunchKernelGGL(k_dirichlet_scale, dim3((c->N + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->N, c->brow,
+                       c->bcol, c->bvals, c->bc_mask, c->residual, c->dsol, c->scale, do_scale);
+    CHECK_LAUNCH();
+    adhk_kt_end(c);
+    return 0;
+}
+
+/* A <- S A S (scale_linear_system.c:104-110) and the 3x3 block-Jacobi inverse of the scaled
+ * diagonal block (adjugate; |det| <= 1e-300 falls back to the inverted diagonal).  With T = float the
+ * scaled matrix and inverse are also written in FP32 for the mixed-precision Krylov solve. */
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_scale_prec(int N, const int *__restrict__ brow, const int *__restrict__ bcol,
+                                                    const int *__restrict__ diag_blk, double *__restrict__ bv,
+                                                    const double *__restrict__ scale, double *__restrict__ minv,
+                                                    T *__restrict__ bv_t, T *__restrict__ minv_t)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double si[3] = {scale[3 * n], scale[3 * n + 1], scale[3 * n + 2]};
+    for (int j = brow[n]; j < brow[n + 1]; j++) {
+        int m = bcol[j];
+        double sj[3] = {scale[3 * m], scale[3 * m + 1], scale[3 * m + 2]};
+        double *a = bv + 9 * (size_t)j;
+        for (int r = 0; r < 3; r++)
+            for (int cc = 0; cc < 3; cc++) {
+                double v = a[3 * r + cc] * (si[r] * sj[cc]);
+                a[3 * r + cc] = v;
+                if (bv_t) bv_t[9 * (size_t)j + 3 * r + cc] = (T)v;
+            }
+    }
+    int d = diag_blk[n];
+    double a[9];
+    for (int k = 0; k < 9; k++) a[k] = (d >= 0) ? bv[9 * (size_t)d + k] : 0.0;
+    double c00 = a[4] * a[8] - a[5] * a[7];
+    double c01 = a[5] * a[6] - a[3] * a[8];
+    double c02 = a[3] * a[7] - a[4] * a[6];
+    double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    double m[9];
+    if (fabs(det) > 1e-300) {
+        double id = 1.0 / det;
+        m[0] = c00 * id;
+        m[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+        m[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+        m[3] = c01 * id;
+        m[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+        m[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+        m[6] = c02 * id;
+        m[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+        m[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+    } else {
+        for (int k = 0; k < 9; k++) m[k] = 0.0;
+        for (int k = 0; k < 3; k++) m[4 * k] = (a[4 * k] != 0.0) ? 1.0 / a[4 * k] : 1.0;
+    }
+    for (int k = 0; k < 9; k++) {
+        minv[9 * (size_t)n + k] = m[k];
+        if (minv_t) minv_t[9 * (size_t)n + k] = (T)m[k];
+    }
+}
+
+extern "C" int adhk_launch_scale_prec(adhk_ctx *c, int mixed)
+{
+    adhk_kt_begin(c, ADHK_KT_PREP);
+    if (mixed)
+        hipLaunchKernelGGL(k_scale_prec<float>, dim3((c->N + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->N, c->brow,
+                           c->bcol, c->diag_blk, c->bvals, c->scale, c->minv, c->bvals32, c->minv32);
+    else
+        hipLaunchKernelGGL(k_scale_prec<double>, dim3((c->N + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->N,
+                           c->brow, c->bcol, c->diag_blk, c->bvals, c->scale, c->minv, (double *)NULL,
+                           (double *)NULL);
+    CHECK_LAUNCH();
+    adhk_kt_end(c);
+    return 0;
+}
+
+/* ---------------- BiCGStab (bcgstab_solver.c:65-335) ---------------- */
+/* b <- M^-1 b (kept in s until the first iteration), x0 = dsol, x = M^-1 b, p = Ap = 0   (:128-147) */
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_kinit1(int N, const T *__restrict__ minv, const double *__restrict__ b,
+                                                const double *__restrict__ dsol, double *__restrict__ x0,
+                                                T *__restrict__ x, T *__restrict__ bp, T *__restrict__ p,
+                                                T *__restrict__ Ap)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double z0, z1, z2;
+    apply_minv<T>(minv, n, b[3 * n], b[3 * n + 1], b[3 * n + 2], z0, z1, z2);
+    double z[3] = {z0, z1, z2};
+    for (int k = 0; k < 3; k++) {
+        int i = 3 * n + k;
+        x0[i] = dsol[i];
+        T zz = (T)z[k];
+        x[i] = zz;
+        bp[i] = zz;
+        p[i] = (T)0;
+        Ap[i] = (T)0;
+    }
+}
+
+/* r = b - M^-1 A x, q = r, p = r (first direction, since p = Ap = 0), rnorm, bnorm, rho  (:155-207) */
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_kinit2(int N, const int *__restrict__ brow, const int *__restrict__ bcol,
+                                                const T *__restrict__ bv, const T *__restrict__ minv,
+                                                const T *__restrict__ x, const T *__restrict__ bp, T *__restrict__ r,
+                                                T *__restrict__ q, T *__restrict__ p, double *partials,
+                                                unsigned int *counter, adhk_scalars *sc)
+{
+    double vmax[2] = {0.0, 0.0}, vsum[1] = {0.0};
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+        double y0, y1, y2, z0, z1, z2;
+        block_row<T>(n, brow, bcol, bv, ColPlain<T>{x}, y0, y1, y2);
+        apply_minv<T>(minv, n, y0, y1, y2, z0, z1, z2);
+        double z[3] = {z0, z1, z2};
+        for (int k = 0; k < 3; k++) {
+            int i = 3 * n + k;
+            double bb = (double)bp[i];
+            T rr = (T)(bb + (-1) * z[k]);
+            r[i] = rr;
+            q[i] = rr;
+            p[i] = rr;
+            vmax[0] = fmax(vmax[0], fabs((double)rr));
+            vmax[1] = fmax(vmax[1], fabs(bb));
+            vsum[0] += (double)rr * (double)rr;
+        }
+    }
+    bool l1 = grid_reduce<2, true>(vmax, partials, counter + 0);
+    bool l2 = grid_reduce<1, false>(vsum, partials + 4 * MAX_GRID, counter + 1);
+    (void)l1;
+    (void)l2;
+    /* both reductions finish in some last workgroup; publish from each */
+    if (threadIdx.x == 0 && l1) {
+        sc->rnorm = vmax[0];
+        sc->bnorm = vmax[1];
+    }
+    if (threadIdx.x == 0 && l2) sc->rho = vsum[0];
+}
+
+__global__ void k_kinit3(adhk_scalars *sc, double tol, int max_it)
+{
+    sc->tol = tol;
+    sc->max_it = max_it;
+    sc->rhop = 1.0;
+    sc->alpha = 1.0;
+    sc->omega = 1.0;
+    sc->it = 0;
+    sc->beta = (sc->rho * 1.0) / (1.0 * 1.0);
+    sc->done = !(sc->rnorm > (tol + sc->bnorm * tol) && 0 < max_it);
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_spmv_p(int N, const int *__restrict__ brow, const int *__restrict__ bcol,
+                                                const T *__restrict__ bv, const T *__restrict__ minv,
+                                                const T *__restrict__ p, T *__restrict__ Ap,
+                                                const T *__restrict__ q, double *partials, unsigned int *counter,
+                                                adhk_scalars *sc)
+{
+    if (sc->done) return;
+    double v[1] = {0.0};
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+        double y0, y1, y2, z0, z1, z2;
+        block_row<T>(n, brow, bcol, bv, ColPlain<T>{p}, y0, y1, y2);
+        apply_minv<T>(minv, n, y0, y1, y2, z0, z1, z2);
+        T a0 = (T)z0, a1 = (T)z1, a2 = (T)z2;
+        Ap[3 * n] = a0;
+        Ap[3 * n + 1] = a1;
+        Ap[3 * n + 2] = a2;
+        v[0] += (double)q[3 * n] * (double)a0 + (double)q[3 * n + 1] * (double)a1 + (double)q[3 * n + 2] * (double)a2;
+    }
+    if (grid_reduce<1, false>(v, partials, counter + 2)) {
+        if (threadIdx.x == 0) {
+            sc->gamma = v[0];
+            sc->alpha = sc->rho / v[0];
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_spmv_s(int N, const int *__restrict__ brow, const int *__restrict__ bcol,
+                                                const T *__restrict__ bv, const T *__restrict__ minv,
+                                                const T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ s,
+                                                T *__restrict__ As, const T *__restrict__ q, double *partials,
+                                                unsigned int *counter, adhk_scalars *sc)
+{
+    if (sc->done) return;
+    const double alpha = sc->alpha;
+    ColS<T> col{r, Ap, alpha};
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+        double y0, y1, y2, z0, z1, z2;
+        block_row<T>(n, brow, bcol, bv, col, y0, y1, y2);
+        apply_minv<T>(minv, n, y0, y1, y2, z0, z1, z2);
+        double z[3] = {z0, z1, z2};
+        for (int k = 0; k < 3; k++) {
+            int i = 3 * n + k;
+            T si = col.one(i);
+            T ai = (T)z[k];
+            s[i] = si;
+            As[i] = ai;
+            v[0] += (double)ai * (double)si;
+            v[1] += (double)ai * (double)ai;
+            v[2] += (double)ai * (double)q[i];
+        }
+    }
+    if (grid_reduce<3, false>(v, partials, counter + 3)) {
+        if (threadIdx.x == 0) {
+            double omega = v[0] / v[1];
+            double rhop = sc->rho;
+            double rho = -1.0 * omega * v[2];
+            sc->omega = omega;
+            sc->rhop = rhop;
+            sc->rho = rho;
+            sc->beta = (rho * alpha) / (rhop * omega);
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_update(int ndofs, T *__restrict__ x, T *__restrict__ r, T *__restrict__ p,
+                                                const T *__restrict__ Ap, const T *__restrict__ s,
+                                                const T *__restrict__ As, double *partials, unsigned int *counter,
+                                                adhk_scalars *sc)
+{
+    if (sc->done) return;
+    const double alpha = sc->alpha, omega = sc->omega, beta = sc->beta;
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ndofs; i += gridDim.x * blockDim.x) {
+        double pi = (double)p[i], si = (double)s[i];
+        double xi = (double)x[i] + alpha * pi;
+        x[i] = (T)(xi + omega * si);
+        T rr = (T)(si + (-omega) * (double)As[i]);
+        r[i] = rr;
+        double t = pi + (-omega) * (double)Ap[i];
+        t *= beta;
+        p[i] = (T)(t + 1 * (double)rr);
+        v[0] = fmax(v[0], fabs((double)rr));
+    }
+    if (grid_reduce<1, true>(v, partials, counter + 4)) {
+        if (threadIdx.x == 0) {
+            sc->rnorm = v[0];
+            int it = sc->it + 1;
+            sc->it = it;
+            sc->done = !(v[0] > (sc->tol + sc->bnorm * sc->tol) && it < sc->max_it);
+        }
+    }
+}
+
+/* unscale_linear_system (scale_linear_system.c:137-146): dsol = (x + x0) * s */
+template <typename T>
+__global__ void k_kfinish(int ndofs, const T *__restrict__ x, const double *__restrict__ x0,
+                          const double *__restrict__ scale, double *__restrict__ dsol)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndofs) return;
+    double t = (double)x[i] + x0[i];
+    dsol[i] = t * scale[i];
+}
+
+template <typename T> struct KV {
+    T *r, *p, *Ap, *s, *As, *q, *x;
+    const T *bv, *mi;
+};
+static KV<double> kv64(adhk_ctx *c)
+{
+    KV<double> k;
+    k.r = c->r;
+    k.p = c->pv;
+    k.Ap = c->Ap;
+    k.s = c->s;
+    k.As = c->As;
+    k.q = c->q;
+    k.x = c->x0 + c->ndofs; /* x0 is allocated with 2*ndofs: [x0 | x] */
+    k.bv = c->bvals;
+    k.mi = c->minv;
+    return k;
+}
+static KV<float> kv32(adhk_ctx *c)
+{
+    KV<float> k;
+    k.r = c->r32;
+    k.p = c->p32;
+    k.Ap = c->Ap32;
+    k.s = c->s32;
+    k.As = c->As32;
+    k.q = c->q32;
+    k.x = c->xk32;
+    k.bv = c->bvals32;
+    k.mi = c->minv32;
+    return k;
+}
+
+template <typename T> static int krylov_init(adhk_ctx *c, KV<T> k, double tol, int max_it)
+{
+    int gn = (c->N + TPB - 1) / TPB;
+    /* bp (preconditioned right-hand side) is kept in s until the first iteration overwrites it */
+    hipLaunchKernelGGL(k_kinit1<T>, dim3(gn), dim3(TPB), 0, c->stream, c->N, k.mi, c->residual, c->dsol, c->x0, k.x,
+                       k.s, k.p, k.Ap);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_kinit2<T>, dim3(grid_for(c->N)), dim3(TPB), 0, c->stream, c->N, c->brow, c->bcol, k.bv,
+                       k.mi, k.x, k.s, k.r, k.q, k.p, c->partials, c->counter, c->sc);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_kinit3, dim3(1), dim3(1), 0, c->stream, c->sc, tol, max_it);
+    CHECK_LAUNCH();
+    return 0;
+}
+
+template <typename T> static int krylov_iter(adhk_ctx *c, KV<T> k)
+{
+    adhk_kt_begin(c, ADHK_KT_SPMV_P);
+    hipLaunchKernelGGL(k_spmv_p<T>, dim3(grid_for(c->N)), dim3(TPB), 0, c->stream, c->N, c->brow, c->bcol, k.bv, k.mi,
+                       k.p, k.Ap, k.q, c->partials, c->counter, c->sc);
+    adhk_kt_end(c);
+    CHECK_LAUNCH();
+    adhk_kt_begin(c, ADHK_KT_SPMV_S);
+    hipLaunchKernelGGL(k_spmv_s<T>, dim3(grid_for(c->N)), dim3(TPB), 0, c->stream, c->N, c->brow, c->bcol, k.bv, k.mi,
+                       k.r, k.Ap, k.s, k.As, k.q, c->partials, c->counter, c->sc);
+    adhk_kt_end(c);
+    CHECK_LAUNCH();
+    adhk_kt_begin(c, ADHK_KT_UPDATE);
+    hipLaunchKernelGGL(k_update<T>, dim3(grid_for(c->ndofs)), dim3(TPB), 0, c->stream, c->ndofs, k.x, k.r, k.p, k.Ap,
+                       k.s, k.As, c->partials, c->counter, c->sc);
+    adhk_kt_end(c);
+    CHECK_LAUNCH();
+    return 0;
+}
+
+template <typename T> static int krylov_finish(adhk_ctx *c, KV<T> k)
+{
+    hipLaunchKernelGGL(k_kfinish<T>, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs, k.x, c->x0,
+                       c->scale, c->dsol);
+    CHECK_LAUNCH();
+    return 0;
+}
+
+extern "C" int adhk_launch_krylov_init(adhk_ctx *c, int mixed, double tol, int max_it)
+{
+    return mixed ? krylov_init<float>(c, kv32(c), tol, max_it) : krylov_init<double>(c, kv64(c), tol, max_it);
+}
+extern "C" int adhk_launch_krylov_iter(adhk_ctx *c, int mixed)
+{
+    return mixed ? krylov_iter<float>(c, kv32(c)) : krylov_iter<double>(c, kv64(c));
+}
+extern "C" int adhk_launch_krylov_finish(adhk_ctx *c, int mixed)
+{
+    return mixed ? krylov_finish<float>(c, kv32(c)) : krylov_finish<double>(c, kv64(c));
+}
+
+/* ---------------- Newton updates ---------------- */
+/* increment_function (newton_tools.c:261) then dsol = -dsol (fe_newton.c:553-555) */
+__global__ void k_increment(int n, double *sol, double *dsol)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = dsol[i];
+    sol[i] += d;
+    dsol[i] = -d;
+}
+/* line search step (fe_newton.c:681-741): dsol *= 0.5, sol += dsol */
+__global__ void k_linesearch(int n, double *sol, double *dsol)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = dsol[i] * 0.5;
+    dsol[i] = d;
+    sol[i] += d;
+}
+/* initialize_system's copy (newton_tools.c:33-37) */
+__global__ void k_copy(int n, double *dst, const double *src)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+/* initialize_dirichlet_bc (newton_tools.c:130-142) */
+__global__ void k_apply_dirichlet_sol(int n, double *sol, const int *bc, const double *dir)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && bc[i] == 1) sol[i] = dir[i];
+}
+/* smodel_super_prep_sol (smodel_super_prep_sol.c:2-8) */
+__global__ void k_rotate(int n, double *sol, double *sol_old, double *sol_older)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    sol_older[i] = sol_old[i];
+    sol_old[i] = sol[i];
+}
+
+extern "C" int adhk_launch_increment(adhk_ctx *c)
+{
+    hipLaunchKernelGGL(k_increment, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs, c->sol,
+                       c->dsol);
+    CHECK_LAUNCH();
+    return 0;
+}
+extern "C" int adhk_launch_linesearch(adhk_ctx *c)
+{
+    hipLaunchKernelGGL(k_linesearch, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs, c->sol,
+                       c->dsol);
+    CHECK_LAUNCH();
+    return 0;
+}
+/* sol <- sol_old; wet/dry flags; dry nodes -> u,v Dirichlet; sol[bc] <- dirichlet */
+extern "C" int adhk_launch_init_system(adhk_ctx *c)
+{
+    hipLaunchKernelGGL(k_copy, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs, c->sol,
+                       c->sol_old);
+    CHECK_LAUNCH();
+    if (adhk_launch_wdflag(c)) return -1;
+    if (adhk_launch_dry_bc(c)) return -1;
+    hipLaunchKernelGGL(k_apply_dirichlet_sol, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs,
+                       c->sol, c->bc_mask, c->dirichlet);
+    CHECK_LAUNCH();
+    return 0;
+}
+extern "C" int adhk_launch_rotate(adhk_ctx *c)
+{
+    hipLaunchKernelGGL(k_rotate, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs, c->sol,
+                       c->sol_old, c->sol_older);
+    CHECK_LAUNCH();
+    return 0;
+}

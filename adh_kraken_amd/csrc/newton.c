@@ -1,0 +1,742 @@
+/*
+ * newton.c -- C host driver of the SW2 Newton step on one MI355X.
+ *
+ * Restates the reference's host control flow -- fe_newton
+ * (src/newton/fe_newton.c:44-926), initialize_system / initialize_dirichlet_bc
+ * (newton_tools.c:21, :130), get_residual_norms (:303), the time loop
+ * (src/timeloop/time_loop.c:14-93) -- over device-resident state.  Every
+ * numerical operation runs in a gfx950 kernel; the host only reads back the
+ * three residual norms per residual evaluation and the BiCGStab "done" flag
+ * once per chunk of iterations, and takes the reference's branch decisions
+ * (line search, convergence, time-step cut) on them.
+ */
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "adhk_internal.h"
+
+static __thread char g_err[1024];
+
+void adhk_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+const char *adhk_last_error(void) { return g_err; }
+
+void adhk_solver_defaults(adhk_solver *s)
+{
+    s->tol_nonlin = 1e-8; /* smodel_super_alloc_init.c:63 */
+    s->inc_nonlin = 1e-6; /* :62 */
+    s->max_nonlin_it = 10; /* :65 */
+    s->max_linesearch_cuts = 5; /* :64 */
+    s->lin_tol = 1e-11; /* bcgstab_solver.c:20-21 */
+    s->lin_max_it = 100; /* bcgstab_solver.c:22 */
+    s->precision = ADHK_PREC_FP64;
+    s->lin_check_every = 8;
+    s->mixed_lin_tol = 1e-6;
+}
+
+#define HC(x)                                                                                                     \
+    do {                                                                                                          \
+        hipError_t e_ = (x);                                                                                      \
+        if (e_ != hipSuccess) {                                                                                   \
+            adhk_set_error("%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_));                        \
+            goto fail;                                                                                            \
+        }                                                                                                         \
+    } while (0)
+#define LC(x)                                                                                                     \
+    do {                                                                                                          \
+        if ((x) != 0) goto fail;                                                                                  \
+    } while (0)
+
+/* fr_manningsn_to_rheight, src/friction/fr_manningsn_to_rheight.c:15-32 */
+static double manning_to_rheight(double n, double kn, double g)
+{
+    if (n < 1.E-8) return 0.0;
+    return pow((8.25 * sqrt(g) * n / kn), 6.);
+}
+
+static void ctx_free(adhk_ctx *c)
+{
+    if (!c) return;
+    void *ptrs[] = {c->x, c->y, c->z, c->tri, c->djac, c->grad, c->ks, c->seg, c->seg_kind, c->seg_val, c->sdjac,
+                    c->snrml, c->sol, c->sol_old, c->sol_older, c->dirichlet, c->dsol, c->residual, c->scale,
+                    c->bc_mask, c->wd_flag, c->wd_tmp, c->brow, c->bcol, c->diag_blk, c->cptr, c->contrib, c->bvals,
+                    c->minv, c->bvals32, c->minv32, c->erhs, c->emat, c->r, c->pv, c->Ap, c->s, c->As, c->q, c->x0,
+                    c->r32, c->p32, c->Ap32, c->s32, c->As32, c->q32, c->xk32, c->partials, c->counter, c->sc,
+                    c->flags};
+    for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++)
+        if (ptrs[i]) hipFree(ptrs[i]);
+    for (int i = 0; i < 8; i++)
+        if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    for (int i = 0; i < ADHK_KT_RING; i++) {
+        if (c->kt.a[i]) hipEventDestroy(c->kt.a[i]);
+        if (c->kt.b[i]) hipEventDestroy(c->kt.b[i]);
+    }
+    if (c->stream) hipStreamDestroy(c->stream);
+    free(c);
+}
+
+void adhk_sw2_destroy(adhk_sw2 *c) { ctx_free(c); }
+
+#define DALLOC(ptr, n, T) HC(hipMalloc((void **)&(ptr), ((size_t)(n) > 0 ? (size_t)(n) : 1) * sizeof(T)))
+#define DZERO(ptr, n, T) HC(hipMemsetAsync((ptr), 0, ((size_t)(n) > 0 ? (size_t)(n) : 1) * sizeof(T), c->stream))
+
+adhk_sw2 *adhk_sw2_create(int nnodes, const double *x, const double *y, const double *z, int nelems2d,
+                          const int *tri, const double *mann, int nelems1d, const int *seg, const int *seg_kind,
+                          const double *seg_val, const adhk_params *p)
+{
+    adhk_ctx *c = NULL;
+    double *h_djac = NULL, *h_grad = NULL, *h_ks = NULL, *h_sdjac = NULL, *h_snrml = NULL;
+    int *h_tri = NULL;
+    if (nnodes <= 0 || nelems2d <= 0 || nelems1d < 0 || !x || !y || !z || !tri || !p ||
+        (nelems1d > 0 && (!seg || !seg_kind || !seg_val))) {
+        adhk_set_error("adhk_sw2_create: invalid arguments");
+        return NULL;
+    }
+    if ((long)nnodes * 3 >= 2147483647L || (long)nelems2d * 81 >= 2147483647L) {
+        adhk_set_error("adhk_sw2_create: mesh too large for 32-bit indices");
+        return NULL;
+    }
+    for (long i = 0; i < 3L * nelems2d; i++)
+        if (tri[i] < 0 || tri[i] >= nnodes) {
+            adhk_set_error("adhk_sw2_create: triangle node index out of range at %ld", i);
+            return NULL;
+        }
+    for (long i = 0; i < 2L * nelems1d; i++)
+        if (seg[i] < 0 || seg[i] >= nnodes) {
+            adhk_set_error("adhk_sw2_create: segment node index out of range at %ld", i);
+            return NULL;
+        }
+    for (int i = 0; i < nelems1d; i++)
+        if (seg_kind[i] < ADHK_BC_DEPTH || seg_kind[i] > ADHK_BC_VEL) {
+            adhk_set_error("adhk_sw2_create: unknown segment kind %d", seg_kind[i]);
+            return NULL;
+        }
+    c = (adhk_ctx *)calloc(1, sizeof(adhk_ctx));
+    c->N = nnodes;
+    c->E = nelems2d;
+    c->S = nelems1d;
+    c->ndofs = 3 * nnodes;
+    c->p = *p;
+    HC(hipStreamCreate(&c->stream));
+    for (int i = 0; i < 8; i++) HC(hipEventCreate(&c->ev[i]));
+
+    /* geometry: selem2d_get_triangle_linear_djac_nrml_gradPhi (selem_2d.c:270-351), stored SoA */
+    {
+        const int E = nelems2d, S = nelems1d;
+        h_djac = malloc((size_t)E * sizeof(double));
+        h_grad = malloc((size_t)E * 6 * sizeof(double));
+        h_ks = malloc((size_t)E * sizeof(double));
+        h_tri = malloc((size_t)E * 3 * sizeof(int));
+        h_sdjac = malloc((size_t)(S > 0 ? S : 1) * sizeof(double));
+        h_snrml = malloc((size_t)(S > 0 ? S : 1) * 2 * sizeof(double));
+        for (int e = 0; e < E; e++) {
+            const int *t = tri + 3 * e;
+            double s1x = x[t[1]] - x[t[0]], s1y = y[t[1]] - y[t[0]];
+            double s2x = x[t[2]] - x[t[0]], s2y = y[t[2]] - y[t[0]];
+            double dj = (s1x * s2y) - (s2x * s1y);
+            if (!(fabs(dj) > 2.220446049250313e-16)) {
+                adhk_set_error("adhk_sw2_create: degenerate triangle %d", e);
+                goto fail;
+            }
+            double g6[6] = {(s1y - s2y), s2y, -s1y, (s2x - s1x), -s2x, s1x}; /* gx0 gx1 gx2 gy0 gy1 gy2 */
+            for (int k = 0; k < 6; k++) h_grad[(size_t)k * E + e] = g6[k] * (1. / dj);
+            h_djac[e] = dj * 0.5;
+            for (int a = 0; a < 3; a++) h_tri[(size_t)a * E + e] = t[a];
+            h_ks[e] = (p->fric_mode == ADHK_FRIC_MANNING && mann) ? manning_to_rheight(mann[e], p->manning_units, p->g)
+                                                                   : 0.0;
+        }
+        /* selem1d_load, selem_1d.c:42-56 */
+        for (int e = 0; e < S; e++) {
+            const int *s = seg + 2 * e;
+            double dx = x[s[1]] - x[s[0]], dy = y[s[1]] - y[s[0]];
+            double L = sqrt(dx * dx + dy * dy);
+            if (L < 1e-6) {
+                adhk_set_error("adhk_sw2_create: degenerate segment %d", e);
+                goto fail;
+            }
+            h_sdjac[e] = L;
+            h_snrml[2 * e] = (y[s[1]] - y[s[0]]) / L;
+            h_snrml[2 * e + 1] = -(x[s[1]] - x[s[0]]) / L;
+        }
+        const int N = nnodes;
+        DALLOC(c->x, N, double);
+        DALLOC(c->y, N, double);
+        DALLOC(c->z, N, double);
+        DALLOC(c->tri, 3L * E, int);
+        DALLOC(c->djac, E, double);
+        DALLOC(c->grad, 6L * E, double);
+        DALLOC(c->ks, E, double);
+        DALLOC(c->seg, 2L * S, int);
+        DALLOC(c->seg_kind, S, int);
+        DALLOC(c->seg_val, 2L * S, double);
+        DALLOC(c->sdjac, S, double);
+        DALLOC(c->snrml, 2L * S, double);
+        HC(hipMemcpy(c->x, x, (size_t)N * sizeof(double), hipMemcpyHostToDevice));
+        HC(hipMemcpy(c->y, y, (size_t)N * sizeof(double), hipMemcpyHostToDevice));
+        HC(hipMemcpy(c->z, z, (size_t)N * sizeof(double), hipMemcpyHostToDevice));
+        HC(hipMemcpy(c->tri, h_tri, (size_t)E * 3 * sizeof(int), hipMemcpyHostToDevice));
+        HC(hipMemcpy(c->djac, h_djac, (size_t)E * sizeof(double), hipMemcpyHostToDevice));
+        HC(hipMemcpy(c->grad, h_grad, (size_t)E * 6 * sizeof(double), hipMemcpyHostToDevice));
+        HC(hipMemcpy(c->ks, h_ks, (size_t)E * sizeof(double), hipMemcpyHostToDevice));
+        if (S > 0) {
+            HC(hipMemcpy(c->seg, seg, (size_t)S * 2 * sizeof(int), hipMemcpyHostToDevice));
+            HC(hipMemcpy(c->seg_kind, seg_kind, (size_t)S * sizeof(int), hipMemcpyHostToDevice));
+            HC(hipMemcpy(c->seg_val, seg_val, (size_t)S * 2 * sizeof(double), hipMemcpyHostToDevice));
+            HC(hipMemcpy(c->sdjac, h_sdjac, (size_t)S * sizeof(double), hipMemcpyHostToDevice));
+            HC(hipMemcpy(c->snrml, h_snrml, (size_t)S * 2 * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    {
+        const long nd = c->ndofs, N = c->N, E = c->E, S = c->S;
+        DALLOC(c->sol, nd, double);
+        DALLOC(c->sol_old, nd, double);
+        DALLOC(c->sol_older, nd, double);
+        DALLOC(c->dirichlet, nd, double);
+        DALLOC(c->dsol, nd, double);
+        DALLOC(c->residual, nd, double);
+        DALLOC(c->scale, nd, double);
+        DALLOC(c->bc_mask, nd, int);
+        DALLOC(c->wd_flag, E, int);
+        DALLOC(c->wd_tmp, N, int);
+        DZERO(c->sol, nd, double);
+        DZERO(c->sol_old, nd, double);
+        DZERO(c->sol_older, nd, double);
+        DZERO(c->dirichlet, nd, double);
+        DZERO(c->dsol, nd, double);
+        DZERO(c->residual, nd, double);
+        DZERO(c->scale, nd, double);
+        DZERO(c->bc_mask, nd, int);
+        DZERO(c->wd_flag, E, int);
+        DALLOC(c->erhs, 9 * E + 6 * S, double);
+        DALLOC(c->emat, 81 * E + 36 * S, double);
+        DALLOC(c->r, nd, double);
+        DALLOC(c->pv, nd, double);
+        DALLOC(c->Ap, nd, double);
+        DALLOC(c->s, nd, double);
+        DALLOC(c->As, nd, double);
+        DALLOC(c->q, nd, double);
+        DALLOC(c->x0, 2 * nd, double);
+        DALLOC(c->minv, 9 * N, double);
+        DALLOC(c->partials, 8 * 2048, double);
+        DALLOC(c->counter, 8, unsigned int);
+        DALLOC(c->sc, 1, adhk_scalars);
+        DALLOC(c->flags, 4, int);
+        DZERO(c->counter, 8, unsigned int);
+        DZERO(c->sc, 1, adhk_scalars);
+        DZERO(c->flags, 4, int);
+    }
+    LC(adhk_build_pattern(c));
+    DALLOC(c->bvals, 9L * c->nnzb, double);
+    DZERO(c->bvals, 9L * c->nnzb, double);
+    HC(hipStreamSynchronize(c->stream));
+    free(h_djac);
+    free(h_grad);
+    free(h_ks);
+    free(h_tri);
+    free(h_sdjac);
+    free(h_snrml);
+    return c;
+fail:
+    free(h_djac);
+    free(h_grad);
+    free(h_ks);
+    free(h_tri);
+    free(h_sdjac);
+    free(h_snrml);
+    ctx_free(c);
+    return NULL;
+}
+
+static int ensure_mixed(adhk_ctx *c)
+{
+    if (c->bvals32) return 0;
+    const long nd = c->ndofs, N = c->N;
+    DALLOC(c->bvals32, 9L * c->nnzb, float);
+    DALLOC(c->minv32, 9 * N, float);
+    DALLOC(c->r32, nd, float);
+    DALLOC(c->p32, nd, float);
+    DALLOC(c->Ap32, nd, float);
+    DALLOC(c->s32, nd, float);
+    DALLOC(c->As32, nd, float);
+    DALLOC(c->q32, nd, float);
+    DALLOC(c->xk32, nd, float);
+    return 0;
+fail:
+    return -1;
+}
+
+int adhk_sw2_sizes(const adhk_sw2 *c, int *ndofs, int *nnz, int *nnzb)
+{
+    if (!c) return -1;
+    if (ndofs) *ndofs = c->ndofs;
+    if (nnz) *nnz = 9 * c->nnzb;
+    if (nnzb) *nnzb = c->nnzb;
+    return 0;
+}
+
+static int copy_in(adhk_ctx *c, double *dst, const double *src)
+{
+    if (!src) return 0;
+    HC(hipMemcpy(dst, src, (size_t)c->ndofs * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+fail:
+    return -1;
+}
+static int copy_out(adhk_ctx *c, double *dst, const double *src)
+{
+    if (!dst) return 0;
+    HC(hipStreamSynchronize(c->stream));
+    HC(hipMemcpy(dst, src, (size_t)c->ndofs * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+fail:
+    return -1;
+}
+
+int adhk_sw2_set_state(adhk_sw2 *c, const double *sol, const double *sol_old, const double *sol_older)
+{
+    if (!c) return -1;
+    if (copy_in(c, c->sol, sol) || copy_in(c, c->sol_old, sol_old) || copy_in(c, c->sol_older, sol_older)) return -1;
+    return 0;
+}
+int adhk_sw2_get_state(adhk_sw2 *c, double *sol, double *sol_old, double *sol_older)
+{
+    if (!c) return -1;
+    if (copy_out(c, sol, c->sol) || copy_out(c, sol_old, c->sol_old) || copy_out(c, sol_older, c->sol_older))
+        return -1;
+    return 0;
+}
+int adhk_sw2_set_bc(adhk_sw2 *c, const int *bc_mask, const double *dirichlet)
+{
+    if (!c) return -1;
+    if (bc_mask) HC(hipMemcpy(c->bc_mask, bc_mask, (size_t)c->ndofs * sizeof(int), hipMemcpyHostToDevice));
+    if (copy_in(c, c->dirichlet, dirichlet)) return -1;
+    return 0;
+fail:
+    return -1;
+}
+int adhk_sw2_get_bc(adhk_sw2 *c, int *bc_mask)
+{
+    if (!c || !bc_mask) return -1;
+    HC(hipStreamSynchronize(c->stream));
+    HC(hipMemcpy(bc_mask, c->bc_mask, (size_t)c->ndofs * sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+fail:
+    return -1;
+}
+int adhk_sw2_set_dt(adhk_sw2 *c, double dt)
+{
+    if (!c || !(dt > 0)) return -1;
+    c->p.dt = dt;
+    return 0;
+}
+double adhk_sw2_get_dt(const adhk_sw2 *c) { return c ? c->p.dt : -1.0; }
+int adhk_sw2_get_timing(const adhk_sw2 *c, adhk_timing *t)
+{
+    if (!c || !t) return -1;
+    *t = c->timing;
+    return 0;
+}
+
+static int kt_flush(adhk_ctx *c)
+{
+    adhk_ktimer *k = &c->kt;
+    if (k->n == 0) return 0;
+    HC(hipEventSynchronize(k->b[k->n - 1]));
+    for (int i = 0; i < k->n; i++) {
+        float ms = 0.f;
+        HC(hipEventElapsedTime(&ms, k->a[i], k->b[i]));
+        k->ms[k->kind[i]] += ms;
+        k->cnt[k->kind[i]]++;
+    }
+    k->n = 0;
+    return 0;
+fail:
+    k->n = 0;
+    return -1;
+}
+
+void adhk_kt_begin(adhk_ctx *c, int kind)
+{
+    adhk_ktimer *k = &c->kt;
+    if (!k->on) return;
+    if (k->n == ADHK_KT_RING) kt_flush(c);
+    k->kind[k->n] = kind;
+    (void)hipEventRecord(k->a[k->n], c->stream);
+}
+
+void adhk_kt_end(adhk_ctx *c)
+{
+    adhk_ktimer *k = &c->kt;
+    if (!k->on) return;
+    (void)hipEventRecord(k->b[k->n], c->stream);
+    k->n++;
+}
+
+int adhk_sw2_kernel_timing(adhk_sw2 *c, int on)
+{
+    if (!c) return -1;
+    if (on && !c->kt.a[0])
+        for (int i = 0; i < ADHK_KT_RING; i++) {
+            HC(hipEventCreate(&c->kt.a[i]));
+            HC(hipEventCreate(&c->kt.b[i]));
+        }
+    if (kt_flush(c)) return -1;
+    if (on) {
+        memset(c->kt.ms, 0, sizeof c->kt.ms);
+        memset(c->kt.cnt, 0, sizeof c->kt.cnt);
+    }
+    c->kt.on = on ? 1 : 0;
+    return 0;
+fail:
+    return -1;
+}
+
+int adhk_sw2_kernel_times(adhk_sw2 *c, double *ms, long long *count)
+{
+    if (!c) return -1;
+    if (kt_flush(c)) return -1;
+    for (int i = 0; i < ADHK_KT_NKINDS; i++) {
+        if (ms) ms[i] = c->kt.ms[i];
+        if (count) count[i] = c->kt.cnt[i];
+    }
+    return 0;
+}
+
+int adhk_sw2_geometry(adhk_sw2 *c, double *djac, double *grad, double *sdjac, double *snrml)
+{
+    double *tmp = NULL;
+    const int E = c->E;
+    HC(hipStreamSynchronize(c->stream));
+    if (djac) HC(hipMemcpy(djac, c->djac, (size_t)E * sizeof(double), hipMemcpyDeviceToHost));
+    if (grad) { /* device SoA (gx0,gx1,gx2,gy0,gy1,gy2) -> per element (g0x,g0y,g1x,g1y,g2x,g2y) */
+        tmp = malloc((size_t)E * 6 * sizeof(double));
+        HC(hipMemcpy(tmp, c->grad, (size_t)E * 6 * sizeof(double), hipMemcpyDeviceToHost));
+        for (int e = 0; e < E; e++)
+            for (int a = 0; a < 3; a++) {
+                grad[6 * (size_t)e + 2 * a] = tmp[(size_t)a * E + e];
+                grad[6 * (size_t)e + 2 * a + 1] = tmp[(size_t)(3 + a) * E + e];
+            }
+    }
+    if (sdjac && c->S) HC(hipMemcpy(sdjac, c->sdjac, (size_t)c->S * sizeof(double), hipMemcpyDeviceToHost));
+    if (snrml && c->S) HC(hipMemcpy(snrml, c->snrml, (size_t)c->S * 2 * sizeof(double), hipMemcpyDeviceToHost));
+    free(tmp);
+    return 0;
+fail:
+    free(tmp);
+    return -1;
+}
+
+/* host flags: [0] partially dry element met, [1] NaN in the residual */
+static int read_flags(adhk_ctx *c, int *flags)
+{
+    HC(hipMemcpyAsync(flags, c->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    return 0;
+fail:
+    return -1;
+}
+
+static int check_partial(adhk_ctx *c)
+{
+    int f[4];
+    if (read_flags(c, f)) return -1;
+    if (f[0]) {
+        adhk_set_error("partially dry element met (wet/dry sub-element integration is not on the GPU path); "
+                       "all elements must be all-wet or all-dry");
+        return -1;
+    }
+    return 0;
+}
+
+int adhk_sw2_init(adhk_sw2 *c, int *wd_flag_out)
+{
+    if (!c) return -1;
+    LC(adhk_launch_wdflag(c));
+    LC(adhk_launch_dry_bc(c));
+    if (wd_flag_out) {
+        HC(hipStreamSynchronize(c->stream));
+        HC(hipMemcpy(wd_flag_out, c->wd_flag, (size_t)c->E * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    return 0;
+fail:
+    return -1;
+}
+
+static int residual_dev(adhk_ctx *c)
+{
+    LC(adhk_launch_elem_resid(c));
+    LC(adhk_launch_resid_gather(c));
+    c->timing.residual_calls++;
+    return 0;
+fail:
+    return -1;
+}
+
+int adhk_assemble_residual(adhk_sw2 *c, double *residual)
+{
+    if (!c) return -1;
+    HC(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
+    LC(residual_dev(c));
+    if (check_partial(c)) return -1;
+    return copy_out(c, residual, c->residual);
+fail:
+    return -1;
+}
+
+int adhk_elem_residuals(adhk_sw2 *c, double *elem_rhs)
+{
+    if (!c) return -1;
+    HC(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
+    LC(adhk_launch_elem_resid(c));
+    if (check_partial(c)) return -1;
+    if (elem_rhs)
+        HC(hipMemcpy(elem_rhs, c->erhs, ((size_t)c->E * 9 + (size_t)c->S * 6) * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    return 0;
+fail:
+    return -1;
+}
+
+int adhk_sparsity(adhk_sw2 *c, int *indptr, int *cols)
+{
+    int *di = NULL, *dc = NULL;
+    if (!c) return -1;
+    DALLOC(di, c->ndofs + 1, int);
+    DALLOC(dc, 9L * c->nnzb, int);
+    LC(adhk_launch_csr_pattern(c, di, dc));
+    HC(hipStreamSynchronize(c->stream));
+    if (indptr) HC(hipMemcpy(indptr, di, ((size_t)c->ndofs + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    if (cols) HC(hipMemcpy(cols, dc, 9 * (size_t)c->nnzb * sizeof(int), hipMemcpyDeviceToHost));
+    hipFree(di);
+    hipFree(dc);
+    return 0;
+fail:
+    if (di) hipFree(di);
+    if (dc) hipFree(dc);
+    return -1;
+}
+
+static int export_vals(adhk_ctx *c, double *vals)
+{
+    double *dv = NULL;
+    if (!vals) return 0;
+    DALLOC(dv, 9L * c->nnzb, double);
+    LC(adhk_launch_csr_vals(c, dv));
+    HC(hipStreamSynchronize(c->stream));
+    HC(hipMemcpy(vals, dv, 9 * (size_t)c->nnzb * sizeof(double), hipMemcpyDeviceToHost));
+    hipFree(dv);
+    return 0;
+fail:
+    if (dv) hipFree(dv);
+    return -1;
+}
+
+static int jacobian_dev(adhk_ctx *c)
+{
+    LC(adhk_launch_elem_jac(c));
+    LC(adhk_launch_jac_gather(c));
+    c->timing.jacobian_calls++;
+    return 0;
+fail:
+    return -1;
+}
+
+int adhk_assemble_jacobian(adhk_sw2 *c, double *vals)
+{
+    if (!c) return -1;
+    HC(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
+    LC(jacobian_dev(c));
+    if (check_partial(c)) return -1;
+    return export_vals(c, vals);
+fail:
+    return -1;
+}
+
+int adhk_elem_jacobians(adhk_sw2 *c, double *out)
+{
+    double *tmp = NULL;
+    if (!c) return -1;
+    size_t n = (size_t)c->E * 81 + (size_t)c->S * 36;
+    HC(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
+    LC(adhk_launch_elem_jac(c));
+    if (check_partial(c)) return -1;
+    if (!out) return 0;
+    tmp = malloc(n * sizeof(double));
+    HC(hipMemcpy(tmp, c->emat, n * sizeof(double), hipMemcpyDeviceToHost));
+    /* device layout [elem][col][row] -> row-major [elem][row][col] */
+    for (size_t e = 0; e < (size_t)c->E; e++)
+        for (int r = 0; r < 9; r++)
+            for (int cc = 0; cc < 9; cc++) out[81 * e + 9 * r + cc] = tmp[81 * e + 9 * cc + r];
+    size_t off = 81 * (size_t)c->E;
+    for (size_t e = 0; e < (size_t)c->S; e++)
+        for (int r = 0; r < 6; r++)
+            for (int cc = 0; cc < 6; cc++) out[off + 36 * e + 6 * r + cc] = tmp[off + 36 * e + 6 * cc + r];
+    free(tmp);
+    return 0;
+fail:
+    free(tmp);
+    return -1;
+}
+
+int adhk_check_diag(adhk_sw2 *c)
+{
+    if (!c) return -1;
+    return adhk_launch_check_diag(c);
+}
+int adhk_apply_dirichlet_bc(adhk_sw2 *c)
+{
+    if (!c) return -1;
+    return adhk_launch_dirichlet(c, 0);
+}
+int adhk_get_system(adhk_sw2 *c, double *vals, double *residual)
+{
+    if (!c) return -1;
+    if (export_vals(c, vals)) return -1;
+    return copy_out(c, residual, c->residual);
+}
+
+/* residual norms of get_residual_norms (newton_tools.c:303-373); l2 = sqrt(sum r^2) / ndofs as coded */
+static int norms(adhk_ctx *c, adhk_scalars *hs, int *nan)
+{
+    int f[4];
+    LC(adhk_launch_norms(c));
+    HC(hipMemcpyAsync(hs, c->sc, sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
+    HC(hipMemcpyAsync(f, c->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    if (f[0]) {
+        adhk_set_error("partially dry element met (wet/dry sub-element integration is not on the GPU path); "
+                       "all elements must be all-wet or all-dry");
+        return -1;
+    }
+    *nan = f[1];
+    return 0;
+fail:
+    return -1;
+}
+
+/* solve_linear_sys_bcgstab on the device; returns the iteration count */
+static int krylov(adhk_ctx *c, const adhk_solver *s, int mixed)
+{
+    adhk_scalars hs;
+    double tol = mixed ? s->mixed_lin_tol : s->lin_tol;
+    int chunk = s->lin_check_every > 0 ? s->lin_check_every : 8;
+    LC(adhk_launch_krylov_init(c, mixed, tol, s->lin_max_it));
+    HC(hipMemcpyAsync(&hs, c->sc, sizeof hs, hipMemcpyDeviceToHost, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    int launched = 0;
+    while (!hs.done && launched < s->lin_max_it) {
+        int k = chunk;
+        if (launched + k > s->lin_max_it) k = s->lin_max_it - launched;
+        for (int i = 0; i < k; i++) LC(adhk_launch_krylov_iter(c, mixed));
+        launched += k;
+        HC(hipMemcpyAsync(&hs, c->sc, sizeof hs, hipMemcpyDeviceToHost, c->stream));
+        HC(hipStreamSynchronize(c->stream));
+    }
+    LC(adhk_launch_krylov_finish(c, mixed));
+    c->timing.krylov_calls++;
+    return hs.it;
+fail:
+    return -1;
+}
+
+/* fe_newton, fe_newton.c:44-926 (non-PETSc branch) */
+int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
+{
+    adhk_newton_stats loc;
+    adhk_scalars hs;
+    int nan = 0, it = 0, solv_flag = 1, keep;
+    double rmax, rl2, incmax, old_rl2;
+    const int mixed = (s->precision == ADHK_PREC_MIXED);
+    memset(&loc, 0, sizeof loc);
+    if (!c || !s) return -1;
+    if (mixed && ensure_mixed(c)) return -1;
+    HC(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
+    /* initialize_system + initialize_dirichlet_bc (newton_tools.c:21-53, 130-142) */
+    LC(adhk_launch_init_system(c));
+    LC(residual_dev(c));
+    LC(norms(c, &hs, &nan));
+    rmax = hs.rmax;
+    rl2 = sqrt(hs.rl2sum) / c->ndofs;
+    incmax = hs.incmax;
+    if (nan) solv_flag = 0;
+    do {
+        LC(jacobian_dev(c));
+        LC(adhk_launch_check_diag(c));
+        LC(adhk_launch_dirichlet(c, 1));
+        LC(adhk_launch_scale_prec(c, mixed));
+        int lits = krylov(c, s, mixed);
+        if (lits < 0) goto fail;
+        loc.lin_its_total += lits;
+        if (lits > loc.lin_its_max) loc.lin_its_max = lits;
+        LC(adhk_launch_increment(c));
+        old_rl2 = rl2;
+        LC(residual_dev(c));
+        LC(norms(c, &hs, &nan));
+        rmax = hs.rmax;
+        rl2 = sqrt(hs.rl2sum) / c->ndofs;
+        incmax = hs.incmax;
+        if (nan || isnan(rl2) || isinf(rl2)) solv_flag = 0;
+        int cuts = 0;
+        while (solv_flag && (rmax > s->tol_nonlin) && (rl2 > old_rl2) && (cuts < s->max_linesearch_cuts)) {
+            LC(adhk_launch_linesearch(c));
+            LC(residual_dev(c));
+            LC(norms(c, &hs, &nan));
+            rmax = hs.rmax;
+            rl2 = sqrt(hs.rl2sum) / c->ndofs;
+            incmax = hs.incmax;
+            if (nan) solv_flag = 0;
+            cuts++;
+        }
+        loc.linesearch_cuts_total += cuts;
+        it++;
+        keep = solv_flag && (it < s->max_nonlin_it) && (rmax < 1.0E+20) &&
+               (rmax > s->tol_nonlin && incmax > s->inc_nonlin);
+    } while (keep);
+    loc.newton_its = it;
+    loc.resid_max_norm = rmax;
+    loc.inc_max_norm = incmax;
+    loc.converged = 1;
+    if (!solv_flag || ((rmax > s->tol_nonlin) && (incmax > s->inc_nonlin)) || (rmax >= 1.0E+20)) {
+        c->p.dt *= 0.25; /* tc_scale, tc_scale.c:3-19 (DT_REDUCE_FACTOR, define.h:127) */
+        LC(adhk_launch_init_system(c));
+        HC(hipStreamSynchronize(c->stream));
+        loc.converged = 0;
+    }
+    if (st) *st = loc;
+    return loc.converged;
+fail:
+    if (st) *st = loc;
+    return -1;
+}
+
+/* time_loop / advance_time (time_loop.c:14-93): prep_sol then fe_newton per step */
+int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_stats *acc)
+{
+    adhk_newton_stats tot, st;
+    int ok = 1, k;
+    memset(&tot, 0, sizeof tot);
+    if (!c || !s) return -1000000000;
+    for (k = 0; k < nsteps && ok == 1; k++) {
+        if (adhk_launch_rotate(c)) return -1000000000;
+        ok = adhk_fe_newton(c, s, &st);
+        if (ok < 0) return -1000000000;
+        tot.newton_its += st.newton_its;
+        tot.linesearch_cuts_total += st.linesearch_cuts_total;
+        tot.lin_its_total += st.lin_its_total;
+        if (st.lin_its_max > tot.lin_its_max) tot.lin_its_max = st.lin_its_max;
+        tot.resid_max_norm = st.resid_max_norm;
+        tot.inc_max_norm = st.inc_max_norm;
+    }
+    tot.converged = ok;
+    if (acc) *acc = tot;
+    return ok == 1 ? k : -k;
+}

@@ -1,0 +1,373 @@
+/*
+ * sw2_kernels.hip -- SW2 elemental residual, central-FD elemental Jacobian,
+ * wet/dry flags, and the deterministic gather assembly into the node-block
+ * CSR matrix (gfx950).
+ *
+ * Assembly is atomic-free: every block (and every residual node) sums its
+ * contributions in element order (2-D elements first, then 1-D), which is
+ * the order of the reference's sequential scatter
+ * (assemble_residual.c:69-92, assemble_jacobian.c:20-106).
+ */
+#include <hip/hip_runtime.h>
+
+#include "adhk_internal.h"
+#include "sw2_device.h"
+
+#define CHECK_LAUNCH()                                                                                            \
+    do {                                                                                                          \
+        hipError_t e_ = hipGetLastError();                                                                        \
+        if (e_ != hipSuccess) {                                                                                   \
+            adhk_set_error("%s: %s", __func__, hipGetErrorString(e_));                                            \
+            return -1;                                                                                            \
+        }                                                                                                         \
+    } while (0)
+
+static inline Sw2Par make_par(const adhk_ctx *c)
+{
+    Sw2Par P;
+    P.g = c->p.g;
+    P.dt = c->p.dt;
+    P.alpha = c->p.alpha;
+    P.cf = c->p.cf_const;
+    P.manning = (c->p.fric_mode == ADHK_FRIC_MANNING);
+    return P;
+}
+
+struct MeshView {
+    const double *x, *y, *z, *djac, *grad, *ks;
+    const int *tri, *wd;
+    const double *sol, *sol_old, *sol_older;
+    int E;
+};
+
+__device__ __forceinline__ void gather_tri(const MeshView &m, int e, bool older, TriIn &in)
+{
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int n = m.tri[a * m.E + e];
+        in.x[a] = m.x[n];
+        in.y[a] = m.y[n];
+        in.z[a] = m.z[n];
+        in.h[a] = m.sol[3 * n];
+        in.u[a] = m.sol[3 * n + 1];
+        in.v[a] = m.sol[3 * n + 2];
+        in.ho[a] = m.sol_old[3 * n];
+        in.uo[a] = m.sol_old[3 * n + 1];
+        in.vo[a] = m.sol_old[3 * n + 2];
+        if (older) {
+            in.hoo[a] = m.sol_older[3 * n];
+            in.uoo[a] = m.sol_older[3 * n + 1];
+            in.voo[a] = m.sol_older[3 * n + 2];
+        } else {
+            in.hoo[a] = in.uoo[a] = in.voo[a] = 0.0;
+        }
+        in.gx[a] = m.grad[a * m.E + e];
+        in.gy[a] = m.grad[(3 + a) * m.E + e];
+    }
+    in.djac = m.djac[e];
+    in.ks = m.ks[e];
+    in.wd_flag = m.wd[e];
+}
+
+static MeshView make_view(const adhk_ctx *c)
+{
+    MeshView m;
+    m.x = c->x;
+    m.y = c->y;
+    m.z = c->z;
+    m.djac = c->djac;
+    m.grad = c->grad;
+    m.ks = c->ks;
+    m.tri = c->tri;
+    m.wd = c->wd_flag;
+    m.sol = c->sol;
+    m.sol_old = c->sol_old;
+    m.sol_older = c->sol_older;
+    m.E = c->E;
+    return m;
+}
+
+/* ---------------- residual ---------------- */
+__global__ __launch_bounds__(256) void k_elem_resid(MeshView m, Sw2Par P, double *__restrict__ erhs, int *flags)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.E) return;
+    TriIn in;
+    gather_tri(m, e, P.alpha != 0.0, in);
+    double r[9];
+    int partial = sw2_tri_resid(in, P, 0, -1, 0.0, r);
+    if (partial) flags[0] = 1;
+#pragma unroll
+    for (int k = 0; k < 9; k++) erhs[9 * (size_t)e + k] = r[k];
+}
+
+struct SegView {
+    const int *seg, *kind;
+    const double *val, *sdjac, *nrml, *sol;
+    int S;
+};
+
+__global__ void k_seg_resid(SegView s, Sw2Par P, double *__restrict__ erhs)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= s.S) return;
+    double h[2], u[2], v[2];
+    for (int a = 0; a < 2; a++) {
+        int n = s.seg[2 * e + a];
+        h[a] = s.sol[3 * n];
+        u[a] = s.sol[3 * n + 1];
+        v[a] = s.sol[3 * n + 2];
+    }
+    double er[6];
+    sw2_seg_resid(s.kind[e], s.val[2 * e], s.val[2 * e + 1], s.nrml[2 * e], s.nrml[2 * e + 1], s.sdjac[e], h, u, v,
+                  P, 0, -1, 0.0, er);
+    for (int k = 0; k < 6; k++) erhs[6 * (size_t)e + k] = er[k];
+}
+
+static SegView make_segview(const adhk_ctx *c)
+{
+    SegView s;
+    s.seg = c->seg;
+    s.kind = c->seg_kind;
+    s.val = c->seg_val;
+    s.sdjac = c->sdjac;
+    s.nrml = c->snrml;
+    s.sol = c->sol;
+    s.S = c->S;
+    return s;
+}
+
+extern "C" int adhk_launch_elem_resid(adhk_ctx *c)
+{
+    adhk_kt_begin(c, ADHK_KT_ELEM_RESID);
+    Sw2Par P = make_par(c);
+    if (c->E > 0) {
+        hipLaunchKernelGGL(k_elem_resid, dim3((c->E + 255) / 256), dim3(256), 0, c->stream, make_view(c), P, c->erhs,
+                           c->flags);
+        CHECK_LAUNCH();
+    }
+    if (c->S > 0) {
+        hipLaunchKernelGGL(k_seg_resid, dim3((c->S + 127) / 128), dim3(128), 0, c->stream, make_segview(c), P,
+                           c->erhs + 9 * (size_t)c->E);
+        CHECK_LAUNCH();
+    }
+    adhk_kt_end(c);
+    return 0;
+}
+
+/* residual[3n+k] = sum over the diagonal block's contributions (element order) of -erhs */
+__global__ __launch_bounds__(256) void k_resid_gather(int N, int E, const int *__restrict__ diag_blk,
+                                                      const int *__restrict__ cptr, const int *__restrict__ contrib,
+                                                      const double *__restrict__ erhs, double *__restrict__ residual)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    int b = diag_blk[n];
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+    const long segbase = 9L * E;
+    for (int j = cptr[b]; j < cptr[b + 1]; j++) {
+        int cde = contrib[j];
+        const double *er;
+        if (cde < 9 * E) {
+            int e = cde / 9, a = (cde % 9) / 3;
+            er = erhs + 9L * e + 3 * a;
+        } else {
+            int t = cde - 9 * E;
+            int s = t / 4, a = (t % 4) / 2;
+            er = erhs + segbase + 6L * s + 3 * a;
+        }
+        r0 += -1.0 * er[0];
+        r1 += -1.0 * er[1];
+        r2 += -1.0 * er[2];
+    }
+    residual[3 * n] = r0;
+    residual[3 * n + 1] = r1;
+    residual[3 * n + 2] = r2;
+}
+
+extern "C" int adhk_launch_resid_gather(adhk_ctx *c)
+{
+    adhk_kt_begin(c, ADHK_KT_RESID_GATHER);
+    hipLaunchKernelGGL(k_resid_gather, dim3((c->N + 255) / 256), dim3(256), 0, c->stream, c->N, c->E, c->diag_blk,
+                       c->cptr, c->contrib, c->erhs, c->residual);
+    CHECK_LAUNCH();
+    adhk_kt_end(c);
+    return 0;
+}
+
+/* ---------------- FD Jacobian ---------------- */
+/* one thread per (element, column): column = node i * 3 + var k; two residual
+ * evaluations (+eps, -eps), load_elem_mat/perturb_var assemble_jacobian.c:123-363 */
+__global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, double *__restrict__ emat, int *flags)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 9 * m.E) return;
+    int e = t / 9, col = t % 9;
+    int i = col / 3, k = col % 3;
+    TriIn in;
+    gather_tri(m, e, P.alpha != 0.0, in);
+    double var = (k == 0) ? in.h[i] : ((k == 1) ? in.u[i] : in.v[i]);
+    double eps = num_diff_eps(var, 1e-4);
+    double eps2 = 2.0 * eps;
+    double rp[9], rm[9];
+    int pa = sw2_tri_resid(in, P, i, k, (+1) * eps, rp);
+    int pb = sw2_tri_resid(in, P, i, k, (-1) * eps, rm);
+    if (pa | pb) flags[0] = 1;
+    double *out = emat + 81 * (size_t)e + 9 * col;
+#pragma unroll
+    for (int r = 0; r < 9; r++) out[r] = (rp[r] - rm[r]) / eps2;
+}
+
+__global__ void k_seg_jac(SegView s, Sw2Par P, double *__restrict__ emat)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 6 * s.S) return;
+    int e = t / 6, col = t % 6;
+    int i = col / 3, k = col % 3;
+    double h[2], u[2], v[2];
+    for (int a = 0; a < 2; a++) {
+        int n = s.seg[2 * e + a];
+        h[a] = s.sol[3 * n];
+        u[a] = s.sol[3 * n + 1];
+        v[a] = s.sol[3 * n + 2];
+    }
+    double var = (k == 0) ? h[i] : ((k == 1) ? u[i] : v[i]);
+    double eps = num_diff_eps(var, 1e-4);
+    double eps2 = 2.0 * eps;
+    double rp[6], rm[6];
+    sw2_seg_resid(s.kind[e], s.val[2 * e], s.val[2 * e + 1], s.nrml[2 * e], s.nrml[2 * e + 1], s.sdjac[e], h, u, v,
+                  P, i, k, (+1) * eps, rp);
+    sw2_seg_resid(s.kind[e], s.val[2 * e], s.val[2 * e + 1], s.nrml[2 * e], s.nrml[2 * e + 1], s.sdjac[e], h, u, v,
+                  P, i, k, (-1) * eps, rm);
+    double *out = emat + 36 * (size_t)e + 6 * col;
+    for (int r = 0; r < 6; r++) out[r] = (rp[r] - rm[r]) / eps2;
+}
+
+extern "C" int adhk_launch_elem_jac(adhk_ctx *c)
+{
+    adhk_kt_begin(c, ADHK_KT_ELEM_JAC);
+    Sw2Par P = make_par(c);
+    if (c->E > 0) {
+        long nt = 9L * c->E;
+        hipLaunchKernelGGL(k_elem_jac, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, make_view(c), P,
+                           c->emat, c->flags);
+        CHECK_LAUNCH();
+    }
+    if (c->S > 0) {
+        hipLaunchKernelGGL(k_seg_jac, dim3((6 * c->S + 127) / 128), dim3(128), 0, c->stream, make_segview(c), P,
+                           c->emat + 81 * (size_t)c->E);
+        CHECK_LAUNCH();
+    }
+    adhk_kt_end(c);
+    return 0;
+}
+
+/* bvals[b][3r+cc] = sum over contributions (element order) of the elemental entry
+ * (row a*3+r, col bb*3+cc); emat is stored [elem][col][row] */
+__global__ __launch_bounds__(256) void k_jac_gather(int nnzb, int E, const int *__restrict__ cptr,
+                                                    const int *__restrict__ contrib, const double *__restrict__ emat,
+                                                    double *__restrict__ bvals)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nnzb) return;
+    double acc[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) acc[k] = 0.0;
+    const long segbase = 81L * E;
+    for (int j = cptr[b]; j < cptr[b + 1]; j++) {
+        int cde = contrib[j];
+        const double *em;
+        int a, bb, ld;
+        if (cde < 9 * E) {
+            int e = cde / 9;
+            a = (cde % 9) / 3;
+            bb = cde % 3;
+            em = emat + 81L * e;
+            ld = 9;
+        } else {
+            int t = cde - 9 * E;
+            int s = t / 4;
+            a = (t % 4) / 2;
+            bb = t % 2;
+            em = emat + segbase + 36L * s;
+            ld = 6;
+        }
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) acc[3 * r + cc] += em[(3 * bb + cc) * ld + 3 * a + r];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) bvals[9 * (size_t)b + k] = acc[k];
+}
+
+extern "C" int adhk_launch_jac_gather(adhk_ctx *c)
+{
+    adhk_kt_begin(c, ADHK_KT_JAC_GATHER);
+    hipLaunchKernelGGL(k_jac_gather, dim3((c->nnzb + 255) / 256), dim3(256), 0, c->stream, c->nnzb, c->E, c->cptr,
+                       c->contrib, c->emat, c->bvals);
+    CHECK_LAUNCH();
+    adhk_kt_end(c);
+    return 0;
+}
+
+/* ---------------- wet/dry flags, fe_sw2_wdflag_legacy sw2_wdflag_legacy.c:34-139 ---------------- */
+__global__ void k_wd_zero(int N, int *tmp)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < N) tmp[n] = 0;
+}
+__global__ void k_wd_mark(int E, const int *tri, const double *sol, int *tmp)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    int wet = 0;
+    for (int a = 0; a < 3; a++)
+        if (sol[3 * tri[a * E + e]] > 0.0) wet++;
+    if (wet != 3)
+        for (int a = 0; a < 3; a++) tmp[tri[a * E + e]] = 1; /* every writer stores the same value */
+}
+__global__ void k_wd_flag(int E, const int *tri, const double *sol, const double *sol_old, const int *tmp, int *wd)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    int n0 = tri[e], n1 = tri[E + e], n2 = tri[2 * E + e];
+    int f = (tmp[n0] != 0 || tmp[n1] != 0 || tmp[n2] != 0) ? 1 : 0;
+    if (sol[3 * n0] < 0 && sol[3 * n1] < 0 && sol[3 * n2] < 0 && sol_old[3 * n0] < 0 && sol_old[3 * n1] < 0 &&
+        sol_old[3 * n2] < 0)
+        f = 2;
+    wd[e] = f;
+}
+
+extern "C" int adhk_launch_wdflag(adhk_ctx *c)
+{
+    hipLaunchKernelGGL(k_wd_zero, dim3((c->N + 255) / 256), dim3(256), 0, c->stream, c->N, c->wd_tmp);
+    CHECK_LAUNCH();
+    if (c->E > 0) {
+        hipLaunchKernelGGL(k_wd_mark, dim3((c->E + 255) / 256), dim3(256), 0, c->stream, c->E, c->tri, c->sol,
+                           c->wd_tmp);
+        CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_wd_flag, dim3((c->E + 255) / 256), dim3(256), 0, c->stream, c->E, c->tri, c->sol,
+                           c->sol_old, c->wd_tmp, c->wd_flag);
+        CHECK_LAUNCH();
+    }
+    return 0;
+}
+
+/* fe_sw2_init, fe_sw2_init.c:28-53: nodes dry at the old level get u, v Dirichlet */
+__global__ void k_dry_bc(int N, const double *sol_old, int *bc_mask)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    if (sol_old[3 * n] <= 0.) {
+        bc_mask[3 * n + 1] = 1;
+        bc_mask[3 * n + 2] = 1;
+    }
+}
+
+extern "C" int adhk_launch_dry_bc(adhk_ctx *c)
+{
+    hipLaunchKernelGGL(k_dry_bc, dim3((c->N + 255) / 256), dim3(256), 0, c->stream, c->N, c->sol_old, c->bc_mask);
+    CHECK_LAUNCH();
+    return 0;
+}

@@ -1,0 +1,278 @@
+"""Host-side mirror of the reference's SW2 super-model interface over libadhk.so.
+
+Names follow the reference: an ``SModelSuper`` carries ``sol``, ``sol_old``,
+``sol_older``, ``bc_mask``, ``dirichlet_data`` and the Newton knobs of
+SMODEL_SUPER (src/structs/smodel_super/smodel_super.h:7-113, defaults from
+smodel_super_alloc_init.c:55-100); ``assemble_residual``, ``assemble_jacobian``,
+``fe_newton`` and ``time_loop`` take it like the reference functions
+(residual.h:5, jacobian.h:5, newton.h:15, time_loop.c:14).  Arrays live on the
+GPU inside the context; the properties copy to and from host numpy arrays.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+
+@dataclass
+class Mesh:
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    tri: np.ndarray            # (E, 3) int32, counter-clockwise
+    seg: np.ndarray            # (S, 2) int32, domain on the left
+    seg_kind: np.ndarray       # (S,) int32, BC_DEPTH / BC_FLUX / BC_VEL
+    seg_val: np.ndarray        # (S, 2) float64
+    mann: np.ndarray = None    # (E,) Manning n
+
+    @property
+    def nnodes(self):
+        return int(self.x.shape[0])
+
+    @property
+    def nelems2d(self):
+        return int(self.tri.shape[0])
+
+    @property
+    def nelems1d(self):
+        return int(self.seg.shape[0])
+
+
+@dataclass
+class FlumeSpec:
+    xmin: float = 0.0
+    xmax: float = 200.0
+    ymin: float = 0.0
+    ymax: float = 20.0
+    npx: int = 201
+    npy: int = 21
+    slope: float = 1e-4
+    nbumps: int = 0
+    bump_amp_max: float = 0.0
+    bump_sigma_min: float = 5.0
+    bump_sigma_max: float = 30.0
+    seed: int = 1
+    manning_n: float = 0.03
+    inflow_q: float = 10.0
+    tailwater: float = 1.0
+    ic_flat_wse: int = 0
+
+    def c(self):
+        s = L.FlumeSpec()
+        for k in self.__dataclass_fields__:
+            setattr(s, k, getattr(self, k))
+        return s
+
+
+# the configurations of the workload set (sizes and physics as specified)
+FLUME_C1 = FlumeSpec()  # 200 m x 20 m at 1 m, n = 0.03, Q = 10 m^3/s, tailwater 1.0 m, h0 = 1.0 m
+FLUME_C2 = FlumeSpec(xmax=2000.0, ymax=200.0, npx=2001, npy=201, nbumps=50, bump_amp_max=0.3,
+                     bump_sigma_min=5.0, bump_sigma_max=30.0, seed=1, inflow_q=200.0, tailwater=2.0,
+                     ic_flat_wse=1)
+
+
+def flume_mesh(spec: FlumeSpec):
+    """Rectangular flume (create_rectangular_grid.c:4-445 numbering) with boundary segments."""
+    lib = L.load()
+    cs = spec.c()
+    n, e, s = C.c_int(), C.c_int(), C.c_int()
+    L.check(lib.adhk_flume_counts(C.byref(cs), C.byref(n), C.byref(e), C.byref(s)), "adhk_flume_counts")
+    N, E, S = n.value, e.value, s.value
+    x, y, z = (np.zeros(N) for _ in range(3))
+    tri = np.zeros((E, 3), np.int32)
+    mann = np.zeros(E)
+    seg = np.zeros((S, 2), np.int32)
+    kind = np.zeros(S, np.int32)
+    val = np.zeros((S, 2))
+    L.check(lib.adhk_flume_build(C.byref(cs), L.dptr(x), L.dptr(y), L.dptr(z), L.iptr(tri), L.dptr(mann),
+                                 L.iptr(seg), L.iptr(kind), L.dptr(val)), "adhk_flume_build")
+    sol = np.zeros(3 * N)
+    L.check(lib.adhk_flume_initial_state(C.byref(cs), N, L.dptr(z), L.dptr(sol)), "adhk_flume_initial_state")
+    return Mesh(x, y, z, tri, seg, kind, val, mann), sol
+
+
+@dataclass
+class TimeInfo:  # SDT (src/structs/sdt/sdt.h:7-16)
+    dt: float = 1.0
+    tau_temporal: float = 0.0
+
+
+class SModelSuper:
+    """One SW2 super model on one GPU (SMODEL_SUPER)."""
+
+    def __init__(self, mesh: Mesh, dt=1.0, gravity=9.8, friction="manning", cf_const=0.002,
+                 tau_temporal=0.0, manning_units=1.0):
+        self.lib = L.load()
+        self.mesh = mesh
+        self.gravity = gravity
+        p = L.Params(g=gravity, dt=dt, alpha=tau_temporal, cf_const=cf_const, manning_units=manning_units,
+                     fric_mode=L.FRIC_MANNING if friction == "manning" else L.FRIC_CONST)
+        m = mesh
+        mann = m.mann if m.mann is not None else np.zeros(m.nelems2d)
+        self._keep = [np.ascontiguousarray(a) for a in (m.x, m.y, m.z)]
+        tri = np.ascontiguousarray(m.tri, np.int32)
+        seg = np.ascontiguousarray(m.seg, np.int32)
+        kind = np.ascontiguousarray(m.seg_kind, np.int32)
+        val = np.ascontiguousarray(m.seg_val, np.float64)
+        mann = np.ascontiguousarray(mann, np.float64)
+        h = self.lib.adhk_sw2_create(m.nnodes, L.dptr(self._keep[0]), L.dptr(self._keep[1]), L.dptr(self._keep[2]),
+                                     m.nelems2d, L.iptr(tri), L.dptr(mann), m.nelems1d,
+                                     L.iptr(seg) if m.nelems1d else None, L.iptr(kind) if m.nelems1d else None,
+                                     L.dptr(val) if m.nelems1d else None, C.byref(p))
+        if not h:
+            raise L.AdhkError("adhk_sw2_create failed: " + L.last_error())
+        self.h = C.c_void_p(h)
+        nd, nnz, nnzb = C.c_int(), C.c_int(), C.c_int()
+        self.lib.adhk_sw2_sizes(self.h, C.byref(nd), C.byref(nnz), C.byref(nnzb))
+        self.ndofs, self.nnz, self.nnz_blocks = nd.value, nnz.value, nnzb.value
+        # Newton knobs (smodel_super_alloc_init.c:62-65) and linear solver knobs (bcgstab_solver.c:20-22)
+        self.solver = L.solver_defaults()
+        self.ts = TimeInfo(dt=dt, tau_temporal=tau_temporal)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.adhk_sw2_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- Newton knobs with the reference's names ----
+    tol_nonlin = property(lambda s: s.solver.tol_nonlin, lambda s, v: setattr(s.solver, "tol_nonlin", v))
+    inc_nonlin = property(lambda s: s.solver.inc_nonlin, lambda s, v: setattr(s.solver, "inc_nonlin", v))
+    max_nonlin_it = property(lambda s: s.solver.max_nonlin_it, lambda s, v: setattr(s.solver, "max_nonlin_it", v))
+    max_nonlin_linesearch_cuts = property(lambda s: s.solver.max_linesearch_cuts,
+                                          lambda s, v: setattr(s.solver, "max_linesearch_cuts", v))
+
+    # ---- state ----
+    def _get(self, which):
+        a = np.zeros(self.ndofs)
+        args = [None, None, None]
+        args[which] = L.dptr(a)
+        L.check(self.lib.adhk_sw2_get_state(self.h, *args), "adhk_sw2_get_state")
+        return a
+
+    def _set(self, which, v):
+        v = np.ascontiguousarray(v, np.float64)
+        assert v.shape == (self.ndofs,)
+        args = [None, None, None]
+        args[which] = L.dptr(v)
+        L.check(self.lib.adhk_sw2_set_state(self.h, *args), "adhk_sw2_set_state")
+
+    sol = property(lambda s: s._get(0), lambda s, v: s._set(0, v))
+    sol_old = property(lambda s: s._get(1), lambda s, v: s._set(1, v))
+    sol_older = property(lambda s: s._get(2), lambda s, v: s._set(2, v))
+
+    @property
+    def bc_mask(self):
+        a = np.zeros(self.ndofs, np.int32)
+        L.check(self.lib.adhk_sw2_get_bc(self.h, L.iptr(a)), "adhk_sw2_get_bc")
+        return a
+
+    def set_bc(self, bc_mask=None, dirichlet_data=None):
+        bm = None if bc_mask is None else np.ascontiguousarray(bc_mask, np.int32)
+        dd = None if dirichlet_data is None else np.ascontiguousarray(dirichlet_data, np.float64)
+        L.check(self.lib.adhk_sw2_set_bc(self.h, L.iptr(bm), L.dptr(dd)), "adhk_sw2_set_bc")
+
+    def set_dt(self, dt):
+        L.check(self.lib.adhk_sw2_set_dt(self.h, dt), "adhk_sw2_set_dt")
+        self.ts.dt = dt
+
+    def get_dt(self):
+        return self.lib.adhk_sw2_get_dt(self.h)
+
+    def set_initial(self, sol):
+        """sol = sol_old = sol_older = the given state (a cold start)."""
+        for w in range(3):
+            self._set(w, sol)
+
+    def geometry(self):
+        E, S = self.mesh.nelems2d, self.mesh.nelems1d
+        djac, grad, sd, sn = np.zeros(E), np.zeros(6 * E), np.zeros(max(S, 1)), np.zeros(2 * max(S, 1))
+        L.check(self.lib.adhk_sw2_geometry(self.h, L.dptr(djac), L.dptr(grad), L.dptr(sd), L.dptr(sn)),
+                "adhk_sw2_geometry")
+        return djac, grad.reshape(E, 3, 2), sd[:S], sn[:2 * S].reshape(S, 2)
+
+
+# ---- reference-named operations ----
+def fe_sw2_init(sm: SModelSuper):
+    """fe_sw2_init (fe_sw2_init.c:28): wet/dry flags, dry nodes get u,v Dirichlet.  Returns wd flags."""
+    wd = np.zeros(sm.mesh.nelems2d, np.int32)
+    L.check(sm.lib.adhk_sw2_init(sm.h, L.iptr(wd)), "adhk_sw2_init")
+    return wd
+
+
+def assemble_residual(sm: SModelSuper):
+    """assemble_residual (assemble_residual.c:69): residual = -sum of elemental residuals at sm.sol."""
+    r = np.zeros(sm.ndofs)
+    L.check(sm.lib.adhk_assemble_residual(sm.h, L.dptr(r)), "adhk_assemble_residual")
+    return r
+
+
+def elem_residuals(sm: SModelSuper):
+    E, S = sm.mesh.nelems2d, sm.mesh.nelems1d
+    out = np.zeros(9 * E + 6 * S)
+    L.check(sm.lib.adhk_elem_residuals(sm.h, L.dptr(out)), "adhk_elem_residuals")
+    return out[:9 * E].reshape(E, 9), out[9 * E:].reshape(S, 6)
+
+
+def sparsity(sm: SModelSuper):
+    """slin_sys_init_sparsity_mono (slin_sys_init_sparsity_mono.c:28): CSR pattern built on the GPU."""
+    indptr = np.zeros(sm.ndofs + 1, np.int32)
+    cols = np.zeros(sm.nnz, np.int32)
+    L.check(sm.lib.adhk_sparsity(sm.h, L.iptr(indptr), L.iptr(cols)), "adhk_sparsity")
+    return indptr, cols
+
+
+def assemble_jacobian(sm: SModelSuper):
+    """assemble_jacobian (assemble_jacobian.c:20): central-FD Jacobian values in the CSR order of sparsity()."""
+    vals = np.zeros(sm.nnz)
+    L.check(sm.lib.adhk_assemble_jacobian(sm.h, L.dptr(vals)), "adhk_assemble_jacobian")
+    return vals
+
+
+def elem_jacobians(sm: SModelSuper):
+    E, S = sm.mesh.nelems2d, sm.mesh.nelems1d
+    out = np.zeros(81 * E + 36 * S)
+    L.check(sm.lib.adhk_elem_jacobians(sm.h, L.dptr(out)), "adhk_elem_jacobians")
+    return out[:81 * E].reshape(E, 9, 9), out[81 * E:].reshape(S, 6, 6)
+
+
+def check_diag(sm: SModelSuper):
+    L.check(sm.lib.adhk_check_diag(sm.h), "adhk_check_diag")
+
+
+def apply_Dirichlet_BC(sm: SModelSuper):
+    L.check(sm.lib.adhk_apply_dirichlet_bc(sm.h), "adhk_apply_dirichlet_bc")
+
+
+def get_system(sm: SModelSuper):
+    vals, res = np.zeros(sm.nnz), np.zeros(sm.ndofs)
+    L.check(sm.lib.adhk_get_system(sm.h, L.dptr(vals), L.dptr(res)), "adhk_get_system")
+    return vals, res
+
+
+def fe_newton(sm: SModelSuper, precision=L.PREC_FP64):
+    """fe_newton (fe_newton.c:44): returns (TRUE/FALSE, stats dict)."""
+    sm.solver.precision = precision
+    st = L.NewtonStats()
+    rc = sm.lib.adhk_fe_newton(sm.h, C.byref(sm.solver), C.byref(st))
+    L.check(rc, "adhk_fe_newton")
+    sm.ts.dt = sm.get_dt()
+    return rc, st.as_dict()
+
+
+def time_loop(sm: SModelSuper, nsteps, precision=L.PREC_FP64):
+    """time_loop (time_loop.c:14): nsteps x (prep_sol + fe_newton).  Returns (steps done, stats dict)."""
+    sm.solver.precision = precision
+    st = L.NewtonStats()
+    rc = sm.lib.adhk_time_loop(sm.h, C.byref(sm.solver), nsteps, C.byref(st))
+    if rc <= -1000000000:
+        raise L.AdhkError("adhk_time_loop failed: " + L.last_error())
+    sm.ts.dt = sm.get_dt()
+    return rc, st.as_dict()

@@ -1,0 +1,201 @@
+/*
+ * adhk.h -- C-ABI of libadhk.so, the MI355X (gfx950) implementation of the
+ * AdH-Kraken SW2 Newton step.  Plain pointers and sizes only.
+ *
+ * Two groups of entry points:
+ *
+ *  1. adhk_* : the reference's struct-based call surface (SMODEL_SUPER*,
+ *     SGRID*) restated over an opaque context that owns the mesh and the
+ *     device-resident state.  Each function names the reference function it
+ *     replaces.  A shim compiled inside the reference tree maps the reference
+ *     structs onto these calls.
+ *
+ *  2. The reference's plain-pointer linear-algebra surface (src/la/la.h) with
+ *     the reference's exact names and signatures, operating on host CSR arrays.
+ *
+ * Unknown layout everywhere: dof(node, var) = 3*node + var, var 0 = h (depth),
+ * 1 = u, 2 = v (src/structs/sivar_position/sivar_position.c:61-80).
+ *
+ * Error convention: functions returning int return >= 0 on success and -1 on
+ * a device or argument error; adhk_last_error() then describes it.  The
+ * Newton/time-loop entries return the reference's TRUE(1)/FALSE(0).
+ */
+#ifndef ADHK_H
+#define ADHK_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 1-D boundary element kinds */
+#define ADHK_BC_DEPTH 0 /* tailwater depth, fe_sw2_bc_h (src/models/sw2/sw2_bc_h.c:57); val0 = depth            */
+#define ADHK_BC_FLUX  1 /* explicit flux, fe_sw2_1d_explicit_flow (src/fe/fe_sw2_boundary_integrals.c:56);
+                           val0 = normal flux per unit width, negative = into the domain                         */
+#define ADHK_BC_VEL   2 /* weak velocity, fe_sw2_bc_vel (src/models/sw2/sw2_bc_vel.c:58); val0, val1 = u, v    */
+
+/* friction law */
+#define ADHK_FRIC_CONST   0 /* constant drag coefficient (sw2_body_resid.c:241 hard-codes 0.002) */
+#define ADHK_FRIC_MANNING 1 /* Manning n -> roughness height -> log-law drag coefficient
+                               (fr_manningsn_to_rheight.c:15, fr_bedshstr_drag_coef.c:18)       */
+
+/* precision of the linear system (Jacobian storage, SpMV, BiCGStab) */
+#define ADHK_PREC_FP64  0
+#define ADHK_PREC_MIXED 1 /* FP32 matrix/preconditioner/Krylov vectors; FP64 residual, update, test */
+
+typedef struct adhk_sw2 adhk_sw2;
+
+typedef struct {
+    double g;             /* gravity (smodel_super_alloc_init.c:92: 9.8) */
+    double dt;            /* time step (SDT.dt) */
+    double alpha;         /* tau_temporal (sdt.c:24: 0 = backward Euler) */
+    double cf_const;      /* drag coefficient for ADHK_FRIC_CONST */
+    double manning_units; /* Manning's units constant, 1.0 = SI (sstr_value.c:288) */
+    int fric_mode;        /* ADHK_FRIC_* */
+} adhk_params;
+
+typedef struct {
+    double tol_nonlin;       /* smodel_super_alloc_init.c:63 (1e-8) */
+    double inc_nonlin;       /* :62 (1e-6) */
+    int max_nonlin_it;       /* :65 (10) */
+    int max_linesearch_cuts; /* :64 (5) */
+    double lin_tol;          /* bcgstab_solver.c:20-21 (1e-11, absolute and relative) */
+    int lin_max_it;          /* bcgstab_solver.c:22 (100) */
+    int precision;           /* ADHK_PREC_* */
+    int lin_check_every;     /* BiCGStab iterations launched between convergence read-backs (>=1) */
+    double mixed_lin_tol;    /* inner (FP32) BiCGStab tolerance of ADHK_PREC_MIXED; the FP64 Newton
+                                residual test at tol_nonlin is unchanged (iterative refinement) */
+} adhk_solver;
+
+typedef struct {
+    int newton_its;
+    int linesearch_cuts_total;
+    int lin_its_total;
+    int lin_its_max;
+    double resid_max_norm;
+    double inc_max_norm;
+    int converged; /* fe_newton return value */
+} adhk_newton_stats;
+
+/* per-stage device time of the last adhk_time_loop / adhk_fe_newton call, ms */
+typedef struct {
+    double residual_ms, jacobian_ms, prep_ms, krylov_ms, update_ms, total_ms;
+    int residual_calls, jacobian_calls, krylov_calls;
+} adhk_timing;
+
+/* kernel classes of the per-kernel device timers */
+#define ADHK_KT_SPMV_P 0      /* BiCGStab: Ap = M^-1 A p (+ q.Ap)              */
+#define ADHK_KT_SPMV_S 1      /* BiCGStab: As = M^-1 A s (+ 3 dots)            */
+#define ADHK_KT_UPDATE 2      /* BiCGStab: x, r, p update (+ max|r|)           */
+#define ADHK_KT_ELEM_JAC 3    /* elemental central-FD Jacobians                */
+#define ADHK_KT_JAC_GATHER 4  /* block assembly of the Jacobian                */
+#define ADHK_KT_ELEM_RESID 5  /* elemental residuals                           */
+#define ADHK_KT_RESID_GATHER 6 /* residual assembly                            */
+#define ADHK_KT_PREP 7        /* check_diag, Dirichlet, scaling, block inverse */
+#define ADHK_KT_NKINDS 8
+
+const char *adhk_last_error(void);
+void adhk_solver_defaults(adhk_solver *s);
+
+/* ---- context: mesh upload, geometry (selem_2d.c:270-351, selem_1d.c:42-56),
+ *      CSR pattern on the GPU (slin_sys_init_sparsity_mono.c:28-494) ---- */
+adhk_sw2 *adhk_sw2_create(int nnodes, const double *x, const double *y, const double *z, int nelems2d,
+                          const int *tri, const double *mann, int nelems1d, const int *seg, const int *seg_kind,
+                          const double *seg_val, const adhk_params *p);
+void adhk_sw2_destroy(adhk_sw2 *c);
+int adhk_sw2_sizes(const adhk_sw2 *c, int *ndofs, int *nnz, int *nnz_blocks);
+int adhk_sw2_set_state(adhk_sw2 *c, const double *sol, const double *sol_old, const double *sol_older);
+int adhk_sw2_get_state(adhk_sw2 *c, double *sol, double *sol_old, double *sol_older);
+int adhk_sw2_set_bc(adhk_sw2 *c, const int *bc_mask, const double *dirichlet_data);
+int adhk_sw2_get_bc(adhk_sw2 *c, int *bc_mask);
+int adhk_sw2_set_dt(adhk_sw2 *c, double dt);
+double adhk_sw2_get_dt(const adhk_sw2 *c);
+int adhk_sw2_get_timing(const adhk_sw2 *c, adhk_timing *t);
+/* per-kernel device timers: on = 1 resets and enables (hipEvent pairs around every launch of the
+ * classes above, on the context's stream), on = 0 disables.  kernel_times flushes and returns the
+ * accumulated milliseconds and launch counts per class ([ADHK_KT_NKINDS] each). */
+int adhk_sw2_kernel_timing(adhk_sw2 *c, int on);
+int adhk_sw2_kernel_times(adhk_sw2 *c, double *ms, long long *count);
+int adhk_sw2_geometry(adhk_sw2 *c, double *djac, double *grad_shp, double *seg_djac, double *seg_nrml);
+
+/* replaces fe_sw2_init (src/models/sw2/fe_sw2_init.c:28) incl. fe_sw2_wdflag_legacy (sw2_wdflag_legacy.c:34) */
+int adhk_sw2_init(adhk_sw2 *c, int *wd_flag_out /* [nelems2d] or NULL */);
+
+/* replaces assemble_residual(SMODEL_SUPER*, SGRID*) (src/residual/residual.h:5, assemble_residual.c:69):
+ * residual = -sum of elemental SW2 residuals at the context's sol. residual may be NULL (device only). */
+int adhk_assemble_residual(adhk_sw2 *c, double *residual);
+
+/* elemental residual for one element (the pde[k].resid pointer, sw2_body_resid.c:44 / sw2_bc_h.c:57 /
+ * sw2_bc_vel.c:58) evaluated on the GPU for a batch: elem_rhs [nelems2d*9 + nelems1d*6] */
+int adhk_elem_residuals(adhk_sw2 *c, double *elem_rhs);
+
+/* replaces slin_sys_init_sparsity_mono (slin_sys_init_sparsity_mono.c:28): CSR pattern built on the GPU */
+int adhk_sparsity(adhk_sw2 *c, int *indptr /* ndofs+1 */, int *cols /* nnz */);
+
+/* replaces assemble_jacobian(SMODEL_SUPER*) (src/jacobian/jacobian.h:5, assemble_jacobian.c:20):
+ * central-FD Jacobian at sol, stored in the context; vals (CSR order of adhk_sparsity) may be NULL. */
+int adhk_assemble_jacobian(adhk_sw2 *c, double *vals);
+/* elemental FD Jacobians (load_elem_mat, assemble_jacobian.c:123): [nelems2d*81 + nelems1d*36], row-major */
+int adhk_elem_jacobians(adhk_sw2 *c, double *elem_mat);
+
+/* replace check_diag(SMODEL_SUPER*) and apply_Dirichlet_BC(SMODEL_SUPER*) (la.h, matrix_utilities.c:120, :42)
+ * on the context's system */
+int adhk_check_diag(adhk_sw2 *c);
+int adhk_apply_dirichlet_bc(adhk_sw2 *c);
+int adhk_get_system(adhk_sw2 *c, double *vals /* nnz or NULL */, double *residual /* ndofs or NULL */);
+
+/* replaces fe_newton(SMODEL_SUPER*) (src/newton/newton.h:15, fe_newton.c:44).  Returns TRUE(1)/FALSE(0), -1 on error. */
+int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st);
+
+/* replaces time_loop (src/timeloop/time_loop.c:14) for one super model: nsteps x (prep_sol + fe_newton).
+ * Returns the number of steps completed (negative if a step failed), -1e9 on error. */
+int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_stats *acc);
+
+/* ---- reference src/la/la.h surface, host CSR arrays (exact names and signatures) ---- */
+void split_CSR_mat_vec_mult(double *Ax, int *indptr_diag, int *cols_diag, double *vals_diag, int *indptr_off_diag,
+                            int *cols_off_diag, double *vals_off_diag, double *x, int nrows, int *ghosts,
+                            int nghost); /* la.h:13, matrix_utilities.c:177 */
+void scale_linear_system(int *indptr_diag, int *cols_diag, double *vals_diag, int *indptr_off_diag,
+                         int *cols_off_diag, double *vals_off_diag, double *b, double *x, double *scale_vect,
+                         int local_size, int size, int rank, int *ghosts,
+                         int nghost); /* la.h:26, scale_linear_system.c:31 */
+void unscale_linear_system(double *x, double *x0, double *scale_vect,
+                           int local_size); /* la.h:29, scale_linear_system.c:137 */
+int prep_umfpack(int *indptr_diag, int *cols_diag, double *vals_diag,
+                 int nrow); /* la.h:21, bcgstab_solver.c:371 -- builds the block-Jacobi preconditioner */
+int solve_umfpack(double *x, int *indptr_diag, int *cols_diag, double *vals_diag, double *b,
+                  int nrow); /* la.h:22, bcgstab_solver.c:415 -- applies it */
+void umfpack_clear(void);   /* la.h:23, bcgstab_solver.c:351 */
+void free_bcgstab(void);    /* la.h:19, bcgstab_solver.c:438 */
+int solve_linear_sys_bcgstab(double *x, int *indptr_diag, int *cols_diag, double *vals_diag, int *indptr_off_diag,
+                             int *cols_off_diag, double *vals_off_diag, double *b, double *scale_vect,
+                             int local_size, int size, int rank, int *ghosts,
+                             int nghost); /* la.h:16, bcgstab_solver.c:65 */
+void Screen_print_CSR(int *indptr, int *indices, double *data, int n); /* la.h:11 (declared, never defined there) */
+/* block size used by prep_umfpack/solve_umfpack (3 = nodal SW2 blocks; 1 = point Jacobi) */
+void adhk_la_set_block_size(int bs);
+
+/* ---- synthetic flume meshes (restates create_rectangular_grid.c:4-445 and adds boundary
+ *      segments, seeded Gaussian bed bumps and per-element Manning n) ---- */
+typedef struct {
+    double xmin, xmax, ymin, ymax;
+    int npx, npy;
+    double slope;          /* bed z = -slope * (x - xmin) + bumps */
+    int nbumps;            /* seeded Gaussian bumps */
+    double bump_amp_max, bump_sigma_min, bump_sigma_max;
+    unsigned long long seed;
+    double manning_n;
+    double inflow_q;       /* total discharge at x = xmin (m^3/s) */
+    double tailwater;      /* depth at x = xmax */
+    int ic_flat_wse;       /* initial state: 1 = flat water surface at the tailwater level, 0 = depth = tailwater */
+} adhk_flume_spec;
+int adhk_flume_counts(const adhk_flume_spec *f, int *nnodes, int *nelems2d, int *nelems1d);
+int adhk_flume_build(const adhk_flume_spec *f, double *x, double *y, double *z, int *tri, double *mann, int *seg,
+                     int *seg_kind, double *seg_val);
+/* initial state with u = v = 0: ic_flat_wse = 1 -> h = (tailwater - slope*(xmax - xmin)) - z, a flat water
+ * surface at the tailwater level of the bump-free bed; ic_flat_wse = 0 -> h = tailwater everywhere */
+int adhk_flume_initial_state(const adhk_flume_spec *f, int nnodes, const double *z, double *sol);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,91 @@
+"""Independent numpy statement of the SW2 triangle residual by numerical quadrature.
+
+Pins the closed-form P1 integrals of the oracle (src/fe/fe_integrals_2d.c and
+fe_integrals_1d.c as used by sw2_body_resid.c) against the weak form they
+integrate, evaluated with a 7-point degree-5 triangle rule (exact for the
+cubic integrands h*u*u) and 3-point Gauss-Legendre on edges.  Only the
+all-wet branch is stated.  The SUPG combination (sw2_body_resid.c:921-1142)
+is the reference's algebra on element averages; its integrals (time
+differences and the GLS convection term) are taken by quadrature here too.
+"""
+import numpy as np
+
+S15 = np.sqrt(15.0)
+A1, B1 = (6 - S15) / 21, (9 + 2 * S15) / 21
+A2, B2 = (6 + S15) / 21, (9 - 2 * S15) / 21
+W1, W2 = (155 - S15) / 1200, (155 + S15) / 1200
+BARY = np.array([[1 / 3, 1 / 3, 1 / 3],
+                 [A1, A1, B1], [A1, B1, A1], [B1, A1, A1],
+                 [A2, A2, B2], [A2, B2, A2], [B2, A2, A2]])
+WTS = np.array([9 / 40, W1, W1, W1, W2, W2, W2])
+GL_X = np.array([0.5 - np.sqrt(0.15), 0.5, 0.5 + np.sqrt(0.15)])
+GL_W = np.array([5 / 18, 8 / 18, 5 / 18])
+
+
+def p1_grad(x, y):
+    area2 = (x[1] - x[0]) * (y[2] - y[0]) - (x[2] - x[0]) * (y[1] - y[0])
+    gx = np.array([y[1] - y[2], y[2] - y[0], y[0] - y[1]]) / area2
+    gy = np.array([x[2] - x[1], x[0] - x[2], x[1] - x[0]]) / area2
+    return 0.5 * area2, gx, gy
+
+
+def quad(area, f_at_points):
+    return area * np.dot(WTS, f_at_points)
+
+
+def body_residual(x, y, z, h, u, v, ho, uo, vo, dt, g, cf, dll=1e-6):
+    """9-vector [node*3 + (c, x, y)] of the SW2 body residual for an all-wet triangle."""
+    A, gx, gy = p1_grad(x, y)
+    phi = BARY  # (7 points, 3 basis)
+    hq, uq, vq = phi @ h, phi @ u, phi @ v
+    zx, zy = gx @ z, gy @ z
+    r = np.zeros(9)
+    for i in range(3):
+        # advection: -dt int grad(phi_i) . (h v), (h u u, h u v), (h u v, h v v)
+        r[3 * i] += -dt * quad(A, gx[i] * hq * uq + gy[i] * hq * vq)
+        r[3 * i + 1] += -dt * quad(A, gx[i] * hq * uq * uq + gy[i] * hq * uq * vq)
+        r[3 * i + 2] += -dt * quad(A, gx[i] * hq * uq * vq + gy[i] * hq * vq * vq)
+        # temporal: consistent continuity, lumped momentum (backward Euler weights 1, -1)
+        r[3 * i] += quad(A, phi[:, i] * (hq - phi @ ho))
+        r[3 * i + 1] += A / 3 * (u[i] * max(h[i], dll) - uo[i] * max(ho[i], dll))
+        r[3 * i + 2] += A / 3 * (v[i] * max(h[i], dll) - vo[i] * max(ho[i], dll))
+        # friction on the nodal interpolant of u|v|: dt (cf/2) int phi_i I(u|v|)
+        sp = np.sqrt(u ** 2 + v ** 2)
+        r[3 * i + 1] += dt * cf / 2 * quad(A, phi[:, i] * (phi @ (u * sp)))
+        r[3 * i + 2] += dt * cf / 2 * quad(A, phi[:, i] * (phi @ (v * sp)))
+        # pressure -1/2 g dt int dphi_i h^2, bed slope g dt int phi_i h grad z
+        r[3 * i + 1] += -0.5 * g * dt * gx[i] * quad(A, hq ** 2) + g * dt * zx * quad(A, phi[:, i] * hq)
+        r[3 * i + 2] += -0.5 * g * dt * gy[i] * quad(A, hq ** 2) + g * dt * zy * quad(A, phi[:, i] * hq)
+    # boundary pressure 1/2 g dt int_{edges} phi_i h^2 n
+    for a, b in ((0, 1), (1, 2), (2, 0)):
+        L = np.hypot(x[b] - x[a], y[b] - y[a])
+        nx, ny = (y[b] - y[a]) / L, -(x[b] - x[a]) / L
+        for s, w in zip(GL_X, GL_W):
+            hs = (1 - s) * h[a] + s * h[b]
+            for node, ph in ((a, 1 - s), (b, s)):
+                r[3 * node + 1] += 0.5 * g * dt * w * L * ph * hs * hs * nx
+                r[3 * node + 2] += 0.5 * g * dt * w * L * ph * hs * hs * ny
+    # SUPG on element averages
+    hb, ub, vb = h.mean(), u.mean(), v.mean()
+    if abs(hb) >= np.finfo(float).eps:
+        hden = max(hb, np.finfo(np.float32).eps)
+        tau = 0.5 * np.sqrt(A) / np.sqrt(ub * ub + vb * vb + g * hden + np.finfo(float).eps)
+        dhdt = quad(A, hq - phi @ ho)
+        dudt = quad(A, uq - phi @ uo)
+        dvdt = quad(A, vq - phi @ vo)
+        hx, hy, ux, uy, vx, vy = gx @ h, gy @ h, gx @ u, gy @ u, gx @ v, gy @ v
+        a0 = dt * quad(A, np.full(7, ub * hx + hb * ux + vb * hy + hb * vy))
+        a1 = dt * quad(A, np.full(7, ub * ux + vb * uy))
+        a2 = dt * quad(A, np.full(7, ub * vx + vb * vy))
+        res = cf / (2 * g)
+        vmag = np.hypot(ub, vb)
+        xf, yf = g * res * ub * vmag * dt, g * res * vb * vmag * dt
+        xp, yp = g * (hx + zx) * dt, g * (hy + zy) * dt
+        c_eq = dhdt + a0
+        x_eq = dudt + a1 + (xp + xf / hden) * A
+        y_eq = dvdt + a2 + (yp + yf / hden) * A
+        for i in range(3):
+            r[3 * i] += tau * (gx[i] * (ub * c_eq + hb * x_eq) + gy[i] * (vb * c_eq + hb * y_eq))
+            r[3 * i + 1] += tau * (gx[i] * (g * hb * c_eq + ub * hb * x_eq) + gy[i] * (vb * hb * x_eq))
+            r[3 * i + 2] += tau * (gy[i] * (g * hb * c_eq + vb * hb * y_eq) + gx[i] * (ub * hb * y_eq))
+    return r

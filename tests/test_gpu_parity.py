@@ -1,0 +1,208 @@
+"""GPU parity: the gfx950 path through libadhk.so against the CPU oracle.
+
+Tolerances (from the workload statement): elemental/assembled residual 1e-12
+relative, FD Jacobian 1e-7 relative, sparsity pattern bit-exact, depth after N
+time steps 1e-8 relative; mixed precision depth 1e-6 relative to FP64.
+Relative errors are max-norm errors divided by the max-norm of the oracle.
+"""
+import numpy as np
+import pytest
+
+import adh_kraken_amd as A
+import oracle as O
+from adh_kraken_amd import _lib as L
+from helpers import perturbed_state, rel_err, small_flume
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1():
+    mesh, sol0 = A.flume_mesh(A.FLUME_C1)
+    return mesh, sol0
+
+
+def make_pair(mesh, sol, sol_old=None, dt=1.0, friction="manning"):
+    sm = A.SModelSuper(mesh, dt=dt, friction=friction)
+    orc = O.Oracle(mesh, dt=dt, friction=friction, threads=8)
+    so = sol if sol_old is None else sol_old
+    sm.sol, sm.sol_old, sm.sol_older = sol, so, so
+    orc.set_state(sol, so, so)
+    return sm, orc
+
+
+def test_geometry_matches(gpu, c1):
+    mesh, sol0 = c1
+    sm, orc = make_pair(mesh, sol0)
+    g1, g2 = sm.geometry(), orc.geometry()
+    for a, b in zip(g1, g2):
+        assert np.array_equal(a, b)
+
+
+def test_pattern_bit_exact(gpu, c1):
+    mesh, sol0 = c1
+    sm, orc = make_pair(mesh, sol0)
+    ip_g, c_g = A.sparsity(sm)
+    ip_o, c_o = orc.pattern()
+    assert sm.nnz == orc.nnz == 257949
+    assert np.array_equal(ip_g, ip_o)
+    assert np.array_equal(c_g, c_o)
+
+
+@pytest.mark.parametrize("friction", ["manning", "const"])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_elem_residuals(gpu, c1, friction, seed):
+    mesh, sol0 = c1
+    s, so = perturbed_state(sol0, seed=seed)
+    sm, orc = make_pair(mesh, s, so, friction=friction)
+    eg, sg = A.elem_residuals(sm)
+    eo = np.array([orc.body_resid(e, s) for e in range(mesh.nelems2d)])
+    so_ = np.array([orc.seg_resid(e, s) for e in range(mesh.nelems1d)])
+    assert rel_err(eg, eo) <= 1e-12
+    assert rel_err(sg, so_) <= 1e-12
+
+
+@pytest.mark.parametrize("seed", [0, 3])
+def test_assembled_residual(gpu, c1, seed):
+    mesh, sol0 = c1
+    s, so = perturbed_state(sol0, seed=seed)
+    sm, orc = make_pair(mesh, s, so)
+    assert rel_err(A.assemble_residual(sm), orc.residual()) <= 1e-12
+
+
+def test_elem_jacobians(gpu, c1):
+    mesh, sol0 = c1
+    s, so = perturbed_state(sol0, seed=5)
+    sm, orc = make_pair(mesh, s, so)
+    jg, jsg = A.elem_jacobians(sm)
+    idx = np.arange(0, mesh.nelems2d, 7)
+    jo = np.array([orc.elem_jac_tri(int(e)) for e in idx])
+    assert rel_err(jg[idx], jo) <= 1e-7
+    jso = np.array([orc.elem_jac_seg(e) for e in range(mesh.nelems1d)])
+    assert rel_err(jsg, jso) <= 1e-7
+
+
+def test_assembled_jacobian(gpu, c1):
+    mesh, sol0 = c1
+    s, so = perturbed_state(sol0, seed=6)
+    sm, orc = make_pair(mesh, s, so)
+    assert rel_err(A.assemble_jacobian(sm), orc.jacobian()) <= 1e-7
+
+
+def test_check_diag_and_dirichlet(gpu):
+    _, (mesh, sol0) = small_flume()
+    s, so = perturbed_state(sol0, seed=2)
+    sm, orc = make_pair(mesh, s, so)
+    bc = np.zeros(sm.ndofs, np.int32)
+    bc[[1, 2, 7, 30, 31]] = 1
+    dd = np.zeros(sm.ndofs)
+    sm.set_bc(bc, dd)
+    orc.set_bc(bc, dd)
+    A.assemble_residual(sm)
+    orc.residual()
+    A.assemble_jacobian(sm)
+    orc.jacobian()
+    A.check_diag(sm)
+    orc.check_diag()
+    A.apply_Dirichlet_BC(sm)
+    orc.apply_dirichlet()
+    vg, rg = A.get_system(sm)
+    vo, ro = orc.system()
+    assert np.array_equal(sm.bc_mask, orc.bc_mask())
+    assert rel_err(vg, vo) <= 1e-7
+    assert rel_err(rg, ro) <= 1e-12
+    ip_, cols = A.sparsity(sm)
+    for r in np.nonzero(bc)[0]:
+        row = vg[ip_[r]:ip_[r + 1]]
+        assert np.array_equal(row, (cols[ip_[r]:ip_[r + 1]] == r).astype(float))
+        assert rg[r] == 0.0
+
+
+def test_newton_one_step(gpu, c1):
+    mesh, sol0 = c1
+    sm, orc = make_pair(mesh, sol0)
+    ng, sg = A.time_loop(sm, 1)
+    no, so = orc.time_loop(1)
+    assert ng == no == 1
+    assert sg["newton_its"] == so["newton_its"]
+    hg, ho = sm.sol[0::3], orc.state()[0][0::3]
+    assert rel_err(hg, ho) <= 1e-8
+
+
+def test_time_loop_c1_depth(gpu, c1):
+    mesh, sol0 = c1
+    sm, orc = make_pair(mesh, sol0)
+    ng, sg = A.time_loop(sm, 20)
+    no, so = orc.time_loop(20)
+    assert ng == no == 20
+    gs, os_ = sm.sol, orc.state()[0]
+    assert rel_err(gs[0::3], os_[0::3]) <= 1e-8
+    assert rel_err(gs[1::3], os_[1::3]) <= 1e-6
+
+
+def test_mixed_precision_c1(gpu, c1):
+    mesh, sol0 = c1
+    a = A.SModelSuper(mesh, dt=1.0)
+    b = A.SModelSuper(mesh, dt=1.0)
+    a.set_initial(sol0)
+    b.set_initial(sol0)
+    na, _ = A.time_loop(a, 10, precision=L.PREC_FP64)
+    nb, stb = A.time_loop(b, 10, precision=L.PREC_MIXED)
+    assert na == nb == 10
+    assert stb["resid_max_norm"] <= 1e-8 or stb["inc_max_norm"] <= 1e-6
+    assert rel_err(b.sol[0::3], a.sol[0::3]) <= 1e-6
+
+
+def test_la_surface_matches_oracle(gpu):
+    _, (mesh, sol0) = small_flume()
+    s, so = perturbed_state(sol0, seed=9)
+    sm, orc = make_pair(mesh, s, so)
+    orc.residual()
+    vals = orc.jacobian()
+    ip_, cols = orc.pattern()
+    n = orc.ndofs
+    lib = L.load()
+    x = np.random.default_rng(0).uniform(-1, 1, n)
+    y = np.zeros(n)
+    lib.split_CSR_mat_vec_mult(L.dptr(y), L.iptr(ip_), L.iptr(cols), L.dptr(vals), None, None, None, L.dptr(x), n,
+                               None, 0)
+    assert rel_err(y, O.spmv(ip_, cols, vals, x)) <= 1e-13
+    b = orc.system()[1]
+    vo, bo, xo, so_ = O.scale_linear_system(ip_, cols, vals, b, np.zeros(n))
+    vg, bg, xg, sg = vals.copy(), b.copy(), np.zeros(n), np.zeros(n)
+    lib.scale_linear_system(L.iptr(ip_), L.iptr(cols), L.dptr(vg), None, None, None, L.dptr(bg), L.dptr(xg),
+                            L.dptr(sg), n, n, 0, None, 0)
+    assert np.array_equal(sg, so_)
+    assert rel_err(vg, vo) <= 1e-15 and rel_err(bg, bo) <= 1e-15
+    minv = O.block_jacobi(ip_, cols, vo)
+    x_o, it_o, _, _ = O.bicgstab(ip_, cols, vo, minv, bo, so_)
+    assert lib.prep_umfpack(L.iptr(ip_), L.iptr(cols), L.dptr(vg), n) == 0
+    xg2, bg2 = np.zeros(n), bg.copy()
+    lib.solve_linear_sys_bcgstab(L.dptr(xg2), L.iptr(ip_), L.iptr(cols), L.dptr(vg), None, None, None, L.dptr(bg2),
+                                 L.dptr(sg), n, n, 0, None, 0)
+    assert rel_err(xg2, x_o) <= 1e-9
+    # A x = b of the unscaled system
+    r = O.spmv(ip_, cols, vals, xg2) - b
+    assert np.abs(r).max() <= 1e-8 * max(1.0, np.abs(b).max())
+    lib.free_bcgstab()
+
+
+def test_c2_full_size_pattern_and_residual(gpu):
+    mesh, sol0 = A.flume_mesh(A.FLUME_C2)
+    assert (mesh.nnodes, mesh.nelems2d, mesh.nelems1d) == (402201, 800000, 400)
+    sm = A.SModelSuper(mesh, dt=2.0)
+    assert sm.ndofs == 1206603 and sm.nnz == 25259409
+    ip_, cols = A.sparsity(sm)
+    # sorted, unique rows; row length 3 * (node degree + 1); symmetric pattern
+    d = np.diff(ip_)
+    assert d.min() >= 9 and d.max() == 21
+    sm.set_initial(sol0)
+    eg, _ = A.elem_residuals(sm)
+    orc = O.Oracle(mesh, dt=2.0, threads=8)
+    orc.set_initial(sol0)
+    rng = np.random.default_rng(11)
+    idx = rng.choice(mesh.nelems2d, 2000, replace=False)
+    eo = np.array([orc.body_resid(int(e), sol0) for e in idx])
+    assert rel_err(eg[idx], eo) <= 1e-12
+    # lake at rest (flat water surface, u = v = 0, sol = sol_old): every body residual vanishes to rounding
+    assert np.abs(eg).max() <= 1e-9
