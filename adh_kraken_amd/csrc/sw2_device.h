@@ -87,20 +87,16 @@ __device__ __forceinline__ void sw2_temporal(const TriIn &in, const double *h, c
 __device__ __forceinline__ int sw2_tri_resid(const TriIn &in, const Sw2Par &P, int pnode, int pvar, double pe,
                                              double r[9])
 {
-    double h[3] = {in.h[0], in.h[1], in.h[2]};
-    double u[3] = {in.u[0], in.u[1], in.u[2]};
-    double v[3] = {in.v[0], in.v[1], in.v[2]};
-    bool pflag = true;
-    /* perturbation, sw2_body_resid.c:107-120 */
-    if (pvar == 0) {
-        h[pnode] += pe;
-    } else if (pvar == 1) {
-        u[pnode] += pe;
-        pflag = false;
-    } else if (pvar == 2) {
-        v[pnode] += pe;
-        pflag = false;
+    /* perturbation, sw2_body_resid.c:107-120 (selects instead of a dynamic index keep the
+     * element in registers; an unperturbed entry gets + 0.0, which leaves it unchanged) */
+    double h[3], u[3], v[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        h[a] = in.h[a] + ((pvar == 0 && pnode == a) ? pe : 0.0);
+        u[a] = in.u[a] + ((pvar == 1 && pnode == a) ? pe : 0.0);
+        v[a] = in.v[a] + ((pvar == 2 && pnode == a) ? pe : 0.0);
     }
+    const bool pflag = !(pvar == 1 || pvar == 2);
     const int dh = n_dry3(h[0], h[1], h[2]);
     const int dho = n_dry3(in.ho[0], in.ho[1], in.ho[2]);
     if ((dh == 1 || dh == 2) || (dho == 1 || dho == 2)) return 1;

@@ -42,6 +42,9 @@ struct MeshView {
 
 __device__ __forceinline__ void gather_tri(const MeshView &m, int e, bool older, TriIn &in)
 {
+    /* with alpha == 0 the t(n-1) level is unused; point it at the old level (already in cache)
+     * instead of branching, which keeps the element in registers */
+    const double *soo = older ? m.sol_older : m.sol_old;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         int n = m.tri[a * m.E + e];
@@ -54,13 +57,9 @@ __device__ __forceinline__ void gather_tri(const MeshView &m, int e, bool older,
         in.ho[a] = m.sol_old[3 * n];
         in.uo[a] = m.sol_old[3 * n + 1];
         in.vo[a] = m.sol_old[3 * n + 2];
-        if (older) {
-            in.hoo[a] = m.sol_older[3 * n];
-            in.uoo[a] = m.sol_older[3 * n + 1];
-            in.voo[a] = m.sol_older[3 * n + 2];
-        } else {
-            in.hoo[a] = in.uoo[a] = in.voo[a] = 0.0;
-        }
+        in.hoo[a] = soo[3 * n];
+        in.uoo[a] = soo[3 * n + 1];
+        in.voo[a] = soo[3 * n + 2];
         in.gx[a] = m.grad[a * m.E + e];
         in.gy[a] = m.grad[(3 + a) * m.E + e];
     }
@@ -165,7 +164,8 @@ __global__ __launch_bounds__(256) void k_resid_gather(int N, int E, const int *_
     int b = diag_blk[n];
     double r0 = 0.0, r1 = 0.0, r2 = 0.0;
     const long segbase = 9L * E;
-    for (int j = cptr[b]; j < cptr[b + 1]; j++) {
+    const int j0 = (b >= 0) ? cptr[b] : 0, j1 = (b >= 0) ? cptr[b + 1] : 0; /* a node in no element has no row */
+    for (int j = j0; j < j1; j++) {
         int cde = contrib[j];
         const double *er;
         if (cde < 9 * E) {
@@ -206,7 +206,13 @@ __global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, double *
     int i = col / 3, k = col % 3;
     TriIn in;
     gather_tri(m, e, P.alpha != 0.0, in);
-    double var = (k == 0) ? in.h[i] : ((k == 1) ? in.u[i] : in.v[i]);
+    double var = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        var = (a == i && k == 0) ? in.h[a] : var;
+        var = (a == i && k == 1) ? in.u[a] : var;
+        var = (a == i && k == 2) ? in.v[a] : var;
+    }
     double eps = num_diff_eps(var, 1e-4);
     double eps2 = 2.0 * eps;
     double rp[9], rm[9];

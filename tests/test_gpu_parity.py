@@ -196,13 +196,17 @@ def test_c2_full_size_pattern_and_residual(gpu):
     # sorted, unique rows; row length 3 * (node degree + 1); symmetric pattern
     d = np.diff(ip_)
     assert d.min() >= 9 and d.max() == 21
+    # lake at rest (flat water surface, u = v = 0, sol = sol_old): every body residual vanishes to rounding
     sm.set_initial(sol0)
     eg, _ = A.elem_residuals(sm)
+    assert np.abs(eg).max() <= 1e-9
+    # a perturbed state: sampled elements against the oracle
+    s, so = perturbed_state(sol0, seed=12)
+    sm.sol, sm.sol_old, sm.sol_older = s, so, so
+    eg, _ = A.elem_residuals(sm)
     orc = O.Oracle(mesh, dt=2.0, threads=8)
-    orc.set_initial(sol0)
+    orc.set_state(s, so, so)
     rng = np.random.default_rng(11)
     idx = rng.choice(mesh.nelems2d, 2000, replace=False)
-    eo = np.array([orc.body_resid(int(e), sol0) for e in idx])
+    eo = np.array([orc.body_resid(int(e), s) for e in idx])
     assert rel_err(eg[idx], eo) <= 1e-12
-    # lake at rest (flat water surface, u = v = 0, sol = sol_old): every body residual vanishes to rounding
-    assert np.abs(eg).max() <= 1e-9
