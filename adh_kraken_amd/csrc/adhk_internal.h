@@ -7,8 +7,15 @@
  *   triangles             tri                       SoA int [3][E]
  *   geometry              djac [E], grad [6][E]     (gx0,gx1,gx2,gy0,gy1,gy2)
  *   state                 sol, sol_old, sol_older   AoS double [3N] (h,u,v per node)
- *   matrix                block CSR of 3x3 node blocks: brow [N+1], bcol [nnzb],
- *                         bvals [nnzb][9] (row-major block), diag_blk [N]
+ *   pattern               block CSR of 3x3 node blocks: brow [N+1], bcol [nnzb], diag_blk [N]
+ *   matrix values         sliced ELL of 3x3 blocks (SELL-64): node n sits in slice n/64 at lane
+ *                         n%64; slice s owns slot columns slice_ptr[s] .. slice_ptr[s+1]-1 (its
+ *                         widest block row); block (slot column c, lane l) has column node
+ *                         scol[c*64 + l] and entry e (row-major 3x3) at bvals[(c*9 + e)*64 + l],
+ *                         so a wavefront reads 64 consecutive values per load.  Padding slots
+ *                         hold zeros and their own node as column.  sell2blk / blk2sell map
+ *                         slots to block-CSR indices; diag_slot [N] is the diagonal's slot.
+ *   block-Jacobi          minv[(s*9 + e)*64 + l]
  *   assembly lists        cptr [nnzb+1], contrib [npairs]: contributions to each
  *                         block in element order (2-D first, then 1-D)
  *   elemental buffers     erhs [E*9 + S*6], emat [E*81 + S*36] stored [elem][col][row]
@@ -62,14 +69,18 @@ struct adhk_sw2 {
     int *bc_mask, *wd_flag, *wd_tmp;
     /* block CSR pattern and assembly lists */
     int *brow, *bcol, *diag_blk, *cptr, *contrib;
+    /* SELL-64 layout of the matrix values */
+    int nslice, nslot;
+    int *slice_ptr, *scol, *sell2blk, *blk2sell, *diag_slot;
     /* matrix values and preconditioner */
     double *bvals, *minv;
     float *bvals32, *minv32;
     /* elemental buffers */
     double *erhs, *emat;
-    /* Krylov vectors (FP64 and FP32 sets) */
-    double *r, *pv, *Ap, *s, *As, *q, *x0;
-    float *r32, *p32, *Ap32, *s32, *As32, *q32, *xk32;
+    /* Krylov vectors (FP64 and FP32 sets); p and Ap are double-buffered: one BiCGStab kernel
+     * reads the previous direction while it writes the next */
+    double *r, *pv, *Ap, *s, *As, *q, *x0, *pv2, *Ap2, *xk;
+    float *r32, *p32, *Ap32, *s32, *As32, *q32, *xk32, *p32b, *Ap32b;
     /* reductions */
     double *partials;
     unsigned int *counter;
@@ -101,7 +112,9 @@ int adhk_launch_check_diag(adhk_ctx *c);
 int adhk_launch_dirichlet(adhk_ctx *c, int do_scale);
 int adhk_launch_scale_prec(adhk_ctx *c, int mixed);
 int adhk_launch_krylov_init(adhk_ctx *c, int mixed, double tol, int max_it);
-int adhk_launch_krylov_iter(adhk_ctx *c, int mixed);
+int adhk_launch_krylov_first(adhk_ctx *c, int mixed);
+int adhk_launch_krylov_iter(adhk_ctx *c, int mixed, int cur);
+int adhk_launch_krylov_last(adhk_ctx *c, int mixed, int cur);
 int adhk_launch_krylov_finish(adhk_ctx *c, int mixed);
 int adhk_launch_increment(adhk_ctx *c);
 int adhk_launch_linesearch(adhk_ctx *c);

@@ -9,10 +9,11 @@
  * (bcgstab_solver.c:371-424); this path uses the 3x3 nodal block-Jacobi
  * inverse instead.
  *
- * Every BiCGStab iteration is three HBM-bound kernels:
- *   k_spmv_p : Ap = M^-1 A p                           + q.Ap        -> alpha
- *   k_spmv_s : s = r - alpha Ap, As = M^-1 A s          + As.s, As.As, As.q -> omega, rho, beta
- *   k_update : x += alpha p + omega s, r = s - omega As, p = r + beta (p - omega Ap) + max|r| -> done
+ * Every BiCGStab iteration is two HBM-bound kernels over the SELL-64 block matrix:
+ *   k_iter_a : finish the previous iteration (x += alpha p + omega s, r = s - omega As, max|r| and the
+ *              stop test), form p = r + beta (p - omega Ap) -- recomputed on the fly for the columns,
+ *              so no extra pass -- and Ap = M^-1 A p with q.Ap -> alpha
+ *   k_iter_s : s = r - alpha Ap, As = M^-1 A s with As.s, As.As, As.q -> omega, rho, beta
  * Scalars live in device memory; every kernel exits at once when the solver
  * has converged, so the host launches iterations in chunks and reads the
  * "done" flag between chunks.  Reductions are deterministic: per-workgroup
@@ -110,7 +111,7 @@ __device__ __forceinline__ bool grid_reduce(double (&v)[K], double *partials, un
     return true;
 }
 
-/* ---- per-node block row product y = A_n x (x given by a functor for the column values) ---- */
+/* ---- SELL-64 block row product y = A_n x (column values from a functor) ---- */
 template <typename T> struct ColPlain {
     const T *x;
     __device__ __forceinline__ void get(int m, double &a, double &b, double &c) const
@@ -120,14 +121,28 @@ template <typename T> struct ColPlain {
         c = (double)x[3 * m + 2];
     }
 };
-/* s = r + (-alpha) Ap evaluated on the fly (explicit fma so the stored and the recomputed value agree) */
+
+/* BiCGStab vector updates with explicit fused multiply-adds, so that a value recomputed for a
+ * neighbouring column is bit-identical to the value its owner row stores */
+template <typename T> __device__ __forceinline__ T upd_s(double r, double Ap, double alpha)
+{
+    return (T)__fma_rn(-alpha, Ap, r); /* s = r - alpha Ap            (bcgstab_solver.c:247-248) */
+}
+template <typename T> __device__ __forceinline__ T upd_r(double s, double As, double omega)
+{
+    return (T)__fma_rn(-omega, As, s); /* r = s - omega As            (:300-301) */
+}
+template <typename T>
+__device__ __forceinline__ T upd_p(double p, double Ap, double r_new, double omega, double beta)
+{
+    return (T)__fma_rn(__fma_rn(-omega, Ap, p), beta, r_new); /* p = (p - omega Ap) beta + r (:219-221) */
+}
+
+/* s = r - alpha Ap evaluated on the fly for the columns */
 template <typename T> struct ColS {
     const T *r, *Ap;
     double alpha;
-    __device__ __forceinline__ T one(int i) const
-    {
-        return (T)fma(-alpha, (double)Ap[i], (double)r[i]);
-    }
+    __device__ __forceinline__ T one(int i) const { return upd_s<T>((double)r[i], (double)Ap[i], alpha); }
     __device__ __forceinline__ void get(int m, double &a, double &b, double &c) const
     {
         a = (double)one(3 * m);
@@ -135,30 +150,54 @@ template <typename T> struct ColS {
         c = (double)one(3 * m + 2);
     }
 };
+/* next direction p' = (p - omega Ap) beta + (s - omega As), evaluated on the fly for the columns */
+template <typename T> struct ColP {
+    const T *s, *As, *p, *Ap;
+    double omega, beta;
+    __device__ __forceinline__ void one(int i, T &rr, T &pp) const
+    {
+        rr = upd_r<T>((double)s[i], (double)As[i], omega);
+        pp = upd_p<T>((double)p[i], (double)Ap[i], (double)rr, omega, beta);
+    }
+    __device__ __forceinline__ void get(int m, double &a, double &b, double &c) const
+    {
+        T rr, pp;
+        one(3 * m, rr, pp);
+        a = (double)pp;
+        one(3 * m + 1, rr, pp);
+        b = (double)pp;
+        one(3 * m + 2, rr, pp);
+        c = (double)pp;
+    }
+};
+
+struct Sell {
+    const int *slice_ptr, *scol;
+};
 
 template <typename T, typename Col>
-__device__ __forceinline__ void block_row(int n, const int *__restrict__ brow, const int *__restrict__ bcol,
-                                          const T *__restrict__ bv, const Col &col, double &y0, double &y1,
-                                          double &y2)
+__device__ __forceinline__ void block_row(int n, const Sell &S, const T *__restrict__ bv, const Col &col, double &y0,
+                                          double &y1, double &y2)
 {
     y0 = 0.0;
     y1 = 0.0;
     y2 = 0.0;
-    int j0 = brow[n], j1 = brow[n + 1];
-    for (int j = j0; j < j1; j++) {
-        int m = bcol[j];
+    const int sl = n >> 6, lane = n & 63;
+    const int c0 = S.slice_ptr[sl], c1 = S.slice_ptr[sl + 1];
+    for (int cs = c0; cs < c1; cs++) {
+        int m = S.scol[cs * 64 + lane];
         double x0, x1, x2;
         col.get(m, x0, x1, x2);
-        const T *a = bv + 9 * (size_t)j;
+        const T *a = bv + (size_t)cs * 9 * 64 + lane;
         y0 += (double)a[0] * x0;
-        y0 += (double)a[1] * x1;
-        y0 += (double)a[2] * x2;
-        y1 += (double)a[3] * x0;
-        y1 += (double)a[4] * x1;
-        y1 += (double)a[5] * x2;
-        y2 += (double)a[6] * x0;
-        y2 += (double)a[7] * x1;
-        y2 += (double)a[8] * x2;
+        y0 += (double)a[64] * x1;
+        y0 += (double)a[128] * x2;
+        y1 += (double)a[192] * x0;
+        y1 += (double)a[256] * x1;
+        y1 += (double)a[320] * x2;
+        y2 += (double)a[384] * x0;
+        y2 += (double)a[448] * x1;
+        y2 += (double)a[512] * x2;
     }
 }
 
@@ -166,11 +205,13 @@ template <typename T>
 __device__ __forceinline__ void apply_minv(const T *__restrict__ mi, int n, double y0, double y1, double y2,
                                            double &z0, double &z1, double &z2)
 {
-    const T *m = mi + 9 * (size_t)n;
-    z0 = (double)m[0] * y0 + (double)m[1] * y1 + (double)m[2] * y2;
-    z1 = (double)m[3] * y0 + (double)m[4] * y1 + (double)m[5] * y2;
-    z2 = (double)m[6] * y0 + (double)m[7] * y1 + (double)m[8] * y2;
+    const T *m = mi + (size_t)(n >> 6) * 9 * 64 + (n & 63);
+    z0 = (double)m[0] * y0 + (double)m[64] * y1 + (double)m[128] * y2;
+    z1 = (double)m[192] * y0 + (double)m[256] * y1 + (double)m[320] * y2;
+    z2 = (double)m[384] * y0 + (double)m[448] * y1 + (double)m[512] * y2;
 }
+
+__device__ __forceinline__ size_t sell_val(int cs, int e, int lane) { return ((size_t)cs * 9 + e) * 64 + lane; }
 
 /* ---------------- Newton-level kernels ---------------- */
 /* get_residual_norms, newton_tools.c:303-373: max|r|, sum r^2, max|dsol| over non-Dirichlet dofs */
@@ -211,13 +252,15 @@ extern "C" int adhk_launch_norms(adhk_ctx *c)
 }
 
 /* check_diag, matrix_utilities.c:120-152 */
-__global__ void k_check_diag(int N, const int *diag_blk, const double *bv, int *bc)
+__global__ void k_check_diag(int N, const int *__restrict__ slice_ptr, const int *__restrict__ diag_slot,
+                             const double *__restrict__ bv, int *__restrict__ bc)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    int d = diag_blk[n];
+    int d = diag_slot[n];
+    int cs = slice_ptr[n >> 6] + d;
     for (int r = 0; r < 3; r++) {
-        double a = (d >= 0) ? bv[9 * (size_t)d + 4 * r] : 0.0;
+        double a = (d >= 0) ? bv[sell_val(cs, 4 * r, n & 63)] : 0.0;
         if (fabs(a) < D_SMALL) bc[3 * n + r] = 1;
     }
 }
@@ -225,38 +268,41 @@ __global__ void k_check_diag(int N, const int *diag_blk, const double *bv, int *
 extern "C" int adhk_launch_check_diag(adhk_ctx *c)
 {
     adhk_kt_begin(c, ADHK_KT_PREP);
-    hipLaunchKernelGGL(k_check_diag, dim3((c->N + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->N, c->diag_blk,
-                       c->bvals, c->bc_mask);
+    hipLaunchKernelGGL(k_check_diag, dim3((c->N + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->N, c->slice_ptr,
+                       c->diag_slot, c->bvals, c->bc_mask);
     CHECK_LAUNCH();
     adhk_kt_end(c);
     return 0;
 }
 
 /* apply_Dirichlet_BC (matrix_utilities.c:42-105): Dirichlet columns -> 0, Dirichlet rows -> identity,
- * residual -> 0; then the row maxima of scale_linear_system (scale_linear_system.c:31-121):
+ * residual -> 0; then (do_scale) the row maxima of scale_linear_system (scale_linear_system.c:31-121):
  * s = 1/sqrt(max|a|) (1/sqrt(eps) for a vanishing row), b *= s, x = dsol = 0 */
 __global__ __launch_bounds__(TPB) void k_dirichlet_scale(int N, const int *__restrict__ brow,
-                                                         const int *__restrict__ bcol, double *__restrict__ bv,
+                                                         const int *__restrict__ slice_ptr,
+                                                         const int *__restrict__ scol, double *__restrict__ bv,
                                                          const int *__restrict__ bc, double *__restrict__ res,
                                                          double *__restrict__ dsol, double *__restrict__ scale,
                                                          int do_scale)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
+    const int lane = n & 63, c0 = slice_ptr[n >> 6], deg = brow[n + 1] - brow[n];
     int brc[3] = {bc[3 * n], bc[3 * n + 1], bc[3 * n + 2]};
     double mx[3] = {0., 0., 0.};
-    for (int j = brow[n]; j < brow[n + 1]; j++) {
-        int m = bcol[j];
+    for (int k = 0; k < deg; k++) {
+        const int cs = c0 + k;
+        const int m = scol[cs * 64 + lane];
         int bcc[3] = {bc[3 * m], bc[3 * m + 1], bc[3 * m + 2]};
-        double *a = bv + 9 * (size_t)j;
         for (int r = 0; r < 3; r++)
             for (int cc = 0; cc < 3; cc++) {
-                double v = a[3 * r + cc];
+                double *ap = bv + sell_val(cs, 3 * r + cc, lane);
+                double v = *ap;
                 if (brc[r] == 1)
                     v = (m == n && cc == r) ? 1.0 : 0.0;
                 else if (bcc[cc] == 1)
                     v = 0.0;
-                a[3 * r + cc] = v;
+                *ap = v;
                 double t = fabs(v);
                 if (t > mx[r]) mx[r] = t;
             }
@@ -281,7 +327,7 @@ extern "C" int adhk_launch_dirichlet(adhk_ctx *c, int do_scale)
 {
     adhk_kt_begin(c, ADHK_KT_PREP);
     hipLaunchKernelGGL(k_dirichlet_scale, dim3((c->N + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->N, c->brow,
-                       c->bcol, c->bvals, c->bc_mask, c->residual, c->dsol, c->scale, do_scale);
+                       c->slice_ptr, c->scol, c->bvals, c->bc_mask, c->residual, c->dsol, c->scale, do_scale);
     CHECK_LAUNCH();
     adhk_kt_end(c);
     return 0;
@@ -291,28 +337,31 @@ extern "C" int adhk_launch_dirichlet(adhk_ctx *c, int do_scale)
  * diagonal block (adjugate; |det| <= 1e-300 falls back to the inverted diagonal).  With T = float the
  * scaled matrix and inverse are also written in FP32 for the mixed-precision Krylov solve. */
 template <typename T>
-__global__ __launch_bounds__(TPB) void k_scale_prec(int N, const int *__restrict__ brow, const int *__restrict__ bcol,
-                                                    const int *__restrict__ diag_blk, double *__restrict__ bv,
+__global__ __launch_bounds__(TPB) void k_scale_prec(int N, const int *__restrict__ brow,
+                                                    const int *__restrict__ slice_ptr, const int *__restrict__ scol,
+                                                    const int *__restrict__ diag_slot, double *__restrict__ bv,
                                                     const double *__restrict__ scale, double *__restrict__ minv,
                                                     T *__restrict__ bv_t, T *__restrict__ minv_t)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
+    const int lane = n & 63, c0 = slice_ptr[n >> 6], w = slice_ptr[(n >> 6) + 1] - c0, deg = brow[n + 1] - brow[n];
     double si[3] = {scale[3 * n], scale[3 * n + 1], scale[3 * n + 2]};
-    for (int j = brow[n]; j < brow[n + 1]; j++) {
-        int m = bcol[j];
+    for (int k = 0; k < w; k++) {
+        const int cs = c0 + k;
+        const int m = scol[cs * 64 + lane];
         double sj[3] = {scale[3 * m], scale[3 * m + 1], scale[3 * m + 2]};
-        double *a = bv + 9 * (size_t)j;
         for (int r = 0; r < 3; r++)
             for (int cc = 0; cc < 3; cc++) {
-                double v = a[3 * r + cc] * (si[r] * sj[cc]);
-                a[3 * r + cc] = v;
-                if (bv_t) bv_t[9 * (size_t)j + 3 * r + cc] = (T)v;
+                const size_t ix = sell_val(cs, 3 * r + cc, lane);
+                double v = (k < deg) ? bv[ix] * (si[r] * sj[cc]) : 0.0;
+                if (k < deg) bv[ix] = v;
+                if (bv_t) bv_t[ix] = (T)v;
             }
     }
-    int d = diag_blk[n];
+    int d = diag_slot[n];
     double a[9];
-    for (int k = 0; k < 9; k++) a[k] = (d >= 0) ? bv[9 * (size_t)d + k] : 0.0;
+    for (int k = 0; k < 9; k++) a[k] = (d >= 0) ? bv[sell_val(c0 + d, k, lane)] : 0.0;
     double c00 = a[4] * a[8] - a[5] * a[7];
     double c01 = a[5] * a[6] - a[3] * a[8];
     double c02 = a[3] * a[7] - a[4] * a[6];
@@ -333,9 +382,10 @@ __global__ __launch_bounds__(TPB) void k_scale_prec(int N, const int *__restrict
         for (int k = 0; k < 9; k++) m[k] = 0.0;
         for (int k = 0; k < 3; k++) m[4 * k] = (a[4 * k] != 0.0) ? 1.0 / a[4 * k] : 1.0;
     }
+    double *mo = minv + (size_t)(n >> 6) * 9 * 64 + lane;
     for (int k = 0; k < 9; k++) {
-        minv[9 * (size_t)n + k] = m[k];
-        if (minv_t) minv_t[9 * (size_t)n + k] = (T)m[k];
+        mo[k * 64] = m[k];
+        if (minv_t) minv_t[(size_t)(n >> 6) * 9 * 64 + lane + k * 64] = (T)m[k];
     }
 }
 
@@ -344,23 +394,26 @@ extern "C" int adhk_launch_scale_prec(adhk_ctx *c, int mixed)
     adhk_kt_begin(c, ADHK_KT_PREP);
     if (mixed)
         hipLaunchKernelGGL(k_scale_prec<float>, dim3((c->N + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->N, c->brow,
-                           c->bcol, c->diag_blk, c->bvals, c->scale, c->minv, c->bvals32, c->minv32);
+                           c->slice_ptr, c->scol, c->diag_slot, c->bvals, c->scale, c->minv, c->bvals32, c->minv32);
     else
         hipLaunchKernelGGL(k_scale_prec<double>, dim3((c->N + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->N,
-                           c->brow, c->bcol, c->diag_blk, c->bvals, c->scale, c->minv, (double *)NULL,
+                           c->brow, c->slice_ptr, c->scol, c->diag_slot, c->bvals, c->scale, c->minv, (double *)NULL,
                            (double *)NULL);
     CHECK_LAUNCH();
     adhk_kt_end(c);
     return 0;
 }
 
-/* ---------------- BiCGStab (bcgstab_solver.c:65-335) ---------------- */
-/* b <- M^-1 b (kept in s until the first iteration), x0 = dsol, x = M^-1 b, p = Ap = 0   (:128-147) */
+/* ---------------- BiCGStab (bcgstab_solver.c:65-335) ----------------
+ * Kernel sequence: init1, init2, A<first>, S, then {A, S} per iteration.  A of iteration k+1
+ * finishes iteration k (x and r updates and the ||r||_inf test) and starts k+1 (new direction p and
+ * Ap = M^-1 A p); S computes s and As = M^-1 A s.  p and Ap are double-buffered. */
+
+/* b <- M^-1 b (kept in s until the first S overwrites it), x0 = dsol, x = M^-1 b   (:128-147) */
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_kinit1(int N, const T *__restrict__ minv, const double *__restrict__ b,
                                                 const double *__restrict__ dsol, double *__restrict__ x0,
-                                                T *__restrict__ x, T *__restrict__ bp, T *__restrict__ p,
-                                                T *__restrict__ Ap)
+                                                T *__restrict__ x, T *__restrict__ bp)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
@@ -373,23 +426,20 @@ __global__ __launch_bounds__(TPB) void k_kinit1(int N, const T *__restrict__ min
         T zz = (T)z[k];
         x[i] = zz;
         bp[i] = zz;
-        p[i] = (T)0;
-        Ap[i] = (T)0;
     }
 }
 
-/* r = b - M^-1 A x, q = r, p = r (first direction, since p = Ap = 0), rnorm, bnorm, rho  (:155-207) */
+/* r = b - M^-1 A x, q = r, rnorm, bnorm, rho = r.q  (:155-207) */
 template <typename T>
-__global__ __launch_bounds__(TPB) void k_kinit2(int N, const int *__restrict__ brow, const int *__restrict__ bcol,
-                                                const T *__restrict__ bv, const T *__restrict__ minv,
+__global__ __launch_bounds__(TPB) void k_kinit2(int N, Sell S, const T *__restrict__ bv, const T *__restrict__ minv,
                                                 const T *__restrict__ x, const T *__restrict__ bp, T *__restrict__ r,
-                                                T *__restrict__ q, T *__restrict__ p, double *partials,
-                                                unsigned int *counter, adhk_scalars *sc)
+                                                T *__restrict__ q, double *partials, unsigned int *counter,
+                                                adhk_scalars *sc)
 {
     double vmax[2] = {0.0, 0.0}, vsum[1] = {0.0};
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
         double y0, y1, y2, z0, z1, z2;
-        block_row<T>(n, brow, bcol, bv, ColPlain<T>{x}, y0, y1, y2);
+        block_row<T>(n, S, bv, ColPlain<T>{x}, y0, y1, y2);
         apply_minv<T>(minv, n, y0, y1, y2, z0, z1, z2);
         double z[3] = {z0, z1, z2};
         for (int k = 0; k < 3; k++) {
@@ -398,7 +448,6 @@ __global__ __launch_bounds__(TPB) void k_kinit2(int N, const int *__restrict__ b
             T rr = (T)(bb + (-1) * z[k]);
             r[i] = rr;
             q[i] = rr;
-            p[i] = rr;
             vmax[0] = fmax(vmax[0], fabs((double)rr));
             vmax[1] = fmax(vmax[1], fabs(bb));
             vsum[0] += (double)rr * (double)rr;
@@ -406,9 +455,6 @@ __global__ __launch_bounds__(TPB) void k_kinit2(int N, const int *__restrict__ b
     }
     bool l1 = grid_reduce<2, true>(vmax, partials, counter + 0);
     bool l2 = grid_reduce<1, false>(vsum, partials + 4 * MAX_GRID, counter + 1);
-    (void)l1;
-    (void)l2;
-    /* both reductions finish in some last workgroup; publish from each */
     if (threadIdx.x == 0 && l1) {
         sc->rnorm = vmax[0];
         sc->bnorm = vmax[1];
@@ -428,36 +474,64 @@ __global__ void k_kinit3(adhk_scalars *sc, double tol, int max_it)
     sc->done = !(sc->rnorm > (tol + sc->bnorm * tol) && 0 < max_it);
 }
 
-template <typename T>
-__global__ __launch_bounds__(TPB) void k_spmv_p(int N, const int *__restrict__ brow, const int *__restrict__ bcol,
-                                                const T *__restrict__ bv, const T *__restrict__ minv,
-                                                const T *__restrict__ p, T *__restrict__ Ap,
-                                                const T *__restrict__ q, double *partials, unsigned int *counter,
-                                                adhk_scalars *sc)
+/* A: [FIRST] p1 = r0 (p = Ap = 0 at start); otherwise finish iteration k -- x += alpha p + omega s,
+ * r = s - omega As, ||r||_inf and the stop test (:294-307, :210) -- and form p' = (p - omega Ap) beta + r.
+ * Then Ap' = M^-1 A p' and q.Ap' -> alpha = rho / (q.Ap')  (:215-246). */
+template <typename T, bool FIRST>
+__global__ __launch_bounds__(TPB) void k_iter_a(int N, Sell S, const T *__restrict__ bv, const T *__restrict__ minv,
+                                                T *__restrict__ x, T *__restrict__ r, const T *__restrict__ s,
+                                                const T *__restrict__ As, const T *__restrict__ p_old,
+                                                const T *__restrict__ Ap_old, T *__restrict__ p_new,
+                                                T *__restrict__ Ap_new, const T *__restrict__ q, double *partials,
+                                                unsigned int *counter, adhk_scalars *sc)
 {
     if (sc->done) return;
-    double v[1] = {0.0};
+    const double alpha = sc->alpha, omega = sc->omega, beta = sc->beta;
+    double vmax[1] = {0.0}, vsum[1] = {0.0};
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
-        double y0, y1, y2, z0, z1, z2;
-        block_row<T>(n, brow, bcol, bv, ColPlain<T>{p}, y0, y1, y2);
+        double y0, y1, y2;
+        if (FIRST) {
+            for (int k = 0; k < 3; k++) p_new[3 * n + k] = r[3 * n + k];
+            block_row<T>(n, S, bv, ColPlain<T>{r}, y0, y1, y2);
+        } else {
+            ColP<T> col{s, As, p_old, Ap_old, omega, beta};
+            for (int k = 0; k < 3; k++) {
+                const int i = 3 * n + k;
+                T rr, pp;
+                col.one(i, rr, pp);
+                x[i] = (T)__fma_rn(omega, (double)s[i], __fma_rn(alpha, (double)p_old[i], (double)x[i]));
+                r[i] = rr;
+                p_new[i] = pp;
+                vmax[0] = fmax(vmax[0], fabs((double)rr));
+            }
+            block_row<T>(n, S, bv, col, y0, y1, y2);
+        }
+        double z0, z1, z2;
         apply_minv<T>(minv, n, y0, y1, y2, z0, z1, z2);
         T a0 = (T)z0, a1 = (T)z1, a2 = (T)z2;
-        Ap[3 * n] = a0;
-        Ap[3 * n + 1] = a1;
-        Ap[3 * n + 2] = a2;
-        v[0] += (double)q[3 * n] * (double)a0 + (double)q[3 * n + 1] * (double)a1 + (double)q[3 * n + 2] * (double)a2;
+        Ap_new[3 * n] = a0;
+        Ap_new[3 * n + 1] = a1;
+        Ap_new[3 * n + 2] = a2;
+        vsum[0] += (double)q[3 * n] * (double)a0 + (double)q[3 * n + 1] * (double)a1 +
+                   (double)q[3 * n + 2] * (double)a2;
     }
-    if (grid_reduce<1, false>(v, partials, counter + 2)) {
-        if (threadIdx.x == 0) {
-            sc->gamma = v[0];
-            sc->alpha = sc->rho / v[0];
-        }
+    bool lm = false;
+    if (!FIRST) lm = grid_reduce<1, true>(vmax, partials, counter + 2);
+    bool ls = grid_reduce<1, false>(vsum, partials + 4 * MAX_GRID, counter + 3);
+    if (threadIdx.x == 0 && lm) {
+        sc->rnorm = vmax[0];
+        sc->done = !(vmax[0] > (sc->tol + sc->bnorm * sc->tol) && sc->it < sc->max_it);
+    }
+    if (threadIdx.x == 0 && ls) {
+        sc->gamma = vsum[0];
+        sc->alpha = sc->rho / vsum[0];
     }
 }
 
+/* S: s = r - alpha Ap, As = M^-1 A s, omega = (As.s)/(As.As), rho' = -omega (As.q),
+ * beta' = (rho' alpha)/(rho omega), iteration count + 1   (:247-291) */
 template <typename T>
-__global__ __launch_bounds__(TPB) void k_spmv_s(int N, const int *__restrict__ brow, const int *__restrict__ bcol,
-                                                const T *__restrict__ bv, const T *__restrict__ minv,
+__global__ __launch_bounds__(TPB) void k_iter_s(int N, Sell S, const T *__restrict__ bv, const T *__restrict__ minv,
                                                 const T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ s,
                                                 T *__restrict__ As, const T *__restrict__ q, double *partials,
                                                 unsigned int *counter, adhk_scalars *sc)
@@ -468,7 +542,7 @@ __global__ __launch_bounds__(TPB) void k_spmv_s(int N, const int *__restrict__ b
     double v[3] = {0.0, 0.0, 0.0};
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
         double y0, y1, y2, z0, z1, z2;
-        block_row<T>(n, brow, bcol, bv, col, y0, y1, y2);
+        block_row<T>(n, S, bv, col, y0, y1, y2);
         apply_minv<T>(minv, n, y0, y1, y2, z0, z1, z2);
         double z[3] = {z0, z1, z2};
         for (int k = 0; k < 3; k++) {
@@ -482,7 +556,7 @@ __global__ __launch_bounds__(TPB) void k_spmv_s(int N, const int *__restrict__ b
             v[2] += (double)ai * (double)q[i];
         }
     }
-    if (grid_reduce<3, false>(v, partials, counter + 3)) {
+    if (grid_reduce<3, false>(v, partials, counter + 4)) {
         if (threadIdx.x == 0) {
             double omega = v[0] / v[1];
             double rhop = sc->rho;
@@ -491,36 +565,7 @@ __global__ __launch_bounds__(TPB) void k_spmv_s(int N, const int *__restrict__ b
             sc->rhop = rhop;
             sc->rho = rho;
             sc->beta = (rho * alpha) / (rhop * omega);
-        }
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(TPB) void k_update(int ndofs, T *__restrict__ x, T *__restrict__ r, T *__restrict__ p,
-                                                const T *__restrict__ Ap, const T *__restrict__ s,
-                                                const T *__restrict__ As, double *partials, unsigned int *counter,
-                                                adhk_scalars *sc)
-{
-    if (sc->done) return;
-    const double alpha = sc->alpha, omega = sc->omega, beta = sc->beta;
-    double v[1] = {0.0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ndofs; i += gridDim.x * blockDim.x) {
-        double pi = (double)p[i], si = (double)s[i];
-        double xi = (double)x[i] + alpha * pi;
-        x[i] = (T)(xi + omega * si);
-        T rr = (T)(si + (-omega) * (double)As[i]);
-        r[i] = rr;
-        double t = pi + (-omega) * (double)Ap[i];
-        t *= beta;
-        p[i] = (T)(t + 1 * (double)rr);
-        v[0] = fmax(v[0], fabs((double)rr));
-    }
-    if (grid_reduce<1, true>(v, partials, counter + 4)) {
-        if (threadIdx.x == 0) {
-            sc->rnorm = v[0];
-            int it = sc->it + 1;
-            sc->it = it;
-            sc->done = !(v[0] > (sc->tol + sc->bnorm * sc->tol) && it < sc->max_it);
+            sc->it = sc->it + 1;
         }
     }
 }
@@ -537,19 +582,21 @@ __global__ void k_kfinish(int ndofs, const T *__restrict__ x, const double *__re
 }
 
 template <typename T> struct KV {
-    T *r, *p, *Ap, *s, *As, *q, *x;
+    T *r, *p[2], *Ap[2], *s, *As, *q, *x;
     const T *bv, *mi;
 };
 static KV<double> kv64(adhk_ctx *c)
 {
     KV<double> k;
     k.r = c->r;
-    k.p = c->pv;
-    k.Ap = c->Ap;
+    k.p[0] = c->pv;
+    k.p[1] = c->pv2;
+    k.Ap[0] = c->Ap;
+    k.Ap[1] = c->Ap2;
     k.s = c->s;
     k.As = c->As;
     k.q = c->q;
-    k.x = c->x0 + c->ndofs; /* x0 is allocated with 2*ndofs: [x0 | x] */
+    k.x = c->xk;
     k.bv = c->bvals;
     k.mi = c->minv;
     return k;
@@ -558,8 +605,10 @@ static KV<float> kv32(adhk_ctx *c)
 {
     KV<float> k;
     k.r = c->r32;
-    k.p = c->p32;
-    k.Ap = c->Ap32;
+    k.p[0] = c->p32;
+    k.p[1] = c->p32b;
+    k.Ap[0] = c->Ap32;
+    k.Ap[1] = c->Ap32b;
     k.s = c->s32;
     k.As = c->As32;
     k.q = c->q32;
@@ -569,36 +618,46 @@ static KV<float> kv32(adhk_ctx *c)
     return k;
 }
 
+static Sell sell_of(adhk_ctx *c)
+{
+    Sell S;
+    S.slice_ptr = c->slice_ptr;
+    S.scol = c->scol;
+    return S;
+}
+
 template <typename T> static int krylov_init(adhk_ctx *c, KV<T> k, double tol, int max_it)
 {
     int gn = (c->N + TPB - 1) / TPB;
-    /* bp (preconditioned right-hand side) is kept in s until the first iteration overwrites it */
     hipLaunchKernelGGL(k_kinit1<T>, dim3(gn), dim3(TPB), 0, c->stream, c->N, k.mi, c->residual, c->dsol, c->x0, k.x,
-                       k.s, k.p, k.Ap);
+                       k.s);
     CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_kinit2<T>, dim3(grid_for(c->N)), dim3(TPB), 0, c->stream, c->N, c->brow, c->bcol, k.bv,
-                       k.mi, k.x, k.s, k.r, k.q, k.p, c->partials, c->counter, c->sc);
+    hipLaunchKernelGGL(k_kinit2<T>, dim3(grid_for(c->N)), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.mi, k.x,
+                       k.s, k.r, k.q, c->partials, c->counter, c->sc);
     CHECK_LAUNCH();
     hipLaunchKernelGGL(k_kinit3, dim3(1), dim3(1), 0, c->stream, c->sc, tol, max_it);
     CHECK_LAUNCH();
     return 0;
 }
 
-template <typename T> static int krylov_iter(adhk_ctx *c, KV<T> k)
+template <typename T, bool FIRST> static int launch_a(adhk_ctx *c, KV<T> k, int cur)
 {
+    /* reads direction buffer cur, writes 1 - cur (FIRST writes buffer 1) */
+    int nw = FIRST ? 1 : 1 - cur;
     adhk_kt_begin(c, ADHK_KT_SPMV_P);
-    hipLaunchKernelGGL(k_spmv_p<T>, dim3(grid_for(c->N)), dim3(TPB), 0, c->stream, c->N, c->brow, c->bcol, k.bv, k.mi,
-                       k.p, k.Ap, k.q, c->partials, c->counter, c->sc);
+    hipLaunchKernelGGL((k_iter_a<T, FIRST>), dim3(grid_for(c->N)), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv,
+                       k.mi, k.x, k.r, k.s, k.As, k.p[cur], k.Ap[cur], k.p[nw], k.Ap[nw], k.q, c->partials,
+                       c->counter, c->sc);
     adhk_kt_end(c);
     CHECK_LAUNCH();
+    return 0;
+}
+
+template <typename T> static int launch_s(adhk_ctx *c, KV<T> k, int cur)
+{
     adhk_kt_begin(c, ADHK_KT_SPMV_S);
-    hipLaunchKernelGGL(k_spmv_s<T>, dim3(grid_for(c->N)), dim3(TPB), 0, c->stream, c->N, c->brow, c->bcol, k.bv, k.mi,
-                       k.r, k.Ap, k.s, k.As, k.q, c->partials, c->counter, c->sc);
-    adhk_kt_end(c);
-    CHECK_LAUNCH();
-    adhk_kt_begin(c, ADHK_KT_UPDATE);
-    hipLaunchKernelGGL(k_update<T>, dim3(grid_for(c->ndofs)), dim3(TPB), 0, c->stream, c->ndofs, k.x, k.r, k.p, k.Ap,
-                       k.s, k.As, c->partials, c->counter, c->sc);
+    hipLaunchKernelGGL(k_iter_s<T>, dim3(grid_for(c->N)), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.mi, k.r,
+                       k.Ap[cur], k.s, k.As, k.q, c->partials, c->counter, c->sc);
     adhk_kt_end(c);
     CHECK_LAUNCH();
     return 0;
@@ -616,9 +675,30 @@ extern "C" int adhk_launch_krylov_init(adhk_ctx *c, int mixed, double tol, int m
 {
     return mixed ? krylov_init<float>(c, kv32(c), tol, max_it) : krylov_init<double>(c, kv64(c), tol, max_it);
 }
-extern "C" int adhk_launch_krylov_iter(adhk_ctx *c, int mixed)
+/* first iteration: A<first> then S; the current direction buffer is then 1 */
+extern "C" int adhk_launch_krylov_first(adhk_ctx *c, int mixed)
 {
-    return mixed ? krylov_iter<float>(c, kv32(c)) : krylov_iter<double>(c, kv64(c));
+    if (mixed) {
+        if (launch_a<float, true>(c, kv32(c), 0)) return -1;
+        return launch_s<float>(c, kv32(c), 1);
+    }
+    if (launch_a<double, true>(c, kv64(c), 0)) return -1;
+    return launch_s<double>(c, kv64(c), 1);
+}
+/* one more iteration from direction buffer cur: A then S; the current buffer becomes 1 - cur */
+extern "C" int adhk_launch_krylov_iter(adhk_ctx *c, int mixed, int cur)
+{
+    if (mixed) {
+        if (launch_a<float, false>(c, kv32(c), cur)) return -1;
+        return launch_s<float>(c, kv32(c), 1 - cur);
+    }
+    if (launch_a<double, false>(c, kv64(c), cur)) return -1;
+    return launch_s<double>(c, kv64(c), 1 - cur);
+}
+/* the A kernel that completes the last iteration (x, r updates and the stop test) */
+extern "C" int adhk_launch_krylov_last(adhk_ctx *c, int mixed, int cur)
+{
+    return mixed ? launch_a<float, false>(c, kv32(c), cur) : launch_a<double, false>(c, kv64(c), cur);
 }
 extern "C" int adhk_launch_krylov_finish(adhk_ctx *c, int mixed)
 {

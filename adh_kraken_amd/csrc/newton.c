@@ -70,8 +70,9 @@ static void ctx_free(adhk_ctx *c)
                     c->snrml, c->sol, c->sol_old, c->sol_older, c->dirichlet, c->dsol, c->residual, c->scale,
                     c->bc_mask, c->wd_flag, c->wd_tmp, c->brow, c->bcol, c->diag_blk, c->cptr, c->contrib, c->bvals,
                     c->minv, c->bvals32, c->minv32, c->erhs, c->emat, c->r, c->pv, c->Ap, c->s, c->As, c->q, c->x0,
-                    c->r32, c->p32, c->Ap32, c->s32, c->As32, c->q32, c->xk32, c->partials, c->counter, c->sc,
-                    c->flags};
+                    c->pv2, c->Ap2, c->xk, c->r32, c->p32, c->Ap32, c->s32, c->As32, c->q32, c->xk32, c->p32b,
+                    c->Ap32b, c->partials, c->counter, c->sc, c->flags, c->slice_ptr, c->scol, c->sell2blk,
+                    c->blk2sell, c->diag_slot};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++)
         if (ptrs[i]) hipFree(ptrs[i]);
     for (int i = 0; i < 8; i++)
@@ -224,8 +225,10 @@ adhk_sw2 *adhk_sw2_create(int nnodes, const double *x, const double *y, const do
         DALLOC(c->s, nd, double);
         DALLOC(c->As, nd, double);
         DALLOC(c->q, nd, double);
-        DALLOC(c->x0, 2 * nd, double);
-        DALLOC(c->minv, 9 * N, double);
+        DALLOC(c->x0, nd, double);
+        DALLOC(c->pv2, nd, double);
+        DALLOC(c->Ap2, nd, double);
+        DALLOC(c->xk, nd, double);
         DALLOC(c->partials, 8 * 2048, double);
         DALLOC(c->counter, 8, unsigned int);
         DALLOC(c->sc, 1, adhk_scalars);
@@ -235,8 +238,10 @@ adhk_sw2 *adhk_sw2_create(int nnodes, const double *x, const double *y, const do
         DZERO(c->flags, 4, int);
     }
     LC(adhk_build_pattern(c));
-    DALLOC(c->bvals, 9L * c->nnzb, double);
-    DZERO(c->bvals, 9L * c->nnzb, double);
+    DALLOC(c->bvals, 9L * 64 * c->nslot, double);
+    DZERO(c->bvals, 9L * 64 * c->nslot, double);
+    DALLOC(c->minv, 9L * 64 * c->nslice, double);
+    DZERO(c->minv, 9L * 64 * c->nslice, double);
     HC(hipStreamSynchronize(c->stream));
     free(h_djac);
     free(h_grad);
@@ -259,9 +264,11 @@ fail:
 static int ensure_mixed(adhk_ctx *c)
 {
     if (c->bvals32) return 0;
-    const long nd = c->ndofs, N = c->N;
-    DALLOC(c->bvals32, 9L * c->nnzb, float);
-    DALLOC(c->minv32, 9 * N, float);
+    const long nd = c->ndofs;
+    DALLOC(c->bvals32, 9L * 64 * c->nslot, float);
+    DALLOC(c->minv32, 9L * 64 * c->nslice, float);
+    DALLOC(c->p32b, nd, float);
+    DALLOC(c->Ap32b, nd, float);
     DALLOC(c->r32, nd, float);
     DALLOC(c->p32, nd, float);
     DALLOC(c->Ap32, nd, float);
@@ -632,16 +639,27 @@ static int krylov(adhk_ctx *c, const adhk_solver *s, int mixed)
     LC(adhk_launch_krylov_init(c, mixed, tol, s->lin_max_it));
     HC(hipMemcpyAsync(&hs, c->sc, sizeof hs, hipMemcpyDeviceToHost, c->stream));
     HC(hipStreamSynchronize(c->stream));
-    int launched = 0;
-    while (!hs.done && launched < s->lin_max_it) {
-        int k = chunk;
-        if (launched + k > s->lin_max_it) k = s->lin_max_it - launched;
-        for (int i = 0; i < k; i++) LC(adhk_launch_krylov_iter(c, mixed));
-        launched += k;
+    if (!hs.done) {
+        /* iteration 1, then chunks; the A kernel of iteration k+1 completes iteration k and tests it */
+        int cur = 1, launched = 1;
+        LC(adhk_launch_krylov_first(c, mixed));
+        while (launched < s->lin_max_it) {
+            int k = chunk;
+            if (launched + k > s->lin_max_it) k = s->lin_max_it - launched;
+            for (int i = 0; i < k; i++) {
+                LC(adhk_launch_krylov_iter(c, mixed, cur));
+                cur = 1 - cur;
+            }
+            launched += k;
+            HC(hipMemcpyAsync(&hs, c->sc, sizeof hs, hipMemcpyDeviceToHost, c->stream));
+            HC(hipStreamSynchronize(c->stream));
+            if (hs.done) break;
+        }
+        if (!hs.done) LC(adhk_launch_krylov_last(c, mixed, cur));
         HC(hipMemcpyAsync(&hs, c->sc, sizeof hs, hipMemcpyDeviceToHost, c->stream));
-        HC(hipStreamSynchronize(c->stream));
     }
     LC(adhk_launch_krylov_finish(c, mixed));
+    HC(hipStreamSynchronize(c->stream));
     c->timing.krylov_calls++;
     return hs.it;
 fail:

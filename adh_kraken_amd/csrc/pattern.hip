@@ -79,6 +79,89 @@ __global__ void k_rowptr(int N, int nnzb, const int *brow_of, const int *bcol, i
     }
 }
 
+/* SELL-64: slice width = widest block row among the slice's 64 nodes */
+__global__ void k_slice_width(int N, int nslice, const int *brow, int *sw)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslice) return;
+    int w = 0;
+    for (int l = 0; l < 64; l++) {
+        int n = s * 64 + l;
+        if (n < N) {
+            int d = brow[n + 1] - brow[n];
+            if (d > w) w = d;
+        }
+    }
+    sw[s] = w;
+}
+
+/* one thread per (slice, lane): the node's blocks in column order, then zero padding */
+__global__ void k_sell_fill(int N, int nslice, const int *slice_ptr, const int *brow, const int *bcol, int *scol,
+                            int *sell2blk, int *blk2sell, int *diag_slot)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nslice * 64) return;
+    int s = t >> 6, l = t & 63, n = t;
+    int c0 = slice_ptr[s], w = slice_ptr[s + 1] - c0;
+    int b0 = 0, d = 0;
+    if (n < N) {
+        b0 = brow[n];
+        d = brow[n + 1] - b0;
+        diag_slot[n] = -1;
+    }
+    for (int k = 0; k < w; k++) {
+        int pos = (c0 + k) * 64 + l;
+        if (k < d) {
+            int b = b0 + k;
+            scol[pos] = bcol[b];
+            sell2blk[pos] = b;
+            blk2sell[b] = pos;
+            if (bcol[b] == n) diag_slot[n] = k;
+        } else {
+            scol[pos] = (n < N) ? n : 0;
+            sell2blk[pos] = -1;
+        }
+    }
+}
+
+static int build_sell(adhk_ctx *c)
+{
+    int *sw = NULL;
+    void *tmp = NULL;
+    size_t tb = 0;
+    c->nslice = (c->N + 63) / 64;
+    CHECK(hipMalloc((void **)&sw, ((size_t)c->nslice + 1) * sizeof(int)));
+    CHECK(hipMemsetAsync(sw, 0, ((size_t)c->nslice + 1) * sizeof(int), c->stream));
+    CHECK(hipMalloc((void **)&c->slice_ptr, ((size_t)c->nslice + 1) * sizeof(int)));
+    hipLaunchKernelGGL(k_slice_width, dim3((c->nslice + 255) / 256), dim3(256), 0, c->stream, c->N, c->nslice, c->brow,
+                       sw);
+    CHECK(hipGetLastError());
+    CHECK(hipcub::DeviceScan::ExclusiveSum(NULL, tb, sw, c->slice_ptr, c->nslice + 1, c->stream));
+    CHECK(hipMalloc(&tmp, tb));
+    CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tb, sw, c->slice_ptr, c->nslice + 1, c->stream));
+    CHECK(hipMemcpyAsync(&c->nslot, c->slice_ptr + c->nslice, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    CHECK(hipStreamSynchronize(c->stream));
+    if ((long)c->nslot * 64 * 9 >= 2147483647L) {
+        adhk_set_error("SELL layout too large for 32-bit indices");
+        goto fail;
+    }
+    CHECK(hipMalloc((void **)&c->scol, (size_t)c->nslot * 64 * sizeof(int)));
+    CHECK(hipMalloc((void **)&c->sell2blk, (size_t)c->nslot * 64 * sizeof(int)));
+    CHECK(hipMalloc((void **)&c->blk2sell, (size_t)c->nnzb * sizeof(int)));
+    CHECK(hipMalloc((void **)&c->diag_slot, (size_t)c->N * sizeof(int)));
+    hipLaunchKernelGGL(k_sell_fill, dim3((c->nslice * 64 + 255) / 256), dim3(256), 0, c->stream, c->N, c->nslice,
+                       c->slice_ptr, c->brow, c->bcol, c->scol, c->sell2blk, c->blk2sell, c->diag_slot);
+    CHECK(hipGetLastError());
+    CHECK(hipStreamSynchronize(c->stream));
+    hipFree(sw);
+    hipFree(tmp);
+    return 0;
+fail:
+    hipFree(sw);
+    hipFree(tmp);
+    return -1;
+}
+
 extern "C" int adhk_build_pattern(adhk_ctx *c)
 {
     long np = 9L * c->E + 4L * c->S;
@@ -140,7 +223,7 @@ extern "C" int adhk_build_pattern(adhk_ctx *c)
     hipFree(brow_of);
     hipFree(tmp);
     (void)hlast;
-    return 0;
+    return build_sell(c);
 fail:
     hipFree(keys);
     hipFree(keys_s);
@@ -167,15 +250,18 @@ __global__ void k_csr_pattern(int N, int nnzb, const int *brow, const int *bcol,
     if (n == N - 1) indptr[3 * N] = 9 * nnzb;
 }
 
-__global__ void k_csr_vals(int N, const int *brow, const double *bvals, double *vals)
+__global__ void k_csr_vals(int N, const int *brow, const int *blk2sell, const double *bvals, double *vals)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     int b0 = brow[n], deg = brow[n + 1] - b0;
     for (int r = 0; r < 3; r++) {
         int rs = 9 * b0 + 3 * r * deg;
-        for (int j = 0; j < deg; j++)
-            for (int cc = 0; cc < 3; cc++) vals[rs + 3 * j + cc] = bvals[9 * (size_t)(b0 + j) + 3 * r + cc];
+        for (int j = 0; j < deg; j++) {
+            int pos = blk2sell[b0 + j];
+            size_t base = (size_t)(pos >> 6) * 9 * 64 + (pos & 63);
+            for (int cc = 0; cc < 3; cc++) vals[rs + 3 * j + cc] = bvals[base + (size_t)(3 * r + cc) * 64];
+        }
     }
 }
 
@@ -193,8 +279,8 @@ extern "C" int adhk_launch_csr_pattern(adhk_ctx *c, int *d_indptr, int *d_cols)
 
 extern "C" int adhk_launch_csr_vals(adhk_ctx *c, double *d_vals)
 {
-    hipLaunchKernelGGL(k_csr_vals, dim3((c->N + 255) / 256), dim3(256), 0, c->stream, c->N, c->brow, c->bvals,
-                       d_vals);
+    hipLaunchKernelGGL(k_csr_vals, dim3((c->N + 255) / 256), dim3(256), 0, c->stream, c->N, c->brow, c->blk2sell,
+                       c->bvals, d_vals);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         adhk_set_error("csr_vals: %s", hipGetErrorString(e));

@@ -268,19 +268,21 @@ extern "C" int adhk_launch_elem_jac(adhk_ctx *c)
     return 0;
 }
 
-/* bvals[b][3r+cc] = sum over contributions (element order) of the elemental entry
- * (row a*3+r, col bb*3+cc); emat is stored [elem][col][row] */
-__global__ __launch_bounds__(256) void k_jac_gather(int nnzb, int E, const int *__restrict__ cptr,
-                                                    const int *__restrict__ contrib, const double *__restrict__ emat,
-                                                    double *__restrict__ bvals)
+/* one thread per SELL slot: block entry (3r+cc) = sum over the block's contributions (element order)
+ * of the elemental entry (row a*3+r, col bb*3+cc); emat is stored [elem][col][row]; padding slots get 0 */
+__global__ __launch_bounds__(256) void k_jac_gather(int nslot64, int E, const int *__restrict__ sell2blk,
+                                                    const int *__restrict__ cptr, const int *__restrict__ contrib,
+                                                    const double *__restrict__ emat, double *__restrict__ bvals)
 {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nnzb) return;
+    int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= nslot64) return;
+    int b = sell2blk[pos];
     double acc[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) acc[k] = 0.0;
     const long segbase = 81L * E;
-    for (int j = cptr[b]; j < cptr[b + 1]; j++) {
+    const int j0 = (b >= 0) ? cptr[b] : 0, j1 = (b >= 0) ? cptr[b + 1] : 0;
+    for (int j = j0; j < j1; j++) {
         int cde = contrib[j];
         const double *em;
         int a, bb, ld;
@@ -303,15 +305,16 @@ __global__ __launch_bounds__(256) void k_jac_gather(int nnzb, int E, const int *
 #pragma unroll
             for (int cc = 0; cc < 3; cc++) acc[3 * r + cc] += em[(3 * bb + cc) * ld + 3 * a + r];
     }
+    double *out = bvals + (size_t)(pos >> 6) * 9 * 64 + (pos & 63);
 #pragma unroll
-    for (int k = 0; k < 9; k++) bvals[9 * (size_t)b + k] = acc[k];
+    for (int k = 0; k < 9; k++) out[(size_t)k * 64] = acc[k];
 }
 
 extern "C" int adhk_launch_jac_gather(adhk_ctx *c)
 {
     adhk_kt_begin(c, ADHK_KT_JAC_GATHER);
-    hipLaunchKernelGGL(k_jac_gather, dim3((c->nnzb + 255) / 256), dim3(256), 0, c->stream, c->nnzb, c->E, c->cptr,
-                       c->contrib, c->emat, c->bvals);
+    hipLaunchKernelGGL(k_jac_gather, dim3((c->nslot * 64 + 255) / 256), dim3(256), 0, c->stream, c->nslot * 64, c->E,
+                       c->sell2blk, c->cptr, c->contrib, c->emat, c->bvals);
     CHECK_LAUNCH();
     adhk_kt_end(c);
     return 0;
