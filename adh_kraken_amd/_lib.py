@@ -62,6 +62,7 @@ SIGNATURES = [
     ("adhk_sw2_create", P, [C.c_int, dp, dp, dp, C.c_int, ip, dp, C.c_int, ip, ip, dp, C.POINTER(Params)]),
     ("adhk_sw2_destroy", None, [P]),
     ("adhk_sw2_sizes", C.c_int, [P, ip, ip, ip]),
+    ("adhk_sw2_layout", C.c_int, [P, ip, ip]),
     ("adhk_sw2_set_state", C.c_int, [P, dp, dp, dp]),
     ("adhk_sw2_get_state", C.c_int, [P, dp, dp, dp]),
     ("adhk_sw2_set_bc", C.c_int, [P, ip, dp]),
@@ -71,6 +72,7 @@ SIGNATURES = [
     ("adhk_sw2_get_timing", C.c_int, [P, C.POINTER(Timing)]),
     ("adhk_sw2_kernel_timing", C.c_int, [P, C.c_int]),
     ("adhk_sw2_kernel_times", C.c_int, [P, dp, C.POINTER(C.c_longlong)]),
+    ("adhk_copy_probe", C.c_int, [P, C.c_longlong, C.c_int, dp]),
     ("adhk_sw2_geometry", C.c_int, [P, dp, dp, dp, dp]),
     ("adhk_sw2_init", C.c_int, [P, ip]),
     ("adhk_assemble_residual", C.c_int, [P, dp]),

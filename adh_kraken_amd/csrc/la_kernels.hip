@@ -17,7 +17,7 @@
  * Scalars live in device memory; every kernel exits at once when the solver
  * has converged, so the host launches iterations in chunks and reads the
  * "done" flag between chunks.  Reductions are deterministic: per-workgroup
- * partials, and the last workgroup to finish sums them in index order.
+ * partials, summed in index order by a one-workgroup finalize kernel.
  * The block matrix and vectors are templated on the storage type (double for
  * the FP64 path, float for the mixed-precision path); reductions and scalars
  * are always double.
@@ -47,6 +47,33 @@ static inline int grid_for(long n)
     return (int)g;
 }
 
+/* XCD-aware node ranges for the matrix kernels.  Workgroups are dispatched round-robin over the 8
+ * XCDs (workgroup w runs on XCD w % 8), and each XCD has its own L2.  Logical block
+ * (w % 8) * (G / 8) + w / 8 gives every XCD a contiguous eighth of the node range, so the neighbour
+ * gathers of a slice (nodes +-npy away) hit in that XCD's L2 instead of being fetched by several. */
+#define NXCD 8
+static inline int grid_xcd(long nodes)
+{
+    long g = (nodes + TPB - 1) / TPB;
+    g = ((g + NXCD - 1) / NXCD) * NXCD;
+    if (g > MAX_GRID) g = MAX_GRID;
+    if (g < NXCD) g = NXCD;
+    return (int)g;
+}
+struct NodeRange {
+    int begin, end;
+};
+__device__ __forceinline__ NodeRange xcd_range(int N)
+{
+    const int G = gridDim.x, w = blockIdx.x;
+    const int b = (w % NXCD) * (G / NXCD) + w / NXCD;
+    const int per = ((N + G - 1) / G + TPB - 1) / TPB * TPB; /* nodes per logical block, multiple of 256 */
+    NodeRange r;
+    r.begin = b * per;
+    r.end = min(N, (b + 1) * per);
+    return r;
+}
+
 /* ---- deterministic block reductions ---- */
 template <int K, bool MAX> __device__ __forceinline__ void block_reduce(double (&v)[K], double *sh)
 {
@@ -73,43 +100,34 @@ template <int K, bool MAX> __device__ __forceinline__ void block_reduce(double (
     }
 }
 
-/* writes this workgroup's partials; returns true in the last workgroup to finish, whose
- * thread 0 then holds the totals (summed over workgroups in index order) in v */
-template <int K, bool MAX>
-__device__ __forceinline__ bool grid_reduce(double (&v)[K], double *partials, unsigned int *counter)
+/* Cross-workgroup reductions are deterministic and fence-free: thread 0 of every workgroup stores
+ * its K block results at partials[k * MAX_GRID + blockIdx.x]; a one-workgroup finalize kernel that
+ * runs next on the stream sums them in index order (the kernel boundary makes them visible). */
+template <int K, bool MAX> __device__ __forceinline__ void block_partials(double (&v)[K], double *partials)
 {
     __shared__ double sh[(TPB / 64) * K];
-    __shared__ int amlast;
     block_reduce<K, MAX>(v, sh);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < K; k++) partials[k * MAX_GRID + blockIdx.x] = v[k];
-        __threadfence();
-        unsigned t = atomicAdd(counter, 1u);
-        amlast = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!amlast) return false;
-    __threadfence();
-    /* last workgroup: strided partial sums in a fixed order, then the same tree */
-    double w[K];
+}
+
+/* in a one-workgroup kernel: totals over n workgroups' partials, valid in thread 0 */
+template <int K, bool MAX> __device__ __forceinline__ void sum_partials(const double *partials, int n, double (&w)[K])
+{
+    __shared__ double sh[(TPB / 64) * K];
 #pragma unroll
-    for (int k = 0; k < K; k++) w[k] = MAX ? 0.0 : 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += TPB)
+    for (int k = 0; k < K; k++) w[k] = 0.0;
+    for (int b = threadIdx.x; b < n; b += TPB)
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            double pv = __builtin_nontemporal_load(&partials[k * MAX_GRID + b]);
+            double pv = partials[k * MAX_GRID + b];
             w[k] = MAX ? fmax(w[k], pv) : w[k] + pv;
         }
-    __syncthreads();
     block_reduce<K, MAX>(w, sh);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) v[k] = w[k];
-        *counter = 0u;
-    }
-    return true;
 }
+
+#define PART_B (4 * MAX_GRID) /* second partial region */
 
 /* ---- SELL-64 block row product y = A_n x (column values from a functor) ---- */
 template <typename T> struct ColPlain {
@@ -217,8 +235,7 @@ __device__ __forceinline__ size_t sell_val(int cs, int e, int lane) { return ((s
 /* get_residual_norms, newton_tools.c:303-373: max|r|, sum r^2, max|dsol| over non-Dirichlet dofs */
 __global__ __launch_bounds__(TPB) void k_norms(int ndofs, const double *__restrict__ res,
                                                const double *__restrict__ dsol, const int *__restrict__ bc,
-                                               double *partials, unsigned int *counter, adhk_scalars *sc,
-                                               int *flags)
+                                               double *partials)
 {
     double vmax[3] = {0.0, 0.0, 0.0}; /* max|r|, max|dsol|, nan indicator */
     double vsum[1] = {0.0};
@@ -231,22 +248,29 @@ __global__ __launch_bounds__(TPB) void k_norms(int ndofs, const double *__restri
             vsum[0] += r * r;
         }
     }
-    if (grid_reduce<3, true>(vmax, partials, counter + 0)) {
-        if (threadIdx.x == 0) {
-            sc->rmax = vmax[0];
-            sc->incmax = vmax[1];
-            if (vmax[2] != 0.0) flags[1] = 1;
-        }
-    }
-    if (grid_reduce<1, false>(vsum, partials + 4 * MAX_GRID, counter + 1)) {
-        if (threadIdx.x == 0) sc->rl2sum = vsum[0];
+    block_partials<3, true>(vmax, partials);
+    block_partials<1, false>(vsum, partials + PART_B);
+}
+__global__ __launch_bounds__(TPB) void k_norms_fin(int n, const double *partials, adhk_scalars *sc, int *flags)
+{
+    double vmax[3], vsum[1];
+    sum_partials<3, true>(partials, n, vmax);
+    sum_partials<1, false>(partials + PART_B, n, vsum);
+    if (threadIdx.x == 0) {
+        sc->rmax = vmax[0];
+        sc->incmax = vmax[1];
+        if (vmax[2] != 0.0) flags[1] = 1;
+        sc->rl2sum = vsum[0];
     }
 }
 
 extern "C" int adhk_launch_norms(adhk_ctx *c)
 {
-    hipLaunchKernelGGL(k_norms, dim3(grid_for(c->ndofs)), dim3(TPB), 0, c->stream, c->ndofs, c->residual, c->dsol,
-                       c->bc_mask, c->partials, c->counter, c->sc, c->flags);
+    int g = grid_for(c->ndofs);
+    hipLaunchKernelGGL(k_norms, dim3(g), dim3(TPB), 0, c->stream, c->ndofs, c->residual, c->dsol, c->bc_mask,
+                       c->partials);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_norms_fin, dim3(1), dim3(TPB), 0, c->stream, g, c->partials, c->sc, c->flags);
     CHECK_LAUNCH();
     return 0;
 }
@@ -433,11 +457,11 @@ __global__ __launch_bounds__(TPB) void k_kinit1(int N, const T *__restrict__ min
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_kinit2(int N, Sell S, const T *__restrict__ bv, const T *__restrict__ minv,
                                                 const T *__restrict__ x, const T *__restrict__ bp, T *__restrict__ r,
-                                                T *__restrict__ q, double *partials, unsigned int *counter,
-                                                adhk_scalars *sc)
+                                                T *__restrict__ q, double *partials)
 {
     double vmax[2] = {0.0, 0.0}, vsum[1] = {0.0};
-    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    const NodeRange nr = xcd_range(N);
+    for (int n = nr.begin + threadIdx.x; n < nr.end; n += TPB) {
         double y0, y1, y2, z0, z1, z2;
         block_row<T>(n, S, bv, ColPlain<T>{x}, y0, y1, y2);
         apply_minv<T>(minv, n, y0, y1, y2, z0, z1, z2);
@@ -453,25 +477,29 @@ __global__ __launch_bounds__(TPB) void k_kinit2(int N, Sell S, const T *__restri
             vsum[0] += (double)rr * (double)rr;
         }
     }
-    bool l1 = grid_reduce<2, true>(vmax, partials, counter + 0);
-    bool l2 = grid_reduce<1, false>(vsum, partials + 4 * MAX_GRID, counter + 1);
-    if (threadIdx.x == 0 && l1) {
-        sc->rnorm = vmax[0];
-        sc->bnorm = vmax[1];
-    }
-    if (threadIdx.x == 0 && l2) sc->rho = vsum[0];
+    block_partials<2, true>(vmax, partials);
+    block_partials<1, false>(vsum, partials + PART_B);
 }
 
-__global__ void k_kinit3(adhk_scalars *sc, double tol, int max_it)
+__global__ __launch_bounds__(TPB) void k_kinit3(int n, const double *partials, adhk_scalars *sc, double tol,
+                                                int max_it)
 {
-    sc->tol = tol;
-    sc->max_it = max_it;
-    sc->rhop = 1.0;
-    sc->alpha = 1.0;
-    sc->omega = 1.0;
-    sc->it = 0;
-    sc->beta = (sc->rho * 1.0) / (1.0 * 1.0);
-    sc->done = !(sc->rnorm > (tol + sc->bnorm * tol) && 0 < max_it);
+    double vmax[2], vsum[1];
+    sum_partials<2, true>(partials, n, vmax);
+    sum_partials<1, false>(partials + PART_B, n, vsum);
+    if (threadIdx.x == 0) {
+        sc->rnorm = vmax[0];
+        sc->bnorm = vmax[1];
+        sc->rho = vsum[0];
+        sc->tol = tol;
+        sc->max_it = max_it;
+        sc->rhop = 1.0;
+        sc->alpha = 1.0;
+        sc->omega = 1.0;
+        sc->it = 0;
+        sc->beta = (sc->rho * 1.0) / (1.0 * 1.0);
+        sc->done = !(sc->rnorm > (tol + sc->bnorm * tol) && 0 < max_it);
+    }
 }
 
 /* A: [FIRST] p1 = r0 (p = Ap = 0 at start); otherwise finish iteration k -- x += alpha p + omega s,
@@ -483,12 +511,13 @@ __global__ __launch_bounds__(TPB) void k_iter_a(int N, Sell S, const T *__restri
                                                 const T *__restrict__ As, const T *__restrict__ p_old,
                                                 const T *__restrict__ Ap_old, T *__restrict__ p_new,
                                                 T *__restrict__ Ap_new, const T *__restrict__ q, double *partials,
-                                                unsigned int *counter, adhk_scalars *sc)
+                                                const adhk_scalars *sc)
 {
     if (sc->done) return;
     const double alpha = sc->alpha, omega = sc->omega, beta = sc->beta;
     double vmax[1] = {0.0}, vsum[1] = {0.0};
-    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    const NodeRange nr = xcd_range(N);
+    for (int n = nr.begin + threadIdx.x; n < nr.end; n += TPB) {
         double y0, y1, y2;
         if (FIRST) {
             for (int k = 0; k < 3; k++) p_new[3 * n + k] = r[3 * n + k];
@@ -515,14 +544,22 @@ __global__ __launch_bounds__(TPB) void k_iter_a(int N, Sell S, const T *__restri
         vsum[0] += (double)q[3 * n] * (double)a0 + (double)q[3 * n + 1] * (double)a1 +
                    (double)q[3 * n + 2] * (double)a2;
     }
-    bool lm = false;
-    if (!FIRST) lm = grid_reduce<1, true>(vmax, partials, counter + 2);
-    bool ls = grid_reduce<1, false>(vsum, partials + 4 * MAX_GRID, counter + 3);
-    if (threadIdx.x == 0 && lm) {
-        sc->rnorm = vmax[0];
-        sc->done = !(vmax[0] > (sc->tol + sc->bnorm * sc->tol) && sc->it < sc->max_it);
-    }
-    if (threadIdx.x == 0 && ls) {
+    if (!FIRST) block_partials<1, true>(vmax, partials);
+    block_partials<1, false>(vsum, partials + PART_B);
+}
+
+/* finalize A: [not FIRST] ||r||_inf of the completed iteration and the stop test (:210); gamma, alpha */
+template <bool FIRST> __global__ __launch_bounds__(TPB) void k_fin_a(int n, const double *partials, adhk_scalars *sc)
+{
+    if (sc->done) return;
+    double vmax[1], vsum[1];
+    if (!FIRST) sum_partials<1, true>(partials, n, vmax);
+    sum_partials<1, false>(partials + PART_B, n, vsum);
+    if (threadIdx.x == 0) {
+        if (!FIRST) {
+            sc->rnorm = vmax[0];
+            sc->done = !(vmax[0] > (sc->tol + sc->bnorm * sc->tol) && sc->it < sc->max_it);
+        }
         sc->gamma = vsum[0];
         sc->alpha = sc->rho / vsum[0];
     }
@@ -534,13 +571,14 @@ template <typename T>
 __global__ __launch_bounds__(TPB) void k_iter_s(int N, Sell S, const T *__restrict__ bv, const T *__restrict__ minv,
                                                 const T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ s,
                                                 T *__restrict__ As, const T *__restrict__ q, double *partials,
-                                                unsigned int *counter, adhk_scalars *sc)
+                                                const adhk_scalars *sc)
 {
     if (sc->done) return;
     const double alpha = sc->alpha;
     ColS<T> col{r, Ap, alpha};
     double v[3] = {0.0, 0.0, 0.0};
-    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    const NodeRange nr = xcd_range(N);
+    for (int n = nr.begin + threadIdx.x; n < nr.end; n += TPB) {
         double y0, y1, y2, z0, z1, z2;
         block_row<T>(n, S, bv, col, y0, y1, y2);
         apply_minv<T>(minv, n, y0, y1, y2, z0, z1, z2);
@@ -556,17 +594,24 @@ __global__ __launch_bounds__(TPB) void k_iter_s(int N, Sell S, const T *__restri
             v[2] += (double)ai * (double)q[i];
         }
     }
-    if (grid_reduce<3, false>(v, partials, counter + 4)) {
-        if (threadIdx.x == 0) {
-            double omega = v[0] / v[1];
-            double rhop = sc->rho;
-            double rho = -1.0 * omega * v[2];
-            sc->omega = omega;
-            sc->rhop = rhop;
-            sc->rho = rho;
-            sc->beta = (rho * alpha) / (rhop * omega);
-            sc->it = sc->it + 1;
-        }
+    block_partials<3, false>(v, partials);
+}
+
+/* finalize S: omega = (As.s)/(As.As), rho' = -omega (As.q), beta' = (rho' alpha)/(rho omega), it + 1 */
+__global__ __launch_bounds__(TPB) void k_fin_s(int n, const double *partials, adhk_scalars *sc)
+{
+    if (sc->done) return;
+    double v[3];
+    sum_partials<3, false>(partials, n, v);
+    if (threadIdx.x == 0) {
+        double omega = v[0] / v[1];
+        double rhop = sc->rho;
+        double rho = -1.0 * omega * v[2];
+        sc->omega = omega;
+        sc->rhop = rhop;
+        sc->rho = rho;
+        sc->beta = (rho * sc->alpha) / (rhop * omega);
+        sc->it = sc->it + 1;
     }
 }
 
@@ -632,10 +677,11 @@ template <typename T> static int krylov_init(adhk_ctx *c, KV<T> k, double tol, i
     hipLaunchKernelGGL(k_kinit1<T>, dim3(gn), dim3(TPB), 0, c->stream, c->N, k.mi, c->residual, c->dsol, c->x0, k.x,
                        k.s);
     CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_kinit2<T>, dim3(grid_for(c->N)), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.mi, k.x,
-                       k.s, k.r, k.q, c->partials, c->counter, c->sc);
+    const int g = grid_xcd(c->N);
+    hipLaunchKernelGGL(k_kinit2<T>, dim3(g), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.mi, k.x, k.s, k.r,
+                       k.q, c->partials);
     CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_kinit3, dim3(1), dim3(1), 0, c->stream, c->sc, tol, max_it);
+    hipLaunchKernelGGL(k_kinit3, dim3(1), dim3(TPB), 0, c->stream, g, c->partials, c->sc, tol, max_it);
     CHECK_LAUNCH();
     return 0;
 }
@@ -644,21 +690,26 @@ template <typename T, bool FIRST> static int launch_a(adhk_ctx *c, KV<T> k, int 
 {
     /* reads direction buffer cur, writes 1 - cur (FIRST writes buffer 1) */
     int nw = FIRST ? 1 : 1 - cur;
+    const int g = grid_xcd(c->N);
     adhk_kt_begin(c, ADHK_KT_SPMV_P);
-    hipLaunchKernelGGL((k_iter_a<T, FIRST>), dim3(grid_for(c->N)), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv,
-                       k.mi, k.x, k.r, k.s, k.As, k.p[cur], k.Ap[cur], k.p[nw], k.Ap[nw], k.q, c->partials,
-                       c->counter, c->sc);
+    hipLaunchKernelGGL((k_iter_a<T, FIRST>), dim3(g), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.mi, k.x, k.r,
+                       k.s, k.As, k.p[cur], k.Ap[cur], k.p[nw], k.Ap[nw], k.q, c->partials, c->sc);
     adhk_kt_end(c);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_fin_a<FIRST>, dim3(1), dim3(TPB), 0, c->stream, g, c->partials, c->sc);
     CHECK_LAUNCH();
     return 0;
 }
 
 template <typename T> static int launch_s(adhk_ctx *c, KV<T> k, int cur)
 {
+    const int g = grid_xcd(c->N);
     adhk_kt_begin(c, ADHK_KT_SPMV_S);
-    hipLaunchKernelGGL(k_iter_s<T>, dim3(grid_for(c->N)), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.mi, k.r,
-                       k.Ap[cur], k.s, k.As, k.q, c->partials, c->counter, c->sc);
+    hipLaunchKernelGGL(k_iter_s<T>, dim3(g), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.mi, k.r, k.Ap[cur],
+                       k.s, k.As, k.q, c->partials, c->sc);
     adhk_kt_end(c);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_fin_s, dim3(1), dim3(TPB), 0, c->stream, g, c->partials, c->sc);
     CHECK_LAUNCH();
     return 0;
 }
@@ -778,4 +829,41 @@ extern "C" int adhk_launch_rotate(adhk_ctx *c)
                        c->sol_old, c->sol_older);
     CHECK_LAUNCH();
     return 0;
+}
+
+/* ---------------- bandwidth probe ----------------
+ * A plain device copy of a known size on the context stream: gives the achievable HBM ceiling and
+ * the scale of the FETCH_SIZE / WRITE_SIZE counters for a kernel whose traffic is known exactly. */
+__global__ __launch_bounds__(TPB) void k_copy_probe(const double4 *__restrict__ a, double4 *__restrict__ b, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
+}
+
+extern "C" int adhk_copy_probe(adhk_ctx *c, long long nbytes, int reps, double *ms_avg)
+{
+    double4 *a = NULL, *b = NULL;
+    hipEvent_t e0 = NULL, e1 = NULL;
+    long n = (long)(nbytes / (long long)sizeof(double4));
+    float ms = 0.f;
+    int rc = -1;
+    if (!c || n <= 0 || reps <= 0) return -1;
+    if (hipMalloc((void **)&a, n * sizeof(double4)) != hipSuccess) goto done;
+    if (hipMalloc((void **)&b, n * sizeof(double4)) != hipSuccess) goto done;
+    if (hipMemsetAsync(a, 0, n * sizeof(double4), c->stream) != hipSuccess) goto done;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) goto done;
+    hipLaunchKernelGGL(k_copy_probe, dim3(2048), dim3(TPB), 0, c->stream, a, b, n);
+    if (hipEventRecord(e0, c->stream) != hipSuccess) goto done;
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy_probe, dim3(2048), dim3(TPB), 0, c->stream, a, b, n);
+    if (hipEventRecord(e1, c->stream) != hipSuccess) goto done;
+    if (hipEventSynchronize(e1) != hipSuccess) goto done;
+    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) goto done;
+    if (ms_avg) *ms_avg = ms / reps;
+    rc = 0;
+done:
+    if (rc) adhk_set_error("adhk_copy_probe failed");
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    return rc;
 }

@@ -290,6 +290,14 @@ int adhk_sw2_sizes(const adhk_sw2 *c, int *ndofs, int *nnz, int *nnzb)
     return 0;
 }
 
+int adhk_sw2_layout(const adhk_sw2 *c, int *nslice, int *nslot)
+{
+    if (!c) return -1;
+    if (nslice) *nslice = c->nslice;
+    if (nslot) *nslot = c->nslot;
+    return 0;
+}
+
 static int copy_in(adhk_ctx *c, double *dst, const double *src)
 {
     if (!src) return 0;

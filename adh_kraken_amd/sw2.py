@@ -127,6 +127,9 @@ class SModelSuper:
         nd, nnz, nnzb = C.c_int(), C.c_int(), C.c_int()
         self.lib.adhk_sw2_sizes(self.h, C.byref(nd), C.byref(nnz), C.byref(nnzb))
         self.ndofs, self.nnz, self.nnz_blocks = nd.value, nnz.value, nnzb.value
+        nsl, nso = C.c_int(), C.c_int()
+        self.lib.adhk_sw2_layout(self.h, C.byref(nsl), C.byref(nso))
+        self.nslice, self.nslot = nsl.value, nso.value
         # Newton knobs (smodel_super_alloc_init.c:62-65) and linear solver knobs (bcgstab_solver.c:20-22)
         self.solver = L.solver_defaults()
         self.ts = TimeInfo(dt=dt, tau_temporal=tau_temporal)

@@ -103,6 +103,8 @@ adhk_sw2 *adhk_sw2_create(int nnodes, const double *x, const double *y, const do
                           const double *seg_val, const adhk_params *p);
 void adhk_sw2_destroy(adhk_sw2 *c);
 int adhk_sw2_sizes(const adhk_sw2 *c, int *ndofs, int *nnz, int *nnz_blocks);
+/* SELL-64 layout of the device matrix: 64-node slices and 3x3-block slot columns */
+int adhk_sw2_layout(const adhk_sw2 *c, int *nslice, int *nslot);
 int adhk_sw2_set_state(adhk_sw2 *c, const double *sol, const double *sol_old, const double *sol_older);
 int adhk_sw2_get_state(adhk_sw2 *c, double *sol, double *sol_old, double *sol_older);
 int adhk_sw2_set_bc(adhk_sw2 *c, const int *bc_mask, const double *dirichlet_data);
@@ -115,6 +117,9 @@ int adhk_sw2_get_timing(const adhk_sw2 *c, adhk_timing *t);
  * accumulated milliseconds and launch counts per class ([ADHK_KT_NKINDS] each). */
 int adhk_sw2_kernel_timing(adhk_sw2 *c, int on);
 int adhk_sw2_kernel_times(adhk_sw2 *c, double *ms, long long *count);
+/* device copy of nbytes (read nbytes + write nbytes) repeated reps times on the context's stream:
+ * average milliseconds per copy (the achievable HBM ceiling, and a counter-scale reference) */
+int adhk_copy_probe(adhk_sw2 *c, long long nbytes, int reps, double *ms_avg);
 int adhk_sw2_geometry(adhk_sw2 *c, double *djac, double *grad_shp, double *seg_djac, double *seg_nrml);
 
 /* replaces fe_sw2_init (src/models/sw2/fe_sw2_init.c:28) incl. fe_sw2_wdflag_legacy (sw2_wdflag_legacy.c:34) */
