@@ -126,9 +126,14 @@ def main():
     nb = 1 << 30
     L.check(L.load().adhk_copy_probe(sm.h, nb, 10, C.byref(probe_ms)), "copy_probe")
     copy_gbs = 2 * nb / (probe_ms.value / 1e3) / 1e9
-    roofline = {"kernel": {"spmv_p": "k_iter_a", "spmv_s": "k_iter_s"}[roof_kind], "bound": "hbm",
+    kname = {"spmv_p": "k_iter_a", "spmv_s": "k_iter_s"}[roof_kind]
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "sw2_newton_c2_traffic.json")
+    if args.config == "c2" and os.path.exists(tpath):  # HBM bytes per launch from the rocprofv3 counter runs
+        traffic = json.load(open(tpath))["bytes_per_launch"].get(kname)
+    roofline = {"kernel": kname, "bound": "hbm",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": ab,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes": ab,
                 "avg_launch_ms": round(avg_ms, 5), "copy_kernel_gbs": round(copy_gbs, 1)}
     value = args.steps / plain["elapsed"]
     out = {
