@@ -19,6 +19,7 @@
  *   assembly lists        cptr [nnzb+1], contrib [npairs]: contributions to each
  *                         block in element order (2-D first, then 1-D)
  *   elemental buffers     erhs [E*9 + S*6], emat [E*81 + S*36] stored [elem][col][row]
+ *   wet/dry queue         wdq [E+1]: count, then the partially dry elements of the last launch
  */
 #ifndef ADHK_INTERNAL_H
 #define ADHK_INTERNAL_H
@@ -67,6 +68,7 @@ struct adhk_sw2 {
     /* state */
     double *sol, *sol_old, *sol_older, *dirichlet, *dsol, *residual, *scale;
     int *bc_mask, *wd_flag, *wd_tmp;
+    int *wdq; /* partially dry elements of the last elemental launch: [0] count, [1..] elements */
     /* block CSR pattern and assembly lists */
     int *brow, *bcol, *diag_blk, *cptr, *contrib;
     /* SELL-64 layout of the matrix values */
@@ -85,7 +87,7 @@ struct adhk_sw2 {
     double *partials;
     unsigned int *counter;
     adhk_scalars *sc;
-    int *flags; /* [0] = partially dry element seen, [1] = NaN seen */
+    int *flags; /* [0] = unqueued partially dry element (internal guard), [1] = NaN seen */
     /* timing */
     adhk_timing timing;
     hipEvent_t ev[8];

@@ -68,7 +68,7 @@ static void ctx_free(adhk_ctx *c)
     if (!c) return;
     void *ptrs[] = {c->x, c->y, c->z, c->tri, c->djac, c->grad, c->ks, c->seg, c->seg_kind, c->seg_val, c->sdjac,
                     c->snrml, c->sol, c->sol_old, c->sol_older, c->dirichlet, c->dsol, c->residual, c->scale,
-                    c->bc_mask, c->wd_flag, c->wd_tmp, c->brow, c->bcol, c->diag_blk, c->cptr, c->contrib, c->bvals,
+                    c->bc_mask, c->wd_flag, c->wd_tmp, c->wdq, c->brow, c->bcol, c->diag_blk, c->cptr, c->contrib, c->bvals,
                     c->minv, c->bvals32, c->minv32, c->erhs, c->emat, c->r, c->pv, c->Ap, c->s, c->As, c->q, c->x0,
                     c->pv2, c->Ap2, c->xk, c->r32, c->p32, c->Ap32, c->s32, c->As32, c->q32, c->xk32, c->p32b,
                     c->Ap32b, c->partials, c->counter, c->sc, c->flags, c->slice_ptr, c->scol, c->sell2blk,
@@ -208,6 +208,8 @@ adhk_sw2 *adhk_sw2_create(int nnodes, const double *x, const double *y, const do
         DALLOC(c->bc_mask, nd, int);
         DALLOC(c->wd_flag, E, int);
         DALLOC(c->wd_tmp, N, int);
+        DALLOC(c->wdq, E + 1, int);
+        DZERO(c->wdq, E + 1, int);
         DZERO(c->sol, nd, double);
         DZERO(c->sol_old, nd, double);
         DZERO(c->sol_older, nd, double);
@@ -450,7 +452,7 @@ fail:
     return -1;
 }
 
-/* host flags: [0] partially dry element met, [1] NaN in the residual */
+/* host flags: [0] unqueued partially dry element (internal guard), [1] NaN in the residual */
 static int read_flags(adhk_ctx *c, int *flags)
 {
     HC(hipMemcpyAsync(flags, c->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -465,8 +467,8 @@ static int check_partial(adhk_ctx *c)
     int f[4];
     if (read_flags(c, f)) return -1;
     if (f[0]) {
-        adhk_set_error("partially dry element met (wet/dry sub-element integration is not on the GPU path); "
-                       "all elements must be all-wet or all-dry");
+        adhk_set_error("a partially dry element reached the all-wet/all-dry kernel without being queued "
+                       "for the wet/dry kernel (internal error)");
         return -1;
     }
     return 0;
@@ -628,8 +630,8 @@ static int norms(adhk_ctx *c, adhk_scalars *hs, int *nan)
     HC(hipMemcpyAsync(f, c->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HC(hipStreamSynchronize(c->stream));
     if (f[0]) {
-        adhk_set_error("partially dry element met (wet/dry sub-element integration is not on the GPU path); "
-                       "all elements must be all-wet or all-dry");
+        adhk_set_error("a partially dry element reached the all-wet/all-dry kernel without being queued "
+                       "for the wet/dry kernel (internal error)");
         return -1;
     }
     *nan = f[1];

@@ -14,8 +14,8 @@
  * Wet/dry: each wet/dry-wrapped integral (fe_wet_dry_wrapper.c:104) has three
  * outcomes, all nodes wet (integrand on the full triangle), all dry (nothing),
  * or partially dry (integral over the wet sub-triangle).  The first two are
- * handled here; a partially dry field makes the function return 1 and the
- * caller raises an error (partially dry elements are not on this path).
+ * handled here; a partially dry field makes the function return 1.  Callers
+ * route such elements to sw2_tri_resid_general (sw2_wetdry.h) instead.
  */
 #ifndef ADHK_SW2_DEVICE_H
 #define ADHK_SW2_DEVICE_H

@@ -12,6 +12,7 @@
 
 #include "adhk_internal.h"
 #include "sw2_device.h"
+#include "sw2_wetdry.h"
 
 #define CHECK_LAUNCH()                                                                                            \
     do {                                                                                                          \
@@ -87,17 +88,78 @@ static MeshView make_view(const adhk_ctx *c)
 }
 
 /* ---------------- residual ---------------- */
-__global__ __launch_bounds__(256) void k_elem_resid(MeshView m, Sw2Par P, double *__restrict__ erhs, int *flags)
+/* An element whose depth field at any level the residual reads is partially dry goes to the
+ * wet/dry queue (wdq[0] = count, wdq[1..] = elements) and is evaluated by k_elem_resid_wd;
+ * the queue order does not matter, each element owns its output slots. */
+__device__ __forceinline__ bool tri_partial(const TriIn &in, bool older)
+{
+    const int d = n_dry3(in.h[0], in.h[1], in.h[2]);
+    const int dold = n_dry3(in.ho[0], in.ho[1], in.ho[2]);
+    const int dolder = older ? n_dry3(in.hoo[0], in.hoo[1], in.hoo[2]) : 0;
+    return (d == 1 || d == 2) || (dold == 1 || dold == 2) || (dolder == 1 || dolder == 2);
+}
+
+/* the same, with every depth perturbation of the FD Jacobian (h[i] +- eps_i) */
+__device__ __forceinline__ bool tri_partial_fd(const TriIn &in, bool older)
+{
+    if (tri_partial(in, older)) return true;
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double eps = num_diff_eps(in.h[i], 1e-4);
+#pragma unroll
+        for (int sg = 0; sg < 2; sg++) {
+            const double hp = in.h[i] + (sg ? -eps : eps);
+            const double h0 = (i == 0) ? hp : in.h[0], h1 = (i == 1) ? hp : in.h[1], h2 = (i == 2) ? hp : in.h[2];
+            const int d = n_dry3(h0, h1, h2);
+            any = any || (d == 1 || d == 2);
+        }
+    }
+    return any;
+}
+
+__device__ __forceinline__ void wdq_push(int *wdq, int e)
+{
+    const int slot = atomicAdd(wdq, 1);
+    wdq[1 + slot] = e;
+}
+
+__global__ __launch_bounds__(256) void k_elem_resid(MeshView m, Sw2Par P, double *__restrict__ erhs, int *flags,
+                                                    int *wdq)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m.E) return;
     TriIn in;
     gather_tri(m, e, P.alpha != 0.0, in);
+    if (tri_partial(in, P.alpha != 0.0)) {
+        wdq_push(wdq, e);
+        return;
+    }
     double r[9];
     int partial = sw2_tri_resid(in, P, 0, -1, 0.0, r);
-    if (partial) flags[0] = 1;
+    if (partial) flags[0] = 1; /* cannot happen after tri_partial; kept as a guard */
 #pragma unroll
     for (int k = 0; k < 9; k++) erhs[9 * (size_t)e + k] = r[k];
+}
+
+__global__ void k_wdq_reset(int *wdq)
+{
+    if (threadIdx.x == 0) wdq[0] = 0;
+}
+
+/* queued (partially dry) elements: the general wet/dry residual, grid-stride over the queue */
+__global__ __launch_bounds__(128) void k_elem_resid_wd(MeshView m, Sw2Par P, const int *__restrict__ wdq,
+                                                       double *__restrict__ erhs)
+{
+    const int n = wdq[0];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int e = wdq[1 + i];
+        TriIn in;
+        gather_tri(m, e, true, in);
+        double r[9];
+        sw2_tri_resid_general(in, P, 0, -1, 0.0, r);
+        for (int k = 0; k < 9; k++) erhs[9 * (size_t)e + k] = r[k];
+    }
 }
 
 struct SegView {
@@ -136,13 +198,23 @@ static SegView make_segview(const adhk_ctx *c)
     return s;
 }
 
+/* workgroups of the queue kernels: enough for a wet/dry front, at most one per 128 elements */
+static inline unsigned wd_grid(long nwork)
+{
+    long g = (nwork + 127) / 128;
+    return (unsigned)(g < 1 ? 1 : (g > 1024 ? 1024 : g));
+}
+
 extern "C" int adhk_launch_elem_resid(adhk_ctx *c)
 {
     adhk_kt_begin(c, ADHK_KT_ELEM_RESID);
     Sw2Par P = make_par(c);
     if (c->E > 0) {
+        hipLaunchKernelGGL(k_wdq_reset, dim3(1), dim3(64), 0, c->stream, c->wdq);
         hipLaunchKernelGGL(k_elem_resid, dim3((c->E + 255) / 256), dim3(256), 0, c->stream, make_view(c), P, c->erhs,
-                           c->flags);
+                           c->flags, c->wdq);
+        hipLaunchKernelGGL(k_elem_resid_wd, dim3(wd_grid(c->E)), dim3(128), 0, c->stream, make_view(c), P, c->wdq,
+                           c->erhs);
         CHECK_LAUNCH();
     }
     if (c->S > 0) {
@@ -198,7 +270,8 @@ extern "C" int adhk_launch_resid_gather(adhk_ctx *c)
 /* ---------------- FD Jacobian ---------------- */
 /* one thread per (element, column): column = node i * 3 + var k; two residual
  * evaluations (+eps, -eps), load_elem_mat/perturb_var assemble_jacobian.c:123-363 */
-__global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, double *__restrict__ emat, int *flags)
+__global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, double *__restrict__ emat, int *flags,
+                                                  int *wdq)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 9 * m.E) return;
@@ -206,6 +279,11 @@ __global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, double *
     int i = col / 3, k = col % 3;
     TriIn in;
     gather_tri(m, e, P.alpha != 0.0, in);
+    /* the nine threads of an element reach the same decision; column 0 queues it */
+    if (tri_partial_fd(in, P.alpha != 0.0)) {
+        if (col == 0) wdq_push(wdq, e);
+        return;
+    }
     double var = 0.0;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -218,10 +296,31 @@ __global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, double *
     double rp[9], rm[9];
     int pa = sw2_tri_resid(in, P, i, k, (+1) * eps, rp);
     int pb = sw2_tri_resid(in, P, i, k, (-1) * eps, rm);
-    if (pa | pb) flags[0] = 1;
+    if (pa | pb) flags[0] = 1; /* cannot happen after tri_partial_fd; kept as a guard */
     double *out = emat + 81 * (size_t)e + 9 * col;
 #pragma unroll
     for (int r = 0; r < 9; r++) out[r] = (rp[r] - rm[r]) / eps2;
+}
+
+/* queued elements: nine columns each, central differences of the general wet/dry residual */
+__global__ __launch_bounds__(128) void k_elem_jac_wd(MeshView m, Sw2Par P, const int *__restrict__ wdq,
+                                                     double *__restrict__ emat)
+{
+    const long n = 9L * wdq[0];
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int e = wdq[1 + t / 9], col = (int)(t % 9);
+        const int i = col / 3, k = col % 3;
+        TriIn in;
+        gather_tri(m, e, true, in);
+        const double var = (k == 0) ? in.h[i] : ((k == 1) ? in.u[i] : in.v[i]);
+        const double eps = num_diff_eps(var, 1e-4);
+        const double eps2 = 2.0 * eps;
+        double rp[9], rm[9];
+        sw2_tri_resid_general(in, P, i, k, (+1) * eps, rp);
+        sw2_tri_resid_general(in, P, i, k, (-1) * eps, rm);
+        double *out = emat + 81 * (size_t)e + 9 * col;
+        for (int r = 0; r < 9; r++) out[r] = (rp[r] - rm[r]) / eps2;
+    }
 }
 
 __global__ void k_seg_jac(SegView s, Sw2Par P, double *__restrict__ emat)
@@ -255,8 +354,11 @@ extern "C" int adhk_launch_elem_jac(adhk_ctx *c)
     Sw2Par P = make_par(c);
     if (c->E > 0) {
         long nt = 9L * c->E;
+        hipLaunchKernelGGL(k_wdq_reset, dim3(1), dim3(64), 0, c->stream, c->wdq);
         hipLaunchKernelGGL(k_elem_jac, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, make_view(c), P,
-                           c->emat, c->flags);
+                           c->emat, c->flags, c->wdq);
+        hipLaunchKernelGGL(k_elem_jac_wd, dim3(wd_grid(nt)), dim3(128), 0, c->stream, make_view(c), P, c->wdq,
+                           c->emat);
         CHECK_LAUNCH();
     }
     if (c->S > 0) {

@@ -89,3 +89,110 @@ def body_residual(x, y, z, h, u, v, ho, uo, vo, dt, g, cf, dll=1e-6):
             r[3 * i + 1] += tau * (gx[i] * (g * hb * c_eq + ub * hb * x_eq) + gy[i] * (vb * hb * x_eq))
             r[3 * i + 2] += tau * (gy[i] * (g * hb * c_eq + vb * hb * y_eq) + gx[i] * (ub * hb * y_eq))
     return r
+
+
+def wet_pieces(h):
+    """Triangles covering the wet part {h > 0} of the P1 depth field, as 3x3 barycentric vertex rows,
+    and the wet polygon (counter-clockwise when the element is)."""
+    corners = np.eye(3)
+    poly = []
+    for k in range(3):
+        a, b = corners[k], corners[(k + 1) % 3]
+        ha, hb = h[k], h[(k + 1) % 3]
+        if ha > 0:
+            poly.append(a)
+        if (ha > 0) != (hb > 0):
+            poly.append(a + ha / (ha - hb) * (b - a))
+    tris = [np.array([poly[0], poly[i], poly[i + 1]]) for i in range(1, len(poly) - 1)]
+    return tris, poly
+
+
+def wet_quad(A, pieces, fn):
+    """sum over the wet pieces of the 7-point rule; fn(P) gets the points' barycentric coordinates (7x3)."""
+    tot = 0.0
+    for V in pieces:
+        P = BARY @ V
+        tot = tot + A * abs(np.linalg.det(V)) * np.tensordot(WTS, fn(P), axes=(0, 0))
+    return tot
+
+
+def wet_body_residual(x, y, z, h, u, v, ho, uo, vo, dt, g, cf, dll=1e-6):
+    """SW2 body residual of a triangle that may be partly dry (h <= 0 at some nodes).
+
+    What the wet/dry wrapper (fe_wet_dry_wrapper.c:104-304, with RE_DISTRIBUTE) integrates:
+    every wrapped term is the weak form over the wet region {h > 0} of the linear depth,
+    tested with the ORIGINAL P1 basis functions; the lumped momentum storage and the
+    friction are full-element terms scaled by the wet area fraction; the SUPG term uses
+    wet-region averages and integrals.  Stated here by clipping the element and applying the
+    7-point rule on each wet piece (the integrands are at most cubic on a piece).
+    """
+    A, gx, gy = p1_grad(x, y)
+    zx, zy = gx @ z, gy @ z
+    wet, poly = wet_pieces(h)
+    wet_o, _ = wet_pieces(ho)
+    area = wet_quad(A, wet, lambda P: np.ones(len(P)))
+    area_o = wet_quad(A, wet_o, lambda P: np.ones(len(P)))
+    fac, fac_o = area / A, area_o / A
+    r = np.zeros(9)
+
+    def Q(fn, pieces=wet):
+        return wet_quad(A, pieces, fn)
+
+    for i in range(3):
+        # advection
+        r[3 * i] += -dt * Q(lambda P: (gx[i] * (P @ h) * (P @ u) + gy[i] * (P @ h) * (P @ v)))
+        r[3 * i + 1] += -dt * Q(lambda P: (P @ h) * (P @ u) * (gx[i] * (P @ u) + gy[i] * (P @ v)))
+        r[3 * i + 2] += -dt * Q(lambda P: (P @ h) * (P @ v) * (gx[i] * (P @ u) + gy[i] * (P @ v)))
+        # storage: continuity over the wet regions of each level, lumped momentum scaled by the wet fraction
+        r[3 * i] += Q(lambda P: P[:, i] * (P @ h)) - Q(lambda P: P[:, i] * (P @ ho), wet_o)
+        r[3 * i + 1] += A / 3 * (fac * u[i] * max(h[i], dll) - fac_o * uo[i] * max(ho[i], dll))
+        r[3 * i + 2] += A / 3 * (fac * v[i] * max(h[i], dll) - fac_o * vo[i] * max(ho[i], dll))
+        # friction on the whole element, scaled by the wet fraction
+        sp = np.sqrt(u ** 2 + v ** 2)
+        full = [np.eye(3)]
+        r[3 * i + 1] += fac * dt * cf / 2 * wet_quad(A, full, lambda P: P[:, i] * (P @ (u * sp)))
+        r[3 * i + 2] += fac * dt * cf / 2 * wet_quad(A, full, lambda P: P[:, i] * (P @ (v * sp)))
+        # pressure and bed slope over the wet region
+        r[3 * i + 1] += -0.5 * g * dt * gx[i] * Q(lambda P: (P @ h) ** 2) + g * dt * zx * Q(lambda P: P[:, i] * (P @ h))
+        r[3 * i + 2] += -0.5 * g * dt * gy[i] * Q(lambda P: (P @ h) ** 2) + g * dt * zy * Q(lambda P: P[:, i] * (P @ h))
+    # boundary pressure on the wet polygon's edges (h = 0 on the wet/dry line)
+    X = np.array([x, y])
+    for k in range(len(poly)):
+        a, b = poly[k], poly[(k + 1) % len(poly)]
+        pa, pb = X @ a, X @ b
+        L = np.hypot(*(pb - pa))
+        if L == 0.0:
+            continue
+        nx, ny = (pb[1] - pa[1]) / L, -(pb[0] - pa[0]) / L
+        for s, w in zip(GL_X, GL_W):
+            bc = (1 - s) * a + s * b
+            hs = bc @ h
+            for node in range(3):
+                r[3 * node + 1] += 0.5 * g * dt * w * L * bc[node] * hs * hs * nx
+                r[3 * node + 2] += 0.5 * g * dt * w * L * bc[node] * hs * hs * ny
+    # SUPG with wet-region averages
+    hb = Q(lambda P: P @ h) / area if area > 0 else 0.0
+    ub = Q(lambda P: P @ u) / area if area > 0 else 0.0
+    vb = Q(lambda P: P @ v) / area if area > 0 else 0.0
+    if abs(hb) >= np.finfo(float).eps:
+        hden = max(hb, np.finfo(np.float32).eps)
+        tau = 0.5 * np.sqrt(A * fac_o) / np.sqrt(ub * ub + vb * vb + g * hden + np.finfo(float).eps)
+        dhdt = Q(lambda P: P @ h) - Q(lambda P: P @ ho, wet_o)
+        dudt = Q(lambda P: P @ u) - Q(lambda P: P @ uo, wet_o)
+        dvdt = Q(lambda P: P @ v) - Q(lambda P: P @ vo, wet_o)
+        hx, hy, ux, uy, vx, vy = gx @ h, gy @ h, gx @ u, gy @ u, gx @ v, gy @ v
+        a0 = dt * Q(lambda P: (P @ u) * hx + (P @ h) * ux + (P @ v) * hy + (P @ h) * vy)
+        a1 = dt * Q(lambda P: (P @ u) * ux + (P @ v) * uy)
+        a2 = dt * Q(lambda P: (P @ u) * vx + (P @ v) * vy)
+        res = cf / (2 * g)
+        vmag = np.hypot(ub, vb)
+        xf, yf = g * res * ub * vmag * dt * fac, g * res * vb * vmag * dt * fac
+        xp, yp = g * (hx + zx) * dt * fac, g * (hy + zy) * dt * fac
+        c_eq = dhdt + a0
+        x_eq = dudt + a1 + (xp + xf / hden) * A
+        y_eq = dvdt + a2 + (yp + yf / hden) * A
+        for i in range(3):
+            r[3 * i] += tau * (gx[i] * (ub * c_eq + hb * x_eq) + gy[i] * (vb * c_eq + hb * y_eq))
+            r[3 * i + 1] += tau * (gx[i] * (g * hb * c_eq + ub * hb * x_eq) + gy[i] * (vb * hb * x_eq))
+            r[3 * i + 2] += tau * (gy[i] * (g * hb * c_eq + vb * hb * y_eq) + gx[i] * (ub * hb * y_eq))
+    return r

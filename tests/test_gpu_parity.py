@@ -89,6 +89,82 @@ def test_assembled_jacobian(gpu, c1):
     assert rel_err(A.assemble_jacobian(sm), orc.jacobian()) <= 1e-7
 
 
+def wet_dry_state(sol0, seed, hlo=-0.5, hhi=1.0, vamp=0.4):
+    """Random depths on both sides of zero: about a third of the elements are partially dry."""
+    rng = np.random.default_rng(seed)
+    n = len(sol0) // 3
+    s, so = sol0.copy(), sol0.copy()
+    for a in (s, so):
+        a[0::3] = rng.uniform(hlo, hhi, n)
+        a[1::3] = rng.uniform(-vamp, vamp, n)
+        a[2::3] = rng.uniform(-vamp, vamp, n)
+    return s, so
+
+
+@pytest.mark.parametrize("friction", ["manning", "const"])
+def test_elem_residuals_wet_dry(gpu, c1, friction):
+    mesh, sol0 = c1
+    s, so = wet_dry_state(sol0, seed=31)
+    sm, orc = make_pair(mesh, s, so, friction=friction)
+    eg, sg = A.elem_residuals(sm)
+    eo = np.array([orc.body_resid(e, s) for e in range(mesh.nelems2d)])
+    nd = (s[3 * mesh.tri] <= 0).sum(1)
+    assert ((nd == 1) | (nd == 2)).sum() > 1000
+    assert rel_err(eg, eo) <= 1e-12
+    so_ = np.array([orc.seg_resid(e, s) for e in range(mesh.nelems1d)])
+    assert rel_err(sg, so_) <= 1e-12
+
+
+def test_elem_jacobians_wet_dry(gpu, c1):
+    mesh, sol0 = c1
+    s, so = wet_dry_state(sol0, seed=32)
+    sm, orc = make_pair(mesh, s, so)
+    jg, _ = A.elem_jacobians(sm)
+    idx = np.arange(0, mesh.nelems2d, 5)
+    jo = np.array([orc.elem_jac_tri(int(e)) for e in idx])
+    assert rel_err(jg[idx], jo) <= 1e-7
+
+
+def test_assembled_residual_and_jacobian_wet_dry(gpu, c1):
+    mesh, sol0 = c1
+    s, so = wet_dry_state(sol0, seed=33)
+    sm, orc = make_pair(mesh, s, so)
+    assert rel_err(A.assemble_residual(sm), orc.residual()) <= 1e-12
+    assert rel_err(A.assemble_jacobian(sm), orc.jacobian()) <= 1e-7
+
+
+def wet_dry_flume():
+    """100 m x 10 m flume whose bed bumps rise above the still water: 125 dry nodes, 73 partially dry elements."""
+    spec = A.FlumeSpec(xmin=0.0, xmax=100.0, ymin=0.0, ymax=10.0, npx=101, npy=11, slope=1e-3, nbumps=6,
+                       bump_amp_max=1.6, bump_sigma_min=2.0, bump_sigma_max=6.0, seed=3, manning_n=0.03,
+                       inflow_q=2.0, tailwater=1.0, ic_flat_wse=1)
+    return A.flume_mesh(spec)
+
+
+def test_sw2_init_flags_and_dry_bc(gpu, c1):
+    mesh, sol0 = c1
+    s, so = wet_dry_state(sol0, seed=34)
+    sm, orc = make_pair(mesh, s, so)
+    wg = A.fe_sw2_init(sm)
+    orc.sw2_init()
+    assert np.array_equal(wg, orc.wdflag())
+    assert set(np.unique(wg)) == {0, 1, 2}
+    assert np.array_equal(sm.bc_mask, orc.bc_mask())
+
+
+def test_time_loop_wet_dry_depth(gpu):
+    mesh, sol0 = wet_dry_flume()
+    assert (sol0[0::3] <= 0).sum() > 100
+    sm, orc = make_pair(mesh, sol0)
+    ng, sg = A.time_loop(sm, 10)
+    no, so = orc.time_loop(10)
+    assert ng == no == 10
+    assert sg["newton_its"] == so["newton_its"]
+    gs, os_ = sm.sol, orc.state()[0]
+    assert rel_err(gs[0::3], os_[0::3]) <= 1e-8
+    assert rel_err(gs[1::3], os_[1::3]) <= 1e-6
+
+
 def test_check_diag_and_dirichlet(gpu):
     _, (mesh, sol0) = small_flume()
     s, so = perturbed_state(sol0, seed=2)

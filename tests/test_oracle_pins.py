@@ -8,6 +8,9 @@ mathematics fixes, each chosen so that a plausible slip in the restatement
 fails:
 
 * per-element residual vs an independent quadrature statement of the weak form
+* partially dry elements vs the same weak form over the clipped wet region with
+  the original test functions (what the wet/dry wrapper with RE_DISTRIBUTE
+  integrates), for every count of dry nodes at the new and the old level
 * lake at rest (flat water surface, zero velocity) -> zero residual
 * global mass and x-momentum balances (closed forms from partition of unity)
 * geometry closed forms (area, P1 gradients, outward unit normals)
@@ -25,7 +28,7 @@ import scipy.sparse.linalg as spla
 import adh_kraken_amd as A
 import oracle as O
 from helpers import perturbed_state, rel_err, small_flume
-from sw2_quadrature import body_residual
+from sw2_quadrature import body_residual, wet_body_residual
 
 G = 9.8
 
@@ -132,6 +135,72 @@ def test_elem_residual_vs_quadrature(seed):
                             so[3 * t + 1], so[3 * t + 2], 0.7, G, 0.002)
         got = orc.body_resid(e, s)
         assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def wet_dry_state(sol0, seed, hlo=-0.5, hhi=1.0, vamp=0.4):
+    rng = np.random.default_rng(seed)
+    n = len(sol0) // 3
+    s, so = sol0.copy(), sol0.copy()
+    for a in (s, so):
+        a[0::3] = rng.uniform(hlo, hhi, n)
+        a[1::3] = rng.uniform(-vamp, vamp, n)
+        a[2::3] = rng.uniform(-vamp, vamp, n)
+    return s, so
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_wet_dry_elem_residual_vs_wet_region_quadrature(seed):
+    spec, (mesh, sol0) = small_flume(npx=21, npy=9, seed=21 + seed)
+    s, so = wet_dry_state(sol0, seed)
+    orc = O.Oracle(mesh, dt=0.7, friction="const", cf_const=0.002)
+    orc.set_state(s, so, so)
+    seen = set()
+    for e in range(mesh.nelems2d):
+        t = mesh.tri[e]
+        ref = wet_body_residual(mesh.x[t], mesh.y[t], mesh.z[t], s[3 * t], s[3 * t + 1], s[3 * t + 2], so[3 * t],
+                                so[3 * t + 1], so[3 * t + 2], 0.7, G, 0.002)
+        got = orc.body_resid(e, s)
+        assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), e
+        seen.add((int((s[3 * t] <= 0).sum()), int((so[3 * t] <= 0).sum())))
+    # one and two dry nodes at each level, against wet and dry partners
+    assert {(1, 0), (2, 0), (0, 1), (0, 2), (1, 1), (2, 2)} <= seen
+
+
+def test_wdflag_brute_force():
+    """fe_sw2_wdflag_legacy (sw2_wdflag_legacy.c:34-139) from its definition, element by element:
+    2 where all six depths (new and old) are negative, else 1 where a node belongs to any element
+    that is not all wet, else 0; fe_sw2_init (fe_sw2_init.c:28-53) fixes u, v at old-dry nodes."""
+    spec, (mesh, sol0) = small_flume(npx=21, npy=9, seed=22)
+    s, so = wet_dry_state(sol0, 5)
+    orc = O.Oracle(mesh, dt=1.0)
+    orc.set_state(s, so, so)
+    got = orc.wdflag()
+    h, ho = s[0::3], so[0::3]
+    touched = set()
+    for t in mesh.tri:
+        if not all(h[n] > 0 for n in t):
+            touched.update(int(n) for n in t)
+    for e, t in enumerate(mesh.tri):
+        if all(h[n] < 0 for n in t) and all(ho[n] < 0 for n in t):
+            want = 2
+        else:
+            want = 1 if any(int(n) in touched for n in t) else 0
+        assert got[e] == want
+    assert set(np.unique(got)) == {0, 1, 2}
+    orc.sw2_init()
+    bc = orc.bc_mask()
+    assert np.array_equal(bc[1::3], (ho <= 0).astype(bc.dtype)) and np.array_equal(bc[2::3], bc[1::3])
+
+
+def test_wet_region_statement_reduces_to_full_element():
+    spec, (mesh, sol0) = small_flume(seed=5)
+    s, so = perturbed_state(sol0, seed=3)
+    for e in range(0, mesh.nelems2d, 5):
+        t = mesh.tri[e]
+        args = (mesh.x[t], mesh.y[t], mesh.z[t], s[3 * t], s[3 * t + 1], s[3 * t + 2], so[3 * t], so[3 * t + 1],
+                so[3 * t + 2], 0.7, G, 0.002)
+        a, b = wet_body_residual(*args), body_residual(*args)
+        assert np.abs(a - b).max() <= 1e-13 * max(1.0, np.abs(b).max())
 
 
 def test_lake_at_rest():
