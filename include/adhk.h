@@ -8,7 +8,7 @@
  *     SGRID*) restated over an opaque context that owns the mesh and the
  *     device-resident state.  Each function names the reference function it
  *     replaces.  A shim compiled inside the reference tree maps the reference
- *     structs onto these calls (see INTEGRATION.md).
+ *     structs onto these calls.
  *
  *  2. The reference's plain-pointer linear-algebra surface (src/la/la.h) with
  *     the reference's exact names and signatures, operating on host CSR arrays.

@@ -6,9 +6,21 @@
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it.  The product library never links or calls it.
  *
- * Every function cites the reference file:line it follows.  Pins (what checks
- * this restatement independently of retyping it) are listed in DESIGN.md and
- * exercised by tests/test_oracle_*.py.
+ * Every function cites the reference file:line it follows.  The pins (checks of
+ * this restatement that do not retype it) live in tests/test_oracle_pins.py:
+ *   P1 triangle and line integrals, all-wet body residual  -> 7-point / Gauss-Legendre quadrature
+ *   wet/dry wrapper (partially dry elements)                -> quadrature over the clipped wet region
+ *   boundary segments (depth, flux, velocity)               -> line quadrature
+ *   Manning friction chain                                  -> log-law profile integrated numerically
+ *   lake at rest, global mass and x-momentum balances       -> invariants
+ *   geometry, dof map, CSR pattern, wet/dry flags           -> closed forms and brute force
+ *   FD Jacobian                                             -> directional differences of the residual
+ *   block-Jacobi, BiCGStab, Newton                          -> dense inverses, direct sparse solve,
+ *                                                              converged-state residual
+ * Partially pinned: the SUPG combination (sw2_body_resid.c:1074-1079).  Its
+ * integrals are checked by quadrature, but the weights that combine the strong
+ * residuals are the reference's algebra, stated the same way in the quadrature
+ * check, so a slip common to both would not show.
  *
  * Layout conventions follow the reference: dof(node, var) = 3*node + var with
  * var 0 = h (depth), 1 = u, 2 = v (depth-averaged velocity)

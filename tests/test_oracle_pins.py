@@ -13,6 +13,10 @@ fails:
   integrates), for every count of dry nodes at the new and the old level
 * lake at rest (flat water surface, zero velocity) -> zero residual
 * global mass and x-momentum balances (closed forms from partition of unity)
+* boundary segments (tailwater depth, explicit flux, weak velocity) vs
+  Gauss-Legendre line quadrature of their flux and pressure integrals
+* Manning friction: roughness height from the Manning-Strickler relation and the
+  drag coefficient from numerically integrating the log-law velocity profile
 * geometry closed forms (area, P1 gradients, outward unit normals)
 * sparsity pattern vs brute force, and row lengths = 3 x (node degree + 1) on
   the reference's ten test-grid geometries (src/test/run_tests.c:42-47)
@@ -28,7 +32,7 @@ import scipy.sparse.linalg as spla
 import adh_kraken_amd as A
 import oracle as O
 from helpers import perturbed_state, rel_err, small_flume
-from sw2_quadrature import body_residual, wet_body_residual
+from sw2_quadrature import GL_W, GL_X, body_residual, wet_body_residual
 
 G = 9.8
 
@@ -201,6 +205,113 @@ def test_wet_region_statement_reduces_to_full_element():
                 so[3 * t + 2], 0.7, G, 0.002)
         a, b = wet_body_residual(*args), body_residual(*args)
         assert np.abs(a - b).max() <= 1e-13 * max(1.0, np.abs(b).max())
+
+
+def seg_line_quadrature(kind, val, xa, ya, xb, yb, h, u, v, dt, g):
+    """6-vector [node*3 + (c, x, y)] of a boundary segment's residual by 3-point Gauss-Legendre.
+
+    Implicit flow (fe_sw2_1d_implicit_flow): continuity dt int phi h (v.n); momentum
+    dt int phi (u_r, v_r) (v.n) h with (u_r, v_r) the nodal velocity where v.n >= eps at that
+    node, else 0, interpolated linearly.  Pressure: dt g/2 n int phi (h_bc^2 - h^2)."""
+    eps = np.finfo(float).eps
+    L = np.hypot(xb - xa, yb - ya)
+    nx, ny = (yb - ya) / L, -(xb - xa) / L
+    r = np.zeros(6)
+
+    def implicit(hn, un, vn):
+        vdn = un * nx + vn * ny
+        ur = np.where(vdn < eps, 0.0, un)
+        vr = np.where(vdn < eps, 0.0, vn)
+        for s_, w in zip(GL_X, GL_W):
+            ph = np.array([1 - s_, s_])
+            hq, uq, vq, urq, vrq = ph @ hn, ph @ un, ph @ vn, ph @ ur, ph @ vr
+            qn = uq * nx + vq * ny
+            for i in range(2):
+                r[3 * i] += dt * w * L * ph[i] * hq * qn
+                r[3 * i + 1] += dt * w * L * ph[i] * urq * qn * hq
+                r[3 * i + 2] += dt * w * L * ph[i] * vrq * qn * hq
+
+    if kind == O.BC_DEPTH:
+        hb = np.array([val[0], val[0]])
+        if min(hb.min(), h.min()) < eps:
+            return r
+        implicit(hb, u, v)
+        for s_, w in zip(GL_X, GL_W):
+            ph = np.array([1 - s_, s_])
+            dp = 0.5 * g * dt * w * L * ((ph @ hb) ** 2 - (ph @ h) ** 2)
+            for i in range(2):
+                r[3 * i + 1] += ph[i] * dp * nx
+                r[3 * i + 2] += ph[i] * dp * ny
+    elif kind == O.BC_FLUX:
+        q = val[0]
+        hav = max(h.mean(), eps)
+        for i in range(2):
+            r[3 * i] += dt * q * L / 2
+            if q < 0:
+                r[3 * i + 1] += dt * q * L / 2 * (q * nx / hav)
+                r[3 * i + 2] += dt * q * L / 2 * (q * ny / hav)
+        if q >= 0:
+            for s_, w in zip(GL_X, GL_W):
+                ph = np.array([1 - s_, s_])
+                for i in range(2):
+                    r[3 * i + 1] += dt * q * w * L * ph[i] * (ph @ u)
+                    r[3 * i + 2] += dt * q * w * L * ph[i] * (ph @ v)
+    else:
+        implicit(h, np.array([val[0], val[0]]), np.array([val[1], val[1]]))
+    return r
+
+
+def test_seg_residual_vs_line_quadrature():
+    import dataclasses
+    spec, (mesh, sol0) = small_flume(seed=9)
+    s, so = perturbed_state(sol0, seed=2, amp_v=0.4)
+    S = mesh.nelems1d
+    rng = np.random.default_rng(3)
+    variants = [mesh.seg_kind.copy(), np.full(S, O.BC_VEL, np.int32), np.full(S, O.BC_FLUX, np.int32)]
+    vals = [mesh.seg_val.copy(), rng.uniform(-0.4, 0.4, (S, 2)), rng.uniform(-0.8, 0.8, (S, 2))]
+    # a dry tailwater segment (no implicit flux, no pressure) and an inflow/outflow mix
+    vals[0][-1, 0] = 0.0
+    for kind, val in zip(variants, vals):
+        m = dataclasses.replace(mesh, seg_kind=kind, seg_val=val)
+        orc = O.Oracle(m, dt=0.8)
+        orc.set_state(s, so, so)
+        for e in range(S):
+            a, b = m.seg[e]
+            ref = seg_line_quadrature(kind[e], val[e], m.x[a], m.y[a], m.x[b], m.y[b], s[3 * m.seg[e]],
+                                      s[3 * m.seg[e] + 1], s[3 * m.seg[e] + 2], 0.8, G)
+            got = orc.seg_resid(e, s)
+            assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()), (kind[e], e)
+
+
+def log_law_drag(depth, ks, kappa=0.4):
+    """c_f = 2 (u*/U)^2 for the rough-wall log law u(z) = (u*/kappa) ln(z/z0), z0 = ks/29.7, averaged
+    over z0 <= z <= z0 + depth by numerical quadrature (depth >= z0, as the clamp of the law keeps it)."""
+    from scipy.integrate import quad
+    z0 = ks / 29.7
+    depth = max(depth, z0)
+    mean, _ = quad(lambda z: np.log(z / z0) / kappa, z0, z0 + depth, epsabs=0, epsrel=1e-13, limit=200)
+    return 2.0 * (depth / mean) ** 2
+
+
+def test_manning_drag_vs_log_law_profile():
+    """Manning mode at an all-wet element equals constant-drag mode with c_f from the log law at the
+    element's mean depth and k_s from Manning-Strickler, n = k_s^(1/6) / (8.25 sqrt(g))."""
+    spec, (mesh, sol0) = small_flume(seed=4)
+    s, so = perturbed_state(sol0, seed=6, amp_v=0.4)
+    for n_mann, depth_scale in ((0.03, 1.0), (0.06, 0.05), (0.015, 3.0)):
+        m = A.Mesh(mesh.x, mesh.y, mesh.z, mesh.tri, mesh.seg, mesh.seg_kind, mesh.seg_val,
+                   np.full(mesh.nelems2d, n_mann))
+        st = s.copy()
+        st[0::3] *= depth_scale
+        ks = (8.25 * np.sqrt(G) * n_mann) ** 6
+        om = O.Oracle(m, dt=0.9, friction="manning")
+        om.set_state(st, so, so)
+        for e in range(0, mesh.nelems2d, 9):
+            hbar = st[3 * mesh.tri[e]].mean()
+            oc = O.Oracle(m, dt=0.9, friction="const", cf_const=log_law_drag(hbar, ks))
+            oc.set_state(st, so, so)
+            a, b = om.body_resid(e, st), oc.body_resid(e, st)
+            assert np.abs(a - b).max() <= 1e-12 * max(1.0, np.abs(b).max())
 
 
 def test_lake_at_rest():
