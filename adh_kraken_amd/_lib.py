@@ -31,7 +31,7 @@ class Solver(C.Structure):
 class NewtonStats(C.Structure):
     _fields_ = [("newton_its", C.c_int), ("linesearch_cuts_total", C.c_int), ("lin_its_total", C.c_int),
                 ("lin_its_max", C.c_int), ("resid_max_norm", C.c_double), ("inc_max_norm", C.c_double),
-                ("converged", C.c_int)]
+                ("converged", C.c_int), ("warm_accepted", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -54,6 +54,8 @@ class FlumeSpec(C.Structure):
 P = C.c_void_p
 dp = C.POINTER(C.c_double)
 ip = C.POINTER(C.c_int)
+fp = C.POINTER(C.c_float)
+MLP_NIN, MLP_HID, MLP_NOUT, MLP_NNORM = 12, 64, 3, 36
 
 # (name, restype, argtypes) of every symbol include/adhk.h declares
 SIGNATURES = [
@@ -99,6 +101,20 @@ SIGNATURES = [
     ("adhk_flume_counts", C.c_int, [C.POINTER(FlumeSpec), ip, ip, ip]),
     ("adhk_flume_build", C.c_int, [C.POINTER(FlumeSpec), dp, dp, dp, ip, dp, ip, ip, dp]),
     ("adhk_flume_initial_state", C.c_int, [C.POINTER(FlumeSpec), C.c_int, dp, dp]),
+    ("adhk_mlp_param_count", C.c_int, []),
+    ("adhk_mlp_create", P, [C.c_ulonglong]),
+    ("adhk_mlp_destroy", None, [P]),
+    ("adhk_mlp_get_params", C.c_int, [P, fp, fp]),
+    ("adhk_mlp_set_params", C.c_int, [P, fp, fp]),
+    ("adhk_mlp_data_reserve", C.c_int, [P, C.c_longlong]),
+    ("adhk_mlp_collect", C.c_int, [P, P]),
+    ("adhk_mlp_data_size", C.c_longlong, [P]),
+    ("adhk_mlp_fit_normalization", C.c_int, [P]),
+    ("adhk_mlp_train", C.c_int, [P, C.c_int, C.c_int, C.c_double, C.c_ulonglong, dp]),
+    ("adhk_mlp_forward", C.c_int, [P, C.c_int, fp, fp]),
+    ("adhk_mlp_loss_grad", C.c_int, [P, C.c_int, fp, fp, fp, dp]),
+    ("adhk_sw2_features", C.c_int, [P, fp]),
+    ("adhk_sw2_set_warm_start", C.c_int, [P, P]),
 ]
 
 _lib = None
@@ -140,6 +156,13 @@ def dptr(a):
         return None
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "float64 C-contiguous array expected"
     return a.ctypes.data_as(dp)
+
+
+def fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"], "float32 C-contiguous array expected"
+    return a.ctypes.data_as(fp)
 
 
 def iptr(a):

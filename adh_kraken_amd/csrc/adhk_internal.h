@@ -62,7 +62,9 @@ struct adhk_sw2 {
     /* mesh */
     double *x, *y, *z;
     int *tri;
-    double *djac, *grad, *ks;
+    double *djac, *grad, *ks, *mann;
+    float *nstat; /* per-node bed slope x, y and Manning n [3][N], built on first use by the learned start */
+    adhk_mlp *warm; /* learned warm start, or NULL */
     int *seg, *seg_kind;
     double *seg_val, *sdjac, *snrml;
     /* state */
@@ -122,6 +124,12 @@ int adhk_launch_increment(adhk_ctx *c);
 int adhk_launch_linesearch(adhk_ctx *c);
 int adhk_launch_init_system(adhk_ctx *c);
 int adhk_launch_rotate(adhk_ctx *c);
+int adhk_launch_start_from_old(adhk_ctx *c);
+int adhk_launch_copy(adhk_ctx *c, double *dst, const double *src);
+int adhk_launch_dirichlet_sol(adhk_ctx *c);
+
+/* ---- learned warm start (mlp.hip) ---- */
+int adhk_mlp_predict_into(adhk_mlp *m, adhk_ctx *c);
 
 /* error reporting and kernel timers (newton.c) */
 void adhk_set_error(const char *fmt, ...);

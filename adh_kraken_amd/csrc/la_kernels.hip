@@ -823,6 +823,27 @@ extern "C" int adhk_launch_init_system(adhk_ctx *c)
     CHECK_LAUNCH();
     return 0;
 }
+extern "C" int adhk_launch_copy(adhk_ctx *c, double *dst, const double *src)
+{
+    hipLaunchKernelGGL(k_copy, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs, dst, src);
+    CHECK_LAUNCH();
+    return 0;
+}
+/* sol <- sol_old with the Dirichlet values (the start of initialize_system, without the wet/dry flags) */
+extern "C" int adhk_launch_start_from_old(adhk_ctx *c)
+{
+    hipLaunchKernelGGL(k_copy, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs, c->sol,
+                       c->sol_old);
+    CHECK_LAUNCH();
+    return adhk_launch_dirichlet_sol(c);
+}
+extern "C" int adhk_launch_dirichlet_sol(adhk_ctx *c)
+{
+    hipLaunchKernelGGL(k_apply_dirichlet_sol, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs,
+                       c->sol, c->bc_mask, c->dirichlet);
+    CHECK_LAUNCH();
+    return 0;
+}
 extern "C" int adhk_launch_rotate(adhk_ctx *c)
 {
     hipLaunchKernelGGL(k_rotate, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs, c->sol,

@@ -66,7 +66,7 @@ static double manning_to_rheight(double n, double kn, double g)
 static void ctx_free(adhk_ctx *c)
 {
     if (!c) return;
-    void *ptrs[] = {c->x, c->y, c->z, c->tri, c->djac, c->grad, c->ks, c->seg, c->seg_kind, c->seg_val, c->sdjac,
+    void *ptrs[] = {c->x, c->y, c->z, c->tri, c->djac, c->grad, c->ks, c->mann, c->nstat, c->seg, c->seg_kind, c->seg_val, c->sdjac,
                     c->snrml, c->sol, c->sol_old, c->sol_older, c->dirichlet, c->dsol, c->residual, c->scale,
                     c->bc_mask, c->wd_flag, c->wd_tmp, c->wdq, c->brow, c->bcol, c->diag_blk, c->cptr, c->contrib, c->bvals,
                     c->minv, c->bvals32, c->minv32, c->erhs, c->emat, c->r, c->pv, c->Ap, c->s, c->As, c->q, c->x0,
@@ -176,6 +176,7 @@ adhk_sw2 *adhk_sw2_create(int nnodes, const double *x, const double *y, const do
         DALLOC(c->djac, E, double);
         DALLOC(c->grad, 6L * E, double);
         DALLOC(c->ks, E, double);
+        DALLOC(c->mann, E, double);
         DALLOC(c->seg, 2L * S, int);
         DALLOC(c->seg_kind, S, int);
         DALLOC(c->seg_val, 2L * S, double);
@@ -188,6 +189,10 @@ adhk_sw2 *adhk_sw2_create(int nnodes, const double *x, const double *y, const do
         HC(hipMemcpy(c->djac, h_djac, (size_t)E * sizeof(double), hipMemcpyHostToDevice));
         HC(hipMemcpy(c->grad, h_grad, (size_t)E * 6 * sizeof(double), hipMemcpyHostToDevice));
         HC(hipMemcpy(c->ks, h_ks, (size_t)E * sizeof(double), hipMemcpyHostToDevice));
+        if (mann)
+            HC(hipMemcpy(c->mann, mann, (size_t)E * sizeof(double), hipMemcpyHostToDevice));
+        else
+            HC(hipMemset(c->mann, 0, (size_t)E * sizeof(double)));
         if (S > 0) {
             HC(hipMemcpy(c->seg, seg, (size_t)S * 2 * sizeof(int), hipMemcpyHostToDevice));
             HC(hipMemcpy(c->seg_kind, seg_kind, (size_t)S * sizeof(int), hipMemcpyHostToDevice));
@@ -692,6 +697,25 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
     LC(adhk_launch_init_system(c));
     LC(residual_dev(c));
     LC(norms(c, &hs, &nan));
+    if (c->warm) {
+        /* learned start: keep it only when its residual is smaller than the previous level's; the
+         * previous level's residual is kept (in the idle Krylov vector r) for the fallback */
+        adhk_scalars hw;
+        int nanw = 0;
+        LC(adhk_launch_copy(c, c->r, c->residual));
+        LC(adhk_mlp_predict_into(c->warm, c));
+        LC(adhk_launch_dirichlet_sol(c));
+        LC(residual_dev(c));
+        LC(norms(c, &hw, &nanw));
+        if (!nanw && !nan && hw.rmax < hs.rmax) {
+            hs = hw;
+            loc.warm_accepted = 1;
+        } else {
+            HC(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
+            LC(adhk_launch_start_from_old(c));
+            LC(adhk_launch_copy(c, c->residual, c->r));
+        }
+    }
     rmax = hs.rmax;
     rl2 = sqrt(hs.rl2sum) / c->ndofs;
     incmax = hs.incmax;
@@ -763,6 +787,7 @@ int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_st
         if (st.lin_its_max > tot.lin_its_max) tot.lin_its_max = st.lin_its_max;
         tot.resid_max_norm = st.resid_max_norm;
         tot.inc_max_norm = st.inc_max_norm;
+        tot.warm_accepted += st.warm_accepted;
     }
     tot.converged = ok;
     if (acc) *acc = tot;

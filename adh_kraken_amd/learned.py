@@ -1,0 +1,129 @@
+"""Learned warm start of the Newton iterate (host mirror of the adhk_mlp_* entry points).
+
+A per-node MLP (12 -> 64 -> 64 -> 3, GELU, BF16 weights, FP32 arithmetic) predicts
+the change of (h, u, v) over one time step.  Data collection, training (forward,
+backward, Adam) and inference all run in libadhk.so's gfx950 kernels; this module
+only moves handles and host arrays.  With a model attached to a super model,
+fe_newton starts from the predicted iterate when its residual is smaller than the
+previous level's, and otherwise from the previous level, as the reference does
+(smodel_super_prep_sol.c:2-8); the Newton iteration and its 1e-8 test are unchanged.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .sw2 import SModelSuper, time_loop
+
+NPARAM = 5187
+NNORM = 36
+
+
+class WarmStartMLP:
+    def __init__(self, seed=1):
+        self.lib = L.load()
+        h = self.lib.adhk_mlp_create(seed)
+        if not h:
+            raise L.AdhkError("adhk_mlp_create failed: " + L.last_error())
+        self.h = C.c_void_p(h)
+        assert self.lib.adhk_mlp_param_count() == NPARAM
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.adhk_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- parameters ----
+    def get_params(self):
+        p, n = np.zeros(NPARAM, np.float32), np.zeros(NNORM, np.float32)
+        L.check(self.lib.adhk_mlp_get_params(self.h, L.fptr(p), L.fptr(n)), "adhk_mlp_get_params")
+        return p, n
+
+    def set_params(self, params=None, norm=None):
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        n = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        L.check(self.lib.adhk_mlp_set_params(self.h, L.fptr(p), L.fptr(n)), "adhk_mlp_set_params")
+
+    # ---- data and training ----
+    def reserve(self, capacity):
+        L.check(self.lib.adhk_mlp_data_reserve(self.h, int(capacity)), "adhk_mlp_data_reserve")
+
+    def collect(self, sm: SModelSuper):
+        L.check(self.lib.adhk_mlp_collect(self.h, sm.h), "adhk_mlp_collect")
+
+    @property
+    def size(self):
+        return int(self.lib.adhk_mlp_data_size(self.h))
+
+    def fit_normalization(self):
+        L.check(self.lib.adhk_mlp_fit_normalization(self.h), "adhk_mlp_fit_normalization")
+
+    def train(self, steps, batch=8192, lr=1e-3, seed=7):
+        hist = np.zeros(max(steps, 1))
+        L.check(self.lib.adhk_mlp_train(self.h, int(steps), int(batch), float(lr), int(seed), L.dptr(hist)),
+                "adhk_mlp_train")
+        return hist[:steps]
+
+    # ---- checking surface ----
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((len(x), 3), np.float32)
+        L.check(self.lib.adhk_mlp_forward(self.h, len(x), L.fptr(x), L.fptr(y)), "adhk_mlp_forward")
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(NPARAM, np.float32)
+        loss = C.c_double()
+        L.check(self.lib.adhk_mlp_loss_grad(self.h, len(x), L.fptr(x), L.fptr(t), L.fptr(g), C.byref(loss)),
+                "adhk_mlp_loss_grad")
+        return loss.value, g
+
+
+def features(sm: SModelSuper):
+    """(nnodes, 12) float32 features of the super model's (sol_old, sol_older)."""
+    f = np.zeros((sm.mesh.nnodes, 12), np.float32)
+    L.check(sm.lib.adhk_sw2_features(sm.h, L.fptr(f)), "adhk_sw2_features")
+    return f
+
+
+def set_warm_start(sm: SModelSuper, mlp):
+    """Attach a trained model as the Newton warm start of sm (None detaches)."""
+    L.check(sm.lib.adhk_sw2_set_warm_start(sm.h, mlp.h if mlp is not None else None), "adhk_sw2_set_warm_start")
+    sm._warm = mlp  # keep the model alive while attached
+
+
+def collect_trajectory(sm: SModelSuper, mlp: WarmStartMLP, nsteps, precision=L.PREC_FP64):
+    """Advance sm by nsteps plain time steps, adding each step's (features, change) of every node."""
+    its = 0
+    for _ in range(nsteps):
+        n, st = time_loop(sm, 1, precision=precision)
+        if n != 1:
+            raise L.AdhkError(f"time step failed while collecting: {st}")
+        its += st["newton_its"]
+        mlp.collect(sm)
+    return its
+
+
+def train_warm_start(cases, nsteps, steps=2000, batch=8192, lr=2e-3, seed=1, precision=L.PREC_FP64, dt=1.0):
+    """Train a model on solver trajectories: cases = [(mesh, sol0), ...], nsteps steps each.
+
+    Returns (model, loss history, data-generation Newton iterations)."""
+    mlp = WarmStartMLP(seed=seed)
+    mlp.reserve(sum(m.nnodes for m, _ in cases) * nsteps)
+    gen_its = 0
+    for mesh, sol0 in cases:
+        sm = SModelSuper(mesh, dt=dt)
+        sm.set_initial(sol0)
+        gen_its += collect_trajectory(sm, mlp, nsteps, precision)
+        sm.close()
+    mlp.fit_normalization()
+    hist = mlp.train(steps, batch=batch, lr=lr, seed=seed + 100)
+    return mlp, hist, gen_its

@@ -10,6 +10,9 @@ initial condition and exactly K steps are timed between two device syncs.
 depth/velocity error against the FP64 result of the same K steps are reported
 beside it.
 
+A third variant adds the learned warm start to the mixed-precision solver: a per-node MLP trained
+(untimed) on the GPU on trajectories of two other flumes of the same family (bump seeds 2 and 3).
+
 The roofline is for the dominant kernel of the plain run: algorithmic bytes of
 one launch divided by its average launch time, measured with hipEvent pairs on
 the library's stream during the timed region.  The CPU baseline is the oracle
@@ -66,10 +69,13 @@ def cpu_baseline(mesh, sol0, dt, nsteps_sample=5):
                       f"{st['lin_its_total']} BiCGStab its)"}
 
 
-def run_variant(A, L, mesh, sol0, dt, precision, steps, warmup, check_every):
+def run_variant(A, L, mesh, sol0, dt, precision, steps, warmup, check_every, warm=None):
     lib = L.load()
     sm = A.SModelSuper(mesh, dt=dt)
     sm.solver.lin_check_every = check_every
+    if warm is not None:
+        from adh_kraken_amd import learned as LR
+        LR.set_warm_start(sm, warm)
     sm.set_initial(sol0)
     if warmup:
         A.time_loop(sm, warmup, precision=precision)
@@ -100,6 +106,8 @@ def main():
     ap.add_argument("--config", choices=["c2", "c1"], default="c2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--train-steps", type=int, default=20, help="time steps per training trajectory")
     ap.add_argument("--check-every", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -172,6 +180,28 @@ def main():
                         "speed_rel_err_vs_fp64": float(np.abs(umx - u64).max() / max(np.abs(u64).max(), 1e-300)),
                         "newton_its": mx["stats"]["newton_its"], "bicgstab_its": mx["stats"]["lin_its_total"],
                         "kernel_ms": {k: round(v["ms_total"], 3) for k, v in mx["kt"].items()}}
+    if not args.no_learned:
+        from adh_kraken_amd import learned as LR
+        t0 = time.perf_counter()
+        # trajectories of two other flumes of the same family (bed-bump seeds 2 and 3), mixed precision
+        cases = [A.flume_mesh(A.FlumeSpec(**{**spec.__dict__, "seed": sd})) for sd in (2, 3)]
+        mlp, hist, gen_its = LR.train_warm_start(cases, nsteps=args.train_steps, steps=2000, batch=16384, lr=2e-3,
+                                                 seed=5, precision=L.PREC_MIXED, dt=dt)
+        t_train = time.perf_counter() - t0
+        ml = run_variant(A, L, mesh, sol0, dt, L.PREC_MIXED, args.steps, args.warmup, args.check_every, warm=mlp)
+        h64, hml = plain["sol"][0::3], ml["sol"][0::3]
+        vml = args.steps / ml["elapsed"]
+        out["mixed_learned"] = {
+            "value": round(vml, 4), "unit": "timesteps/s", "speedup_vs_fp64": round(vml / value, 3),
+            "variant": "mixed precision + learned per-node MLP warm start (12-64-64-3 GELU, BF16 weights); the "
+                       "start with the smaller max-norm residual of {learned, previous level} is kept",
+            "warm_accepted_steps": ml["stats"]["warm_accepted"], "newton_its": ml["stats"]["newton_its"],
+            "bicgstab_its": ml["stats"]["lin_its_total"],
+            "depth_rel_err_vs_fp64": float(np.abs(hml - h64).max() / np.abs(h64).max()),
+            "training": {"trajectories": "2 flumes x %d steps (bump seeds 2, 3)" % args.train_steps,
+                         "samples": mlp.size, "adam_steps": 2000, "batch": 16384,
+                         "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-10:].mean()),
+                         "seconds_incl_data_generation": round(t_train, 2)}}
     if not args.no_cpu:
         cb = cpu_baseline(mesh, sol0, dt)
         cb["gpu_fp64_speedup"] = round(value / cb["value"], 1)

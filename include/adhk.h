@@ -73,7 +73,8 @@ typedef struct {
     int lin_its_max;
     double resid_max_norm;
     double inc_max_norm;
-    int converged; /* fe_newton return value */
+    int converged;     /* fe_newton return value */
+    int warm_accepted; /* Newton solves that started from the learned iterate (adhk_sw2_set_warm_start) */
 } adhk_newton_stats;
 
 /* per-stage device time of the last adhk_time_loop / adhk_fe_newton call, ms */
@@ -199,6 +200,43 @@ int adhk_flume_build(const adhk_flume_spec *f, double *x, double *y, double *z, 
 /* initial state with u = v = 0: ic_flat_wse = 1 -> h = (tailwater - slope*(xmax - xmin)) - z, a flat water
  * surface at the tailwater level of the bump-free bed; ic_flat_wse = 0 -> h = tailwater everywhere */
 int adhk_flume_initial_state(const adhk_flume_spec *f, int nnodes, const double *z, double *sol);
+
+/* ---- learned warm start of the Newton iterate.  No counterpart in the reference, whose Newton
+ *      solve starts from the previous time level (smodel_super_prep_sol.c:2-8, newton_tools.c:21).
+ *      A per-node MLP 12 -> 64 -> 64 -> 3 (GELU, BF16 weights, FP32 arithmetic) predicts the change
+ *      of (h, u, v) over one step from 12 features: h, hu, hv at t; their change since t-1; the 1-ring
+ *      neighbour means of h, hu, hv at t minus the node's own; the bed slope (x, y) and Manning n, both
+ *      area-weighted over the node's triangles.
+ *      It is trained on the GPU on (features, change) pairs collected from solver trajectories.
+ *      With a model attached, fe_newton evaluates the residual at the previous level and at the
+ *      predicted iterate and starts from the smaller (max norm); iteration and stop test are unchanged. */
+#define ADHK_MLP_NIN 12
+#define ADHK_MLP_HID 64
+#define ADHK_MLP_NOUT 3
+typedef struct adhk_mlp adhk_mlp;
+int adhk_mlp_param_count(void); /* 5187: W1[64][12] b1[64] W2[64][64] b2[64] W3[3][64] b3[3] */
+adhk_mlp *adhk_mlp_create(unsigned long long seed);
+void adhk_mlp_destroy(adhk_mlp *m);
+/* FP32 master parameters [param_count] and normalisation [36]: x mean[12], x 1/std[12], y mean[3],
+ * y std[3], y 1/std[3]; set_params resets the optimiser state */
+int adhk_mlp_get_params(adhk_mlp *m, float *params, float *norm);
+int adhk_mlp_set_params(adhk_mlp *m, const float *params, const float *norm);
+/* device dataset: reserve capacity (samples), append one sample per node of the context: features of
+ * (sol_old, sol_older) -> target sol - sol_old (call after a converged step) */
+int adhk_mlp_data_reserve(adhk_mlp *m, long long capacity);
+int adhk_mlp_collect(adhk_mlp *m, adhk_sw2 *c);
+long long adhk_mlp_data_size(const adhk_mlp *m);
+int adhk_mlp_fit_normalization(adhk_mlp *m);
+/* Adam on minibatches drawn from the dataset (seeded); loss_hist [steps] or NULL */
+int adhk_mlp_train(adhk_mlp *m, int steps, int batch, double lr, unsigned long long seed, double *loss_hist);
+/* checking surface: raw features [n][12] -> raw outputs [n][3]; loss and gradient [param_count] of the
+ * mean squared error on normalised outputs over n samples (n <= 65536) */
+int adhk_mlp_forward(adhk_mlp *m, int n, const float *x, float *y);
+int adhk_mlp_loss_grad(adhk_mlp *m, int n, const float *x, const float *t, float *grad, double *loss);
+/* features [nnodes][12] of the context's (sol_old, sol_older) */
+int adhk_sw2_features(adhk_sw2 *c, float *feat);
+/* attach (or with NULL detach) a model as the Newton warm start of this context */
+int adhk_sw2_set_warm_start(adhk_sw2 *c, adhk_mlp *m);
 
 #ifdef __cplusplus
 }
