@@ -14,6 +14,8 @@
  *              stop test), form p = r + beta (p - omega Ap) -- recomputed on the fly for the columns,
  *              so no extra pass -- and Ap = M^-1 A p with q.Ap -> alpha
  *   k_iter_s : s = r - alpha Ap, As = M^-1 A s with As.s, As.As, As.q -> omega, rho, beta
+ * M^-1 A is formed once per Newton iteration (k_scale_prec folds each row's 3x3 inverse into the
+ * row), so each SpMV pass reads one matrix and no preconditioner.
  * Scalars live in device memory; every kernel exits at once when the solver
  * has converged, so the host launches iterations in chunks and reads the
  * "done" flag between chunks.  Reductions are deterministic: per-workgroup
@@ -357,9 +359,10 @@ extern "C" int adhk_launch_dirichlet(adhk_ctx *c, int do_scale)
     return 0;
 }
 
-/* A <- S A S (scale_linear_system.c:104-110) and the 3x3 block-Jacobi inverse of the scaled
- * diagonal block (adjugate; |det| <= 1e-300 falls back to the inverted diagonal).  With T = float the
- * scaled matrix and inverse are also written in FP32 for the mixed-precision Krylov solve. */
+/* A <- S A S (scale_linear_system.c:104-110), the 3x3 block-Jacobi inverse M^-1 of the scaled
+ * diagonal block (adjugate; |det| <= 1e-300 falls back to the inverted diagonal), and the
+ * left-preconditioned matrix M^-1 A that the Krylov kernels multiply by.  With T = float, M^-1 A and
+ * M^-1 are also written in FP32 for the mixed-precision Krylov solve. */
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_scale_prec(int N, const int *__restrict__ brow,
                                                     const int *__restrict__ slice_ptr, const int *__restrict__ scol,
@@ -380,7 +383,6 @@ __global__ __launch_bounds__(TPB) void k_scale_prec(int N, const int *__restrict
                 const size_t ix = sell_val(cs, 3 * r + cc, lane);
                 double v = (k < deg) ? bv[ix] * (si[r] * sj[cc]) : 0.0;
                 if (k < deg) bv[ix] = v;
-                if (bv_t) bv_t[ix] = (T)v;
             }
     }
     int d = diag_slot[n];
@@ -410,6 +412,22 @@ __global__ __launch_bounds__(TPB) void k_scale_prec(int N, const int *__restrict
     for (int k = 0; k < 9; k++) {
         mo[k * 64] = m[k];
         if (minv_t) minv_t[(size_t)(n >> 6) * 9 * 64 + lane + k * 64] = (T)m[k];
+    }
+    /* fold the inverse into the row: the Krylov kernels apply M^-1 A as one SpMV.  FP64: in place;
+     * mixed: into the FP32 copy (the FP64 values stay the scaled matrix) */
+    for (int k = 0; k < w; k++) {
+        const int cs = c0 + k;
+        double b[9];
+        for (int e = 0; e < 9; e++) b[e] = (k < deg) ? bv[sell_val(cs, e, lane)] : 0.0;
+        for (int r = 0; r < 3; r++)
+            for (int cc = 0; cc < 3; cc++) {
+                const double f = m[3 * r] * b[cc] + m[3 * r + 1] * b[3 + cc] + m[3 * r + 2] * b[6 + cc];
+                const size_t ix = sell_val(cs, 3 * r + cc, lane);
+                if (bv_t)
+                    bv_t[ix] = (T)f;
+                else if (k < deg)
+                    bv[ix] = f;
+            }
     }
 }
 
@@ -453,19 +471,18 @@ __global__ __launch_bounds__(TPB) void k_kinit1(int N, const T *__restrict__ min
     }
 }
 
-/* r = b - M^-1 A x, q = r, rnorm, bnorm, rho = r.q  (:155-207) */
+/* r = M^-1 b - (M^-1 A) x, q = r, rnorm, bnorm, rho = r.q  (:155-207) */
 template <typename T>
-__global__ __launch_bounds__(TPB) void k_kinit2(int N, Sell S, const T *__restrict__ bv, const T *__restrict__ minv,
-                                                const T *__restrict__ x, const T *__restrict__ bp, T *__restrict__ r,
-                                                T *__restrict__ q, double *partials)
+__global__ __launch_bounds__(TPB) void k_kinit2(int N, Sell S, const T *__restrict__ bv, const T *__restrict__ x,
+                                                const T *__restrict__ bp, T *__restrict__ r, T *__restrict__ q,
+                                                double *partials)
 {
     double vmax[2] = {0.0, 0.0}, vsum[1] = {0.0};
     const NodeRange nr = xcd_range(N);
     for (int n = nr.begin + threadIdx.x; n < nr.end; n += TPB) {
-        double y0, y1, y2, z0, z1, z2;
-        block_row<T>(n, S, bv, ColPlain<T>{x}, y0, y1, y2);
-        apply_minv<T>(minv, n, y0, y1, y2, z0, z1, z2);
-        double z[3] = {z0, z1, z2};
+        double y0, y1, y2;
+        block_row<T>(n, S, bv, ColPlain<T>{x}, y0, y1, y2); /* bv holds M^-1 A */
+        double z[3] = {y0, y1, y2};
         for (int k = 0; k < 3; k++) {
             int i = 3 * n + k;
             double bb = (double)bp[i];
@@ -506,8 +523,8 @@ __global__ __launch_bounds__(TPB) void k_kinit3(int n, const double *partials, a
  * r = s - omega As, ||r||_inf and the stop test (:294-307, :210) -- and form p' = (p - omega Ap) beta + r.
  * Then Ap' = M^-1 A p' and q.Ap' -> alpha = rho / (q.Ap')  (:215-246). */
 template <typename T, bool FIRST>
-__global__ __launch_bounds__(TPB) void k_iter_a(int N, Sell S, const T *__restrict__ bv, const T *__restrict__ minv,
-                                                T *__restrict__ x, T *__restrict__ r, const T *__restrict__ s,
+__global__ __launch_bounds__(TPB) void k_iter_a(int N, Sell S, const T *__restrict__ bv, T *__restrict__ x,
+                                                T *__restrict__ r, const T *__restrict__ s,
                                                 const T *__restrict__ As, const T *__restrict__ p_old,
                                                 const T *__restrict__ Ap_old, T *__restrict__ p_new,
                                                 T *__restrict__ Ap_new, const T *__restrict__ q, double *partials,
@@ -535,9 +552,7 @@ __global__ __launch_bounds__(TPB) void k_iter_a(int N, Sell S, const T *__restri
             }
             block_row<T>(n, S, bv, col, y0, y1, y2);
         }
-        double z0, z1, z2;
-        apply_minv<T>(minv, n, y0, y1, y2, z0, z1, z2);
-        T a0 = (T)z0, a1 = (T)z1, a2 = (T)z2;
+        T a0 = (T)y0, a1 = (T)y1, a2 = (T)y2; /* bv holds M^-1 A */
         Ap_new[3 * n] = a0;
         Ap_new[3 * n + 1] = a1;
         Ap_new[3 * n + 2] = a2;
@@ -568,10 +583,9 @@ template <bool FIRST> __global__ __launch_bounds__(TPB) void k_fin_a(int n, cons
 /* S: s = r - alpha Ap, As = M^-1 A s, omega = (As.s)/(As.As), rho' = -omega (As.q),
  * beta' = (rho' alpha)/(rho omega), iteration count + 1   (:247-291) */
 template <typename T>
-__global__ __launch_bounds__(TPB) void k_iter_s(int N, Sell S, const T *__restrict__ bv, const T *__restrict__ minv,
-                                                const T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ s,
-                                                T *__restrict__ As, const T *__restrict__ q, double *partials,
-                                                const adhk_scalars *sc)
+__global__ __launch_bounds__(TPB) void k_iter_s(int N, Sell S, const T *__restrict__ bv, const T *__restrict__ r,
+                                                const T *__restrict__ Ap, T *__restrict__ s, T *__restrict__ As,
+                                                const T *__restrict__ q, double *partials, const adhk_scalars *sc)
 {
     if (sc->done) return;
     const double alpha = sc->alpha;
@@ -579,10 +593,9 @@ __global__ __launch_bounds__(TPB) void k_iter_s(int N, Sell S, const T *__restri
     double v[3] = {0.0, 0.0, 0.0};
     const NodeRange nr = xcd_range(N);
     for (int n = nr.begin + threadIdx.x; n < nr.end; n += TPB) {
-        double y0, y1, y2, z0, z1, z2;
-        block_row<T>(n, S, bv, col, y0, y1, y2);
-        apply_minv<T>(minv, n, y0, y1, y2, z0, z1, z2);
-        double z[3] = {z0, z1, z2};
+        double y0, y1, y2;
+        block_row<T>(n, S, bv, col, y0, y1, y2); /* bv holds M^-1 A */
+        double z[3] = {y0, y1, y2};
         for (int k = 0; k < 3; k++) {
             int i = 3 * n + k;
             T si = col.one(i);
@@ -678,8 +691,8 @@ template <typename T> static int krylov_init(adhk_ctx *c, KV<T> k, double tol, i
                        k.s);
     CHECK_LAUNCH();
     const int g = grid_xcd(c->N);
-    hipLaunchKernelGGL(k_kinit2<T>, dim3(g), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.mi, k.x, k.s, k.r,
-                       k.q, c->partials);
+    hipLaunchKernelGGL(k_kinit2<T>, dim3(g), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.x, k.s, k.r, k.q,
+                       c->partials);
     CHECK_LAUNCH();
     hipLaunchKernelGGL(k_kinit3, dim3(1), dim3(TPB), 0, c->stream, g, c->partials, c->sc, tol, max_it);
     CHECK_LAUNCH();
@@ -692,8 +705,8 @@ template <typename T, bool FIRST> static int launch_a(adhk_ctx *c, KV<T> k, int 
     int nw = FIRST ? 1 : 1 - cur;
     const int g = grid_xcd(c->N);
     adhk_kt_begin(c, ADHK_KT_SPMV_P);
-    hipLaunchKernelGGL((k_iter_a<T, FIRST>), dim3(g), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.mi, k.x, k.r,
-                       k.s, k.As, k.p[cur], k.Ap[cur], k.p[nw], k.Ap[nw], k.q, c->partials, c->sc);
+    hipLaunchKernelGGL((k_iter_a<T, FIRST>), dim3(g), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.x, k.r, k.s,
+                       k.As, k.p[cur], k.Ap[cur], k.p[nw], k.Ap[nw], k.q, c->partials, c->sc);
     adhk_kt_end(c);
     CHECK_LAUNCH();
     hipLaunchKernelGGL(k_fin_a<FIRST>, dim3(1), dim3(TPB), 0, c->stream, g, c->partials, c->sc);
@@ -705,8 +718,8 @@ template <typename T> static int launch_s(adhk_ctx *c, KV<T> k, int cur)
 {
     const int g = grid_xcd(c->N);
     adhk_kt_begin(c, ADHK_KT_SPMV_S);
-    hipLaunchKernelGGL(k_iter_s<T>, dim3(g), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.mi, k.r, k.Ap[cur],
-                       k.s, k.As, k.q, c->partials, c->sc);
+    hipLaunchKernelGGL(k_iter_s<T>, dim3(g), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.r, k.Ap[cur], k.s,
+                       k.As, k.q, c->partials, c->sc);
     adhk_kt_end(c);
     CHECK_LAUNCH();
     hipLaunchKernelGGL(k_fin_s, dim3(1), dim3(TPB), 0, c->stream, g, c->partials, c->sc);
