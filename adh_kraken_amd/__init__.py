@@ -6,4 +6,4 @@ this package is the Python host-side mirror of the reference interface (``sw2``)
 from . import _lib  # noqa: F401
 from .sw2 import (FLUME_C1, FLUME_C2, FlumeSpec, Mesh, SModelSuper, apply_Dirichlet_BC,  # noqa: F401
                   assemble_jacobian, assemble_residual, check_diag, elem_jacobians, elem_residuals, fe_newton,
-                  fe_sw2_init, flume_mesh, get_system, sparsity, time_loop)
+                  fe_sw2_init, flume_ensemble, flume_mesh, get_system, sparsity, time_loop)

@@ -62,6 +62,10 @@ SIGNATURES = [
     ("adhk_last_error", C.c_char_p, []),
     ("adhk_solver_defaults", None, [C.POINTER(Solver)]),
     ("adhk_sw2_create", P, [C.c_int, dp, dp, dp, C.c_int, ip, dp, C.c_int, ip, ip, dp, C.POINTER(Params)]),
+    ("adhk_sw2_create_batch", P, [C.c_int, ip, ip, ip, C.c_int, dp, dp, dp, C.c_int, ip, dp, C.c_int, ip, ip, dp,
+                                  C.POINTER(Params)]),
+    ("adhk_sw2_instances", C.c_int, [P]),
+    ("adhk_sw2_instance_stats", C.c_int, [P, C.POINTER(NewtonStats)]),
     ("adhk_sw2_destroy", None, [P]),
     ("adhk_sw2_sizes", C.c_int, [P, ip, ip, ip]),
     ("adhk_sw2_layout", C.c_int, [P, ip, ip]),

@@ -20,6 +20,8 @@
  *                         block in element order (2-D first, then 1-D)
  *   elemental buffers     erhs [E*9 + S*6], emat [E*81 + S*36] stored [elem][col][row]
  *   wet/dry queue         wdq [E+1]: count, then the partially dry elements of the last launch
+ *   instances             node_inst [N], elem_inst [E], seg_inst [S]; 256-node blocks blk_node/blk_inst;
+ *                         per-instance scalars sc [ninst] and work mask [ninst]
  */
 #ifndef ADHK_INTERNAL_H
 #define ADHK_INTERNAL_H
@@ -88,7 +90,16 @@ struct adhk_sw2 {
     /* reductions */
     double *partials;
     unsigned int *counter;
-    adhk_scalars *sc;
+    adhk_scalars *sc; /* [ninst]: BiCGStab scalars and Newton norms of each instance */
+    /* instances: an ensemble of independent meshes solved as one block-diagonal system (1 = one mesh) */
+    int ninst;
+    int *h_inst_node, *h_inst_elem, *h_inst_seg, *h_inst_blk; /* host [ninst + 1] offsets */
+    int *node_inst, *elem_inst, *seg_inst;                     /* device: instance of each node / element */
+    /* blocks of up to 256 consecutive nodes of one instance: the unit of every reduction */
+    int nblk;
+    int *blk_node, *blk_inst, *inst_blk; /* device [nblk + 1], [nblk], [ninst + 1] */
+    int *work, *h_work;                  /* [ninst]: 1 = the instance takes part in the next launches */
+    adhk_newton_stats *inst_stats;       /* host [ninst]: per-instance statistics of the last solve */
     int *flags; /* [0] = unqueued partially dry element (internal guard), [1] = NaN seen */
     /* timing */
     adhk_timing timing;
@@ -126,6 +137,8 @@ int adhk_launch_init_system(adhk_ctx *c);
 int adhk_launch_rotate(adhk_ctx *c);
 int adhk_launch_start_from_old(adhk_ctx *c);
 int adhk_launch_copy(adhk_ctx *c, double *dst, const double *src);
+int adhk_launch_copy_masked(adhk_ctx *c, double *dst, const double *src);
+int adhk_launch_start_from_old_masked(adhk_ctx *c);
 int adhk_launch_dirichlet_sol(adhk_ctx *c);
 
 /* ---- learned warm start (mlp.hip) ---- */

@@ -49,31 +49,51 @@ static inline int grid_for(long n)
     return (int)g;
 }
 
-/* XCD-aware node ranges for the matrix kernels.  Workgroups are dispatched round-robin over the 8
- * XCDs (workgroup w runs on XCD w % 8), and each XCD has its own L2.  Logical block
- * (w % 8) * (G / 8) + w / 8 gives every XCD a contiguous eighth of the node range, so the neighbour
+/* Reductions and the SELL kernels work on blocks of up to 256 consecutive nodes that never straddle
+ * two instances of an ensemble (c->blk_node, c->blk_inst): every partial sum belongs to one block, so
+ * every scalar can be per instance, and a mesh solved alone or inside an ensemble sums in the same order.
+ *
+ * XCD-aware block ranges.  Workgroups are dispatched round-robin over the 8 XCDs (workgroup w runs on
+ * XCD w % 8), and each XCD has its own L2.  Logical workgroup (w % 8) * (G / 8) + w / 8 takes a
+ * contiguous run of blocks, so every XCD owns a contiguous eighth of the nodes and the neighbour
  * gathers of a slice (nodes +-npy away) hit in that XCD's L2 instead of being fetched by several. */
 #define NXCD 8
-static inline int grid_xcd(long nodes)
+static inline int grid_blk(int nblk)
 {
-    long g = (nodes + TPB - 1) / TPB;
-    g = ((g + NXCD - 1) / NXCD) * NXCD;
+    long g = ((long)nblk + NXCD - 1) / NXCD * NXCD;
     if (g > MAX_GRID) g = MAX_GRID;
     if (g < NXCD) g = NXCD;
     return (int)g;
 }
-struct NodeRange {
-    int begin, end;
-};
-__device__ __forceinline__ NodeRange xcd_range(int N)
+/* Block metadata and the scalars are passed as separate const __restrict__ kernel arguments: the
+ * compiler then knows no store of the kernel can change them and reads them through the scalar
+ * cache (one s_load per block instead of dependent vector loads in front of every workgroup). */
+#define BLK_ARGS const int *__restrict__ blk_node, const int *__restrict__ blk_inst, int nblk, int n_single
+#define BLK_PASS(c) (c)->blk_node, (c)->blk_inst, (c)->nblk, ((c)->ninst == 1 ? (c)->N : 0)
+
+/* bounds and instance of block j; one mesh alone (n_single = its node count): block j is nodes
+ * [256 j, 256 j + 256) of instance 0, with no loads */
+__device__ __forceinline__ void blk_get(const int *__restrict__ blk_node, const int *__restrict__ blk_inst,
+                                        int n_single, int j, int &nb0, int &nb1, int &inst)
+{
+    if (n_single) {
+        nb0 = j << 8;
+        nb1 = min(nb0 + 256, n_single);
+        inst = 0;
+    } else {
+        nb0 = blk_node[j];
+        nb1 = blk_node[j + 1];
+        inst = blk_inst[j];
+        asm volatile("" ::"s"(nb0), "s"(nb1), "s"(inst)); /* keep the three loads ahead of the branch */
+    }
+}
+__device__ __forceinline__ void blk_range(int nblk, int &j0, int &j1)
 {
     const int G = gridDim.x, w = blockIdx.x;
-    const int b = (w % NXCD) * (G / NXCD) + w / NXCD;
-    const int per = ((N + G - 1) / G + TPB - 1) / TPB * TPB; /* nodes per logical block, multiple of 256 */
-    NodeRange r;
-    r.begin = b * per;
-    r.end = min(N, (b + 1) * per);
-    return r;
+    const int lb = (w % NXCD) * (G / NXCD) + w / NXCD;
+    const int pb = (nblk + G - 1) / G;
+    j0 = lb * pb;
+    j1 = min(nblk, j0 + pb);
 }
 
 /* ---- deterministic block reductions ---- */
@@ -100,36 +120,42 @@ template <int K, bool MAX> __device__ __forceinline__ void block_reduce(double (
             v[k] = a;
         }
     }
+    __syncthreads(); /* sh is free again for the next reduction */
 }
 
-/* Cross-workgroup reductions are deterministic and fence-free: thread 0 of every workgroup stores
- * its K block results at partials[k * MAX_GRID + blockIdx.x]; a one-workgroup finalize kernel that
- * runs next on the stream sums them in index order (the kernel boundary makes them visible). */
-template <int K, bool MAX> __device__ __forceinline__ void block_partials(double (&v)[K], double *partials)
+/* Cross-workgroup reductions are deterministic and fence-free: thread 0 stores the K results of
+ * block j at partials[k * stride + j]; a finalize kernel with one workgroup per instance, which runs
+ * next on the stream, sums the instance's blocks in index order (the kernel boundary makes them
+ * visible). */
+template <int K, bool MAX>
+__device__ __forceinline__ void block_partial(double (&v)[K], double *partials, int stride, int j)
 {
     __shared__ double sh[(TPB / 64) * K];
     block_reduce<K, MAX>(v, sh);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < K; k++) partials[k * MAX_GRID + blockIdx.x] = v[k];
+        for (int k = 0; k < K; k++) partials[k * stride + j] = v[k];
 }
 
-/* in a one-workgroup kernel: totals over n workgroups' partials, valid in thread 0 */
-template <int K, bool MAX> __device__ __forceinline__ void sum_partials(const double *partials, int n, double (&w)[K])
+/* in a finalize workgroup: totals over blocks [b0, b1), valid in thread 0 */
+template <int K, bool MAX>
+__device__ __forceinline__ void sum_blocks(const double *partials, int stride, int b0, int b1, double (&w)[K])
 {
     __shared__ double sh[(TPB / 64) * K];
 #pragma unroll
     for (int k = 0; k < K; k++) w[k] = 0.0;
-    for (int b = threadIdx.x; b < n; b += TPB)
+    for (int b = b0 + threadIdx.x; b < b1; b += TPB)
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            double pv = partials[k * MAX_GRID + b];
+            double pv = partials[k * stride + b];
             w[k] = MAX ? fmax(w[k], pv) : w[k] + pv;
         }
     block_reduce<K, MAX>(w, sh);
 }
 
-#define PART_B (4 * MAX_GRID) /* second partial region */
+/* partial regions, in units of the block count (stride) */
+#define PR_A 0 /* 4 rows */
+#define PR_B 4 /* 4 rows */
 
 /* ---- SELL-64 block row product y = A_n x (column values from a functor) ---- */
 template <typename T> struct ColPlain {
@@ -195,30 +221,60 @@ struct Sell {
     const int *slice_ptr, *scol;
 };
 
+/* Slot range [c0, c1) of node n's slice.  UNI: the 64 lanes of the wave hold the 64 nodes of one slice
+ * (blocks start at multiples of 64), so the slice index is wave-uniform and the loads are scalar. */
+template <bool UNI>
+__device__ __forceinline__ void slice_bounds(int n, const Sell &S, int &c0, int &c1)
+{
+    const int sl = UNI ? __builtin_amdgcn_readfirstlane(n >> 6) : (n >> 6);
+    c0 = S.slice_ptr[sl];
+    c1 = S.slice_ptr[sl + 1];
+}
+
+/* y = (row n of the block matrix) x, over the slots [c0, c1) of n's slice.  Slots go in groups of eight
+ * (a triangle-mesh row is rarely wider): the eight column indices first, then the eight neighbours'
+ * values, so a group costs two memory latencies instead of eight dependent pairs; the products still
+ * accumulate in slot order. */
 template <typename T, typename Col>
-__device__ __forceinline__ void block_row(int n, const Sell &S, const T *__restrict__ bv, const Col &col, double &y0,
-                                          double &y1, double &y2)
+__device__ __forceinline__ void block_row_cs(int n, int c0, int c1, const Sell &S, const T *__restrict__ bv,
+                                             const Col &col, double &y0, double &y1, double &y2)
 {
     y0 = 0.0;
     y1 = 0.0;
     y2 = 0.0;
-    const int sl = n >> 6, lane = n & 63;
-    const int c0 = S.slice_ptr[sl], c1 = S.slice_ptr[sl + 1];
-    for (int cs = c0; cs < c1; cs++) {
-        int m = S.scol[cs * 64 + lane];
-        double x0, x1, x2;
-        col.get(m, x0, x1, x2);
-        const T *a = bv + (size_t)cs * 9 * 64 + lane;
-        y0 += (double)a[0] * x0;
-        y0 += (double)a[64] * x1;
-        y0 += (double)a[128] * x2;
-        y1 += (double)a[192] * x0;
-        y1 += (double)a[256] * x1;
-        y1 += (double)a[320] * x2;
-        y2 += (double)a[384] * x0;
-        y2 += (double)a[448] * x1;
-        y2 += (double)a[512] * x2;
+    const int lane = n & 63;
+    for (int cs = c0; cs < c1; cs += 8) {
+        int m[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) m[u] = (cs + u < c1) ? S.scol[(cs + u) * 64 + lane] : n;
+        double xv[8][3];
+#pragma unroll
+        for (int u = 0; u < 8; u++) col.get(m[u], xv[u][0], xv[u][1], xv[u][2]);
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (cs + u >= c1) break;
+            const T *a = bv + (size_t)(cs + u) * 9 * 64 + lane;
+            const double x0 = xv[u][0], x1 = xv[u][1], x2 = xv[u][2];
+            y0 += (double)a[0] * x0;
+            y0 += (double)a[64] * x1;
+            y0 += (double)a[128] * x2;
+            y1 += (double)a[192] * x0;
+            y1 += (double)a[256] * x1;
+            y1 += (double)a[320] * x2;
+            y2 += (double)a[384] * x0;
+            y2 += (double)a[448] * x1;
+            y2 += (double)a[512] * x2;
+        }
     }
+}
+
+template <typename T, typename Col, bool UNI = false>
+__device__ __forceinline__ void block_row(int n, const Sell &S, const T *__restrict__ bv, const Col &col, double &y0,
+                                          double &y1, double &y2)
+{
+    int c0, c1;
+    slice_bounds<UNI>(n, S, c0, c1);
+    block_row_cs<T, Col>(n, c0, c1, S, bv, col, y0, y1, y2);
 }
 
 template <typename T>
@@ -234,55 +290,69 @@ __device__ __forceinline__ void apply_minv(const T *__restrict__ mi, int n, doub
 __device__ __forceinline__ size_t sell_val(int cs, int e, int lane) { return ((size_t)cs * 9 + e) * 64 + lane; }
 
 /* ---------------- Newton-level kernels ---------------- */
-/* get_residual_norms, newton_tools.c:303-373: max|r|, sum r^2, max|dsol| over non-Dirichlet dofs */
-__global__ __launch_bounds__(TPB) void k_norms(int ndofs, const double *__restrict__ res,
+/* get_residual_norms, newton_tools.c:303-373: max|r|, sum r^2, max|dsol| over non-Dirichlet dofs, per
+ * instance (a NaN anywhere in an instance's residual is flagged) */
+__global__ __launch_bounds__(TPB) void k_norms(BLK_ARGS, const double *__restrict__ res,
                                                const double *__restrict__ dsol, const int *__restrict__ bc,
                                                double *partials)
 {
-    double vmax[3] = {0.0, 0.0, 0.0}; /* max|r|, max|dsol|, nan indicator */
-    double vsum[1] = {0.0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ndofs; i += gridDim.x * blockDim.x) {
-        double r = res[i];
-        if (isnan(r)) vmax[2] = 1.0;
-        if (bc[i] != 1) {
-            vmax[0] = fmax(vmax[0], fabs(r));
-            vmax[1] = fmax(vmax[1], fabs(dsol[i]));
-            vsum[0] += r * r;
-        }
+    int j0, j1;
+    blk_range(nblk, j0, j1);
+    for (int j = j0; j < j1; j++) {
+        double vmax[3] = {0.0, 0.0, 0.0}; /* max|r|, max|dsol|, nan indicator */
+        double vsum[1] = {0.0};
+        int nb0, nb1, inst;
+        blk_get(blk_node, blk_inst, n_single, j, nb0, nb1, inst);
+        const int n = nb0 + threadIdx.x;
+        if (n < nb1)
+            for (int k = 0; k < 3; k++) {
+                const int i = 3 * n + k;
+                double r = res[i];
+                if (isnan(r)) vmax[2] = 1.0;
+                if (bc[i] != 1) {
+                    vmax[0] = fmax(vmax[0], fabs(r));
+                    vmax[1] = fmax(vmax[1], fabs(dsol[i]));
+                    vsum[0] += r * r;
+                }
+            }
+        block_partial<3, true>(vmax, partials + PR_A * nblk, nblk, j);
+        block_partial<1, false>(vsum, partials + PR_B * nblk, nblk, j);
     }
-    block_partials<3, true>(vmax, partials);
-    block_partials<1, false>(vsum, partials + PART_B);
 }
-__global__ __launch_bounds__(TPB) void k_norms_fin(int n, const double *partials, adhk_scalars *sc, int *flags)
+/* one workgroup per instance */
+__global__ __launch_bounds__(TPB) void k_norms_fin(int nblk, const int *__restrict__ inst_blk, const double *partials,
+                                                   adhk_scalars *sc)
 {
+    const int i = blockIdx.x;
     double vmax[3], vsum[1];
-    sum_partials<3, true>(partials, n, vmax);
-    sum_partials<1, false>(partials + PART_B, n, vsum);
+    sum_blocks<3, true>(partials + PR_A * nblk, nblk, inst_blk[i], inst_blk[i + 1], vmax);
+    sum_blocks<1, false>(partials + PR_B * nblk, nblk, inst_blk[i], inst_blk[i + 1], vsum);
     if (threadIdx.x == 0) {
-        sc->rmax = vmax[0];
-        sc->incmax = vmax[1];
-        if (vmax[2] != 0.0) flags[1] = 1;
-        sc->rl2sum = vsum[0];
+        sc[i].rmax = vmax[0];
+        sc[i].incmax = vmax[1];
+        sc[i].nanflag = (vmax[2] != 0.0) ? 1 : 0;
+        sc[i].rl2sum = vsum[0];
     }
 }
 
 extern "C" int adhk_launch_norms(adhk_ctx *c)
 {
-    int g = grid_for(c->ndofs);
-    hipLaunchKernelGGL(k_norms, dim3(g), dim3(TPB), 0, c->stream, c->ndofs, c->residual, c->dsol, c->bc_mask,
-                       c->partials);
+    hipLaunchKernelGGL(k_norms, dim3(grid_blk(c->nblk)), dim3(TPB), 0, c->stream, BLK_PASS(c), c->residual, c->dsol,
+                       c->bc_mask, c->partials);
     CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_norms_fin, dim3(1), dim3(TPB), 0, c->stream, g, c->partials, c->sc, c->flags);
+    hipLaunchKernelGGL(k_norms_fin, dim3(c->ninst), dim3(TPB), 0, c->stream, c->nblk, c->inst_blk, c->partials,
+                       c->sc);
     CHECK_LAUNCH();
     return 0;
 }
 
 /* check_diag, matrix_utilities.c:120-152 */
 __global__ void k_check_diag(int N, const int *__restrict__ slice_ptr, const int *__restrict__ diag_slot,
-                             const double *__restrict__ bv, int *__restrict__ bc)
+                             const double *__restrict__ bv, int *__restrict__ bc, const int *__restrict__ node_inst,
+                             const int *__restrict__ work)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+    if (n >= N || !work[node_inst[n]]) return;
     int d = diag_slot[n];
     int cs = slice_ptr[n >> 6] + d;
     for (int r = 0; r < 3; r++) {
@@ -295,7 +365,7 @@ extern "C" int adhk_launch_check_diag(adhk_ctx *c)
 {
     adhk_kt_begin(c, ADHK_KT_PREP);
     hipLaunchKernelGGL(k_check_diag, dim3((c->N + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->N, c->slice_ptr,
-                       c->diag_slot, c->bvals, c->bc_mask);
+                       c->diag_slot, c->bvals, c->bc_mask, c->node_inst, c->work);
     CHECK_LAUNCH();
     adhk_kt_end(c);
     return 0;
@@ -473,37 +543,47 @@ __global__ __launch_bounds__(TPB) void k_kinit1(int N, const T *__restrict__ min
 
 /* r = M^-1 b - (M^-1 A) x, q = r, rnorm, bnorm, rho = r.q  (:155-207) */
 template <typename T>
-__global__ __launch_bounds__(TPB) void k_kinit2(int N, Sell S, const T *__restrict__ bv, const T *__restrict__ x,
-                                                const T *__restrict__ bp, T *__restrict__ r, T *__restrict__ q,
-                                                double *partials)
+__global__ __launch_bounds__(TPB) void k_kinit2(BLK_ARGS, Sell S, const T *__restrict__ bv,
+                                                const T *__restrict__ x, const T *__restrict__ bp, T *__restrict__ r,
+                                                T *__restrict__ q, double *partials)
 {
-    double vmax[2] = {0.0, 0.0}, vsum[1] = {0.0};
-    const NodeRange nr = xcd_range(N);
-    for (int n = nr.begin + threadIdx.x; n < nr.end; n += TPB) {
-        double y0, y1, y2;
-        block_row<T>(n, S, bv, ColPlain<T>{x}, y0, y1, y2); /* bv holds M^-1 A */
-        double z[3] = {y0, y1, y2};
-        for (int k = 0; k < 3; k++) {
-            int i = 3 * n + k;
-            double bb = (double)bp[i];
-            T rr = (T)(bb + (-1) * z[k]);
-            r[i] = rr;
-            q[i] = rr;
-            vmax[0] = fmax(vmax[0], fabs((double)rr));
-            vmax[1] = fmax(vmax[1], fabs(bb));
-            vsum[0] += (double)rr * (double)rr;
+    int j0, j1;
+    blk_range(nblk, j0, j1);
+    for (int j = j0; j < j1; j++) {
+        double vmax[2] = {0.0, 0.0}, vsum[1] = {0.0};
+        int nb0, nb1, inst;
+        blk_get(blk_node, blk_inst, n_single, j, nb0, nb1, inst);
+        const int n = nb0 + threadIdx.x;
+        if (n < nb1) {
+            double y0, y1, y2;
+            block_row<T>(n, S, bv, ColPlain<T>{x}, y0, y1, y2); /* bv holds M^-1 A */
+            double z[3] = {y0, y1, y2};
+            for (int k = 0; k < 3; k++) {
+                int i = 3 * n + k;
+                double bb = (double)bp[i];
+                T rr = (T)(bb + (-1) * z[k]);
+                r[i] = rr;
+                q[i] = rr;
+                vmax[0] = fmax(vmax[0], fabs((double)rr));
+                vmax[1] = fmax(vmax[1], fabs(bb));
+                vsum[0] += (double)rr * (double)rr;
+            }
         }
+        block_partial<2, true>(vmax, partials + PR_A * nblk, nblk, j);
+        block_partial<1, false>(vsum, partials + PR_B * nblk, nblk, j);
     }
-    block_partials<2, true>(vmax, partials);
-    block_partials<1, false>(vsum, partials + PART_B);
 }
 
-__global__ __launch_bounds__(TPB) void k_kinit3(int n, const double *partials, adhk_scalars *sc, double tol,
+/* one workgroup per instance; an instance outside the work mask starts (and stays) converged */
+__global__ __launch_bounds__(TPB) void k_kinit3(int nblk, const int *__restrict__ inst_blk, const double *partials,
+                                                adhk_scalars *sc_all, const int *__restrict__ work, double tol,
                                                 int max_it)
 {
+    const int inst = blockIdx.x;
+    adhk_scalars *sc = sc_all + inst;
     double vmax[2], vsum[1];
-    sum_partials<2, true>(partials, n, vmax);
-    sum_partials<1, false>(partials + PART_B, n, vsum);
+    sum_blocks<2, true>(partials + PR_A * nblk, nblk, inst_blk[inst], inst_blk[inst + 1], vmax);
+    sum_blocks<1, false>(partials + PR_B * nblk, nblk, inst_blk[inst], inst_blk[inst + 1], vsum);
     if (threadIdx.x == 0) {
         sc->rnorm = vmax[0];
         sc->bnorm = vmax[1];
@@ -515,61 +595,80 @@ __global__ __launch_bounds__(TPB) void k_kinit3(int n, const double *partials, a
         sc->omega = 1.0;
         sc->it = 0;
         sc->beta = (sc->rho * 1.0) / (1.0 * 1.0);
-        sc->done = !(sc->rnorm > (tol + sc->bnorm * tol) && 0 < max_it);
+        sc->done = !(sc->rnorm > (tol + sc->bnorm * tol) && 0 < max_it) || !work[inst];
     }
 }
 
 /* A: [FIRST] p1 = r0 (p = Ap = 0 at start); otherwise finish iteration k -- x += alpha p + omega s,
  * r = s - omega As, ||r||_inf and the stop test (:294-307, :210) -- and form p' = (p - omega Ap) beta + r.
- * Then Ap' = M^-1 A p' and q.Ap' -> alpha = rho / (q.Ap')  (:215-246). */
+ * Then Ap' = M^-1 A p' and q.Ap' -> alpha = rho / (q.Ap')  (:215-246).  Blocks of converged
+ * instances are skipped. */
 template <typename T, bool FIRST>
-__global__ __launch_bounds__(TPB) void k_iter_a(int N, Sell S, const T *__restrict__ bv, T *__restrict__ x,
+__global__ __launch_bounds__(TPB) void k_iter_a(BLK_ARGS, Sell S, const T *__restrict__ bv, T *__restrict__ x,
                                                 T *__restrict__ r, const T *__restrict__ s,
                                                 const T *__restrict__ As, const T *__restrict__ p_old,
                                                 const T *__restrict__ Ap_old, T *__restrict__ p_new,
                                                 T *__restrict__ Ap_new, const T *__restrict__ q, double *partials,
-                                                const adhk_scalars *sc)
+                                                const adhk_scalars *__restrict__ sc)
 {
-    if (sc->done) return;
-    const double alpha = sc->alpha, omega = sc->omega, beta = sc->beta;
-    double vmax[1] = {0.0}, vsum[1] = {0.0};
-    const NodeRange nr = xcd_range(N);
-    for (int n = nr.begin + threadIdx.x; n < nr.end; n += TPB) {
-        double y0, y1, y2;
-        if (FIRST) {
-            for (int k = 0; k < 3; k++) p_new[3 * n + k] = r[3 * n + k];
-            block_row<T>(n, S, bv, ColPlain<T>{r}, y0, y1, y2);
-        } else {
-            ColP<T> col{s, As, p_old, Ap_old, omega, beta};
-            for (int k = 0; k < 3; k++) {
-                const int i = 3 * n + k;
-                T rr, pp;
-                col.one(i, rr, pp);
-                x[i] = (T)__fma_rn(omega, (double)s[i], __fma_rn(alpha, (double)p_old[i], (double)x[i]));
-                r[i] = rr;
-                p_new[i] = pp;
-                vmax[0] = fmax(vmax[0], fabs((double)rr));
+    int j0, j1;
+    blk_range(nblk, j0, j1);
+    for (int j = j0; j < j1; j++) {
+        int nb0, nb1, inst;
+        blk_get(blk_node, blk_inst, n_single, j, nb0, nb1, inst);
+        const adhk_scalars *si = sc + inst;
+        if (si->done) continue;
+        const double alpha = si->alpha, omega = si->omega, beta = si->beta;
+        double vmax[1] = {0.0}, vsum[1] = {0.0};
+        const int n = nb0 + threadIdx.x;
+        if (n < nb1) {
+            double y0, y1, y2;
+            /* slice bounds and q before the updates: their stores must not delay the gathers */
+            int c0, c1;
+            if (n_single)
+                slice_bounds<true>(n, S, c0, c1);
+            else
+                slice_bounds<false>(n, S, c0, c1);
+            const double q0 = (double)q[3 * n], q1 = (double)q[3 * n + 1], q2 = (double)q[3 * n + 2];
+            if (FIRST) {
+                for (int k = 0; k < 3; k++) p_new[3 * n + k] = r[3 * n + k];
+                block_row_cs<T>(n, c0, c1, S, bv, ColPlain<T>{r}, y0, y1, y2);
+            } else {
+                ColP<T> col{s, As, p_old, Ap_old, omega, beta};
+                for (int k = 0; k < 3; k++) {
+                    const int i = 3 * n + k;
+                    T rr, pp;
+                    col.one(i, rr, pp);
+                    x[i] = (T)__fma_rn(omega, (double)s[i], __fma_rn(alpha, (double)p_old[i], (double)x[i]));
+                    r[i] = rr;
+                    p_new[i] = pp;
+                    vmax[0] = fmax(vmax[0], fabs((double)rr));
+                }
+                block_row_cs<T>(n, c0, c1, S, bv, col, y0, y1, y2);
             }
-            block_row<T>(n, S, bv, col, y0, y1, y2);
+            T a0 = (T)y0, a1 = (T)y1, a2 = (T)y2; /* bv holds M^-1 A */
+            Ap_new[3 * n] = a0;
+            Ap_new[3 * n + 1] = a1;
+            Ap_new[3 * n + 2] = a2;
+            vsum[0] += q0 * (double)a0 + q1 * (double)a1 + q2 * (double)a2;
         }
-        T a0 = (T)y0, a1 = (T)y1, a2 = (T)y2; /* bv holds M^-1 A */
-        Ap_new[3 * n] = a0;
-        Ap_new[3 * n + 1] = a1;
-        Ap_new[3 * n + 2] = a2;
-        vsum[0] += (double)q[3 * n] * (double)a0 + (double)q[3 * n + 1] * (double)a1 +
-                   (double)q[3 * n + 2] * (double)a2;
+        if (!FIRST) block_partial<1, true>(vmax, partials + PR_A * nblk, nblk, j);
+        block_partial<1, false>(vsum, partials + PR_B * nblk, nblk, j);
     }
-    if (!FIRST) block_partials<1, true>(vmax, partials);
-    block_partials<1, false>(vsum, partials + PART_B);
 }
 
-/* finalize A: [not FIRST] ||r||_inf of the completed iteration and the stop test (:210); gamma, alpha */
-template <bool FIRST> __global__ __launch_bounds__(TPB) void k_fin_a(int n, const double *partials, adhk_scalars *sc)
+/* finalize A, one workgroup per instance: [not FIRST] ||r||_inf of the completed iteration and the
+ * stop test (:210); gamma, alpha */
+template <bool FIRST>
+__global__ __launch_bounds__(TPB) void k_fin_a(int nblk, const int *__restrict__ inst_blk, const double *partials,
+                                               adhk_scalars *sc_all)
 {
+    adhk_scalars *sc = sc_all + blockIdx.x;
     if (sc->done) return;
+    const int b0 = inst_blk[blockIdx.x], b1 = inst_blk[blockIdx.x + 1];
     double vmax[1], vsum[1];
-    if (!FIRST) sum_partials<1, true>(partials, n, vmax);
-    sum_partials<1, false>(partials + PART_B, n, vsum);
+    if (!FIRST) sum_blocks<1, true>(partials + PR_A * nblk, nblk, b0, b1, vmax);
+    sum_blocks<1, false>(partials + PR_B * nblk, nblk, b0, b1, vsum);
     if (threadIdx.x == 0) {
         if (!FIRST) {
             sc->rnorm = vmax[0];
@@ -583,39 +682,57 @@ template <bool FIRST> __global__ __launch_bounds__(TPB) void k_fin_a(int n, cons
 /* S: s = r - alpha Ap, As = M^-1 A s, omega = (As.s)/(As.As), rho' = -omega (As.q),
  * beta' = (rho' alpha)/(rho omega), iteration count + 1   (:247-291) */
 template <typename T>
-__global__ __launch_bounds__(TPB) void k_iter_s(int N, Sell S, const T *__restrict__ bv, const T *__restrict__ r,
-                                                const T *__restrict__ Ap, T *__restrict__ s, T *__restrict__ As,
-                                                const T *__restrict__ q, double *partials, const adhk_scalars *sc)
+__global__ __launch_bounds__(TPB) void k_iter_s(BLK_ARGS, Sell S, const T *__restrict__ bv,
+                                                const T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ s,
+                                                T *__restrict__ As, const T *__restrict__ q, double *partials,
+                                                const adhk_scalars *__restrict__ sc)
 {
-    if (sc->done) return;
-    const double alpha = sc->alpha;
-    ColS<T> col{r, Ap, alpha};
-    double v[3] = {0.0, 0.0, 0.0};
-    const NodeRange nr = xcd_range(N);
-    for (int n = nr.begin + threadIdx.x; n < nr.end; n += TPB) {
-        double y0, y1, y2;
-        block_row<T>(n, S, bv, col, y0, y1, y2); /* bv holds M^-1 A */
-        double z[3] = {y0, y1, y2};
-        for (int k = 0; k < 3; k++) {
-            int i = 3 * n + k;
-            T si = col.one(i);
-            T ai = (T)z[k];
-            s[i] = si;
-            As[i] = ai;
-            v[0] += (double)ai * (double)si;
-            v[1] += (double)ai * (double)ai;
-            v[2] += (double)ai * (double)q[i];
+    int j0, j1;
+    blk_range(nblk, j0, j1);
+    for (int j = j0; j < j1; j++) {
+        int nb0, nb1, inst;
+        blk_get(blk_node, blk_inst, n_single, j, nb0, nb1, inst);
+        const adhk_scalars *si = sc + inst;
+        if (si->done) continue;
+        ColS<T> col{r, Ap, si->alpha};
+        double v[3] = {0.0, 0.0, 0.0};
+        const int n = nb0 + threadIdx.x;
+        if (n < nb1) {
+            /* the row's own r, Ap and q first, so their loads overlap the gathers */
+            T sv[3], qv[3];
+            for (int k = 0; k < 3; k++) {
+                sv[k] = col.one(3 * n + k);
+                qv[k] = q[3 * n + k];
+            }
+            double y0, y1, y2;
+            if (n_single)
+                block_row<T, ColS<T>, true>(n, S, bv, col, y0, y1, y2); /* bv holds M^-1 A */
+            else
+                block_row<T, ColS<T>, false>(n, S, bv, col, y0, y1, y2);
+            double z[3] = {y0, y1, y2};
+            for (int k = 0; k < 3; k++) {
+                int i = 3 * n + k;
+                T ai = (T)z[k];
+                s[i] = sv[k];
+                As[i] = ai;
+                v[0] += (double)ai * (double)sv[k];
+                v[1] += (double)ai * (double)ai;
+                v[2] += (double)ai * (double)qv[k];
+            }
         }
+        block_partial<3, false>(v, partials + PR_A * nblk, nblk, j);
     }
-    block_partials<3, false>(v, partials);
 }
 
-/* finalize S: omega = (As.s)/(As.As), rho' = -omega (As.q), beta' = (rho' alpha)/(rho omega), it + 1 */
-__global__ __launch_bounds__(TPB) void k_fin_s(int n, const double *partials, adhk_scalars *sc)
+/* finalize S, one workgroup per instance: omega = (As.s)/(As.As), rho' = -omega (As.q),
+ * beta' = (rho' alpha)/(rho omega), it + 1 */
+__global__ __launch_bounds__(TPB) void k_fin_s(int nblk, const int *__restrict__ inst_blk, const double *partials,
+                                               adhk_scalars *sc_all)
 {
+    adhk_scalars *sc = sc_all + blockIdx.x;
     if (sc->done) return;
     double v[3];
-    sum_partials<3, false>(partials, n, v);
+    sum_blocks<3, false>(partials + PR_A * nblk, nblk, inst_blk[blockIdx.x], inst_blk[blockIdx.x + 1], v);
     if (threadIdx.x == 0) {
         double omega = v[0] / v[1];
         double rhop = sc->rho;
@@ -690,11 +807,11 @@ template <typename T> static int krylov_init(adhk_ctx *c, KV<T> k, double tol, i
     hipLaunchKernelGGL(k_kinit1<T>, dim3(gn), dim3(TPB), 0, c->stream, c->N, k.mi, c->residual, c->dsol, c->x0, k.x,
                        k.s);
     CHECK_LAUNCH();
-    const int g = grid_xcd(c->N);
-    hipLaunchKernelGGL(k_kinit2<T>, dim3(g), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.x, k.s, k.r, k.q,
-                       c->partials);
+    hipLaunchKernelGGL(k_kinit2<T>, dim3(grid_blk(c->nblk)), dim3(TPB), 0, c->stream, BLK_PASS(c), sell_of(c), k.bv,
+                       k.x, k.s, k.r, k.q, c->partials);
     CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_kinit3, dim3(1), dim3(TPB), 0, c->stream, g, c->partials, c->sc, tol, max_it);
+    hipLaunchKernelGGL(k_kinit3, dim3(c->ninst), dim3(TPB), 0, c->stream, c->nblk, c->inst_blk, c->partials, c->sc,
+                       c->work, tol, max_it);
     CHECK_LAUNCH();
     return 0;
 }
@@ -703,26 +820,25 @@ template <typename T, bool FIRST> static int launch_a(adhk_ctx *c, KV<T> k, int 
 {
     /* reads direction buffer cur, writes 1 - cur (FIRST writes buffer 1) */
     int nw = FIRST ? 1 : 1 - cur;
-    const int g = grid_xcd(c->N);
     adhk_kt_begin(c, ADHK_KT_SPMV_P);
-    hipLaunchKernelGGL((k_iter_a<T, FIRST>), dim3(g), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.x, k.r, k.s,
-                       k.As, k.p[cur], k.Ap[cur], k.p[nw], k.Ap[nw], k.q, c->partials, c->sc);
+    hipLaunchKernelGGL((k_iter_a<T, FIRST>), dim3(grid_blk(c->nblk)), dim3(TPB), 0, c->stream, BLK_PASS(c), sell_of(c),
+                       k.bv, k.x, k.r, k.s, k.As, k.p[cur], k.Ap[cur], k.p[nw], k.Ap[nw], k.q, c->partials, c->sc);
     adhk_kt_end(c);
     CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_fin_a<FIRST>, dim3(1), dim3(TPB), 0, c->stream, g, c->partials, c->sc);
+    hipLaunchKernelGGL(k_fin_a<FIRST>, dim3(c->ninst), dim3(TPB), 0, c->stream, c->nblk, c->inst_blk, c->partials,
+                       c->sc);
     CHECK_LAUNCH();
     return 0;
 }
 
 template <typename T> static int launch_s(adhk_ctx *c, KV<T> k, int cur)
 {
-    const int g = grid_xcd(c->N);
     adhk_kt_begin(c, ADHK_KT_SPMV_S);
-    hipLaunchKernelGGL(k_iter_s<T>, dim3(g), dim3(TPB), 0, c->stream, c->N, sell_of(c), k.bv, k.r, k.Ap[cur], k.s,
-                       k.As, k.q, c->partials, c->sc);
+    hipLaunchKernelGGL(k_iter_s<T>, dim3(grid_blk(c->nblk)), dim3(TPB), 0, c->stream, BLK_PASS(c), sell_of(c), k.bv,
+                       k.r, k.Ap[cur], k.s, k.As, k.q, c->partials, c->sc);
     adhk_kt_end(c);
     CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_fin_s, dim3(1), dim3(TPB), 0, c->stream, g, c->partials, c->sc);
+    hipLaunchKernelGGL(k_fin_s, dim3(c->ninst), dim3(TPB), 0, c->stream, c->nblk, c->inst_blk, c->partials, c->sc);
     CHECK_LAUNCH();
     return 0;
 }
@@ -771,19 +887,19 @@ extern "C" int adhk_launch_krylov_finish(adhk_ctx *c, int mixed)
 
 /* ---------------- Newton updates ---------------- */
 /* increment_function (newton_tools.c:261) then dsol = -dsol (fe_newton.c:553-555) */
-__global__ void k_increment(int n, double *sol, double *dsol)
+__global__ void k_increment(int n, double *sol, double *dsol, const int *node_inst, const int *work)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || !work[node_inst[i / 3]]) return;
     double d = dsol[i];
     sol[i] += d;
     dsol[i] = -d;
 }
 /* line search step (fe_newton.c:681-741): dsol *= 0.5, sol += dsol */
-__global__ void k_linesearch(int n, double *sol, double *dsol)
+__global__ void k_linesearch(int n, double *sol, double *dsol, const int *node_inst, const int *work)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || !work[node_inst[i / 3]]) return;
     double d = dsol[i] * 0.5;
     dsol[i] = d;
     sol[i] += d;
@@ -812,14 +928,14 @@ __global__ void k_rotate(int n, double *sol, double *sol_old, double *sol_older)
 extern "C" int adhk_launch_increment(adhk_ctx *c)
 {
     hipLaunchKernelGGL(k_increment, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs, c->sol,
-                       c->dsol);
+                       c->dsol, c->node_inst, c->work);
     CHECK_LAUNCH();
     return 0;
 }
 extern "C" int adhk_launch_linesearch(adhk_ctx *c)
 {
     hipLaunchKernelGGL(k_linesearch, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs, c->sol,
-                       c->dsol);
+                       c->dsol, c->node_inst, c->work);
     CHECK_LAUNCH();
     return 0;
 }
@@ -833,6 +949,34 @@ extern "C" int adhk_launch_init_system(adhk_ctx *c)
     if (adhk_launch_dry_bc(c)) return -1;
     hipLaunchKernelGGL(k_apply_dirichlet_sol, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs,
                        c->sol, c->bc_mask, c->dirichlet);
+    CHECK_LAUNCH();
+    return 0;
+}
+/* dst <- src for the dofs of instances in the work mask */
+__global__ void k_copy_masked(int n, double *dst, const double *src, const int *node_inst, const int *work)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && work[node_inst[i / 3]]) dst[i] = src[i];
+}
+__global__ void k_dirichlet_sol_masked(int n, double *sol, const int *bc, const double *dir, const int *node_inst,
+                                       const int *work)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && bc[i] == 1 && work[node_inst[i / 3]]) sol[i] = dir[i];
+}
+extern "C" int adhk_launch_copy_masked(adhk_ctx *c, double *dst, const double *src)
+{
+    hipLaunchKernelGGL(k_copy_masked, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs, dst, src,
+                       c->node_inst, c->work);
+    CHECK_LAUNCH();
+    return 0;
+}
+/* sol <- sol_old with the Dirichlet values, for the instances in the work mask */
+extern "C" int adhk_launch_start_from_old_masked(adhk_ctx *c)
+{
+    if (adhk_launch_copy_masked(c, c->sol, c->sol_old)) return -1;
+    hipLaunchKernelGGL(k_dirichlet_sol_masked, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs,
+                       c->sol, c->bc_mask, c->dirichlet, c->node_inst, c->work);
     CHECK_LAUNCH();
     return 0;
 }

@@ -72,9 +72,16 @@ static void ctx_free(adhk_ctx *c)
                     c->minv, c->bvals32, c->minv32, c->erhs, c->emat, c->r, c->pv, c->Ap, c->s, c->As, c->q, c->x0,
                     c->pv2, c->Ap2, c->xk, c->r32, c->p32, c->Ap32, c->s32, c->As32, c->q32, c->xk32, c->p32b,
                     c->Ap32b, c->partials, c->counter, c->sc, c->flags, c->slice_ptr, c->scol, c->sell2blk,
-                    c->blk2sell, c->diag_slot};
+                    c->blk2sell, c->diag_slot, c->node_inst, c->elem_inst, c->seg_inst, c->blk_node,
+                    c->blk_inst, c->inst_blk, c->work};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++)
         if (ptrs[i]) hipFree(ptrs[i]);
+    free(c->h_inst_node);
+    free(c->h_inst_elem);
+    free(c->h_inst_seg);
+    free(c->h_inst_blk);
+    free(c->h_work);
+    free(c->inst_stats);
     for (int i = 0; i < 8; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
     for (int i = 0; i < ADHK_KT_RING; i++) {
@@ -90,32 +97,45 @@ void adhk_sw2_destroy(adhk_sw2 *c) { ctx_free(c); }
 #define DALLOC(ptr, n, T) HC(hipMalloc((void **)&(ptr), ((size_t)(n) > 0 ? (size_t)(n) : 1) * sizeof(T)))
 #define DZERO(ptr, n, T) HC(hipMemsetAsync((ptr), 0, ((size_t)(n) > 0 ? (size_t)(n) : 1) * sizeof(T), c->stream))
 
-adhk_sw2 *adhk_sw2_create(int nnodes, const double *x, const double *y, const double *z, int nelems2d,
-                          const int *tri, const double *mann, int nelems1d, const int *seg, const int *seg_kind,
-                          const double *seg_val, const adhk_params *p)
+static adhk_sw2 *create_impl(int ninst, const int *node_off, const int *elem_off, const int *seg_off, int nnodes,
+                             const double *x, const double *y, const double *z, int nelems2d, const int *tri,
+                             const double *mann, int nelems1d, const int *seg, const int *seg_kind,
+                             const double *seg_val, const adhk_params *p)
 {
     adhk_ctx *c = NULL;
     double *h_djac = NULL, *h_grad = NULL, *h_ks = NULL, *h_sdjac = NULL, *h_snrml = NULL;
-    int *h_tri = NULL;
+    int *h_tri = NULL, *h_ni = NULL, *h_ei = NULL, *h_si = NULL, *h_bn = NULL, *h_bi = NULL;
     if (nnodes <= 0 || nelems2d <= 0 || nelems1d < 0 || !x || !y || !z || !tri || !p ||
         (nelems1d > 0 && (!seg || !seg_kind || !seg_val))) {
         adhk_set_error("adhk_sw2_create: invalid arguments");
         return NULL;
     }
+    if (ninst < 1 || !node_off || !elem_off || !seg_off || node_off[0] != 0 || elem_off[0] != 0 ||
+        seg_off[0] != 0 || node_off[ninst] != nnodes || elem_off[ninst] != nelems2d || seg_off[ninst] != nelems1d) {
+        adhk_set_error("adhk_sw2_create: instance offsets must start at 0 and end at the totals");
+        return NULL;
+    }
+    for (int i = 0; i < ninst; i++)
+        if (node_off[i + 1] <= node_off[i] || elem_off[i + 1] <= elem_off[i] || seg_off[i + 1] < seg_off[i]) {
+            adhk_set_error("adhk_sw2_create: instance %d is empty or its offsets decrease", i);
+            return NULL;
+        }
     if ((long)nnodes * 3 >= 2147483647L || (long)nelems2d * 81 >= 2147483647L) {
         adhk_set_error("adhk_sw2_create: mesh too large for 32-bit indices");
         return NULL;
     }
-    for (long i = 0; i < 3L * nelems2d; i++)
-        if (tri[i] < 0 || tri[i] >= nnodes) {
-            adhk_set_error("adhk_sw2_create: triangle node index out of range at %ld", i);
-            return NULL;
-        }
-    for (long i = 0; i < 2L * nelems1d; i++)
-        if (seg[i] < 0 || seg[i] >= nnodes) {
-            adhk_set_error("adhk_sw2_create: segment node index out of range at %ld", i);
-            return NULL;
-        }
+    for (int k = 0; k < ninst; k++) {
+        for (long i = 3L * elem_off[k]; i < 3L * elem_off[k + 1]; i++)
+            if (tri[i] < node_off[k] || tri[i] >= node_off[k + 1]) {
+                adhk_set_error("adhk_sw2_create: triangle node index out of its instance's range at %ld", i);
+                return NULL;
+            }
+        for (long i = 2L * seg_off[k]; i < 2L * seg_off[k + 1]; i++)
+            if (seg[i] < node_off[k] || seg[i] >= node_off[k + 1]) {
+                adhk_set_error("adhk_sw2_create: segment node index out of its instance's range at %ld", i);
+                return NULL;
+            }
+    }
     for (int i = 0; i < nelems1d; i++)
         if (seg_kind[i] < ADHK_BC_DEPTH || seg_kind[i] > ADHK_BC_VEL) {
             adhk_set_error("adhk_sw2_create: unknown segment kind %d", seg_kind[i]);
@@ -127,6 +147,7 @@ adhk_sw2 *adhk_sw2_create(int nnodes, const double *x, const double *y, const do
     c->S = nelems1d;
     c->ndofs = 3 * nnodes;
     c->p = *p;
+    c->ninst = ninst;
     HC(hipStreamCreate(&c->stream));
     for (int i = 0; i < 8; i++) HC(hipEventCreate(&c->ev[i]));
 
@@ -236,13 +257,71 @@ adhk_sw2 *adhk_sw2_create(int nnodes, const double *x, const double *y, const do
         DALLOC(c->pv2, nd, double);
         DALLOC(c->Ap2, nd, double);
         DALLOC(c->xk, nd, double);
-        DALLOC(c->partials, 8 * 2048, double);
         DALLOC(c->counter, 8, unsigned int);
-        DALLOC(c->sc, 1, adhk_scalars);
+        DALLOC(c->sc, ninst, adhk_scalars);
         DALLOC(c->flags, 4, int);
         DZERO(c->counter, 8, unsigned int);
-        DZERO(c->sc, 1, adhk_scalars);
+        DZERO(c->sc, ninst, adhk_scalars);
         DZERO(c->flags, 4, int);
+    }
+    /* instances and their 256-node blocks */
+    {
+        const int N = c->N, E = c->E, S = c->S;
+        int nb = 0;
+        for (int k = 0; k < ninst; k++) nb += (node_off[k + 1] - node_off[k] + 255) / 256;
+        c->nblk = nb;
+        c->h_inst_node = malloc((size_t)(ninst + 1) * sizeof(int));
+        c->h_inst_elem = malloc((size_t)(ninst + 1) * sizeof(int));
+        c->h_inst_seg = malloc((size_t)(ninst + 1) * sizeof(int));
+        c->h_inst_blk = malloc((size_t)(ninst + 1) * sizeof(int));
+        c->h_work = malloc((size_t)ninst * sizeof(int));
+        c->inst_stats = calloc((size_t)ninst, sizeof(adhk_newton_stats));
+        h_ni = malloc((size_t)N * sizeof(int));
+        h_ei = malloc((size_t)E * sizeof(int));
+        h_si = malloc((size_t)(S > 0 ? S : 1) * sizeof(int));
+        h_bn = malloc((size_t)(nb + 1) * sizeof(int));
+        h_bi = malloc((size_t)nb * sizeof(int));
+        if (!c->h_inst_node || !c->h_inst_elem || !c->h_inst_seg || !c->h_inst_blk || !c->h_work ||
+            !c->inst_stats || !h_ni || !h_ei || !h_si || !h_bn || !h_bi) {
+            adhk_set_error("adhk_sw2_create: out of host memory");
+            goto fail;
+        }
+        int b = 0;
+        for (int k = 0; k < ninst; k++) {
+            c->h_inst_node[k] = node_off[k];
+            c->h_inst_elem[k] = elem_off[k];
+            c->h_inst_seg[k] = seg_off[k];
+            c->h_inst_blk[k] = b;
+            c->h_work[k] = 1;
+            for (int n = node_off[k]; n < node_off[k + 1]; n++) h_ni[n] = k;
+            for (int e = elem_off[k]; e < elem_off[k + 1]; e++) h_ei[e] = k;
+            for (int e = seg_off[k]; e < seg_off[k + 1]; e++) h_si[e] = k;
+            for (int n = node_off[k]; n < node_off[k + 1]; n += 256) {
+                h_bn[b] = n;
+                h_bi[b] = k;
+                b++;
+            }
+        }
+        h_bn[nb] = N;
+        c->h_inst_node[ninst] = N;
+        c->h_inst_elem[ninst] = E;
+        c->h_inst_seg[ninst] = S;
+        c->h_inst_blk[ninst] = nb;
+        DALLOC(c->node_inst, N, int);
+        DALLOC(c->elem_inst, E, int);
+        DALLOC(c->seg_inst, S, int);
+        DALLOC(c->blk_node, nb + 1, int);
+        DALLOC(c->blk_inst, nb, int);
+        DALLOC(c->inst_blk, ninst + 1, int);
+        DALLOC(c->work, ninst, int);
+        HC(hipMemcpy(c->node_inst, h_ni, (size_t)N * sizeof(int), hipMemcpyHostToDevice));
+        HC(hipMemcpy(c->elem_inst, h_ei, (size_t)E * sizeof(int), hipMemcpyHostToDevice));
+        if (S > 0) HC(hipMemcpy(c->seg_inst, h_si, (size_t)S * sizeof(int), hipMemcpyHostToDevice));
+        HC(hipMemcpy(c->blk_node, h_bn, (size_t)(nb + 1) * sizeof(int), hipMemcpyHostToDevice));
+        HC(hipMemcpy(c->blk_inst, h_bi, (size_t)nb * sizeof(int), hipMemcpyHostToDevice));
+        HC(hipMemcpy(c->inst_blk, c->h_inst_blk, (size_t)(ninst + 1) * sizeof(int), hipMemcpyHostToDevice));
+        HC(hipMemcpy(c->work, c->h_work, (size_t)ninst * sizeof(int), hipMemcpyHostToDevice));
+        DALLOC(c->partials, 8L * nb, double);
     }
     LC(adhk_build_pattern(c));
     DALLOC(c->bvals, 9L * 64 * c->nslot, double);
@@ -256,6 +335,11 @@ adhk_sw2 *adhk_sw2_create(int nnodes, const double *x, const double *y, const do
     free(h_tri);
     free(h_sdjac);
     free(h_snrml);
+    free(h_ni);
+    free(h_ei);
+    free(h_si);
+    free(h_bn);
+    free(h_bi);
     return c;
 fail:
     free(h_djac);
@@ -264,8 +348,30 @@ fail:
     free(h_tri);
     free(h_sdjac);
     free(h_snrml);
+    free(h_ni);
+    free(h_ei);
+    free(h_si);
+    free(h_bn);
+    free(h_bi);
     ctx_free(c);
     return NULL;
+}
+
+adhk_sw2 *adhk_sw2_create(int nnodes, const double *x, const double *y, const double *z, int nelems2d,
+                          const int *tri, const double *mann, int nelems1d, const int *seg, const int *seg_kind,
+                          const double *seg_val, const adhk_params *p)
+{
+    const int no[2] = {0, nnodes}, eo[2] = {0, nelems2d}, so[2] = {0, nelems1d};
+    return create_impl(1, no, eo, so, nnodes, x, y, z, nelems2d, tri, mann, nelems1d, seg, seg_kind, seg_val, p);
+}
+
+adhk_sw2 *adhk_sw2_create_batch(int ninst, const int *node_off, const int *elem_off, const int *seg_off, int nnodes,
+                                const double *x, const double *y, const double *z, int nelems2d, const int *tri,
+                                const double *mann, int nelems1d, const int *seg, const int *seg_kind,
+                                const double *seg_val, const adhk_params *p)
+{
+    return create_impl(ninst, node_off, elem_off, seg_off, nnodes, x, y, z, nelems2d, tri, mann, nelems1d, seg,
+                       seg_kind, seg_val, p);
 }
 
 static int ensure_mixed(adhk_ctx *c)
@@ -626,12 +732,12 @@ int adhk_get_system(adhk_sw2 *c, double *vals, double *residual)
     return copy_out(c, residual, c->residual);
 }
 
-/* residual norms of get_residual_norms (newton_tools.c:303-373); l2 = sqrt(sum r^2) / ndofs as coded */
-static int norms(adhk_ctx *c, adhk_scalars *hs, int *nan)
+/* residual norms of get_residual_norms (newton_tools.c:303-373), one set per instance */
+static int norms(adhk_ctx *c, adhk_scalars *hs)
 {
     int f[4];
     LC(adhk_launch_norms(c));
-    HC(hipMemcpyAsync(hs, c->sc, sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
+    HC(hipMemcpyAsync(hs, c->sc, (size_t)c->ninst * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
     HC(hipMemcpyAsync(f, c->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HC(hipStreamSynchronize(c->stream));
     if (f[0]) {
@@ -639,22 +745,37 @@ static int norms(adhk_ctx *c, adhk_scalars *hs, int *nan)
                        "for the wet/dry kernel (internal error)");
         return -1;
     }
-    *nan = f[1];
     return 0;
 fail:
     return -1;
 }
 
-/* solve_linear_sys_bcgstab on the device; returns the iteration count */
-static int krylov(adhk_ctx *c, const adhk_solver *s, int mixed)
+/* upload the work mask (the instances that take part in the following launches) */
+static int set_work(adhk_ctx *c, const int *mask)
 {
-    adhk_scalars hs;
+    int same = 1;
+    for (int i = 0; i < c->ninst; i++)
+        if (c->h_work[i] != mask[i]) same = 0;
+    if (same) return 0;
+    memcpy(c->h_work, mask, (size_t)c->ninst * sizeof(int));
+    HC(hipMemcpyAsync(c->work, c->h_work, (size_t)c->ninst * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HC(hipStreamSynchronize(c->stream)); /* h_work is reused by the next call */
+    return 0;
+fail:
+    return -1;
+}
+
+/* solve_linear_sys_bcgstab on the device for the instances in the work mask; its[i] = iterations */
+static int krylov(adhk_ctx *c, const adhk_solver *s, int mixed, adhk_scalars *hs, int *its)
+{
+    const int K = c->ninst;
     double tol = mixed ? s->mixed_lin_tol : s->lin_tol;
-    int chunk = s->lin_check_every > 0 ? s->lin_check_every : 8;
+    int chunk = s->lin_check_every > 0 ? s->lin_check_every : 8, all_done = 1;
     LC(adhk_launch_krylov_init(c, mixed, tol, s->lin_max_it));
-    HC(hipMemcpyAsync(&hs, c->sc, sizeof hs, hipMemcpyDeviceToHost, c->stream));
+    HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
     HC(hipStreamSynchronize(c->stream));
-    if (!hs.done) {
+    for (int i = 0; i < K; i++) all_done &= hs[i].done;
+    if (!all_done) {
         /* iteration 1, then chunks; the A kernel of iteration k+1 completes iteration k and tests it */
         int cur = 1, launched = 1;
         LC(adhk_launch_krylov_first(c, mixed));
@@ -666,121 +787,227 @@ static int krylov(adhk_ctx *c, const adhk_solver *s, int mixed)
                 cur = 1 - cur;
             }
             launched += k;
-            HC(hipMemcpyAsync(&hs, c->sc, sizeof hs, hipMemcpyDeviceToHost, c->stream));
+            HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
             HC(hipStreamSynchronize(c->stream));
-            if (hs.done) break;
+            all_done = 1;
+            for (int i = 0; i < K; i++) all_done &= hs[i].done;
+            if (all_done) break;
         }
-        if (!hs.done) LC(adhk_launch_krylov_last(c, mixed, cur));
-        HC(hipMemcpyAsync(&hs, c->sc, sizeof hs, hipMemcpyDeviceToHost, c->stream));
+        if (!all_done) LC(adhk_launch_krylov_last(c, mixed, cur));
+        HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
     }
     LC(adhk_launch_krylov_finish(c, mixed));
     HC(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < K; i++) its[i] = c->h_work[i] ? hs[i].it : 0;
     c->timing.krylov_calls++;
-    return hs.it;
+    return 0;
 fail:
     return -1;
 }
 
-/* fe_newton, fe_newton.c:44-926 (non-PETSc branch) */
+/* fe_newton, fe_newton.c:44-926 (non-PETSc branch), for every instance of the context at once.  Each
+ * instance follows its own Newton sequence -- its own norms, line search, stopping test and iteration
+ * count -- exactly as when it is solved alone: launches cover all instances, and the work mask keeps
+ * the updates of an instance that has stopped (or is not in a line search) from being applied.
+ * Returns TRUE(1) when every instance converged, FALSE(0) otherwise. */
 int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
 {
-    adhk_newton_stats loc;
-    adhk_scalars hs;
-    int nan = 0, it = 0, solv_flag = 1, keep;
-    double rmax, rl2, incmax, old_rl2;
-    const int mixed = (s->precision == ADHK_PREC_MIXED);
-    memset(&loc, 0, sizeof loc);
     if (!c || !s) return -1;
+    const int K = c->ninst;
+    const int mixed = (s->precision == ADHK_PREC_MIXED);
+    adhk_scalars *hs = NULL, *hw = NULL;
+    int *act = NULL, *ls = NULL, *solv = NULL, *its = NULL, *lits = NULL, *cuts = NULL;
+    double *rmax = NULL, *rl2 = NULL, *incmax = NULL, *old_rl2 = NULL;
+    int all_ok = 1;
     if (mixed && ensure_mixed(c)) return -1;
+    hs = calloc((size_t)K, sizeof *hs);
+    hw = calloc((size_t)K, sizeof *hw);
+    act = calloc((size_t)K, sizeof(int));
+    ls = calloc((size_t)K, sizeof(int));
+    solv = calloc((size_t)K, sizeof(int));
+    its = calloc((size_t)K, sizeof(int));
+    lits = calloc((size_t)K, sizeof(int));
+    cuts = calloc((size_t)K, sizeof(int));
+    rmax = calloc((size_t)K, sizeof(double));
+    rl2 = calloc((size_t)K, sizeof(double));
+    incmax = calloc((size_t)K, sizeof(double));
+    old_rl2 = calloc((size_t)K, sizeof(double));
+    if (!hs || !hw || !act || !ls || !solv || !its || !lits || !cuts || !rmax || !rl2 || !incmax || !old_rl2) {
+        adhk_set_error("adhk_fe_newton: out of host memory");
+        goto fail;
+    }
+    memset(c->inst_stats, 0, (size_t)K * sizeof(adhk_newton_stats));
+    for (int i = 0; i < K; i++) act[i] = solv[i] = 1;
+    LC(set_work(c, act));
     HC(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
     /* initialize_system + initialize_dirichlet_bc (newton_tools.c:21-53, 130-142) */
     LC(adhk_launch_init_system(c));
     LC(residual_dev(c));
-    LC(norms(c, &hs, &nan));
+    LC(norms(c, hs));
     if (c->warm) {
-        /* learned start: keep it only when its residual is smaller than the previous level's; the
-         * previous level's residual is kept (in the idle Krylov vector r) for the fallback */
-        adhk_scalars hw;
-        int nanw = 0;
+        /* learned start: an instance keeps it only when its residual is smaller than the previous
+         * level's; the previous level's residual is kept (in the idle Krylov vector r) for the fallback */
+        int any_reject = 0;
         LC(adhk_launch_copy(c, c->r, c->residual));
         LC(adhk_mlp_predict_into(c->warm, c));
         LC(adhk_launch_dirichlet_sol(c));
         LC(residual_dev(c));
-        LC(norms(c, &hw, &nanw));
-        if (!nanw && !nan && hw.rmax < hs.rmax) {
-            hs = hw;
-            loc.warm_accepted = 1;
-        } else {
-            HC(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
-            LC(adhk_launch_start_from_old(c));
-            LC(adhk_launch_copy(c, c->residual, c->r));
+        LC(norms(c, hw));
+        for (int i = 0; i < K; i++) {
+            ls[i] = !(!hw[i].nanflag && !hs[i].nanflag && hw[i].rmax < hs[i].rmax); /* reject mask */
+            if (!ls[i]) {
+                hs[i] = hw[i];
+                c->inst_stats[i].warm_accepted = 1;
+            }
+            any_reject |= ls[i];
+        }
+        if (any_reject) {
+            LC(set_work(c, ls));
+            LC(adhk_launch_start_from_old_masked(c));
+            LC(adhk_launch_copy_masked(c, c->residual, c->r));
+            LC(set_work(c, act));
         }
     }
-    rmax = hs.rmax;
-    rl2 = sqrt(hs.rl2sum) / c->ndofs;
-    incmax = hs.incmax;
-    if (nan) solv_flag = 0;
-    do {
+    for (int i = 0; i < K; i++) {
+        const double nd = 3.0 * (c->h_inst_node[i + 1] - c->h_inst_node[i]);
+        rmax[i] = hs[i].rmax;
+        rl2[i] = sqrt(hs[i].rl2sum) / nd;
+        incmax[i] = hs[i].incmax;
+        if (hs[i].nanflag) solv[i] = 0;
+    }
+    for (;;) {
+        int any = 0;
+        for (int i = 0; i < K; i++) any |= act[i];
+        if (!any) break;
+        LC(set_work(c, act));
         LC(jacobian_dev(c));
         LC(adhk_launch_check_diag(c));
         LC(adhk_launch_dirichlet(c, 1));
         LC(adhk_launch_scale_prec(c, mixed));
-        int lits = krylov(c, s, mixed);
-        if (lits < 0) goto fail;
-        loc.lin_its_total += lits;
-        if (lits > loc.lin_its_max) loc.lin_its_max = lits;
+        LC(krylov(c, s, mixed, hw, lits));
         LC(adhk_launch_increment(c));
-        old_rl2 = rl2;
+        for (int i = 0; i < K; i++)
+            if (act[i]) {
+                c->inst_stats[i].lin_its_total += lits[i];
+                if (lits[i] > c->inst_stats[i].lin_its_max) c->inst_stats[i].lin_its_max = lits[i];
+                old_rl2[i] = rl2[i];
+                cuts[i] = 0;
+            }
         LC(residual_dev(c));
-        LC(norms(c, &hs, &nan));
-        rmax = hs.rmax;
-        rl2 = sqrt(hs.rl2sum) / c->ndofs;
-        incmax = hs.incmax;
-        if (nan || isnan(rl2) || isinf(rl2)) solv_flag = 0;
-        int cuts = 0;
-        while (solv_flag && (rmax > s->tol_nonlin) && (rl2 > old_rl2) && (cuts < s->max_linesearch_cuts)) {
+        LC(norms(c, hs));
+        for (int i = 0; i < K; i++)
+            if (act[i]) {
+                const double nd = 3.0 * (c->h_inst_node[i + 1] - c->h_inst_node[i]);
+                rmax[i] = hs[i].rmax;
+                rl2[i] = sqrt(hs[i].rl2sum) / nd;
+                incmax[i] = hs[i].incmax;
+                if (hs[i].nanflag || isnan(rl2[i]) || isinf(rl2[i])) solv[i] = 0;
+            }
+        /* line search (fe_newton.c:681-741), per instance */
+        for (;;) {
+            int anyls = 0;
+            for (int i = 0; i < K; i++) {
+                ls[i] = act[i] && solv[i] && (rmax[i] > s->tol_nonlin) && (rl2[i] > old_rl2[i]) &&
+                        (cuts[i] < s->max_linesearch_cuts);
+                anyls |= ls[i];
+            }
+            if (!anyls) break;
+            LC(set_work(c, ls));
             LC(adhk_launch_linesearch(c));
             LC(residual_dev(c));
-            LC(norms(c, &hs, &nan));
-            rmax = hs.rmax;
-            rl2 = sqrt(hs.rl2sum) / c->ndofs;
-            incmax = hs.incmax;
-            if (nan) solv_flag = 0;
-            cuts++;
+            LC(norms(c, hs));
+            for (int i = 0; i < K; i++)
+                if (ls[i]) {
+                    const double nd = 3.0 * (c->h_inst_node[i + 1] - c->h_inst_node[i]);
+                    rmax[i] = hs[i].rmax;
+                    rl2[i] = sqrt(hs[i].rl2sum) / nd;
+                    incmax[i] = hs[i].incmax;
+                    if (hs[i].nanflag) solv[i] = 0;
+                    cuts[i]++;
+                    c->inst_stats[i].linesearch_cuts_total++;
+                }
         }
-        loc.linesearch_cuts_total += cuts;
-        it++;
-        keep = solv_flag && (it < s->max_nonlin_it) && (rmax < 1.0E+20) &&
-               (rmax > s->tol_nonlin && incmax > s->inc_nonlin);
-    } while (keep);
-    loc.newton_its = it;
-    loc.resid_max_norm = rmax;
-    loc.inc_max_norm = incmax;
-    loc.converged = 1;
-    if (!solv_flag || ((rmax > s->tol_nonlin) && (incmax > s->inc_nonlin)) || (rmax >= 1.0E+20)) {
-        c->p.dt *= 0.25; /* tc_scale, tc_scale.c:3-19 (DT_REDUCE_FACTOR, define.h:127) */
+        for (int i = 0; i < K; i++)
+            if (act[i]) {
+                its[i]++;
+                act[i] = solv[i] && (its[i] < s->max_nonlin_it) && (rmax[i] < 1.0E+20) &&
+                         (rmax[i] > s->tol_nonlin && incmax[i] > s->inc_nonlin);
+            }
+    }
+    for (int i = 0; i < K; i++) {
+        adhk_newton_stats *t = &c->inst_stats[i];
+        t->newton_its = its[i];
+        t->resid_max_norm = rmax[i];
+        t->inc_max_norm = incmax[i];
+        t->converged = !(!solv[i] || ((rmax[i] > s->tol_nonlin) && (incmax[i] > s->inc_nonlin)) ||
+                         (rmax[i] >= 1.0E+20));
+        all_ok &= t->converged;
+    }
+    if (!all_ok) {
+        /* tc_scale (tc_scale.c:3-19, DT_REDUCE_FACTOR define.h:127): cut dt and restart from the old level */
+        for (int i = 0; i < K; i++) act[i] = 1;
+        LC(set_work(c, act));
+        c->p.dt *= 0.25;
         LC(adhk_launch_init_system(c));
         HC(hipStreamSynchronize(c->stream));
-        loc.converged = 0;
     }
-    if (st) *st = loc;
-    return loc.converged;
+    if (st) {
+        adhk_newton_stats tot;
+        memset(&tot, 0, sizeof tot);
+        for (int i = 0; i < K; i++) {
+            const adhk_newton_stats *t = &c->inst_stats[i];
+            tot.newton_its += t->newton_its;
+            tot.linesearch_cuts_total += t->linesearch_cuts_total;
+            tot.lin_its_total += t->lin_its_total;
+            if (t->lin_its_max > tot.lin_its_max) tot.lin_its_max = t->lin_its_max;
+            if (t->resid_max_norm > tot.resid_max_norm) tot.resid_max_norm = t->resid_max_norm;
+            if (t->inc_max_norm > tot.inc_max_norm) tot.inc_max_norm = t->inc_max_norm;
+            tot.warm_accepted += t->warm_accepted;
+        }
+        tot.converged = all_ok;
+        *st = tot;
+    }
+    free(hs); free(hw); free(act); free(ls); free(solv); free(its); free(lits); free(cuts);
+    free(rmax); free(rl2); free(incmax); free(old_rl2);
+    return all_ok;
 fail:
-    if (st) *st = loc;
+    free(hs); free(hw); free(act); free(ls); free(solv); free(its); free(lits); free(cuts);
+    free(rmax); free(rl2); free(incmax); free(old_rl2);
     return -1;
 }
 
-/* time_loop / advance_time (time_loop.c:14-93): prep_sol then fe_newton per step */
+/* time_loop / advance_time (time_loop.c:14-93): prep_sol then fe_newton per step; per-instance
+ * totals accumulate in inst_stats (adhk_sw2_instance_stats) */
 int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_stats *acc)
 {
     adhk_newton_stats tot, st;
+    adhk_newton_stats *run = NULL;
     int ok = 1, k;
     memset(&tot, 0, sizeof tot);
     if (!c || !s) return -1000000000;
+    run = calloc((size_t)c->ninst, sizeof *run);
+    if (!run) return -1000000000;
     for (k = 0; k < nsteps && ok == 1; k++) {
-        if (adhk_launch_rotate(c)) return -1000000000;
+        if (adhk_launch_rotate(c)) {
+            free(run);
+            return -1000000000;
+        }
         ok = adhk_fe_newton(c, s, &st);
-        if (ok < 0) return -1000000000;
+        if (ok < 0) {
+            free(run);
+            return -1000000000;
+        }
+        for (int i = 0; i < c->ninst; i++) {
+            const adhk_newton_stats *t = &c->inst_stats[i];
+            run[i].newton_its += t->newton_its;
+            run[i].linesearch_cuts_total += t->linesearch_cuts_total;
+            run[i].lin_its_total += t->lin_its_total;
+            if (t->lin_its_max > run[i].lin_its_max) run[i].lin_its_max = t->lin_its_max;
+            run[i].resid_max_norm = t->resid_max_norm;
+            run[i].inc_max_norm = t->inc_max_norm;
+            run[i].warm_accepted += t->warm_accepted;
+            run[i].converged = t->converged;
+        }
         tot.newton_its += st.newton_its;
         tot.linesearch_cuts_total += st.linesearch_cuts_total;
         tot.lin_its_total += st.lin_its_total;
@@ -789,7 +1016,18 @@ int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_st
         tot.inc_max_norm = st.inc_max_norm;
         tot.warm_accepted += st.warm_accepted;
     }
+    memcpy(c->inst_stats, run, (size_t)c->ninst * sizeof *run);
+    free(run);
     tot.converged = ok;
     if (acc) *acc = tot;
     return ok == 1 ? k : -k;
+}
+
+int adhk_sw2_instances(const adhk_sw2 *c) { return c ? c->ninst : -1; }
+
+int adhk_sw2_instance_stats(const adhk_sw2 *c, adhk_newton_stats *out)
+{
+    if (!c || !out) return -1;
+    memcpy(out, c->inst_stats, (size_t)c->ninst * sizeof *out);
+    return 0;
 }

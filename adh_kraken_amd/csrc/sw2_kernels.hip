@@ -38,6 +38,7 @@ struct MeshView {
     const double *x, *y, *z, *djac, *grad, *ks;
     const int *tri, *wd;
     const double *sol, *sol_old, *sol_older;
+    const int *elem_inst, *work; /* elements of instances outside the work mask are skipped */
     int E;
 };
 
@@ -83,6 +84,8 @@ static MeshView make_view(const adhk_ctx *c)
     m.sol = c->sol;
     m.sol_old = c->sol_old;
     m.sol_older = c->sol_older;
+    m.elem_inst = c->elem_inst;
+    m.work = c->work;
     m.E = c->E;
     return m;
 }
@@ -128,7 +131,7 @@ __global__ __launch_bounds__(256) void k_elem_resid(MeshView m, Sw2Par P, double
                                                     int *wdq)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= m.E) return;
+    if (e >= m.E || !m.work[m.elem_inst[e]]) return;
     TriIn in;
     gather_tri(m, e, P.alpha != 0.0, in);
     if (tri_partial(in, P.alpha != 0.0)) {
@@ -165,13 +168,14 @@ __global__ __launch_bounds__(128) void k_elem_resid_wd(MeshView m, Sw2Par P, con
 struct SegView {
     const int *seg, *kind;
     const double *val, *sdjac, *nrml, *sol;
+    const int *seg_inst, *work;
     int S;
 };
 
 __global__ void k_seg_resid(SegView s, Sw2Par P, double *__restrict__ erhs)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= s.S) return;
+    if (e >= s.S || !s.work[s.seg_inst[e]]) return;
     double h[2], u[2], v[2];
     for (int a = 0; a < 2; a++) {
         int n = s.seg[2 * e + a];
@@ -194,6 +198,8 @@ static SegView make_segview(const adhk_ctx *c)
     s.sdjac = c->sdjac;
     s.nrml = c->snrml;
     s.sol = c->sol;
+    s.seg_inst = c->seg_inst;
+    s.work = c->work;
     s.S = c->S;
     return s;
 }
@@ -276,6 +282,7 @@ __global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, double *
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 9 * m.E) return;
     int e = t / 9, col = t % 9;
+    if (!m.work[m.elem_inst[e]]) return;
     int i = col / 3, k = col % 3;
     TriIn in;
     gather_tri(m, e, P.alpha != 0.0, in);
@@ -328,6 +335,7 @@ __global__ void k_seg_jac(SegView s, Sw2Par P, double *__restrict__ emat)
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 6 * s.S) return;
     int e = t / 6, col = t % 6;
+    if (!s.work[s.seg_inst[e]]) return;
     int i = col / 3, k = col % 3;
     double h[2], u[2], v[2];
     for (int a = 0; a < 2; a++) {

@@ -103,7 +103,9 @@ class SModelSuper:
     """One SW2 super model on one GPU (SMODEL_SUPER)."""
 
     def __init__(self, mesh: Mesh, dt=1.0, gravity=9.8, friction="manning", cf_const=0.002,
-                 tau_temporal=0.0, manning_units=1.0):
+                 tau_temporal=0.0, manning_units=1.0, instances=None):
+        """instances: None for one mesh, or (node_off, elem_off, seg_off) of an ensemble whose arrays are
+        concatenated in mesh (see SModelSuper.ensemble)."""
         self.lib = L.load()
         self.mesh = mesh
         self.gravity = gravity
@@ -117,10 +119,20 @@ class SModelSuper:
         kind = np.ascontiguousarray(m.seg_kind, np.int32)
         val = np.ascontiguousarray(m.seg_val, np.float64)
         mann = np.ascontiguousarray(mann, np.float64)
-        h = self.lib.adhk_sw2_create(m.nnodes, L.dptr(self._keep[0]), L.dptr(self._keep[1]), L.dptr(self._keep[2]),
-                                     m.nelems2d, L.iptr(tri), L.dptr(mann), m.nelems1d,
-                                     L.iptr(seg) if m.nelems1d else None, L.iptr(kind) if m.nelems1d else None,
-                                     L.dptr(val) if m.nelems1d else None, C.byref(p))
+        segargs = (L.iptr(seg) if m.nelems1d else None, L.iptr(kind) if m.nelems1d else None,
+                   L.dptr(val) if m.nelems1d else None, C.byref(p))
+        if instances is None:
+            h = self.lib.adhk_sw2_create(m.nnodes, L.dptr(self._keep[0]), L.dptr(self._keep[1]),
+                                         L.dptr(self._keep[2]), m.nelems2d, L.iptr(tri), L.dptr(mann), m.nelems1d,
+                                         *segargs)
+            self.offsets = (np.array([0, m.nnodes], np.int32), np.array([0, m.nelems2d], np.int32),
+                            np.array([0, m.nelems1d], np.int32))
+        else:
+            self.offsets = tuple(np.ascontiguousarray(o, np.int32) for o in instances)
+            no, eo, so = self.offsets
+            h = self.lib.adhk_sw2_create_batch(len(no) - 1, L.iptr(no), L.iptr(eo), L.iptr(so), m.nnodes,
+                                               L.dptr(self._keep[0]), L.dptr(self._keep[1]), L.dptr(self._keep[2]),
+                                               m.nelems2d, L.iptr(tri), L.dptr(mann), m.nelems1d, *segargs)
         if not h:
             raise L.AdhkError("adhk_sw2_create failed: " + L.last_error())
         self.h = C.c_void_p(h)
@@ -133,6 +145,37 @@ class SModelSuper:
         # Newton knobs (smodel_super_alloc_init.c:62-65) and linear solver knobs (bcgstab_solver.c:20-22)
         self.solver = L.solver_defaults()
         self.ts = TimeInfo(dt=dt, tau_temporal=tau_temporal)
+
+    @classmethod
+    def ensemble(cls, meshes, **kw):
+        """One context for several independent meshes (an ensemble): arrays concatenated, node ids offset."""
+        no, eo, so = [0], [0], [0]
+        for m in meshes:
+            no.append(no[-1] + m.nnodes)
+            eo.append(eo[-1] + m.nelems2d)
+            so.append(so[-1] + m.nelems1d)
+        cat = Mesh(np.concatenate([m.x for m in meshes]), np.concatenate([m.y for m in meshes]),
+                   np.concatenate([m.z for m in meshes]),
+                   np.concatenate([m.tri + o for m, o in zip(meshes, no)]).astype(np.int32),
+                   np.concatenate([m.seg + o for m, o in zip(meshes, no)]).astype(np.int32),
+                   np.concatenate([m.seg_kind for m in meshes]).astype(np.int32),
+                   np.concatenate([m.seg_val for m in meshes]),
+                   np.concatenate([m.mann if m.mann is not None else np.zeros(m.nelems2d) for m in meshes]))
+        return cls(cat, instances=(no, eo, so), **kw)
+
+    @property
+    def ninst(self):
+        return len(self.offsets[0]) - 1
+
+    def instance_slice(self, i):
+        """dof slice of instance i in the state vectors"""
+        no = self.offsets[0]
+        return slice(3 * int(no[i]), 3 * int(no[i + 1]))
+
+    def instance_stats(self):
+        st = (L.NewtonStats * self.ninst)()
+        L.check(self.lib.adhk_sw2_instance_stats(self.h, st), "adhk_sw2_instance_stats")
+        return [x.as_dict() for x in st]
 
     def close(self):
         if getattr(self, "h", None):
@@ -279,3 +322,34 @@ def time_loop(sm: SModelSuper, nsteps, precision=L.PREC_FP64):
         raise L.AdhkError("adhk_time_loop failed: " + L.last_error())
     sm.ts.dt = sm.get_dt()
     return rc, st.as_dict()
+
+
+def flume_ensemble(n=64, seed=4, tri_min=5_000, tri_max=200_000):
+    """Uneven flumes of the batched-ensemble workload, all from one seed.
+
+    Per instance: length uniform [200, 4000] m, width uniform [20, 200] m, triangle count log-uniform
+    [tri_min, tri_max] (node spacing sqrt(2 L W / T)), Manning n uniform [0.02, 0.05], inflow uniform
+    [1, 50] m^3/s per 10 m of width, tailwater depth uniform [0.5, 3] m, bed slope 1e-4 plus seeded bumps
+    (at most 0.3 m, capped so the initial flat water surface leaves at least 0.1 m of depth).
+    Returns [(mesh, sol0, spec)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        Lx = rng.uniform(200.0, 4000.0)
+        W = rng.uniform(20.0, 200.0)
+        T = float(np.exp(rng.uniform(np.log(tri_min), np.log(tri_max))))
+        h = np.sqrt(2.0 * Lx * W / T)
+        npx = max(3, int(round(Lx / h)) + 1)
+        npy = max(3, int(round(W / h)) + 1)
+        mann = rng.uniform(0.02, 0.05)
+        q10 = rng.uniform(1.0, 50.0)
+        tail = rng.uniform(0.5, 3.0)
+        slope = 1e-4
+        amp = float(np.clip(tail - slope * Lx - 0.1, 0.0, 0.3))
+        spec = FlumeSpec(xmin=0.0, xmax=Lx, ymin=0.0, ymax=W, npx=npx, npy=npy, slope=slope,
+                         nbumps=8 if amp > 0 else 0, bump_amp_max=amp, bump_sigma_min=5.0, bump_sigma_max=30.0,
+                         seed=int(rng.integers(1, 2**62)), manning_n=mann, inflow_q=q10 * W / 10.0, tailwater=tail,
+                         ic_flat_wse=1)
+        mesh, sol0 = flume_mesh(spec)
+        out.append((mesh, sol0, spec))
+    return out

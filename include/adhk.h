@@ -102,6 +102,19 @@ void adhk_solver_defaults(adhk_solver *s);
 adhk_sw2 *adhk_sw2_create(int nnodes, const double *x, const double *y, const double *z, int nelems2d,
                           const int *tri, const double *mann, int nelems1d, const int *seg, const int *seg_kind,
                           const double *seg_val, const adhk_params *p);
+/* An ensemble of independent meshes (instances) in one context: arrays are the instances' arrays
+ * concatenated, with node ids already offset; instance i owns nodes [node_off[i], node_off[i+1]),
+ * triangles [elem_off[i], elem_off[i+1]) and segments [seg_off[i], seg_off[i+1]) (offsets [ninst+1],
+ * starting at 0).  Every launch covers all instances; each instance keeps its own BiCGStab scalars,
+ * Newton norms, line search and stopping test, so its iterates are those of the instance solved alone.
+ * No counterpart in the reference (one SMODEL_SUPER per call); for many small meshes at once. */
+adhk_sw2 *adhk_sw2_create_batch(int ninst, const int *node_off, const int *elem_off, const int *seg_off, int nnodes,
+                                const double *x, const double *y, const double *z, int nelems2d, const int *tri,
+                                const double *mann, int nelems1d, const int *seg, const int *seg_kind,
+                                const double *seg_val, const adhk_params *p);
+int adhk_sw2_instances(const adhk_sw2 *c);
+/* per-instance statistics [ninst] of the last adhk_fe_newton (or totals of the last adhk_time_loop) */
+int adhk_sw2_instance_stats(const adhk_sw2 *c, adhk_newton_stats *out);
 void adhk_sw2_destroy(adhk_sw2 *c);
 int adhk_sw2_sizes(const adhk_sw2 *c, int *ndofs, int *nnz, int *nnz_blocks);
 /* SELL-64 layout of the device matrix: 64-node slices and 3x3-block slot columns */
@@ -149,7 +162,8 @@ int adhk_check_diag(adhk_sw2 *c);
 int adhk_apply_dirichlet_bc(adhk_sw2 *c);
 int adhk_get_system(adhk_sw2 *c, double *vals /* nnz or NULL */, double *residual /* ndofs or NULL */);
 
-/* replaces fe_newton(SMODEL_SUPER*) (src/newton/newton.h:15, fe_newton.c:44).  Returns TRUE(1)/FALSE(0), -1 on error. */
+/* replaces fe_newton(SMODEL_SUPER*) (src/newton/newton.h:15, fe_newton.c:44).  Returns TRUE(1)/FALSE(0), -1 on error.
+ * For an ensemble: TRUE when every instance converged; st holds sums over the instances. */
 int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st);
 
 /* replaces time_loop (src/timeloop/time_loop.c:14) for one super model: nsteps x (prep_sol + fe_newton).
