@@ -43,6 +43,13 @@ class Timing(C.Structure):
                 ("residual_calls", C.c_int), ("jacobian_calls", C.c_int), ("krylov_calls", C.c_int)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("chunk", C.c_int), ("graphs", C.c_int), ("xcd_map", C.c_int), ("elem_tpb", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class FlumeSpec(C.Structure):
     _fields_ = [("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double), ("ymax", C.c_double),
                 ("npx", C.c_int), ("npy", C.c_int), ("slope", C.c_double), ("nbumps", C.c_int),
@@ -65,6 +72,7 @@ SIGNATURES = [
     ("adhk_sw2_create_batch", P, [C.c_int, ip, ip, ip, C.c_int, dp, dp, dp, C.c_int, ip, dp, C.c_int, ip, ip, dp,
                                   C.POINTER(Params)]),
     ("adhk_sw2_instances", C.c_int, [P]),
+    ("adhk_sw2_instance_status", C.c_int, [P, ip, ip]),
     ("adhk_sw2_instance_stats", C.c_int, [P, C.POINTER(NewtonStats)]),
     ("adhk_sw2_destroy", None, [P]),
     ("adhk_sw2_sizes", C.c_int, [P, ip, ip, ip]),
@@ -80,6 +88,9 @@ SIGNATURES = [
     ("adhk_sw2_kernel_times", C.c_int, [P, dp, C.POINTER(C.c_longlong)]),
     ("adhk_copy_probe", C.c_int, [P, C.c_longlong, C.c_int, dp]),
     ("adhk_sw2_geometry", C.c_int, [P, dp, dp, dp, dp]),
+    ("adhk_sw2_get_tuning", C.c_int, [P, C.POINTER(Tuning)]),
+    ("adhk_sw2_set_tuning", C.c_int, [P, C.POINTER(Tuning)]),
+    ("adhk_sw2_autotune", C.c_int, [P, C.POINTER(Solver), C.c_int, C.POINTER(Tuning), dp]),
     ("adhk_sw2_init", C.c_int, [P, ip]),
     ("adhk_assemble_residual", C.c_int, [P, dp]),
     ("adhk_elem_residuals", C.c_int, [P, dp]),

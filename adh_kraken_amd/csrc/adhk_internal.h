@@ -100,6 +100,11 @@ struct adhk_sw2 {
     int *blk_node, *blk_inst, *inst_blk; /* device [nblk + 1], [nblk], [ninst + 1] */
     int *work, *h_work;                  /* [ninst]: 1 = the instance takes part in the next launches */
     adhk_newton_stats *inst_stats;       /* host [ninst]: per-instance statistics of the last solve */
+    int *alive, *steps_done;             /* host [ninst]: ensemble members still stepping, steps completed */
+    /* schedule choices (adhk_sw2_autotune); none of them changes a result */
+    adhk_tuning tune;
+    hipGraphExec_t gchunk[2]; /* captured BiCGStab chunk (FP64, mixed) and its length */
+    int gchunk_len[2];
     int *flags; /* [0] = unqueued partially dry element (internal guard), [1] = NaN seen */
     /* timing */
     adhk_timing timing;

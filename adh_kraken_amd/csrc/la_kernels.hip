@@ -68,8 +68,9 @@ static inline int grid_blk(int nblk)
 /* Block metadata and the scalars are passed as separate const __restrict__ kernel arguments: the
  * compiler then knows no store of the kernel can change them and reads them through the scalar
  * cache (one s_load per block instead of dependent vector loads in front of every workgroup). */
-#define BLK_ARGS const int *__restrict__ blk_node, const int *__restrict__ blk_inst, int nblk, int n_single
-#define BLK_PASS(c) (c)->blk_node, (c)->blk_inst, (c)->nblk, ((c)->ninst == 1 ? (c)->N : 0)
+#define BLK_ARGS                                                                                                \
+    const int *__restrict__ blk_node, const int *__restrict__ blk_inst, int nblk, int n_single, int xcd_map
+#define BLK_PASS(c) (c)->blk_node, (c)->blk_inst, (c)->nblk, ((c)->ninst == 1 ? (c)->N : 0), (c)->tune.xcd_map
 
 /* bounds and instance of block j; one mesh alone (n_single = its node count): block j is nodes
  * [256 j, 256 j + 256) of instance 0, with no loads */
@@ -87,10 +88,10 @@ __device__ __forceinline__ void blk_get(const int *__restrict__ blk_node, const 
         asm volatile("" ::"s"(nb0), "s"(nb1), "s"(inst)); /* keep the three loads ahead of the branch */
     }
 }
-__device__ __forceinline__ void blk_range(int nblk, int &j0, int &j1)
+__device__ __forceinline__ void blk_range(int nblk, int xcd_map, int &j0, int &j1)
 {
     const int G = gridDim.x, w = blockIdx.x;
-    const int lb = (w % NXCD) * (G / NXCD) + w / NXCD;
+    const int lb = xcd_map ? (w % NXCD) * (G / NXCD) + w / NXCD : w;
     const int pb = (nblk + G - 1) / G;
     j0 = lb * pb;
     j1 = min(nblk, j0 + pb);
@@ -297,7 +298,7 @@ __global__ __launch_bounds__(TPB) void k_norms(BLK_ARGS, const double *__restric
                                                double *partials)
 {
     int j0, j1;
-    blk_range(nblk, j0, j1);
+    blk_range(nblk, xcd_map, j0, j1);
     for (int j = j0; j < j1; j++) {
         double vmax[3] = {0.0, 0.0, 0.0}; /* max|r|, max|dsol|, nan indicator */
         double vsum[1] = {0.0};
@@ -548,7 +549,7 @@ __global__ __launch_bounds__(TPB) void k_kinit2(BLK_ARGS, Sell S, const T *__res
                                                 T *__restrict__ q, double *partials)
 {
     int j0, j1;
-    blk_range(nblk, j0, j1);
+    blk_range(nblk, xcd_map, j0, j1);
     for (int j = j0; j < j1; j++) {
         double vmax[2] = {0.0, 0.0}, vsum[1] = {0.0};
         int nb0, nb1, inst;
@@ -612,7 +613,7 @@ __global__ __launch_bounds__(TPB) void k_iter_a(BLK_ARGS, Sell S, const T *__res
                                                 const adhk_scalars *__restrict__ sc)
 {
     int j0, j1;
-    blk_range(nblk, j0, j1);
+    blk_range(nblk, xcd_map, j0, j1);
     for (int j = j0; j < j1; j++) {
         int nb0, nb1, inst;
         blk_get(blk_node, blk_inst, n_single, j, nb0, nb1, inst);
@@ -688,7 +689,7 @@ __global__ __launch_bounds__(TPB) void k_iter_s(BLK_ARGS, Sell S, const T *__res
                                                 const adhk_scalars *__restrict__ sc)
 {
     int j0, j1;
-    blk_range(nblk, j0, j1);
+    blk_range(nblk, xcd_map, j0, j1);
     for (int j = j0; j < j1; j++) {
         int nb0, nb1, inst;
         blk_get(blk_node, blk_inst, n_single, j, nb0, nb1, inst);

@@ -486,13 +486,14 @@ __global__ __launch_bounds__(256) void k_predict(int N, double *__restrict__ sol
                                                  const double *__restrict__ stm1, const int *__restrict__ brow,
                                                  const int *__restrict__ bcol, const float *__restrict__ nstat,
                                                  const unsigned short *__restrict__ wbf,
-                                                 const float *__restrict__ norm)
+                                                 const float *__restrict__ norm, const int *__restrict__ node_inst,
+                                                 const int *__restrict__ work)
 {
     __shared__ float W[NPARAM];
     load_weights(wbf, W);
     __syncthreads();
     int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+    if (n >= N || !work[node_inst[n]]) return;
     float f[NIN], y[NOUT];
     node_features(n, N, st, stm1, brow, bcol, nstat, f);
     #pragma unroll
@@ -875,7 +876,7 @@ extern "C" int adhk_mlp_predict_into(adhk_mlp *m, adhk_ctx *c)
     if (ensure_static(c)) return -1;
     HC(hipStreamSynchronize(m->stream));
     hipLaunchKernelGGL(k_predict, dim3((c->N + 255) / 256), dim3(256), 0, c->stream, c->N, c->sol, c->sol_old,
-                       c->sol_older, c->brow, c->bcol, c->nstat, m->wbf, m->norm);
+                       c->sol_older, c->brow, c->bcol, c->nstat, m->wbf, m->norm, c->node_inst, c->work);
     HC(hipGetLastError());
     return 0;
 fail:

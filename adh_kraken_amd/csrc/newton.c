@@ -10,11 +10,13 @@
  * once per chunk of iterations, and takes the reference's branch decisions
  * (line search, convergence, time-step cut) on them.
  */
+#define _POSIX_C_SOURCE 199309L /* clock_gettime */
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "adhk_internal.h"
 
@@ -82,6 +84,10 @@ static void ctx_free(adhk_ctx *c)
     free(c->h_inst_blk);
     free(c->h_work);
     free(c->inst_stats);
+    free(c->alive);
+    free(c->steps_done);
+    for (int i = 0; i < 2; i++)
+        if (c->gchunk[i]) hipGraphExecDestroy(c->gchunk[i]);
     for (int i = 0; i < 8; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
     for (int i = 0; i < ADHK_KT_RING; i++) {
@@ -148,6 +154,10 @@ static adhk_sw2 *create_impl(int ninst, const int *node_off, const int *elem_off
     c->ndofs = 3 * nnodes;
     c->p = *p;
     c->ninst = ninst;
+    c->tune.chunk = 0;
+    c->tune.graphs = 0;
+    c->tune.xcd_map = 1;
+    c->tune.elem_tpb = 256;
     HC(hipStreamCreate(&c->stream));
     for (int i = 0; i < 8; i++) HC(hipEventCreate(&c->ev[i]));
 
@@ -276,13 +286,15 @@ static adhk_sw2 *create_impl(int ninst, const int *node_off, const int *elem_off
         c->h_inst_blk = malloc((size_t)(ninst + 1) * sizeof(int));
         c->h_work = malloc((size_t)ninst * sizeof(int));
         c->inst_stats = calloc((size_t)ninst, sizeof(adhk_newton_stats));
+        c->alive = malloc((size_t)ninst * sizeof(int));
+        c->steps_done = calloc((size_t)ninst, sizeof(int));
         h_ni = malloc((size_t)N * sizeof(int));
         h_ei = malloc((size_t)E * sizeof(int));
         h_si = malloc((size_t)(S > 0 ? S : 1) * sizeof(int));
         h_bn = malloc((size_t)(nb + 1) * sizeof(int));
         h_bi = malloc((size_t)nb * sizeof(int));
         if (!c->h_inst_node || !c->h_inst_elem || !c->h_inst_seg || !c->h_inst_blk || !c->h_work ||
-            !c->inst_stats || !h_ni || !h_ei || !h_si || !h_bn || !h_bi) {
+            !c->inst_stats || !c->alive || !c->steps_done || !h_ni || !h_ei || !h_si || !h_bn || !h_bi) {
             adhk_set_error("adhk_sw2_create: out of host memory");
             goto fail;
         }
@@ -293,6 +305,7 @@ static adhk_sw2 *create_impl(int ninst, const int *node_off, const int *elem_off
             c->h_inst_seg[k] = seg_off[k];
             c->h_inst_blk[k] = b;
             c->h_work[k] = 1;
+            c->alive[k] = 1;
             for (int n = node_off[k]; n < node_off[k + 1]; n++) h_ni[n] = k;
             for (int e = elem_off[k]; e < elem_off[k + 1]; e++) h_ei[e] = k;
             for (int e = seg_off[k]; e < seg_off[k + 1]; e++) h_si[e] = k;
@@ -432,6 +445,10 @@ fail:
 int adhk_sw2_set_state(adhk_sw2 *c, const double *sol, const double *sol_old, const double *sol_older)
 {
     if (!c) return -1;
+    for (int i = 0; i < c->ninst; i++) {
+        c->alive[i] = 1;
+        c->steps_done[i] = 0;
+    }
     if (copy_in(c, c->sol, sol) || copy_in(c, c->sol_old, sol_old) || copy_in(c, c->sol_older, sol_older)) return -1;
     return 0;
 }
@@ -765,12 +782,52 @@ fail:
     return -1;
 }
 
+static void drop_graphs(adhk_ctx *c)
+{
+    for (int i = 0; i < 2; i++) {
+        if (c->gchunk[i]) (void)hipGraphExecDestroy(c->gchunk[i]);
+        c->gchunk[i] = NULL;
+        c->gchunk_len[i] = 0;
+    }
+}
+
+/* capture `chunk` BiCGStab iterations (direction buffer 1 at the start; chunk is even, so also at the
+ * end) as one graph; the kernels read every scalar and flag from device memory, so one capture serves
+ * every replay */
+static int capture_chunk(adhk_ctx *c, int mixed, int chunk)
+{
+    hipGraph_t g = NULL;
+    int cur = 1, rc = 0;
+    if (c->gchunk[mixed] && c->gchunk_len[mixed] == chunk) return 0;
+    if (c->gchunk[mixed]) (void)hipGraphExecDestroy(c->gchunk[mixed]);
+    c->gchunk[mixed] = NULL;
+    c->gchunk_len[mixed] = 0;
+    HC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < chunk && rc == 0; i++) {
+        rc = adhk_launch_krylov_iter(c, mixed, cur);
+        cur = 1 - cur;
+    }
+    {
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) goto fail;
+        HC(e);
+    }
+    HC(hipGraphInstantiate(&c->gchunk[mixed], g, NULL, NULL, 0));
+    HC(hipGraphDestroy(g));
+    c->gchunk_len[mixed] = chunk;
+    return 0;
+fail:
+    if (g) (void)hipGraphDestroy(g);
+    return -1;
+}
+
 /* solve_linear_sys_bcgstab on the device for the instances in the work mask; its[i] = iterations */
 static int krylov(adhk_ctx *c, const adhk_solver *s, int mixed, adhk_scalars *hs, int *its)
 {
     const int K = c->ninst;
     double tol = mixed ? s->mixed_lin_tol : s->lin_tol;
-    int chunk = s->lin_check_every > 0 ? s->lin_check_every : 8, all_done = 1;
+    int chunk = c->tune.chunk > 0 ? c->tune.chunk : (s->lin_check_every > 0 ? s->lin_check_every : 8), all_done = 1;
+    const int use_graph = c->tune.graphs && !c->kt.on && chunk % 2 == 0;
     LC(adhk_launch_krylov_init(c, mixed, tol, s->lin_max_it));
     HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
     HC(hipStreamSynchronize(c->stream));
@@ -782,9 +839,14 @@ static int krylov(adhk_ctx *c, const adhk_solver *s, int mixed, adhk_scalars *hs
         while (launched < s->lin_max_it) {
             int k = chunk;
             if (launched + k > s->lin_max_it) k = s->lin_max_it - launched;
-            for (int i = 0; i < k; i++) {
-                LC(adhk_launch_krylov_iter(c, mixed, cur));
-                cur = 1 - cur;
+            if (use_graph && k == chunk && cur == 1) {
+                LC(capture_chunk(c, mixed, chunk));
+                HC(hipGraphLaunch(c->gchunk[mixed], c->stream));
+            } else {
+                for (int i = 0; i < k; i++) {
+                    LC(adhk_launch_krylov_iter(c, mixed, cur));
+                    cur = 1 - cur;
+                }
             }
             launched += k;
             HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
@@ -802,6 +864,147 @@ static int krylov(adhk_ctx *c, const adhk_solver *s, int mixed, adhk_scalars *hs
     c->timing.krylov_calls++;
     return 0;
 fail:
+    return -1;
+}
+
+int adhk_sw2_get_tuning(const adhk_sw2 *c, adhk_tuning *t)
+{
+    if (!c || !t) return -1;
+    *t = c->tune;
+    return 0;
+}
+
+int adhk_sw2_set_tuning(adhk_sw2 *c, const adhk_tuning *t)
+{
+    if (!c || !t) return -1;
+    if (t->chunk < 0 || t->chunk % 2 || t->chunk > 100 || (t->graphs != 0 && t->graphs != 1) ||
+        (t->xcd_map != 0 && t->xcd_map != 1) || (t->elem_tpb != 64 && t->elem_tpb != 128 && t->elem_tpb != 256)) {
+        adhk_set_error("adhk_sw2_set_tuning: chunk even in [0, 100], graphs and xcd_map 0/1, elem_tpb 64/128/256");
+        return -1;
+    }
+    drop_graphs(c);
+    c->tune = *t;
+    return 0;
+}
+
+static double now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+/* Schedule autotuner.  Builds the Newton system of the current state (residual, FD Jacobian,
+ * Dirichlet rows, scaling, preconditioner -- work arrays only: sol, sol_old and bc_mask are not
+ * touched) and times, on it: the elemental Jacobian kernel at each workgroup size, then complete
+ * BiCGStab solves under every (XCD map, chunk, graph) schedule.  Each schedule must reproduce the
+ * default's iteration counts (the schedules do not change arithmetic); the fastest is kept. */
+int adhk_sw2_autotune(adhk_sw2 *c, const adhk_solver *s, int ninner, adhk_tuning *chosen, double *ms)
+{
+    static const int tpbs[3] = {256, 128, 64};
+    static const int chunks[3] = {8, 4, 16};
+    const int K = c ? c->ninst : 0;
+    adhk_tuning base, best;
+    adhk_scalars *hs = NULL;
+    int *its = NULL, *its_ref = NULL, *all = NULL, saved_kt;
+    hipEvent_t e0 = NULL, e1 = NULL;
+    double best_e = 1e300, best_k = 1e300, base_k = 0.0;
+    if (!c || !s || ninner < 1) return -1;
+    const int mixed = (s->precision == ADHK_PREC_MIXED);
+    if (mixed && ensure_mixed(c)) return -1;
+    saved_kt = c->kt.on;
+    c->kt.on = 0;
+    base = c->tune;
+    best = base;
+    hs = calloc((size_t)K, sizeof *hs);
+    its = calloc((size_t)K, sizeof(int));
+    its_ref = calloc((size_t)K, sizeof(int));
+    all = calloc((size_t)K, sizeof(int));
+    if (!hs || !its || !its_ref || !all) goto fail;
+    for (int i = 0; i < K; i++) all[i] = 1;
+    LC(set_work(c, all));
+    HC(hipEventCreate(&e0));
+    HC(hipEventCreate(&e1));
+    HC(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
+    LC(residual_dev(c));
+    /* 1. elemental Jacobian: workgroup size */
+    for (int t = 0; t < 3; t++) {
+        float f = 0.f;
+        c->tune.elem_tpb = tpbs[t];
+        LC(adhk_launch_elem_jac(c));
+        HC(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < 3; r++) LC(adhk_launch_elem_jac(c));
+        HC(hipEventRecord(e1, c->stream));
+        HC(hipEventSynchronize(e1));
+        HC(hipEventElapsedTime(&f, e0, e1));
+        if (f / 3.0 < best_e * 0.98) { /* a change must win by 2% */
+            best_e = f / 3.0;
+            best.elem_tpb = tpbs[t];
+        }
+    }
+    c->tune.elem_tpb = best.elem_tpb;
+    /* 2. the Newton system of the current state */
+    LC(jacobian_dev(c));
+    LC(adhk_launch_dirichlet(c, 1));
+    LC(adhk_launch_scale_prec(c, mixed));
+    /* 3. BiCGStab schedules: complete solves, best of ninner */
+    for (int xm = 1; xm >= 0; xm--)
+        for (int ci = 0; ci < 3; ci++)
+            for (int g = 0; g <= 1; g++) {
+                double tbest = 1e300;
+                c->tune.xcd_map = xm;
+                c->tune.chunk = chunks[ci];
+                c->tune.graphs = g;
+                for (int r = 0; r <= ninner; r++) { /* the first solve is a warm-up (and the capture) */
+                    double t0 = now_ms();
+                    LC(krylov(c, s, mixed, hs, its));
+                    double t1 = now_ms();
+                    if (r > 0 && t1 - t0 < tbest) tbest = t1 - t0;
+                }
+                if (xm == 1 && ci == 0 && g == 0) {
+                    memcpy(its_ref, its, (size_t)K * sizeof(int));
+                    base_k = tbest;
+                } else {
+                    for (int i = 0; i < K; i++)
+                        if (its[i] != its_ref[i]) {
+                            adhk_set_error("adhk_sw2_autotune: schedule changed an iteration count (internal error)");
+                            goto fail;
+                        }
+                }
+                if (tbest < best_k * 0.98) {
+                    best_k = tbest;
+                    best.xcd_map = xm;
+                    best.chunk = chunks[ci];
+                    best.graphs = g;
+                }
+            }
+    drop_graphs(c);
+    c->tune = best;
+    c->kt.on = saved_kt;
+    if (chosen) *chosen = best;
+    if (ms) {
+        ms[0] = best_e;
+        ms[1] = best_k;
+        ms[2] = base_k;
+        ms[3] = (double)its_ref[0];
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    free(hs);
+    free(its);
+    free(its_ref);
+    free(all);
+    return 0;
+fail:
+    drop_graphs(c);
+    c->tune = base;
+    c->kt.on = saved_kt;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    free(hs);
+    free(its);
+    free(its_ref);
+    free(all);
     return -1;
 }
 
@@ -837,7 +1040,10 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
         goto fail;
     }
     memset(c->inst_stats, 0, (size_t)K * sizeof(adhk_newton_stats));
-    for (int i = 0; i < K; i++) act[i] = solv[i] = 1;
+    for (int i = 0; i < K; i++) {
+        act[i] = c->alive[i];
+        solv[i] = 1;
+    }
     LC(set_work(c, act));
     HC(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
     /* initialize_system + initialize_dirichlet_bc (newton_tools.c:21-53, 130-142) */
@@ -936,6 +1142,7 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
     }
     for (int i = 0; i < K; i++) {
         adhk_newton_stats *t = &c->inst_stats[i];
+        if (!c->alive[i]) continue;
         t->newton_its = its[i];
         t->resid_max_norm = rmax[i];
         t->inc_max_norm = incmax[i];
@@ -943,12 +1150,21 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
                          (rmax[i] >= 1.0E+20));
         all_ok &= t->converged;
     }
-    if (!all_ok) {
+    if (!all_ok && K == 1) {
         /* tc_scale (tc_scale.c:3-19, DT_REDUCE_FACTOR define.h:127): cut dt and restart from the old level */
-        for (int i = 0; i < K; i++) act[i] = 1;
+        act[0] = 1;
         LC(set_work(c, act));
         c->p.dt *= 0.25;
         LC(adhk_launch_init_system(c));
+        HC(hipStreamSynchronize(c->stream));
+    } else if (!all_ok) {
+        /* ensemble: a failed member restarts from its old level and leaves the following steps */
+        for (int i = 0; i < K; i++) {
+            ls[i] = c->alive[i] && !c->inst_stats[i].converged;
+            if (ls[i]) c->alive[i] = 0;
+        }
+        LC(set_work(c, ls));
+        LC(adhk_launch_start_from_old_masked(c));
         HC(hipStreamSynchronize(c->stream));
     }
     if (st) {
@@ -987,7 +1203,10 @@ int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_st
     if (!c || !s) return -1000000000;
     run = calloc((size_t)c->ninst, sizeof *run);
     if (!run) return -1000000000;
-    for (k = 0; k < nsteps && ok == 1; k++) {
+    for (k = 0; k < nsteps && (ok == 1 || c->ninst > 1); k++) {
+        int any = 0;
+        for (int i = 0; i < c->ninst; i++) any |= c->alive[i];
+        if (!any) break;
         if (adhk_launch_rotate(c)) {
             free(run);
             return -1000000000;
@@ -1006,7 +1225,8 @@ int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_st
             run[i].resid_max_norm = t->resid_max_norm;
             run[i].inc_max_norm = t->inc_max_norm;
             run[i].warm_accepted += t->warm_accepted;
-            run[i].converged = t->converged;
+            run[i].converged = c->alive[i] ? t->converged : 0;
+            if (c->alive[i] && t->converged) c->steps_done[i]++;
         }
         tot.newton_its += st.newton_its;
         tot.linesearch_cuts_total += st.linesearch_cuts_total;
@@ -1018,9 +1238,27 @@ int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_st
     }
     memcpy(c->inst_stats, run, (size_t)c->ninst * sizeof *run);
     free(run);
+    if (c->ninst > 1) { /* an ensemble completes when every member has */
+        ok = 1;
+        for (int i = 0; i < c->ninst; i++) ok &= c->alive[i];
+        if (!ok) {
+            int mn = nsteps;
+            for (int i = 0; i < c->ninst; i++)
+                if (c->steps_done[i] < mn) mn = c->steps_done[i];
+            k = mn;
+        }
+    }
     tot.converged = ok;
     if (acc) *acc = tot;
     return ok == 1 ? k : -k;
+}
+
+int adhk_sw2_instance_status(const adhk_sw2 *c, int *alive, int *steps_done)
+{
+    if (!c) return -1;
+    if (alive) memcpy(alive, c->alive, (size_t)c->ninst * sizeof(int));
+    if (steps_done) memcpy(steps_done, c->steps_done, (size_t)c->ninst * sizeof(int));
+    return 0;
 }
 
 int adhk_sw2_instances(const adhk_sw2 *c) { return c ? c->ninst : -1; }

@@ -217,7 +217,8 @@ extern "C" int adhk_launch_elem_resid(adhk_ctx *c)
     Sw2Par P = make_par(c);
     if (c->E > 0) {
         hipLaunchKernelGGL(k_wdq_reset, dim3(1), dim3(64), 0, c->stream, c->wdq);
-        hipLaunchKernelGGL(k_elem_resid, dim3((c->E + 255) / 256), dim3(256), 0, c->stream, make_view(c), P, c->erhs,
+        const int tpb = c->tune.elem_tpb;
+        hipLaunchKernelGGL(k_elem_resid, dim3((c->E + tpb - 1) / tpb), dim3(tpb), 0, c->stream, make_view(c), P, c->erhs,
                            c->flags, c->wdq);
         hipLaunchKernelGGL(k_elem_resid_wd, dim3(wd_grid(c->E)), dim3(128), 0, c->stream, make_view(c), P, c->wdq,
                            c->erhs);
@@ -363,7 +364,8 @@ extern "C" int adhk_launch_elem_jac(adhk_ctx *c)
     if (c->E > 0) {
         long nt = 9L * c->E;
         hipLaunchKernelGGL(k_wdq_reset, dim3(1), dim3(64), 0, c->stream, c->wdq);
-        hipLaunchKernelGGL(k_elem_jac, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, make_view(c), P,
+        const int tpb = c->tune.elem_tpb;
+        hipLaunchKernelGGL(k_elem_jac, dim3((unsigned)((nt + tpb - 1) / tpb)), dim3(tpb), 0, c->stream, make_view(c), P,
                            c->emat, c->flags, c->wdq);
         hipLaunchKernelGGL(k_elem_jac_wd, dim3(wd_grid(nt)), dim3(128), 0, c->stream, make_view(c), P, c->wdq,
                            c->emat);

@@ -172,6 +172,35 @@ class SModelSuper:
         no = self.offsets[0]
         return slice(3 * int(no[i]), 3 * int(no[i + 1]))
 
+    # ---- schedule choices (none changes a result) ----
+    @property
+    def tuning(self):
+        t = L.Tuning()
+        L.check(self.lib.adhk_sw2_get_tuning(self.h, C.byref(t)), "adhk_sw2_get_tuning")
+        return t.as_dict()
+
+    @tuning.setter
+    def tuning(self, d):
+        t = L.Tuning(**{**self.tuning, **d})
+        L.check(self.lib.adhk_sw2_set_tuning(self.h, C.byref(t)), "adhk_sw2_set_tuning")
+
+    def autotune(self, precision=L.PREC_FP64, ninner=2):
+        """Time every schedule on this model's own Newton system at the current state; keep the fastest.
+        Returns (choice, {elem_jac_ms, krylov_ms, default_krylov_ms, krylov_its})."""
+        self.solver.precision = precision
+        t = L.Tuning()
+        ms = np.zeros(4)
+        L.check(self.lib.adhk_sw2_autotune(self.h, C.byref(self.solver), ninner, C.byref(t), L.dptr(ms)),
+                "adhk_sw2_autotune")
+        return t.as_dict(), {"elem_jac_ms": ms[0], "krylov_ms": ms[1], "default_krylov_ms": ms[2],
+                             "krylov_its": int(ms[3])}
+
+    def instance_status(self):
+        """(alive [ninst], steps_done [ninst]) of an ensemble's members"""
+        a, d = np.zeros(self.ninst, np.int32), np.zeros(self.ninst, np.int32)
+        L.check(self.lib.adhk_sw2_instance_status(self.h, L.iptr(a), L.iptr(d)), "adhk_sw2_instance_status")
+        return a, d
+
     def instance_stats(self):
         st = (L.NewtonStats * self.ninst)()
         L.check(self.lib.adhk_sw2_instance_stats(self.h, st), "adhk_sw2_instance_stats")

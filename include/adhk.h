@@ -83,6 +83,16 @@ typedef struct {
     int residual_calls, jacobian_calls, krylov_calls;
 } adhk_timing;
 
+/* Schedule choices of a context.  None changes a computed value: the iterations, the reduction order
+ * and every operand are the same under all of them, only their launch and read-back pattern differs.
+ * Defaults: chunk 0 (use adhk_solver.lin_check_every), graphs 0, xcd_map 1, elem_tpb 256. */
+typedef struct {
+    int chunk;    /* BiCGStab iterations launched between "done" read-backs (even; 0 = the solver's) */
+    int graphs;   /* 1 = replay each chunk as a captured hipGraph (when per-kernel timers are off) */
+    int xcd_map;  /* 1 = XCD-contiguous block ranges for the matrix kernels, 0 = workgroup order */
+    int elem_tpb; /* workgroup size of the elemental residual / Jacobian kernels (64, 128 or 256) */
+} adhk_tuning;
+
 /* kernel classes of the per-kernel device timers */
 #define ADHK_KT_SPMV_P 0      /* BiCGStab: Ap = M^-1 A p (+ q.Ap)              */
 #define ADHK_KT_SPMV_S 1      /* BiCGStab: As = M^-1 A s (+ 3 dots)            */
@@ -113,6 +123,11 @@ adhk_sw2 *adhk_sw2_create_batch(int ninst, const int *node_off, const int *elem_
                                 const double *mann, int nelems1d, const int *seg, const int *seg_kind,
                                 const double *seg_val, const adhk_params *p);
 int adhk_sw2_instances(const adhk_sw2 *c);
+/* Ensemble members are independent: a member whose Newton solve fails is reset to its previous level
+ * (as fe_newton's failure path does, fe_newton.c:869-915) and drops out of the following steps while
+ * the others continue.  alive [ninst], steps_done [ninst] (either may be NULL).  adhk_sw2_set_state
+ * re-admits every member.  (A single mesh keeps the reference's behaviour: dt cut, time loop ends.) */
+int adhk_sw2_instance_status(const adhk_sw2 *c, int *alive, int *steps_done);
 /* per-instance statistics [ninst] of the last adhk_fe_newton (or totals of the last adhk_time_loop) */
 int adhk_sw2_instance_stats(const adhk_sw2 *c, adhk_newton_stats *out);
 void adhk_sw2_destroy(adhk_sw2 *c);
@@ -135,6 +150,15 @@ int adhk_sw2_kernel_times(adhk_sw2 *c, double *ms, long long *count);
  * average milliseconds per copy (the achievable HBM ceiling, and a counter-scale reference) */
 int adhk_copy_probe(adhk_sw2 *c, long long nbytes, int reps, double *ms_avg);
 int adhk_sw2_geometry(adhk_sw2 *c, double *djac, double *grad_shp, double *seg_djac, double *seg_nrml);
+
+/* schedule choices (see adhk_tuning); no counterpart in the reference */
+int adhk_sw2_get_tuning(const adhk_sw2 *c, adhk_tuning *t);
+int adhk_sw2_set_tuning(adhk_sw2 *c, const adhk_tuning *t);
+/* Autotuner: times each candidate schedule on this context's own system -- the Jacobian and Krylov
+ * solve of the current state, ninner BiCGStab iterations per candidate -- and keeps the fastest.
+ * The state is restored afterwards.  chosen (or NULL) receives the choice, ms[4] (or NULL) the best
+ * times of the element kernels and of one BiCGStab iteration. */
+int adhk_sw2_autotune(adhk_sw2 *c, const adhk_solver *s, int ninner, adhk_tuning *chosen, double *ms);
 
 /* replaces fe_sw2_init (src/models/sw2/fe_sw2_init.c:28) incl. fe_sw2_wdflag_legacy (sw2_wdflag_legacy.c:34) */
 int adhk_sw2_init(adhk_sw2 *c, int *wd_flag_out /* [nelems2d] or NULL */);

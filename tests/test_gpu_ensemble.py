@@ -52,6 +52,28 @@ def test_ensemble_reproduces_standalone_solves(gpu, precision):
     assert st["lin_its_total"] == sum(p["lin_its_total"] for p in per)
 
 
+def test_failed_member_drops_out_and_the_others_continue(gpu):
+    cases = uneven_meshes()[:3]
+    sols = [s.copy() for _, s in cases]
+    sols[1][:] = np.nan  # this member cannot converge
+    ens = A.SModelSuper.ensemble([m for m, _ in cases], dt=1.0)
+    ens.set_initial(np.concatenate(sols))
+    n, st = A.time_loop(ens, 4)
+    alive, done = ens.instance_status()
+    assert list(alive) == [1, 0, 1] and list(done) == [4, 0, 4]
+    assert n == 0  # the ensemble as a whole did not complete (its slowest member did no step)
+    sol = ens.sol
+    for i in (0, 2):
+        mesh, sol0 = cases[i]
+        sm = A.SModelSuper(mesh, dt=1.0)
+        sm.set_initial(sol0)
+        assert A.time_loop(sm, 4)[0] == 4
+        assert np.array_equal(sol[ens.instance_slice(i)], sm.sol), i
+    assert ens.get_dt() == 1.0  # no member's failure cuts the shared time step
+    ens.set_initial(np.concatenate([s for _, s in cases]))
+    assert list(ens.instance_status()[0]) == [1, 1, 1]
+
+
 def test_ensemble_of_the_batched_workload_converges(gpu):
     ens_cases = A.flume_ensemble(n=8, seed=4, tri_min=5_000, tri_max=20_000)
     ens = A.SModelSuper.ensemble([m for m, _, _ in ens_cases], dt=1.0)
