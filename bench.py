@@ -68,7 +68,7 @@ def cpu_baseline(mesh, sol0, dt, nsteps_sample=5):
                       f"{st['lin_its_total']} BiCGStab its)"}
 
 
-def run_variant(A, L, mesh, sol0, dt, precision, steps, warmup, check_every, warm=None):
+def run_variant(A, L, mesh, sol0, dt, precision, steps, warmup, check_every, warm=None, tune=True):
     lib = L.load()
     sm = A.SModelSuper(mesh, dt=dt)
     sm.solver.lin_check_every = check_every
@@ -76,6 +76,11 @@ def run_variant(A, L, mesh, sol0, dt, precision, steps, warmup, check_every, war
         from adh_kraken_amd import learned as LR
         LR.set_warm_start(sm, warm)
     sm.set_initial(sol0)
+    tuned = None
+    if tune:  # schedule choices timed on this model's own system (they never change a result)
+        tuned, _ = sm.autotune(precision=precision, ninner=2)
+        tuned["graphs"] = 0  # the timed region runs with per-kernel event timers, which replay eagerly
+        sm.tuning = tuned
     if warmup:
         A.time_loop(sm, warmup, precision=precision)
     sm.set_initial(sol0)
@@ -94,7 +99,48 @@ def run_variant(A, L, mesh, sol0, dt, precision, steps, warmup, check_every, war
         raise RuntimeError(f"time loop stopped after {nsteps} of {steps} steps: {st}")
     kt = {L.KT_NAMES[i]: {"ms_total": ms[i], "launches": int(cnt[i]),
                           "ms_avg": (ms[i] / cnt[i]) if cnt[i] else None} for i in range(8)}
-    return {"sm": sm, "elapsed": t1 - t0, "stats": st, "kt": kt, "sol": sol_end}
+    return {"sm": sm, "elapsed": t1 - t0, "stats": st, "kt": kt, "sol": sol_end, "tuning": tuned}
+
+
+def ensemble_run(A, L, steps):
+    """64 independent uneven flumes (length 200-4000 m, width 20-200 m, 5k-200k triangles, seeded
+    Manning n, inflow, tailwater, bumps), mixed precision, dt 1 s: one context with every member in
+    each launch, against the same members solved one after another."""
+    cases = A.flume_ensemble(n=64, seed=4)
+    e0 = np.concatenate([s for _, s, _ in cases])
+    ens = A.SModelSuper.ensemble([m for m, _, _ in cases], dt=1.0)
+    ens.set_initial(e0)
+    tuned, _ = ens.autotune(precision=L.PREC_MIXED, ninner=2)
+    ens.set_initial(e0)
+    A.time_loop(ens, 1, precision=L.PREC_MIXED)  # warm-up
+    ens.set_initial(e0)
+    _ = ens.sol
+    t0 = time.perf_counter()
+    _, st = A.time_loop(ens, steps, precision=L.PREC_MIXED)
+    sol_b = ens.sol
+    t_batch = time.perf_counter() - t0
+    alive, done = ens.instance_status()
+    t_seq, same = 0.0, 0
+    for i, (m, s0, _) in enumerate(cases):
+        sm = A.SModelSuper(m, dt=1.0)
+        sm.set_initial(s0)
+        A.time_loop(sm, 1, precision=L.PREC_MIXED)
+        sm.set_initial(s0)
+        _ = sm.sol
+        t0 = time.perf_counter()
+        A.time_loop(sm, steps, precision=L.PREC_MIXED)
+        s1 = sm.sol
+        t_seq += time.perf_counter() - t0
+        same += int(np.array_equal(s1, sol_b[ens.instance_slice(i)]))
+        sm.close()
+    return {"members": len(cases), "triangles": int(sum(m.nelems2d for m, _, _ in cases)),
+            "dofs": int(ens.ndofs), "steps": steps, "precision": "mixed",
+            "batched_ensemble_steps_per_s": round(steps / t_batch, 3),
+            "sequential_ensemble_steps_per_s": round(steps / t_seq, 3),
+            "batched_speedup": round(t_seq / t_batch, 2),
+            "members_completed": int(alive.sum()), "members_failed_at_step_1": int((done == 0).sum()),
+            "members_bit_identical_to_standalone": same, "schedule": tuned,
+            "newton_its": st["newton_its"], "bicgstab_its": st["lin_its_total"]}
 
 
 def main():
@@ -107,6 +153,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--train-steps", type=int, default=20, help="time steps per training trajectory")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--check-every", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -164,6 +212,7 @@ def main():
         "newton_its": plain["stats"]["newton_its"], "bicgstab_its": plain["stats"]["lin_its_total"],
         "final_resid_max": plain["stats"]["resid_max_norm"],
         "kernel_ms": {k: round(v["ms_total"], 3) for k, v in kt.items()},
+        "schedule": plain["tuning"],
         "roofline": roofline,
     }
     if not args.no_mixed:
@@ -201,6 +250,8 @@ def main():
                          "samples": mlp.size, "adam_steps": 2000, "batch": 16384,
                          "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-10:].mean()),
                          "seconds_incl_data_generation": round(t_train, 2)}}
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_run(A, L, args.ensemble_steps)
     if not args.no_cpu:
         cb = cpu_baseline(mesh, sol0, dt)
         cb["gpu_fp64_speedup"] = round(value / cb["value"], 1)
