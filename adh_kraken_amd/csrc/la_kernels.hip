@@ -235,7 +235,8 @@ __device__ __forceinline__ void slice_bounds(int n, const Sell &S, int &c0, int 
 /* y = (row n of the block matrix) x, over the slots [c0, c1) of n's slice.  Slots go in groups of eight
  * (a triangle-mesh row is rarely wider): the eight column indices first, then the eight neighbours'
  * values, so a group costs two memory latencies instead of eight dependent pairs; the products still
- * accumulate in slot order. */
+ * accumulate in slot order.  Every multiply-add here and in the dot products is an explicit fused
+ * multiply-add, so each kernel that uses them rounds identically whatever the compiler contracts. */
 template <typename T, typename Col>
 __device__ __forceinline__ void block_row_cs(int n, int c0, int c1, const Sell &S, const T *__restrict__ bv,
                                              const Col &col, double &y0, double &y1, double &y2)
@@ -256,15 +257,15 @@ __device__ __forceinline__ void block_row_cs(int n, int c0, int c1, const Sell &
             if (cs + u >= c1) break;
             const T *a = bv + (size_t)(cs + u) * 9 * 64 + lane;
             const double x0 = xv[u][0], x1 = xv[u][1], x2 = xv[u][2];
-            y0 += (double)a[0] * x0;
-            y0 += (double)a[64] * x1;
-            y0 += (double)a[128] * x2;
-            y1 += (double)a[192] * x0;
-            y1 += (double)a[256] * x1;
-            y1 += (double)a[320] * x2;
-            y2 += (double)a[384] * x0;
-            y2 += (double)a[448] * x1;
-            y2 += (double)a[512] * x2;
+            y0 = __fma_rn((double)a[0], x0, y0);
+            y0 = __fma_rn((double)a[64], x1, y0);
+            y0 = __fma_rn((double)a[128], x2, y0);
+            y1 = __fma_rn((double)a[192], x0, y1);
+            y1 = __fma_rn((double)a[256], x1, y1);
+            y1 = __fma_rn((double)a[320], x2, y1);
+            y2 = __fma_rn((double)a[384], x0, y2);
+            y2 = __fma_rn((double)a[448], x1, y2);
+            y2 = __fma_rn((double)a[512], x2, y2);
         }
     }
 }
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(TPB) void k_norms(BLK_ARGS, const double *__restric
                 if (bc[i] != 1) {
                     vmax[0] = fmax(vmax[0], fabs(r));
                     vmax[1] = fmax(vmax[1], fabs(dsol[i]));
-                    vsum[0] += r * r;
+                    vsum[0] = __fma_rn(r, r, vsum[0]);
                 }
             }
         block_partial<3, true>(vmax, partials + PR_A * nblk, nblk, j);
@@ -567,7 +568,7 @@ __global__ __launch_bounds__(TPB) void k_kinit2(BLK_ARGS, Sell S, const T *__res
                 q[i] = rr;
                 vmax[0] = fmax(vmax[0], fabs((double)rr));
                 vmax[1] = fmax(vmax[1], fabs(bb));
-                vsum[0] += (double)rr * (double)rr;
+                vsum[0] = __fma_rn((double)rr, (double)rr, vsum[0]);
             }
         }
         block_partial<2, true>(vmax, partials + PR_A * nblk, nblk, j);
@@ -651,9 +652,77 @@ __global__ __launch_bounds__(TPB) void k_iter_a(BLK_ARGS, Sell S, const T *__res
             Ap_new[3 * n] = a0;
             Ap_new[3 * n + 1] = a1;
             Ap_new[3 * n + 2] = a2;
-            vsum[0] += q0 * (double)a0 + q1 * (double)a1 + q2 * (double)a2;
+            vsum[0] = __fma_rn(q2, (double)a2, __fma_rn(q1, (double)a1, __fma_rn(q0, (double)a0, vsum[0])));
         }
         if (!FIRST) block_partial<1, true>(vmax, partials + PR_A * nblk, nblk, j);
+        block_partial<1, false>(vsum, partials + PR_B * nblk, nblk, j);
+    }
+}
+
+/* Split schedule of the A step: k_upd finishes iteration k and stores the new direction, then k_spmv_p
+ * multiplies it, gathering one vector per column instead of the four that k_iter_a recomputes p from.
+ * Same functions, same bits as k_iter_a. */
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_upd(BLK_ARGS, T *__restrict__ x, T *__restrict__ r, const T *__restrict__ s,
+                                             const T *__restrict__ As, const T *__restrict__ p_old,
+                                             const T *__restrict__ Ap_old, T *__restrict__ p_new, double *partials,
+                                             const adhk_scalars *__restrict__ sc)
+{
+    int j0, j1;
+    blk_range(nblk, xcd_map, j0, j1);
+    for (int j = j0; j < j1; j++) {
+        int nb0, nb1, inst;
+        blk_get(blk_node, blk_inst, n_single, j, nb0, nb1, inst);
+        const adhk_scalars *si = sc + inst;
+        if (si->done) continue;
+        const double alpha = si->alpha, omega = si->omega, beta = si->beta;
+        double vmax[1] = {0.0};
+        const int n = nb0 + threadIdx.x;
+        if (n < nb1) {
+            ColP<T> col{s, As, p_old, Ap_old, omega, beta};
+            for (int k = 0; k < 3; k++) {
+                const int i = 3 * n + k;
+                T rr, pp;
+                col.one(i, rr, pp);
+                x[i] = (T)__fma_rn(omega, (double)s[i], __fma_rn(alpha, (double)p_old[i], (double)x[i]));
+                r[i] = rr;
+                p_new[i] = pp;
+                vmax[0] = fmax(vmax[0], fabs((double)rr));
+            }
+        }
+        block_partial<1, true>(vmax, partials + PR_A * nblk, nblk, j);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_spmv_p(BLK_ARGS, Sell S, const T *__restrict__ bv, const T *__restrict__ p,
+                                                T *__restrict__ Ap, const T *__restrict__ q, double *partials,
+                                                const adhk_scalars *__restrict__ sc)
+{
+    int j0, j1;
+    blk_range(nblk, xcd_map, j0, j1);
+    for (int j = j0; j < j1; j++) {
+        int nb0, nb1, inst;
+        blk_get(blk_node, blk_inst, n_single, j, nb0, nb1, inst);
+        const adhk_scalars *si = sc + inst;
+        if (si->done) continue;
+        double vsum[1] = {0.0};
+        const int n = nb0 + threadIdx.x;
+        if (n < nb1) {
+            int c0, c1;
+            if (n_single)
+                slice_bounds<true>(n, S, c0, c1);
+            else
+                slice_bounds<false>(n, S, c0, c1);
+            const double q0 = (double)q[3 * n], q1 = (double)q[3 * n + 1], q2 = (double)q[3 * n + 2];
+            double y0, y1, y2;
+            block_row_cs<T>(n, c0, c1, S, bv, ColPlain<T>{p}, y0, y1, y2); /* bv holds M^-1 A */
+            T a0 = (T)y0, a1 = (T)y1, a2 = (T)y2;
+            Ap[3 * n] = a0;
+            Ap[3 * n + 1] = a1;
+            Ap[3 * n + 2] = a2;
+            vsum[0] = __fma_rn(q2, (double)a2, __fma_rn(q1, (double)a1, __fma_rn(q0, (double)a0, vsum[0])));
+        }
         block_partial<1, false>(vsum, partials + PR_B * nblk, nblk, j);
     }
 }
@@ -716,9 +785,9 @@ __global__ __launch_bounds__(TPB) void k_iter_s(BLK_ARGS, Sell S, const T *__res
                 T ai = (T)z[k];
                 s[i] = sv[k];
                 As[i] = ai;
-                v[0] += (double)ai * (double)sv[k];
-                v[1] += (double)ai * (double)ai;
-                v[2] += (double)ai * (double)qv[k];
+                v[0] = __fma_rn((double)ai, (double)sv[k], v[0]);
+                v[1] = __fma_rn((double)ai, (double)ai, v[1]);
+                v[2] = __fma_rn((double)ai, (double)qv[k], v[2]);
             }
         }
         block_partial<3, false>(v, partials + PR_A * nblk, nblk, j);
@@ -821,11 +890,25 @@ template <typename T, bool FIRST> static int launch_a(adhk_ctx *c, KV<T> k, int 
 {
     /* reads direction buffer cur, writes 1 - cur (FIRST writes buffer 1) */
     int nw = FIRST ? 1 : 1 - cur;
-    adhk_kt_begin(c, ADHK_KT_SPMV_P);
-    hipLaunchKernelGGL((k_iter_a<T, FIRST>), dim3(grid_blk(c->nblk)), dim3(TPB), 0, c->stream, BLK_PASS(c), sell_of(c),
-                       k.bv, k.x, k.r, k.s, k.As, k.p[cur], k.Ap[cur], k.p[nw], k.Ap[nw], k.q, c->partials, c->sc);
-    adhk_kt_end(c);
-    CHECK_LAUNCH();
+    if (!FIRST && c->tune.split) {
+        adhk_kt_begin(c, ADHK_KT_UPDATE);
+        hipLaunchKernelGGL(k_upd<T>, dim3(grid_blk(c->nblk)), dim3(TPB), 0, c->stream, BLK_PASS(c), k.x, k.r, k.s, k.As,
+                           k.p[cur], k.Ap[cur], k.p[nw], c->partials, c->sc);
+        adhk_kt_end(c);
+        CHECK_LAUNCH();
+        adhk_kt_begin(c, ADHK_KT_SPMV_P);
+        hipLaunchKernelGGL(k_spmv_p<T>, dim3(grid_blk(c->nblk)), dim3(TPB), 0, c->stream, BLK_PASS(c), sell_of(c),
+                           k.bv, k.p[nw], k.Ap[nw], k.q, c->partials, c->sc);
+        adhk_kt_end(c);
+        CHECK_LAUNCH();
+    } else {
+        adhk_kt_begin(c, ADHK_KT_SPMV_P);
+        hipLaunchKernelGGL((k_iter_a<T, FIRST>), dim3(grid_blk(c->nblk)), dim3(TPB), 0, c->stream, BLK_PASS(c),
+                           sell_of(c), k.bv, k.x, k.r, k.s, k.As, k.p[cur], k.Ap[cur], k.p[nw], k.Ap[nw], k.q,
+                           c->partials, c->sc);
+        adhk_kt_end(c);
+        CHECK_LAUNCH();
+    }
     hipLaunchKernelGGL(k_fin_a<FIRST>, dim3(c->ninst), dim3(TPB), 0, c->stream, c->nblk, c->inst_blk, c->partials,
                        c->sc);
     CHECK_LAUNCH();

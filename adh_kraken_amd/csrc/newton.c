@@ -158,6 +158,7 @@ static adhk_sw2 *create_impl(int ninst, const int *node_off, const int *elem_off
     c->tune.graphs = 0;
     c->tune.xcd_map = 1;
     c->tune.elem_tpb = 256;
+    c->tune.split = 0;
     HC(hipStreamCreate(&c->stream));
     for (int i = 0; i < 8; i++) HC(hipEventCreate(&c->ev[i]));
 
@@ -878,8 +879,10 @@ int adhk_sw2_set_tuning(adhk_sw2 *c, const adhk_tuning *t)
 {
     if (!c || !t) return -1;
     if (t->chunk < 0 || t->chunk % 2 || t->chunk > 100 || (t->graphs != 0 && t->graphs != 1) ||
-        (t->xcd_map != 0 && t->xcd_map != 1) || (t->elem_tpb != 64 && t->elem_tpb != 128 && t->elem_tpb != 256)) {
-        adhk_set_error("adhk_sw2_set_tuning: chunk even in [0, 100], graphs and xcd_map 0/1, elem_tpb 64/128/256");
+        (t->xcd_map != 0 && t->xcd_map != 1) || (t->elem_tpb != 64 && t->elem_tpb != 128 && t->elem_tpb != 256) ||
+        (t->split != 0 && t->split != 1)) {
+        adhk_set_error("adhk_sw2_set_tuning: chunk even in [0, 100], graphs, xcd_map and split 0/1, "
+                       "elem_tpb 64/128/256");
         return -1;
     }
     drop_graphs(c);
@@ -948,10 +951,12 @@ int adhk_sw2_autotune(adhk_sw2 *c, const adhk_solver *s, int ninner, adhk_tuning
     LC(adhk_launch_dirichlet(c, 1));
     LC(adhk_launch_scale_prec(c, mixed));
     /* 3. BiCGStab schedules: complete solves, best of ninner */
+    for (int sp = 0; sp <= 1; sp++)
     for (int xm = 1; xm >= 0; xm--)
         for (int ci = 0; ci < 3; ci++)
             for (int g = 0; g <= 1; g++) {
                 double tbest = 1e300;
+                c->tune.split = sp;
                 c->tune.xcd_map = xm;
                 c->tune.chunk = chunks[ci];
                 c->tune.graphs = g;
@@ -961,7 +966,7 @@ int adhk_sw2_autotune(adhk_sw2 *c, const adhk_solver *s, int ninner, adhk_tuning
                     double t1 = now_ms();
                     if (r > 0 && t1 - t0 < tbest) tbest = t1 - t0;
                 }
-                if (xm == 1 && ci == 0 && g == 0) {
+                if (sp == 0 && xm == 1 && ci == 0 && g == 0) {
                     memcpy(its_ref, its, (size_t)K * sizeof(int));
                     base_k = tbest;
                 } else {
@@ -973,6 +978,7 @@ int adhk_sw2_autotune(adhk_sw2 *c, const adhk_solver *s, int ninner, adhk_tuning
                 }
                 if (tbest < best_k * 0.98) {
                     best_k = tbest;
+                    best.split = sp;
                     best.xcd_map = xm;
                     best.chunk = chunks[ci];
                     best.graphs = g;

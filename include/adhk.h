@@ -85,18 +85,20 @@ typedef struct {
 
 /* Schedule choices of a context.  None changes a computed value: the iterations, the reduction order
  * and every operand are the same under all of them, only their launch and read-back pattern differs.
- * Defaults: chunk 0 (use adhk_solver.lin_check_every), graphs 0, xcd_map 1, elem_tpb 256. */
+ * Defaults: chunk 0 (use adhk_solver.lin_check_every), graphs 0, xcd_map 1, elem_tpb 256, split 0. */
 typedef struct {
     int chunk;    /* BiCGStab iterations launched between "done" read-backs (even; 0 = the solver's) */
     int graphs;   /* 1 = replay each chunk as a captured hipGraph (when per-kernel timers are off) */
     int xcd_map;  /* 1 = XCD-contiguous block ranges for the matrix kernels, 0 = workgroup order */
     int elem_tpb; /* workgroup size of the elemental residual / Jacobian kernels (64, 128 or 256) */
+    int split;    /* 1 = BiCGStab's A step as two kernels: store the new direction, then multiply it
+                     (one gathered vector per column instead of four) */
 } adhk_tuning;
 
 /* kernel classes of the per-kernel device timers */
 #define ADHK_KT_SPMV_P 0      /* BiCGStab: Ap = M^-1 A p (+ q.Ap)              */
 #define ADHK_KT_SPMV_S 1      /* BiCGStab: As = M^-1 A s (+ 3 dots)            */
-#define ADHK_KT_UPDATE 2      /* BiCGStab: x, r, p update (+ max|r|)           */
+#define ADHK_KT_UPDATE 2      /* BiCGStab: x, r, p update (+ max|r|), split A   */
 #define ADHK_KT_ELEM_JAC 3    /* elemental central-FD Jacobians                */
 #define ADHK_KT_JAC_GATHER 4  /* block assembly of the Jacobian                */
 #define ADHK_KT_ELEM_RESID 5  /* elemental residuals                           */

@@ -18,6 +18,8 @@ SCHEDULES = [
     {"graphs": 1, "chunk": 4, "xcd_map": 0},
     {"graphs": 0, "chunk": 16, "elem_tpb": 64},
     {"graphs": 1, "chunk": 16, "elem_tpb": 128},
+    {"split": 1},
+    {"split": 1, "graphs": 1, "chunk": 8, "xcd_map": 0},
 ]
 
 
@@ -36,7 +38,7 @@ def test_schedules_reproduce_the_default(gpu, precision):
     n0, st0, s0, _ = run(mesh, sol0, 6, precision)
     for t in SCHEDULES:
         n, st, s, sm = run(mesh, sol0, 6, precision, t)
-        assert sm.tuning["graphs"] == t["graphs"]
+        assert all(sm.tuning[k] == v for k, v in t.items())
         assert n == n0 and st["newton_its"] == st0["newton_its"] and st["lin_its_total"] == st0["lin_its_total"], t
         assert np.array_equal(s, s0), t
 
