@@ -33,7 +33,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(kind, N, nslot, ndofs, mixed):
+def algorithmic_bytes(kind, N, nslot, ndofs, mixed, split=False):
     """HBM bytes of one launch with every operand moved once.
 
     matrix: the left-preconditioned M^-1 A in SELL-64, nslot*64 slots x (9 values + 1 column index) + slice
@@ -42,6 +42,8 @@ def algorithmic_bytes(kind, N, nslot, ndofs, mixed):
     vb = 4 if mixed else 8
     nslice = (N + 63) // 64
     mat = nslot * 64 * (9 * vb + 4) + (nslice + 1) * 4
+    if kind == "spmv_p" and split:  # k_spmv_p: read p q, write Ap
+        return mat + 3 * ndofs * vb
     if kind == "spmv_p":  # k_iter_a: read x s As p Ap q, write x r p' Ap'
         return mat + 10 * ndofs * vb
     if kind == "spmv_s":  # k_iter_s: read r Ap q, write s As
@@ -174,14 +176,15 @@ def main():
     kt = plain["kt"]
     dom = max(("spmv_p", "spmv_s", "elem_jac"), key=lambda k: kt[k]["ms_total"])
     roof_kind = dom if dom != "elem_jac" else "spmv_p"
-    ab = algorithmic_bytes(roof_kind, sm.mesh.nnodes, sm.nslot, sm.ndofs, False)
+    split = bool(plain["tuning"] and plain["tuning"].get("split"))
+    ab = algorithmic_bytes(roof_kind, sm.mesh.nnodes, sm.nslot, sm.ndofs, False, split)
     avg_ms = kt[roof_kind]["ms_avg"]
     achieved = ab / (avg_ms / 1e3) / 1e9
     probe_ms = C.c_double()
     nb = 1 << 30
     L.check(L.load().adhk_copy_probe(sm.h, nb, 10, C.byref(probe_ms)), "copy_probe")
     copy_gbs = 2 * nb / (probe_ms.value / 1e3) / 1e9
-    kname = {"spmv_p": "k_iter_a", "spmv_s": "k_iter_s"}[roof_kind]
+    kname = {"spmv_p": "k_spmv_p" if split else "k_iter_a", "spmv_s": "k_iter_s"}[roof_kind]
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "sw2_newton_c2_traffic.json")
     if args.config == "c2" and os.path.exists(tpath):  # HBM bytes per launch from the rocprofv3 counter runs
