@@ -86,6 +86,7 @@ SIGNATURES = [
     ("adhk_sw2_get_dt", C.c_double, [P]),
     ("adhk_sw2_get_timing", C.c_int, [P, C.POINTER(Timing)]),
     ("adhk_sw2_kernel_timing", C.c_int, [P, C.c_int]),
+    ("adhk_sw2_kernel_timing_mask", C.c_int, [P, C.c_uint, C.c_int]),
     ("adhk_sw2_kernel_times", C.c_int, [P, dp, C.POINTER(C.c_longlong)]),
     ("adhk_copy_probe", C.c_int, [P, C.c_longlong, C.c_int, dp]),
     ("adhk_sw2_geometry", C.c_int, [P, dp, dp, dp, dp]),

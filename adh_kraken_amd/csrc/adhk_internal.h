@@ -47,10 +47,13 @@ typedef struct {
  *                        seg: CONTRIB_SEG_BASE + (s*2 + a)*2 + b                */
 #define ADHK_NPARTIAL_MAX 8192
 
-/* per-kernel device timers (hipEvent pairs around launches on the context stream) */
-#define ADHK_KT_RING 256
+/* per-kernel device timers: hipEvent pairs around launches on the context stream, kept in a ring
+ * that is drained from its oldest end, so that reading a pair back never waits on recent work */
+#define ADHK_KT_RING 4096
 typedef struct {
-    int on, n;
+    int on, head, n, pending, stride;
+    unsigned mask;                    /* classes timed: bit k for class k */
+    long long seen[ADHK_KT_NKINDS];   /* launches per class since enabled (1 in stride is timed) */
     int kind[ADHK_KT_RING];
     hipEvent_t a[ADHK_KT_RING], b[ADHK_KT_RING];
     double ms[ADHK_KT_NKINDS];

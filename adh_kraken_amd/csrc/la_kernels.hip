@@ -902,7 +902,8 @@ template <typename T, bool FIRST> static int launch_a(adhk_ctx *c, KV<T> k, int 
         adhk_kt_end(c);
         CHECK_LAUNCH();
     } else {
-        adhk_kt_begin(c, ADHK_KT_SPMV_P);
+        /* under the split schedule the class times k_spmv_p alone: its first iteration's k_iter_a is not timed */
+        if (!c->tune.split) adhk_kt_begin(c, ADHK_KT_SPMV_P);
         hipLaunchKernelGGL((k_iter_a<T, FIRST>), dim3(grid_blk(c->nblk)), dim3(TPB), 0, c->stream, BLK_PASS(c),
                            sell_of(c), k.bv, k.x, k.r, k.s, k.As, k.p[cur], k.Ap[cur], k.p[nw], k.Ap[nw], k.q,
                            c->partials, c->sc);

@@ -492,58 +492,78 @@ int adhk_sw2_get_timing(const adhk_sw2 *c, adhk_timing *t)
     return 0;
 }
 
-static int kt_flush(adhk_ctx *c)
+/* accumulate the oldest pair of the ring (waiting for it if it has not run yet) and drop it */
+static int kt_pop(adhk_ctx *c)
 {
     adhk_ktimer *k = &c->kt;
-    if (k->n == 0) return 0;
-    HC(hipEventSynchronize(k->b[k->n - 1]));
-    for (int i = 0; i < k->n; i++) {
-        float ms = 0.f;
-        HC(hipEventElapsedTime(&ms, k->a[i], k->b[i]));
-        k->ms[k->kind[i]] += ms;
-        k->cnt[k->kind[i]]++;
-    }
-    k->n = 0;
+    const int i = k->head;
+    float ms = 0.f;
+    k->head = (k->head + 1) % ADHK_KT_RING;
+    k->n--;
+    HC(hipEventSynchronize(k->b[i]));
+    HC(hipEventElapsedTime(&ms, k->a[i], k->b[i]));
+    k->ms[k->kind[i]] += ms;
+    k->cnt[k->kind[i]]++;
     return 0;
 fail:
-    k->n = 0;
     return -1;
+}
+
+static int kt_flush(adhk_ctx *c)
+{
+    int rc = 0;
+    while (c->kt.n > 0)
+        if (kt_pop(c)) rc = -1;
+    return rc;
 }
 
 void adhk_kt_begin(adhk_ctx *c, int kind)
 {
     adhk_ktimer *k = &c->kt;
-    if (!k->on) return;
-    if (k->n == ADHK_KT_RING) kt_flush(c);
-    k->kind[k->n] = kind;
-    (void)hipEventRecord(k->a[k->n], c->stream);
+    if (!k->on || !((k->mask >> kind) & 1u)) return;
+    if (k->seen[kind]++ % k->stride) return;
+    if (k->n == ADHK_KT_RING) (void)kt_pop(c); /* the oldest pair finished long ago */
+    const int i = (k->head + k->n) % ADHK_KT_RING;
+    k->kind[i] = kind;
+    (void)hipEventRecord(k->a[i], c->stream);
+    k->pending = 1;
 }
 
 void adhk_kt_end(adhk_ctx *c)
 {
     adhk_ktimer *k = &c->kt;
-    if (!k->on) return;
-    (void)hipEventRecord(k->b[k->n], c->stream);
+    if (!k->pending) return;
+    (void)hipEventRecord(k->b[(k->head + k->n) % ADHK_KT_RING], c->stream);
     k->n++;
+    k->pending = 0;
 }
 
-int adhk_sw2_kernel_timing(adhk_sw2 *c, int on)
+int adhk_sw2_kernel_timing_mask(adhk_sw2 *c, unsigned mask, int stride)
 {
-    if (!c) return -1;
-    if (on && !c->kt.a[0])
+    if (!c || stride < 1) return -1;
+    if (mask && !c->kt.a[0])
         for (int i = 0; i < ADHK_KT_RING; i++) {
             HC(hipEventCreate(&c->kt.a[i]));
             HC(hipEventCreate(&c->kt.b[i]));
         }
     if (kt_flush(c)) return -1;
-    if (on) {
+    if (mask) {
         memset(c->kt.ms, 0, sizeof c->kt.ms);
         memset(c->kt.cnt, 0, sizeof c->kt.cnt);
+        memset(c->kt.seen, 0, sizeof c->kt.seen);
     }
-    c->kt.on = on ? 1 : 0;
+    c->kt.on = mask ? 1 : 0;
+    c->kt.mask = mask;
+    c->kt.stride = stride;
+    c->kt.pending = 0;
     return 0;
 fail:
     return -1;
+}
+
+int adhk_sw2_kernel_timing(adhk_sw2 *c, int on)
+{
+    return adhk_sw2_kernel_timing_mask(c, on ? (1u << ADHK_KT_NKINDS) - 1u : 0u, 1);
 }
 
 int adhk_sw2_kernel_times(adhk_sw2 *c, double *ms, long long *count)
@@ -828,7 +848,8 @@ static int krylov(adhk_ctx *c, const adhk_solver *s, int mixed, adhk_scalars *hs
     const int K = c->ninst;
     double tol = mixed ? s->mixed_lin_tol : s->lin_tol;
     int chunk = c->tune.chunk > 0 ? c->tune.chunk : (s->lin_check_every > 0 ? s->lin_check_every : 8), all_done = 1;
-    const int use_graph = c->tune.graphs && !c->kt.on && chunk % 2 == 0;
+    const unsigned krylov_kt = (1u << ADHK_KT_SPMV_P) | (1u << ADHK_KT_SPMV_S) | (1u << ADHK_KT_UPDATE);
+    const int use_graph = c->tune.graphs && !(c->kt.on && (c->kt.mask & krylov_kt)) && chunk % 2 == 0;
     LC(adhk_launch_krylov_init(c, mixed, tol, s->lin_max_it));
     HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
     HC(hipStreamSynchronize(c->stream));

@@ -147,6 +147,10 @@ int adhk_sw2_get_timing(const adhk_sw2 *c, adhk_timing *t);
  * classes above, on the context's stream), on = 0 disables.  kernel_times flushes and returns the
  * accumulated milliseconds and launch counts per class ([ADHK_KT_NKINDS] each). */
 int adhk_sw2_kernel_timing(adhk_sw2 *c, int on);
+/* as kernel_timing, for the classes whose bits are set in mask (bit k = class k; 0 disables), timing
+ * the first launch of each class and then every stride-th one (stride >= 1; 1 times every launch).
+ * Each timed launch adds two event records to the stream; sampling keeps that cost off the rest. */
+int adhk_sw2_kernel_timing_mask(adhk_sw2 *c, unsigned mask, int stride);
 int adhk_sw2_kernel_times(adhk_sw2 *c, double *ms, long long *count);
 /* device copy of nbytes (read nbytes + write nbytes) repeated reps times on the context's stream:
  * average milliseconds per copy (the achievable HBM ceiling, and a counter-scale reference) */
