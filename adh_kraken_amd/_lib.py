@@ -59,6 +59,15 @@ class FlumeSpec(C.Structure):
                 ("tailwater", C.c_double), ("ic_flat_wse", C.c_int)]
 
 
+class ChannelSpec(C.Structure):
+    _fields_ = [("length", C.c_double), ("width", C.c_double), ("spacing", C.c_double),
+                ("chan_width", C.c_double), ("chan_depth", C.c_double), ("bank_width", C.c_double),
+                ("meander_amp", C.c_double), ("meander_wavelength", C.c_double), ("slope", C.c_double),
+                ("nbumps", C.c_int), ("bump_amp_max", C.c_double), ("bump_sigma_min", C.c_double),
+                ("bump_sigma_max", C.c_double), ("seed", C.c_ulonglong), ("n_channel", C.c_double),
+                ("n_floodplain", C.c_double), ("inflow_q", C.c_double), ("fp_depth_out", C.c_double)]
+
+
 P = C.c_void_p
 dp = C.POINTER(C.c_double)
 ip = C.POINTER(C.c_int)
@@ -118,6 +127,18 @@ SIGNATURES = [
     ("adhk_flume_counts", C.c_int, [C.POINTER(FlumeSpec), ip, ip, ip]),
     ("adhk_flume_build", C.c_int, [C.POINTER(FlumeSpec), dp, dp, dp, ip, dp, ip, ip, dp]),
     ("adhk_flume_initial_state", C.c_int, [C.POINTER(FlumeSpec), C.c_int, dp, dp]),
+    ("adhk_sw2_set_series", C.c_int, [P, C.c_int, C.c_int, dp, dp]),
+    ("adhk_sw2_bind_series", C.c_int, [P, C.c_int, ip, C.c_int, C.c_double]),
+    ("adhk_sw2_set_time", C.c_int, [P, C.c_double]),
+    ("adhk_sw2_get_time", C.c_double, [P]),
+    ("adhk_series_average", C.c_int, [C.c_int, dp, dp, C.c_double, C.c_double, dp]),
+    ("adhk_poisson_disk_bound", C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
+    ("adhk_poisson_disk", C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_ulonglong,
+                                    C.c_int, dp, dp]),
+    ("adhk_rcm_order", C.c_int, [C.c_int, C.c_int, ip, ip]),
+    ("adhk_channel_bed", C.c_int, [C.POINTER(ChannelSpec), C.c_int, dp, dp, dp]),
+    ("adhk_channel_roughness", C.c_int, [C.POINTER(ChannelSpec), C.c_int, dp, dp, C.c_int, ip, dp]),
+    ("adhk_channel_section_weight", C.c_double, [C.POINTER(ChannelSpec), C.c_double, C.c_double]),
     ("adhk_mlp_param_count", C.c_int, []),
     ("adhk_mlp_create", P, [C.c_ulonglong]),
     ("adhk_mlp_destroy", None, [P]),

@@ -109,12 +109,23 @@ struct adhk_sw2 {
     hipGraphExec_t gchunk[2]; /* captured BiCGStab chunk (FP64, mixed) and its length */
     int gchunk_len[2];
     int *flags; /* [0] = unqueued partially dry element (internal guard), [1] = NaN seen */
+    /* time-series boundary values (series.c): tables, segment bindings, time of the old level */
+    double t_prev;
+    int nseries;
+    struct adhk_series_tab *series;
+    int *h_seg_ser;                  /* host [S]: series driving the segment's first value, -1 = none */
+    double *h_seg_scale, *h_seg_val; /* host [S], [2S] */
     /* timing */
     adhk_timing timing;
     hipEvent_t ev[8];
     adhk_ktimer kt;
 };
 typedef struct adhk_sw2 adhk_ctx;
+typedef struct adhk_series_tab adhk_series_tab;
+
+/* ---- time-series boundary values (series.c) ---- */
+int adhk_series_apply(adhk_ctx *c, double t1, double t2);
+void adhk_series_free(adhk_ctx *c);
 
 /* ---- launchers (sw2_kernels.hip) ---- */
 int adhk_launch_wdflag(adhk_ctx *c);

@@ -86,6 +86,7 @@ static void ctx_free(adhk_ctx *c)
     free(c->inst_stats);
     free(c->alive);
     free(c->steps_done);
+    adhk_series_free(c);
     for (int i = 0; i < 2; i++)
         if (c->gchunk[i]) hipGraphExecDestroy(c->gchunk[i]);
     for (int i = 0; i < 8; i++)
@@ -1225,7 +1226,7 @@ int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_st
 {
     adhk_newton_stats tot, st;
     adhk_newton_stats *run = NULL;
-    int ok = 1, k;
+    int ok = 1, k, forcing_failed = 0;
     memset(&tot, 0, sizeof tot);
     if (!c || !s) return -1000000000;
     run = calloc((size_t)c->ninst, sizeof *run);
@@ -1234,6 +1235,14 @@ int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_st
         int any = 0;
         for (int i = 0; i < c->ninst; i++) any |= c->alive[i];
         if (!any) break;
+        /* forcing over the step (sseries_setall_ts_valuesAVG, time_loop.c:76); a step the series do
+         * not cover is not taken */
+        const double dt_step = c->p.dt;
+        if (adhk_series_apply(c, c->t_prev, c->t_prev + dt_step)) {
+            ok = 0;
+            forcing_failed = 1;
+            break;
+        }
         if (adhk_launch_rotate(c)) {
             free(run);
             return -1000000000;
@@ -1243,6 +1252,7 @@ int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_st
             free(run);
             return -1000000000;
         }
+        if (ok == 1 || c->ninst > 1) c->t_prev += dt_step;
         for (int i = 0; i < c->ninst; i++) {
             const adhk_newton_stats *t = &c->inst_stats[i];
             run[i].newton_its += t->newton_its;
@@ -1265,7 +1275,7 @@ int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_st
     }
     memcpy(c->inst_stats, run, (size_t)c->ninst * sizeof *run);
     free(run);
-    if (c->ninst > 1) { /* an ensemble completes when every member has */
+    if (c->ninst > 1 && !forcing_failed) { /* an ensemble completes when every member has */
         ok = 1;
         for (int i = 0; i < c->ninst; i++) ok &= c->alive[i];
         if (!ok) {

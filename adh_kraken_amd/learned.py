@@ -112,16 +112,20 @@ def collect_trajectory(sm: SModelSuper, mlp: WarmStartMLP, nsteps, precision=L.P
     return its
 
 
-def train_warm_start(cases, nsteps, steps=2000, batch=8192, lr=2e-3, seed=1, precision=L.PREC_FP64, dt=1.0):
-    """Train a model on solver trajectories: cases = [(mesh, sol0), ...], nsteps steps each.
+def train_warm_start(cases, nsteps, steps=2000, batch=8192, lr=2e-3, seed=1, precision=L.PREC_FP64, dt=1.0,
+                     prepare=None):
+    """Train a model on solver trajectories: cases = [(mesh, sol0), ...], nsteps steps each;
+    prepare(sm, i), when given, sets up case i's model (e.g. its boundary time series) before it runs.
 
     Returns (model, loss history, data-generation Newton iterations)."""
     mlp = WarmStartMLP(seed=seed)
     mlp.reserve(sum(m.nnodes for m, _ in cases) * nsteps)
     gen_its = 0
-    for mesh, sol0 in cases:
+    for i, (mesh, sol0) in enumerate(cases):
         sm = SModelSuper(mesh, dt=dt)
         sm.set_initial(sol0)
+        if prepare is not None:
+            prepare(sm, i)
         gen_its += collect_trajectory(sm, mlp, nsteps, precision)
         sm.close()
     mlp.fit_normalization()

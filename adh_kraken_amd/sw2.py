@@ -261,6 +261,27 @@ class SModelSuper:
     def get_dt(self):
         return self.lib.adhk_sw2_get_dt(self.h)
 
+    # time series (SSERIES, src/structs/sseries/sseries.c): boundary values that follow a hydrograph
+    def set_series(self, sid, t, v):
+        t = np.ascontiguousarray(t, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        L.check(self.lib.adhk_sw2_set_series(self.h, int(sid), t.size, L.dptr(t), L.dptr(v)), "adhk_sw2_set_series")
+
+    def bind_series(self, segs, sid, scale=1.0):
+        """Segments whose first value becomes scale * (series sid averaged over each step); sid -1 unbinds."""
+        segs = np.ascontiguousarray(segs, np.int32)
+        L.check(self.lib.adhk_sw2_bind_series(self.h, segs.size, L.iptr(segs), int(sid), float(scale)),
+                "adhk_sw2_bind_series")
+
+    @property
+    def time(self):
+        """Time of the old level (t_prev): 0 at creation, + dt per completed step."""
+        return self.lib.adhk_sw2_get_time(self.h)
+
+    @time.setter
+    def time(self, t):
+        L.check(self.lib.adhk_sw2_set_time(self.h, float(t)), "adhk_sw2_set_time")
+
     def set_initial(self, sol):
         """sol = sol_old = sol_older = the given state (a cold start)."""
         for w in range(3):
@@ -351,6 +372,16 @@ def time_loop(sm: SModelSuper, nsteps, precision=L.PREC_FP64):
         raise L.AdhkError("adhk_time_loop failed: " + L.last_error())
     sm.ts.dt = sm.get_dt()
     return rc, st.as_dict()
+
+
+def series_average(t, v, t1, t2):
+    """Average over [t1, t2] of the piecewise-linear series (t, v), as the time loop evaluates it."""
+    t = np.ascontiguousarray(t, np.float64)
+    v = np.ascontiguousarray(v, np.float64)
+    out = C.c_double()
+    L.check(L.load().adhk_series_average(t.size, L.dptr(t), L.dptr(v), float(t1), float(t2), C.byref(out)),
+            "adhk_series_average")
+    return out.value
 
 
 def flume_ensemble(n=64, seed=4, tri_min=5_000, tri_max=200_000):

@@ -32,6 +32,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 KT_STRIDE = 16
+T_START = time.perf_counter()
+
+
+def log(msg):
+    """Progress on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench {time.perf_counter() - T_START:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def algorithmic_bytes(kind, N, nslot, ndofs, mixed, split=False):
@@ -52,7 +58,7 @@ def algorithmic_bytes(kind, N, nslot, ndofs, mixed, split=False):
     raise ValueError(kind)
 
 
-def cpu_baseline(mesh, sol0, dt, nsteps_sample=5):
+def cpu_baseline(mesh, sol0, dt, nsteps_sample=5, seg_val=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the checker doubles as the CPU baseline ("port")
     try:
@@ -62,24 +68,29 @@ def cpu_baseline(mesh, sol0, dt, nsteps_sample=5):
     threads = min(threads, 16)
     orc = O.Oracle(mesh, dt=dt, threads=threads)
     orc.set_initial(sol0)
+    if seg_val is not None:  # boundary values of the first step of a time-series forcing
+        orc.set_seg_values(seg_val)
     t0 = time.perf_counter()
     n, st = orc.time_loop(nsteps_sample)
     t1 = time.perf_counter()
     return {"value": round(nsteps_sample / (t1 - t0), 5), "unit": "timesteps/s", "cores": threads, "kind": "port",
-            "sample": f"{nsteps_sample} time steps of the same 1.2M-dof workload from the same initial state, "
+            "sample": f"{nsteps_sample} time steps of the same {3 * mesh.nnodes}-dof workload from the same initial state, "
                       f"OpenMP C oracle ({t1 - t0:.1f} s, {st['newton_its']} Newton its, "
                       f"{st['lin_its_total']} BiCGStab its)"}
 
 
-def run_variant(A, L, mesh, sol0, dt, precision, steps, warmup, check_every, warm=None, schedule=None):
+def run_variant(A, L, mesh, sol0, dt, precision, steps, warmup, check_every, warm=None, schedule=None, prepare=None):
     lib = L.load()
     sm = A.SModelSuper(mesh, dt=dt)
     sm.solver.lin_check_every = check_every
+    if prepare is not None:  # e.g. the inflow hydrograph
+        prepare(sm)
     if warm is not None:
         from adh_kraken_amd import learned as LR
         LR.set_warm_start(sm, warm)
     sm.set_initial(sol0)
     if schedule is None:  # schedule choices timed on this model's own system (they never change a result)
+        log(f"autotune (precision {precision})")
         tuned, _ = sm.autotune(precision=precision, ninner=2)
     else:
         tuned = {**sm.tuning, **schedule}
@@ -89,10 +100,12 @@ def run_variant(A, L, mesh, sol0, dt, precision, steps, warmup, check_every, war
         A.time_loop(sm, warmup, precision=precision)
     sm.set_initial(sol0)
     sm.set_dt(dt)
+    sm.time = 0.0
     # event pairs around the first and then every KT_STRIDE-th launch of each kernel class: timing every
     # launch would add two stream records per kernel (about 16% of the step on this workload)
     L.check(lib.adhk_sw2_kernel_timing_mask(sm.h, 0xFF, KT_STRIDE), "kernel_timing")
     _ = sm.sol  # the state read-back waits for the library's stream
+    log(f"timed run: {steps} steps, precision {precision}{', learned warm start' if warm is not None else ''}")
     t0 = time.perf_counter()
     nsteps, st = A.time_loop(sm, steps, precision=precision)
     sol_end = sm.sol
@@ -126,6 +139,7 @@ def ensemble_run(A, L, steps):
     sol_b = ens.sol
     t_batch = time.perf_counter() - t0
     alive, done = ens.instance_status()
+    log("ensemble: the same members one after another")
     t_seq, same = 0.0, 0
     for i, (m, s0, _) in enumerate(cases):
         sm = A.SModelSuper(m, dt=1.0)
@@ -152,13 +166,18 @@ def ensemble_run(A, L, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 50; c1 100, c3 20, c5 200)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", choices=["c2", "c1"], default="c2")
+    ap.add_argument("--config", choices=["c2", "c1", "c3", "c5"], default="c2",
+                    help="c2: the 1.2M-dof flume (the metric); c1: the 200 m flume; c3: the 10 km x 2 km "
+                         "unstructured channel with floodplain (FP64 against mixed precision; no learned leg); "
+                         "c5: the 8 km meandering channel under a rising hydrograph, learned warm start trained "
+                         "on channels of the c3 family")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--train-steps", type=int, default=20, help="time steps per training trajectory")
+    ap.add_argument("--train-steps", type=int, default=None,
+                    help="time steps per training trajectory (default 20; c5 200)")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--check-every", type=int, default=8)
@@ -173,13 +192,48 @@ def main():
     import adh_kraken_amd as A
     from adh_kraken_amd import _lib as L
 
-    spec = A.FLUME_C2 if args.config == "c2" else A.FLUME_C1
-    dt = 2.0 if args.config == "c2" else 1.0
-    mesh, sol0 = A.flume_mesh(spec)
+    if args.steps is None:
+        args.steps = {"c1": 100, "c2": 50, "c3": 20, "c5": 200}[args.config]
+    if args.train_steps is None:
+        args.train_steps = 200 if args.config == "c5" else 20
+    prepare = None
+    if args.config == "c5":
+        t0 = time.perf_counter()
+        spec = A.CHANNEL_C5
+        mesh, sol0 = A.channel_mesh(spec)
+        mesh_s = time.perf_counter() - t0
+        dt = 5.0
+        args.no_ensemble = True
+        hydro = A.C5_HYDROGRAPH
+        prepare = lambda sm: A.bind_hydrograph(sm, *hydro)  # noqa: E731
+        workload = (f"SW2 meandering channel 8 km x 1.8 km (sine centreline, amplitude 300 m, wavelength 2 km), "
+                    f"Poisson-disk points at 3 m, Delaunay, RCM order: {mesh.nnodes} nodes, {mesh.nelems2d} "
+                    f"triangles, {3 * mesh.nnodes} dofs; 200 m channel 3 m deep, 50 seeded bumps, Manning n "
+                    f"0.025 / 0.06; inflow hydrograph 50 -> 600 m3/s over 1000 s, dt 5 s "
+                    f"(mesh built in {mesh_s:.1f} s, not timed)")
+    elif args.config == "c3":
+        t0 = time.perf_counter()
+        spec = A.CHANNEL_C3
+        mesh, sol0 = A.channel_mesh(spec)
+        mesh_s = time.perf_counter() - t0
+        dt = 5.0
+        args.no_learned = args.no_ensemble = True
+        workload = (f"SW2 channel with floodplain 10 km x 2 km, Poisson-disk points at 3 m minimum spacing, Delaunay, "
+                    f"RCM order: {mesh.nnodes} nodes, {mesh.nelems2d} triangles, {3 * mesh.nnodes} dofs; 400 m "
+                    f"channel 3 m deep, 50 seeded bumps, Manning n 0.025 / 0.06, Q 800 m3/s, dt 5 s "
+                    f"(mesh built in {mesh_s:.1f} s, not timed)")
+    else:
+        spec = A.FLUME_C2 if args.config == "c2" else A.FLUME_C1
+        dt = 2.0 if args.config == "c2" else 1.0
+        mesh, sol0 = A.flume_mesh(spec)
+        workload = ("SW2 flume 2000 m x 200 m at 1 m, 402201 nodes, 800000 triangles, 1206603 dofs, "
+                    "dt 2 s, Manning n 0.03, 50 seeded bed bumps, Q 200 m3/s, tailwater 2 m"
+                    if args.config == "c2" else "SW2 flume 200 m x 20 m, 12663 dofs, dt 1 s")
 
+    log(f"workload ready: {mesh.nnodes} nodes, {mesh.nelems2d} triangles")
     schedule = json.loads(args.schedule) if args.schedule else None
     plain = run_variant(A, L, mesh, sol0, dt, L.PREC_FP64, args.steps, args.warmup, args.check_every,
-                        schedule=schedule)
+                        schedule=schedule, prepare=prepare)
     sm = plain["sm"]
     kt = plain["kt"]
     # the dominant kernel: the BiCGStab SpMV with the larger time per iteration (together they take
@@ -204,7 +258,9 @@ def main():
                 "avg_launch_ms": round(avg_ms, 5), "copy_kernel_gbs": round(copy_gbs, 1)}
     value = args.steps / plain["elapsed"]
     out = {
-        "metric": "SW2 timesteps/s @1.2M dofs: FP64 plain vs mixed+learned, equal 1e-8 Newton tol",
+        "metric": ("SW2 timesteps/s @1.2M dofs: FP64 plain vs mixed+learned, equal 1e-8 Newton tol"
+                   if args.config == "c2" else f"SW2 timesteps/s, {args.config} workload: FP64 plain vs mixed, "
+                                               f"equal 1e-8 Newton tol"),
         "value": round(value, 4),
         "unit": "timesteps/s",
         "n_gpus": 1,
@@ -216,9 +272,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": ("SW2 flume 2000 m x 200 m at 1 m, 402201 nodes, 800000 triangles, 1206603 dofs, "
-                                "dt 2 s, Manning n 0.03, 50 seeded bed bumps, Q 200 m3/s, tailwater 2 m"
-                                if args.config == "c2" else "SW2 flume 200 m x 20 m, 12663 dofs, dt 1 s"),
+        "config": {"workload": workload,
                    "variant": "plain FP64: FD Jacobian, BiCGStab + 3x3 block-Jacobi folded into the matrix (tol 1e-11, max 100)",
                    "newton_tol": sm.solver.tol_nonlin, "inc_nonlin": sm.solver.inc_nonlin},
         "newton_its": plain["stats"]["newton_its"], "bicgstab_its": plain["stats"]["lin_its_total"],
@@ -230,7 +284,7 @@ def main():
     }
     if not args.no_mixed:
         mx = run_variant(A, L, mesh, sol0, dt, L.PREC_MIXED, args.steps, args.warmup, args.check_every,
-                         schedule=schedule)
+                         schedule=schedule, prepare=prepare)
         h64, hmx = plain["sol"][0::3], mx["sol"][0::3]
         u64 = np.hypot(plain["sol"][1::3], plain["sol"][2::3])
         umx = np.hypot(mx["sol"][1::3], mx["sol"][2::3])
@@ -245,13 +299,27 @@ def main():
     if not args.no_learned:
         from adh_kraken_amd import learned as LR
         t0 = time.perf_counter()
-        # trajectories of two other flumes of the same family (bed-bump seeds 2 and 3), mixed precision
-        cases = [A.flume_mesh(A.FlumeSpec(**{**spec.__dict__, "seed": sd})) for sd in (2, 3)]
+        if args.config == "c5":
+            # trajectories of two straight channels of the c3 family (2 km x 0.5 km, 100 m channel, seeds 11
+            # and 12) under the same hydrograph per unit channel width (25 -> 300 m3/s), mixed precision
+            tr_specs = [A.ChannelSpec(length=2000.0, width=500.0, spacing=3.0, chan_width=100.0, chan_depth=3.0,
+                                      bank_width=20.0, slope=1e-4, nbumps=10, seed=sd, n_channel=0.025,
+                                      n_floodplain=0.06, inflow_q=25.0, fp_depth_out=1.5) for sd in (11, 12)]
+            cases = [A.channel_mesh(ts) for ts in tr_specs]
+            tq = (hydro[0], hydro[1] * 100.0 / 200.0)
+            tr_prepare = lambda sm_, i: A.bind_hydrograph(sm_, *tq)  # noqa: E731
+            tr_desc = "2 channels of the c3 family (2 km x 0.5 km, seeds 11, 12) x %d steps" % args.train_steps
+        else:
+            # trajectories of two other flumes of the same family (bed-bump seeds 2 and 3), mixed precision
+            cases = [A.flume_mesh(A.FlumeSpec(**{**spec.__dict__, "seed": sd})) for sd in (2, 3)]
+            tr_prepare = None
+            tr_desc = "2 flumes x %d steps (bump seeds 2, 3)" % args.train_steps
+        log("learned warm start: trajectories and training")
         mlp, hist, gen_its = LR.train_warm_start(cases, nsteps=args.train_steps, steps=2000, batch=16384, lr=2e-3,
-                                                 seed=5, precision=L.PREC_MIXED, dt=dt)
+                                                 seed=5, precision=L.PREC_MIXED, dt=dt, prepare=tr_prepare)
         t_train = time.perf_counter() - t0
         ml = run_variant(A, L, mesh, sol0, dt, L.PREC_MIXED, args.steps, args.warmup, args.check_every, warm=mlp,
-                         schedule=schedule)
+                         schedule=schedule, prepare=prepare)
         h64, hml = plain["sol"][0::3], ml["sol"][0::3]
         vml = args.steps / ml["elapsed"]
         out["mixed_learned"] = {
@@ -261,14 +329,23 @@ def main():
             "warm_accepted_steps": ml["stats"]["warm_accepted"], "newton_its": ml["stats"]["newton_its"],
             "bicgstab_its": ml["stats"]["lin_its_total"],
             "depth_rel_err_vs_fp64": float(np.abs(hml - h64).max() / np.abs(h64).max()),
-            "training": {"trajectories": "2 flumes x %d steps (bump seeds 2, 3)" % args.train_steps,
+            "plain_mixed_newton_its": out["mixed"]["newton_its"] if "mixed" in out else None,
+            "speedup_vs_plain_mixed": round(vml / out["mixed"]["value"], 3) if "mixed" in out else None,
+            "training": {"trajectories": tr_desc,
                          "samples": mlp.size, "adam_steps": 2000, "batch": 16384,
                          "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-10:].mean()),
                          "seconds_incl_data_generation": round(t_train, 2)}}
     if not args.no_ensemble:
+        log("ensemble")
         out["ensemble"] = ensemble_run(A, L, args.ensemble_steps)
     if not args.no_cpu:
-        cb = cpu_baseline(mesh, sol0, dt)
+        seg_val = None
+        if args.config == "c5":
+            idx, w_in = A.inflow_segments(mesh)
+            seg_val = mesh.seg_val.copy()
+            seg_val[idx, 0] = -A.series_average(hydro[0], hydro[1], 0.0, dt) / w_in
+        log("CPU baseline (oracle)")
+        cb = cpu_baseline(mesh, sol0, dt, nsteps_sample=1 if args.config in ("c3", "c5") else 5, seg_val=seg_val)
         cb["gpu_fp64_speedup"] = round(value / cb["value"], 1)
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)

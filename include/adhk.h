@@ -245,6 +245,57 @@ int adhk_flume_build(const adhk_flume_spec *f, double *x, double *y, double *z, 
  * surface at the tailwater level of the bump-free bed; ic_flat_wse = 0 -> h = tailwater everywhere */
 int adhk_flume_initial_state(const adhk_flume_spec *f, int nnodes, const double *z, double *sol);
 
+/* ---- time-series boundary values (sseries.c, time_loop.c:74-76).  A series is a piecewise-linear
+ *      table of (time, value); each step of adhk_time_loop sets every segment bound to a series to
+ *      scale * (the series' average over the step [t, t + dt]), as sseries_set_values_average
+ *      (sseries.c:622-668) does for the reference's boundary strings.  The context's time t starts at 0
+ *      and advances by dt with every completed step.  A step whose interval leaves a series' time range
+ *      fails (returns as a Newton failure would, with the message in adhk_last_error). */
+int adhk_sw2_set_series(adhk_sw2 *c, int id, int npts, const double *t, const double *v);
+/* bind segments to series id (-1 unbinds: the segment keeps its last value); the segment's first
+ * value (depth for ADHK_BC_DEPTH, flux per unit width for ADHK_BC_FLUX, u for ADHK_BC_VEL) becomes
+ * scale * average */
+int adhk_sw2_bind_series(adhk_sw2 *c, int nsegs, const int *segs, int id, double scale);
+int adhk_sw2_set_time(adhk_sw2 *c, double t);
+double adhk_sw2_get_time(const adhk_sw2 *c);
+/* average over [t1, t2] of the piecewise-linear series (t, v) (its value at t1 when t2 == t1): the
+ * evaluation the time loop uses, on host arrays */
+int adhk_series_average(int npts, const double *t, const double *v, double t1, double t2, double *value);
+
+/* ---- unstructured channel meshes (Poisson-disk points, Delaunay by the caller, RCM order).
+ *      The reference reads meshes from .geo files (sgrid_read.c) or builds rectangular ones
+ *      (create_rectangular_grid.c); these generate the channel-with-floodplain workloads. */
+typedef struct {
+    double length, width;         /* domain [0, length] x [0, width] (m) */
+    double spacing;               /* Poisson-disk minimum point distance (m) */
+    double chan_width, chan_depth; /* main channel: width at mid-bank, depth below the floodplain (m) */
+    double bank_width;            /* bed ramps linearly from channel to floodplain across this width */
+    double meander_amp, meander_wavelength; /* centreline y = width/2 + amp sin(2 pi x / wavelength) */
+    double slope;                 /* bed falls by slope per m of x */
+    int nbumps;                   /* seeded Gaussian bed bumps, as in the flume */
+    double bump_amp_max, bump_sigma_min, bump_sigma_max;
+    unsigned long long seed;
+    double n_channel, n_floodplain; /* Manning n by element centroid (channel when weight >= 1/2) */
+    double inflow_q;              /* discharge through the channel section at x = 0 (m^3/s) */
+    double fp_depth_out;          /* initial flat water surface: floodplain depth at x = length (m) */
+} adhk_channel_spec;
+/* upper bound on the number of points adhk_poisson_disk can return for this rectangle */
+int adhk_poisson_disk_bound(double x0, double x1, double y0, double y1, double r);
+/* Poisson-disk sample (Bridson's algorithm, 30 tries per active point) of [x0,x1] x [y0,y1] with
+ * minimum distance r, starting from the corners and evenly spaced edge points; returns the count
+ * (-1 on bad arguments or when more than max_n points would be needed) */
+int adhk_poisson_disk(double x0, double x1, double y0, double y1, double r, unsigned long long seed, int max_n,
+                      double *x, double *y);
+/* reverse Cuthill-McKee order of the triangle mesh's node graph: perm[new] = old */
+int adhk_rcm_order(int nnodes, int nelems, const int *tri, int *perm);
+/* bed elevation of the channel workload at the given points */
+int adhk_channel_bed(const adhk_channel_spec *s, int nnodes, const double *x, const double *y, double *z);
+/* Manning n per triangle (channel or floodplain by centroid) */
+int adhk_channel_roughness(const adhk_channel_spec *s, int nnodes, const double *x, const double *y, int nelems,
+                           const int *tri, double *mann);
+/* channel section weight at (x, y): 1 in the channel, 0 on the floodplain, linear across the banks */
+double adhk_channel_section_weight(const adhk_channel_spec *s, double x, double y);
+
 /* ---- learned warm start of the Newton iterate.  No counterpart in the reference, whose Newton
  *      solve starts from the previous time level (smodel_super_prep_sol.c:2-8, newton_tools.c:21).
  *      A per-node MLP 12 -> 64 -> 64 -> 3 (GELU, BF16 weights, FP32 arithmetic) predicts the change

@@ -156,6 +156,10 @@ class Oracle:
     def set_initial(self, sol):
         self.set_state(sol, sol, sol)
 
+    def set_seg_values(self, val):
+        """Boundary values (S, 2) for the following calls (the context reads this array in place)."""
+        self._arr["val"][...] = np.asarray(val, np.float64).reshape(self._arr["val"].shape)
+
     def state(self):
         a, b, c = (np.zeros(self.ndofs) for _ in range(3))
         self.lib.or_get_state(self.h, _d(a), _d(b), _d(c))
