@@ -128,10 +128,12 @@ SIGNATURES = [
     ("adhk_flume_build", C.c_int, [C.POINTER(FlumeSpec), dp, dp, dp, ip, dp, ip, ip, dp]),
     ("adhk_flume_initial_state", C.c_int, [C.POINTER(FlumeSpec), C.c_int, dp, dp]),
     ("adhk_sw2_set_series", C.c_int, [P, C.c_int, C.c_int, dp, dp]),
-    ("adhk_sw2_bind_series", C.c_int, [P, C.c_int, ip, C.c_int, C.c_double]),
+    ("adhk_sw2_bind_series", C.c_int, [P, C.c_int, ip, C.c_int, C.c_int, C.c_double]),
     ("adhk_sw2_set_time", C.c_int, [P, C.c_double]),
     ("adhk_sw2_get_time", C.c_double, [P]),
     ("adhk_series_average", C.c_int, [C.c_int, dp, dp, C.c_double, C.c_double, dp]),
+    ("adhk_geo_counts", C.c_int, [C.c_char_p, ip, ip, ip, ip]),
+    ("adhk_geo_read", C.c_int, [C.c_char_p, C.c_int, dp, dp, dp, C.c_int, ip, ip, C.c_int, ip, ip]),
     ("adhk_poisson_disk_bound", C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
     ("adhk_poisson_disk", C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_ulonglong,
                                     C.c_int, dp, dp]),

@@ -113,8 +113,8 @@ struct adhk_sw2 {
     double t_prev;
     int nseries;
     struct adhk_series_tab *series;
-    int *h_seg_ser;                  /* host [S]: series driving the segment's first value, -1 = none */
-    double *h_seg_scale, *h_seg_val; /* host [S], [2S] */
+    int *h_seg_ser;                  /* host [2S]: series driving each segment value, -1 = none */
+    double *h_seg_scale, *h_seg_val; /* host [2S], [2S] */
     /* timing */
     adhk_timing timing;
     hipEvent_t ev[8];

@@ -145,12 +145,22 @@ __device__ __forceinline__ void sum_blocks(const double *partials, int stride, i
     __shared__ double sh[(TPB / 64) * K];
 #pragma unroll
     for (int k = 0; k < K; k++) w[k] = 0.0;
-    for (int b = b0 + threadIdx.x; b < b1; b += TPB)
+    /* a thread combines blocks b0 + tid, b0 + tid + TPB, ... in that order; the loads of eight of
+     * them are issued together, so the finalizer waits for one memory round trip per 8 * TPB blocks
+     * instead of one per TPB */
+    constexpr int U = 8;
+    for (int b = b0 + threadIdx.x; b < b1; b += U * TPB) {
+        double t[U][K];
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            double pv = partials[k * stride + b];
-            w[k] = MAX ? fmax(w[k], pv) : w[k] + pv;
-        }
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int k = 0; k < K; k++) t[u][k] = b + u * TPB < b1 ? partials[k * stride + b + u * TPB] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                if (b + u * TPB < b1) w[k] = MAX ? fmax(w[k], t[u][k]) : w[k] + t[u][k];
+    }
     block_reduce<K, MAX>(w, sh);
 }
 
@@ -734,11 +744,12 @@ __global__ __launch_bounds__(TPB) void k_fin_a(int nblk, const int *__restrict__
                                                adhk_scalars *sc_all)
 {
     adhk_scalars *sc = sc_all + blockIdx.x;
-    if (sc->done) return;
+    const int done = sc->done; /* tested after the sums, so that their loads do not wait for it */
     const int b0 = inst_blk[blockIdx.x], b1 = inst_blk[blockIdx.x + 1];
     double vmax[1], vsum[1];
     if (!FIRST) sum_blocks<1, true>(partials + PR_A * nblk, nblk, b0, b1, vmax);
     sum_blocks<1, false>(partials + PR_B * nblk, nblk, b0, b1, vsum);
+    if (done) return;
     if (threadIdx.x == 0) {
         if (!FIRST) {
             sc->rnorm = vmax[0];
@@ -800,9 +811,10 @@ __global__ __launch_bounds__(TPB) void k_fin_s(int nblk, const int *__restrict__
                                                adhk_scalars *sc_all)
 {
     adhk_scalars *sc = sc_all + blockIdx.x;
-    if (sc->done) return;
+    const int done = sc->done;
     double v[3];
     sum_blocks<3, false>(partials + PR_A * nblk, nblk, inst_blk[blockIdx.x], inst_blk[blockIdx.x + 1], v);
+    if (done) return;
     if (threadIdx.x == 0) {
         double omega = v[0] / v[1];
         double rhop = sc->rho;

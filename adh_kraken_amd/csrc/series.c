@@ -1,6 +1,6 @@
 /*
  * series.c -- time-series boundary values: piecewise-linear (time, value) tables whose average over
- * each time step drives the first value of the boundary segments bound to them.
+ * each time step drives the values of the boundary segments bound to them.
  *
  * Restates the reference's series evaluation: sseries_set_values_average (sseries.c:622-668) sets a
  * series to its average over (t1, t2) -- the area under the piecewise-linear interpolant divided by
@@ -111,19 +111,20 @@ int adhk_sw2_set_series(adhk_sw2 *c, int id, int npts, const double *t, const do
     return 0;
 }
 
-int adhk_sw2_bind_series(adhk_sw2 *c, int nsegs, const int *segs, int id, double scale)
+int adhk_sw2_bind_series(adhk_sw2 *c, int nsegs, const int *segs, int component, int id, double scale)
 {
-    if (!c || nsegs < 0 || (nsegs > 0 && !segs) || id < -1 || id >= c->nseries || (id >= 0 && c->series[id].n < 1)) {
+    if (!c || nsegs < 0 || (nsegs > 0 && !segs) || component < 0 || component > 1 || id < -1 || id >= c->nseries ||
+        (id >= 0 && c->series[id].n < 1)) {
         adhk_set_error("bind_series: bad arguments (series %d of %d defined)", id, c ? c->nseries : 0);
         return -1;
     }
     if (!c->h_seg_ser) {
         const int S = c->S;
-        c->h_seg_ser = (int *)malloc((size_t)(S > 0 ? S : 1) * sizeof(int));
-        c->h_seg_scale = (double *)malloc((size_t)(S > 0 ? S : 1) * sizeof(double));
+        c->h_seg_ser = (int *)malloc((size_t)(S > 0 ? 2 * S : 1) * sizeof(int));
+        c->h_seg_scale = (double *)malloc((size_t)(S > 0 ? 2 * S : 1) * sizeof(double));
         c->h_seg_val = (double *)malloc((size_t)(S > 0 ? 2 * S : 1) * sizeof(double));
         if (!c->h_seg_ser || !c->h_seg_scale || !c->h_seg_val) return -1;
-        for (int i = 0; i < S; i++) {
+        for (int i = 0; i < 2 * S; i++) {
             c->h_seg_ser[i] = -1;
             c->h_seg_scale[i] = 0.0;
         }
@@ -134,8 +135,8 @@ int adhk_sw2_bind_series(adhk_sw2 *c, int nsegs, const int *segs, int id, double
             adhk_set_error("bind_series: segment %d out of range [0, %d)", segs[i], c->S);
             return -1;
         }
-        c->h_seg_ser[segs[i]] = id;
-        c->h_seg_scale[segs[i]] = scale;
+        c->h_seg_ser[2 * segs[i] + component] = id;
+        c->h_seg_scale[2 * segs[i] + component] = scale;
     }
     return 0;
 fail:
@@ -162,7 +163,7 @@ int adhk_series_apply(adhk_ctx *c, double t1, double t2)
         free(need);
         return -1;
     }
-    for (int s = 0; s < c->S; s++)
+    for (int s = 0; s < 2 * c->S; s++)
         if (c->h_seg_ser[s] >= 0) need[c->h_seg_ser[s]] = 1;
     for (int k = 0; k < c->nseries; k++)
         if (need[k] && ser_average(&c->series[k], t1, t2, &avg[k])) {
@@ -172,8 +173,8 @@ int adhk_series_apply(adhk_ctx *c, double t1, double t2)
             free(need);
             return -1;
         }
-    for (int s = 0; s < c->S; s++)
-        if (c->h_seg_ser[s] >= 0) c->h_seg_val[2 * s] = c->h_seg_scale[s] * avg[c->h_seg_ser[s]];
+    for (int s = 0; s < 2 * c->S; s++)
+        if (c->h_seg_ser[s] >= 0) c->h_seg_val[s] = c->h_seg_scale[s] * avg[c->h_seg_ser[s]];
     free(avg);
     free(need);
     HC(hipMemcpyAsync(c->seg_val, c->h_seg_val, (size_t)c->S * 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));

@@ -267,11 +267,12 @@ class SModelSuper:
         v = np.ascontiguousarray(v, np.float64)
         L.check(self.lib.adhk_sw2_set_series(self.h, int(sid), t.size, L.dptr(t), L.dptr(v)), "adhk_sw2_set_series")
 
-    def bind_series(self, segs, sid, scale=1.0):
-        """Segments whose first value becomes scale * (series sid averaged over each step); sid -1 unbinds."""
+    def bind_series(self, segs, sid, scale=1.0, component=0):
+        """Segments whose value `component` becomes scale * (series sid averaged over each step); sid -1
+        unbinds."""
         segs = np.ascontiguousarray(segs, np.int32)
-        L.check(self.lib.adhk_sw2_bind_series(self.h, segs.size, L.iptr(segs), int(sid), float(scale)),
-                "adhk_sw2_bind_series")
+        L.check(self.lib.adhk_sw2_bind_series(self.h, segs.size, L.iptr(segs), int(component), int(sid),
+                                              float(scale)), "adhk_sw2_bind_series")
 
     @property
     def time(self):

@@ -252,15 +252,23 @@ int adhk_flume_initial_state(const adhk_flume_spec *f, int nnodes, const double 
  *      and advances by dt with every completed step.  A step whose interval leaves a series' time range
  *      fails (returns as a Newton failure would, with the message in adhk_last_error). */
 int adhk_sw2_set_series(adhk_sw2 *c, int id, int npts, const double *t, const double *v);
-/* bind segments to series id (-1 unbinds: the segment keeps its last value); the segment's first
- * value (depth for ADHK_BC_DEPTH, flux per unit width for ADHK_BC_FLUX, u for ADHK_BC_VEL) becomes
- * scale * average */
-int adhk_sw2_bind_series(adhk_sw2 *c, int nsegs, const int *segs, int id, double scale);
+/* bind value `component` (0 or 1) of the segments to series id (-1 unbinds: the value stays as it
+ * was last set); it becomes scale * average.  Value 0 is the depth of ADHK_BC_DEPTH, the flux per
+ * unit width of ADHK_BC_FLUX and u of ADHK_BC_VEL; value 1 is v of ADHK_BC_VEL */
+int adhk_sw2_bind_series(adhk_sw2 *c, int nsegs, const int *segs, int component, int id, double scale);
 int adhk_sw2_set_time(adhk_sw2 *c, double t);
 double adhk_sw2_get_time(const adhk_sw2 *c);
 /* average over [t1, t2] of the piecewise-linear series (t, v) (its value at t1 when t2 == t1): the
  * evaluation the time loop uses, on host arrays */
 int adhk_series_average(int npts, const double *t, const double *v, double t1, double t2, double *value);
+
+/* ---- AdH .geo mesh files (sgrid_read.c:84-130, :540-617): ND, TRI and SEG cards, 1-based ids.
+ *      counts: nother = QUAD/TET/PRISM cards (3-D models, not read); returns -2 when node ids are not
+ *      1..N.  read: nodes by id, elements in file order, 0-based; *_flag = the bflag field or -1.
+ *      Returns 0, -1 on bad arguments or counts, or -(line + 2) at a malformed card. */
+int adhk_geo_counts(const char *path, int *nnodes, int *ntri, int *nseg, int *nother);
+int adhk_geo_read(const char *path, int nnodes, double *x, double *y, double *z, int ntri, int *tri, int *tri_flag,
+                  int nseg, int *seg, int *seg_flag);
 
 /* ---- unstructured channel meshes (Poisson-disk points, Delaunay by the caller, RCM order).
  *      The reference reads meshes from .geo files (sgrid_read.c) or builds rectangular ones
