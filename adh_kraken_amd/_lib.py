@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libadhk.so")
 
-BC_DEPTH, BC_FLUX, BC_VEL = 0, 1, 2
+BC_DEPTH, BC_FLUX, BC_VEL, BC_DEPTH_NODAL = 0, 1, 2, 3
 FRIC_CONST, FRIC_MANNING = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 KT_NAMES = ["spmv_p", "spmv_s", "krylov_update", "elem_jac", "jac_gather", "elem_resid", "resid_gather", "prep"]
@@ -65,7 +65,8 @@ class ChannelSpec(C.Structure):
                 ("meander_amp", C.c_double), ("meander_wavelength", C.c_double), ("slope", C.c_double),
                 ("nbumps", C.c_int), ("bump_amp_max", C.c_double), ("bump_sigma_min", C.c_double),
                 ("bump_sigma_max", C.c_double), ("seed", C.c_ulonglong), ("n_channel", C.c_double),
-                ("n_floodplain", C.c_double), ("inflow_q", C.c_double), ("fp_depth_out", C.c_double)]
+                ("n_floodplain", C.c_double), ("inflow_q", C.c_double), ("fp_depth_out", C.c_double),
+                ("meander_phase", C.c_double)]
 
 
 P = C.c_void_p

@@ -4,13 +4,15 @@
 The points, the node order, the bed and the roughness come from libadhk.so (meshgen.c); the
 triangulation is scipy's Delaunay (Qhull), the one step done outside the library.  The domain is the
 rectangle [0, length] x [0, width]; a main channel (straight, or meandering about the sine centreline
-y = width/2 + amp sin(2 pi x / wavelength)) is cut chan_depth below the floodplain with linear banks.
+y = width/2 + amp sin(2 pi (x - phase) / wavelength)) is cut chan_depth below the floodplain with linear banks.
 Boundary conditions, restating what the flume uses (fe_sw2_1d_explicit_flow,
 fe_sw2_boundary_integrals.c:56-94, and fe_sw2_bc_h, sw2_bc_h.c:57-178):
   * inflow: the x = 0 edges whose midpoint lies in the channel carry the flux q = -Q / (their total
     length) per unit width (the floodplain edges at x = 0 are walls);
-  * outflow: every x = length edge carries the depth condition, with the depth of a flat water surface
-    (floodplain depth fp_depth_out over the bump-free floodplain) at the edge's mean bed elevation;
+  * outflow: every x = length edge carries the depth condition per node (ADHK_BC_DEPTH_NODAL): the depth
+    of a flat water surface (floodplain depth fp_depth_out over the bump-free floodplain) over the bed at
+    each of its nodes, so that a lake at rest stays at rest where the bed varies along the outflow (the
+    channel banks);
   * the long sides are walls (no boundary element: natural no-flux).
 The initial state is that flat water surface with u = v = 0.
 """
@@ -43,6 +45,7 @@ class ChannelSpec:
     n_floodplain: float = 0.06
     inflow_q: float = 100.0
     fp_depth_out: float = 1.5
+    meander_phase: float = 0.0
 
     def c(self):
         s = L.ChannelSpec()
@@ -61,13 +64,14 @@ CHANNEL_C3 = ChannelSpec(length=10_000.0, width=2_000.0, spacing=3.0, chan_width
                          fp_depth_out=1.5)
 
 
-# 8 km meandering channel (sine centreline, amplitude 300 m, wavelength 2 km) in a 1.8 km wide floodplain at
-# 3 m minimum point spacing (about 1M points, 2M triangles): a 200 m channel 3 m deep, 50 seeded bumps, Manning n
+# 8 km meandering channel (sine centreline, amplitude 300 m, wavelength 2 km, a quarter wavelength of phase so
+# that it meets the inflow and outflow sides at right angles) in a 1.8 km wide floodplain at 3 m minimum
+# point spacing (about 1M points, 2M triangles): a 200 m channel 3 m deep, 50 seeded bumps, Manning n
 # 0.025 / 0.06; the inflow follows the hydrograph C5_HYDROGRAPH (50 -> 600 m^3/s over 200 steps of 5 s)
 CHANNEL_C5 = ChannelSpec(length=8_000.0, width=1_800.0, spacing=3.0, chan_width=200.0, chan_depth=3.0,
-                         bank_width=20.0, meander_amp=300.0, meander_wavelength=2_000.0, slope=1e-4, nbumps=50,
-                         bump_amp_max=0.3, bump_sigma_min=5.0, bump_sigma_max=30.0, seed=5, n_channel=0.025,
-                         n_floodplain=0.06, inflow_q=50.0, fp_depth_out=1.5)
+                         bank_width=20.0, meander_amp=300.0, meander_wavelength=2_000.0, meander_phase=500.0,
+                         slope=1e-4, nbumps=50, bump_amp_max=0.3, bump_sigma_min=5.0, bump_sigma_max=30.0, seed=5,
+                         n_channel=0.025, n_floodplain=0.06, inflow_q=50.0, fp_depth_out=1.5)
 C5_HYDROGRAPH = (np.array([0.0, 1000.0]), np.array([50.0, 600.0]))
 
 
@@ -173,10 +177,12 @@ def channel_mesh(spec: ChannelSpec):
     w_in = np.hypot(x[inflow[:, 0]] - x[inflow[:, 1]], y[inflow[:, 0]] - y[inflow[:, 1]]).sum()
     wse = -spec.slope * spec.length + spec.fp_depth_out
     seg = np.ascontiguousarray(np.concatenate([inflow, outflow]), np.int32)
-    kind = np.concatenate([np.full(inflow.shape[0], L.BC_FLUX), np.full(outflow.shape[0], L.BC_DEPTH)]).astype(np.int32)
+    kind = np.concatenate([np.full(inflow.shape[0], L.BC_FLUX),
+                           np.full(outflow.shape[0], L.BC_DEPTH_NODAL)]).astype(np.int32)
     val = np.zeros((seg.shape[0], 2))
     val[:inflow.shape[0], 0] = -spec.inflow_q / w_in
-    val[inflow.shape[0]:, 0] = wse - 0.5 * (z[outflow[:, 0]] + z[outflow[:, 1]])
+    val[inflow.shape[0]:, 0] = wse - z[outflow[:, 0]]
+    val[inflow.shape[0]:, 1] = wse - z[outflow[:, 1]]
     h0 = wse - z
     if h0.min() <= 0.0:
         raise ValueError(f"initial water surface leaves dry nodes (min depth {h0.min():.3g} m)")

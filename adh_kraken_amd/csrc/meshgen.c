@@ -306,7 +306,7 @@ static double channel_weight(const adhk_channel_spec *s, double x, double y)
 {
     double yc = 0.5 * s->width;
     if (s->meander_amp != 0.0 && s->meander_wavelength > 0.0)
-        yc += s->meander_amp * sin(TWO_PI * x / s->meander_wavelength);
+        yc += s->meander_amp * sin(TWO_PI * (x - s->meander_phase) / s->meander_wavelength);
     const double d = fabs(y - yc);
     if (s->bank_width <= 0.0) return d <= 0.5 * s->chan_width ? 1.0 : 0.0;
     double w = (0.5 * s->chan_width + 0.5 * s->bank_width - d) / s->bank_width;

@@ -144,7 +144,7 @@ static adhk_sw2 *create_impl(int ninst, const int *node_off, const int *elem_off
             }
     }
     for (int i = 0; i < nelems1d; i++)
-        if (seg_kind[i] < ADHK_BC_DEPTH || seg_kind[i] > ADHK_BC_VEL) {
+        if (seg_kind[i] < ADHK_BC_DEPTH || seg_kind[i] > ADHK_BC_DEPTH_NODAL) {
             adhk_set_error("adhk_sw2_create: unknown segment kind %d", seg_kind[i]);
             return NULL;
         }

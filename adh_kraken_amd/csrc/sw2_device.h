@@ -373,8 +373,8 @@ __device__ __forceinline__ void sw2_seg_resid(int kind, double val0, double val1
 #pragma unroll
     for (int k = 0; k < 6; k++) er[k] = 0.0;
     const double dt = P.dt, g = P.g;
-    if (kind == ADHK_BC_DEPTH) { /* fe_sw2_bc_h, sw2_bc_h.c:57-178 */
-        double hd[2] = {val0, val0};
+    if (kind == ADHK_BC_DEPTH || kind == ADHK_BC_DEPTH_NODAL) { /* fe_sw2_bc_h, sw2_bc_h.c:57-178 */
+        double hd[2] = {val0, kind == ADHK_BC_DEPTH ? val0 : val1};
         if (hd[0] < D_SMALL || hd[1] < D_SMALL || h[0] < D_SMALL || h[1] < D_SMALL) return;
         implicit_flow(er, nx, ny, djac, dt, u, v, hd);
         if (pflag) {

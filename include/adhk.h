@@ -32,6 +32,9 @@ extern "C" {
 #define ADHK_BC_FLUX  1 /* explicit flux, fe_sw2_1d_explicit_flow (src/fe/fe_sw2_boundary_integrals.c:56);
                            val0 = normal flux per unit width, negative = into the domain                         */
 #define ADHK_BC_VEL   2 /* weak velocity, fe_sw2_bc_vel (src/models/sw2/sw2_bc_vel.c:58); val0, val1 = u, v    */
+#define ADHK_BC_DEPTH_NODAL 3 /* fe_sw2_bc_h with a depth per node: val0 at the segment's first node, val1 at
+                                 its second (a water-surface elevation minus the bed: exact for a lake at rest
+                                 on a bed that varies along the boundary, where one depth per segment is not) */
 
 /* friction law */
 #define ADHK_FRIC_CONST   0 /* constant drag coefficient (sw2_body_resid.c:241 hard-codes 0.002) */
@@ -278,7 +281,7 @@ typedef struct {
     double spacing;               /* Poisson-disk minimum point distance (m) */
     double chan_width, chan_depth; /* main channel: width at mid-bank, depth below the floodplain (m) */
     double bank_width;            /* bed ramps linearly from channel to floodplain across this width */
-    double meander_amp, meander_wavelength; /* centreline y = width/2 + amp sin(2 pi x / wavelength) */
+    double meander_amp, meander_wavelength; /* centreline y = width/2 + amp sin(2 pi (x - phase) / wavelength) */
     double slope;                 /* bed falls by slope per m of x */
     int nbumps;                   /* seeded Gaussian bed bumps, as in the flume */
     double bump_amp_max, bump_sigma_min, bump_sigma_max;
@@ -286,6 +289,7 @@ typedef struct {
     double n_channel, n_floodplain; /* Manning n by element centroid (channel when weight >= 1/2) */
     double inflow_q;              /* discharge through the channel section at x = 0 (m^3/s) */
     double fp_depth_out;          /* initial flat water surface: floodplain depth at x = length (m) */
+    double meander_phase;         /* x offset of the centreline's sine (m) */
 } adhk_channel_spec;
 /* upper bound on the number of points adhk_poisson_disk can return for this rectangle */
 int adhk_poisson_disk_bound(double x0, double x1, double y0, double y1, double r);

@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-BC_DEPTH, BC_FLUX, BC_VEL = 0, 1, 2
+BC_DEPTH, BC_FLUX, BC_VEL, BC_DEPTH_NODAL = 0, 1, 2, 3
 FRIC_CONST, FRIC_MANNING = 0, 1
 
 

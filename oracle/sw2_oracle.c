@@ -759,8 +759,8 @@ void or_sw2_seg_resid(const or_ctx *c, int ie, double perturbation, int pnode, i
         vel[pnode].y += psign * perturbation;
         PRESSURE_FLAG = 0;
     }
-    if (kind == OR_BC_DEPTH) { /* sw2_bc_h.c:57-178 */
-        double hd[2] = {val[0], val[0]};
+    if (kind == OR_BC_DEPTH || kind == OR_BC_DEPTH_NODAL) { /* sw2_bc_h.c:57-178 */
+        double hd[2] = {val[0], kind == OR_BC_DEPTH ? val[0] : val[1]};
         if (hd[0] < SMALL || hd[1] < SMALL || h[0] < SMALL || h[1] < SMALL) return;
         implicit_flow(er, n, djac, dt, vel, hd);
         if (PRESSURE_FLAG) {

@@ -38,6 +38,7 @@ extern "C" {
 #define OR_BC_DEPTH 0 /* fe_sw2_bc_h: tailwater depth, src/models/sw2/sw2_bc_h.c:57  */
 #define OR_BC_FLUX  1 /* explicit flow, src/fe/fe_sw2_boundary_integrals.c:56 (val0 = flux per unit width, <0 = inflow) */
 #define OR_BC_VEL   2 /* fe_sw2_bc_vel: weak velocity, src/models/sw2/sw2_bc_vel.c:58 (val0,val1 = u,v) */
+#define OR_BC_DEPTH_NODAL 3 /* fe_sw2_bc_h with a depth per node (val0 first node, val1 second) */
 
 #define OR_FRIC_CONST   0 /* drag coefficient constant (sw2_body_resid.c:241 hard-codes 0.002) */
 #define OR_FRIC_MANNING 1 /* Manning n -> roughness height -> log-law drag (src/friction)    */

@@ -81,12 +81,12 @@ def test_boundary_segments(chan):
     dx, dy = x[s[:, 1]] - x[s[:, 0]], y[s[:, 1]] - y[s[:, 0]]
     ln = np.hypot(dx, dy)
     nx = dy / ln  # outward normal of a segment with the domain on its left (selem_1d.c:53-54)
-    fl, dp = mesh.seg_kind == L.BC_FLUX, mesh.seg_kind == L.BC_DEPTH
+    fl, dp = mesh.seg_kind == L.BC_FLUX, mesh.seg_kind == L.BC_DEPTH_NODAL
     assert np.allclose(nx[fl], -1.0) and np.allclose(nx[dp], 1.0)
     assert abs(-(mesh.seg_val[fl, 0] * ln[fl]).sum() - SPEC.inflow_q) <= 1e-9 * SPEC.inflow_q
-    # the depth condition continues the initial flat surface
-    h_mid = 0.5 * (sol0[0::3][s[dp, 0]] + sol0[0::3][s[dp, 1]])
-    assert np.allclose(mesh.seg_val[dp, 0], h_mid, rtol=0, atol=1e-12)
+    # the depth condition continues the initial flat surface at both nodes of every outflow segment
+    assert np.allclose(mesh.seg_val[dp, 0], sol0[0::3][s[dp, 0]], rtol=0, atol=1e-12)
+    assert np.allclose(mesh.seg_val[dp, 1], sol0[0::3][s[dp, 1]], rtol=0, atol=1e-12)
     # inflow only across the channel section
     cs = SPEC.c()
     lib = L.load()

@@ -231,8 +231,8 @@ def seg_line_quadrature(kind, val, xa, ya, xb, yb, h, u, v, dt, g):
                 r[3 * i + 1] += dt * w * L * ph[i] * urq * qn * hq
                 r[3 * i + 2] += dt * w * L * ph[i] * vrq * qn * hq
 
-    if kind == O.BC_DEPTH:
-        hb = np.array([val[0], val[0]])
+    if kind in (O.BC_DEPTH, O.BC_DEPTH_NODAL):
+        hb = np.array([val[0], val[0] if kind == O.BC_DEPTH else val[1]])
         if min(hb.min(), h.min()) < eps:
             return r
         implicit(hb, u, v)
@@ -267,8 +267,10 @@ def test_seg_residual_vs_line_quadrature():
     s, so = perturbed_state(sol0, seed=2, amp_v=0.4)
     S = mesh.nelems1d
     rng = np.random.default_rng(3)
-    variants = [mesh.seg_kind.copy(), np.full(S, O.BC_VEL, np.int32), np.full(S, O.BC_FLUX, np.int32)]
-    vals = [mesh.seg_val.copy(), rng.uniform(-0.4, 0.4, (S, 2)), rng.uniform(-0.8, 0.8, (S, 2))]
+    variants = [mesh.seg_kind.copy(), np.full(S, O.BC_VEL, np.int32), np.full(S, O.BC_FLUX, np.int32),
+                np.full(S, O.BC_DEPTH_NODAL, np.int32)]
+    vals = [mesh.seg_val.copy(), rng.uniform(-0.4, 0.4, (S, 2)), rng.uniform(-0.8, 0.8, (S, 2)),
+            rng.uniform(0.5, 1.5, (S, 2))]
     # a dry tailwater segment (no implicit flux, no pressure) and an inflow/outflow mix
     vals[0][-1, 0] = 0.0
     for kind, val in zip(variants, vals):
@@ -321,6 +323,33 @@ def test_lake_at_rest():
         orc.set_initial(sol0)
         for e in range(mesh.nelems2d):
             assert np.abs(orc.body_resid(e, sol0)).max() <= 1e-12
+
+
+def test_lake_at_rest_with_nodal_depth_boundary():
+    """A flat water surface at rest over a bed that varies along the boundary: with the depth condition
+    given per node (surface minus bed) every boundary segment's residual vanishes; with one depth per
+    segment (its mean) the pressure pair leaves a net force wherever the bed varies along the segment."""
+    import dataclasses
+    spec, (mesh, sol0) = small_flume(nbumps=6, flat=1)
+    depth = mesh.seg_kind == O.BC_DEPTH
+    wse = (sol0[0::3] + mesh.z)[0]
+    val = mesh.seg_val.copy()
+    val[depth, 0] = wse - mesh.z[mesh.seg[depth, 0]]
+    val[depth, 1] = wse - mesh.z[mesh.seg[depth, 1]]
+    kind = mesh.seg_kind.copy()
+    kind[depth] = O.BC_DEPTH_NODAL
+    orc = O.Oracle(dataclasses.replace(mesh, seg_kind=kind, seg_val=val), dt=2.0)
+    orc.set_initial(sol0)
+    dz = np.abs(mesh.z[mesh.seg[depth, 0]] - mesh.z[mesh.seg[depth, 1]])
+    assert dz.max() > 1e-3  # the bed does vary along the outflow
+    for e in np.flatnonzero(depth):
+        assert np.abs(orc.seg_resid(e, sol0)).max() <= 1e-12
+    val2 = mesh.seg_val.copy()
+    val2[depth, 0] = 0.5 * (val[depth, 0] + val[depth, 1])
+    orc2 = O.Oracle(dataclasses.replace(mesh, seg_val=val2), dt=2.0)
+    orc2.set_initial(sol0)
+    worst = max(np.abs(orc2.seg_resid(e, sol0)).max() for e in np.flatnonzero(depth))
+    assert worst > 1e-8
 
 
 def boundary_edges(mesh):
