@@ -25,7 +25,8 @@ class Params(C.Structure):
 class Solver(C.Structure):
     _fields_ = [("tol_nonlin", C.c_double), ("inc_nonlin", C.c_double), ("max_nonlin_it", C.c_int),
                 ("max_linesearch_cuts", C.c_int), ("lin_tol", C.c_double), ("lin_max_it", C.c_int),
-                ("precision", C.c_int), ("lin_check_every", C.c_int), ("mixed_lin_tol", C.c_double)]
+                ("precision", C.c_int), ("lin_check_every", C.c_int), ("mixed_lin_tol", C.c_double),
+                ("warm_gate", C.c_int)]
 
 
 class NewtonStats(C.Structure):
@@ -128,6 +129,7 @@ SIGNATURES = [
     ("adhk_flume_counts", C.c_int, [C.POINTER(FlumeSpec), ip, ip, ip]),
     ("adhk_flume_build", C.c_int, [C.POINTER(FlumeSpec), dp, dp, dp, ip, dp, ip, ip, dp]),
     ("adhk_flume_initial_state", C.c_int, [C.POINTER(FlumeSpec), C.c_int, dp, dp]),
+    ("adhk_sw2_grid_mass", C.c_int, [P, C.c_double, dp]),
     ("adhk_sw2_set_series", C.c_int, [P, C.c_int, C.c_int, dp, dp]),
     ("adhk_sw2_bind_series", C.c_int, [P, C.c_int, ip, C.c_int, C.c_int, C.c_double]),
     ("adhk_sw2_set_time", C.c_int, [P, C.c_double]),

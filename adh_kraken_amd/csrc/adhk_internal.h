@@ -80,7 +80,8 @@ struct adhk_sw2 {
     int *brow, *bcol, *diag_blk, *cptr, *contrib;
     /* SELL-64 layout of the matrix values */
     int nslice, nslot;
-    int *slice_ptr, *scol, *sell2blk, *blk2sell, *diag_slot;
+    int *slice_ptr, *scol, *sell2blk, *blk2sell, *diag_slot; /* a row's diagonal block is its slot 0 */
+    unsigned char *diag_id; /* [N]: 1 = the folded diagonal block of M^-1 A is the identity */
     /* matrix values and preconditioner */
     double *bvals, *minv;
     float *bvals32, *minv32;

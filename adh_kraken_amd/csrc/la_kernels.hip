@@ -230,6 +230,7 @@ template <typename T> struct ColP {
 
 struct Sell {
     const int *slice_ptr, *scol;
+    const unsigned char *diag_id; /* [N]: 1 = the folded diagonal block is the identity */
 };
 
 /* Slot range [c0, c1) of node n's slice.  UNI: the 64 lanes of the wave hold the 64 nodes of one slice
@@ -242,20 +243,29 @@ __device__ __forceinline__ void slice_bounds(int n, const Sell &S, int &c0, int 
     c1 = S.slice_ptr[sl + 1];
 }
 
-/* y = (row n of the block matrix) x, over the slots [c0, c1) of n's slice.  Slots go in groups of eight
- * (a triangle-mesh row is rarely wider): the eight column indices first, then the eight neighbours'
- * values, so a group costs two memory latencies instead of eight dependent pairs; the products still
- * accumulate in slot order.  Every multiply-add here and in the dot products is an explicit fused
- * multiply-add, so each kernel that uses them rounds identically whatever the compiler contracts. */
+/* y = (row n of M^-1 A) x over the slots [c0, c1) of n's slice.  Slot c0 of every row holds its
+ * diagonal block, which the block-Jacobi fold makes the identity: y starts at x_n and the slot is not
+ * read (1/7 of the matrix traffic on a triangle mesh), unless the block was singular and kept the
+ * fallback inverse (diag_id 0), when it is multiplied like the others.  The off-diagonal slots go in
+ * groups of eight (a triangle-mesh row is rarely wider): the eight column indices first, then the eight
+ * neighbours' values, so a group costs two memory latencies instead of eight dependent pairs; the
+ * products still accumulate in slot order.  Every multiply-add here and in the dot products is an
+ * explicit fused multiply-add, so each kernel that uses them rounds identically whatever the compiler
+ * contracts. */
 template <typename T, typename Col>
 __device__ __forceinline__ void block_row_cs(int n, int c0, int c1, const Sell &S, const T *__restrict__ bv,
                                              const Col &col, double &y0, double &y1, double &y2)
 {
-    y0 = 0.0;
-    y1 = 0.0;
-    y2 = 0.0;
     const int lane = n & 63;
-    for (int cs = c0; cs < c1; cs += 8) {
+    col.get(n, y0, y1, y2);
+    if (!S.diag_id[n]) {
+        const T *a = bv + (size_t)c0 * 9 * 64 + lane;
+        const double x0 = y0, x1 = y1, x2 = y2;
+        y0 = __fma_rn((double)a[128], x2, __fma_rn((double)a[64], x1, (double)a[0] * x0));
+        y1 = __fma_rn((double)a[320], x2, __fma_rn((double)a[256], x1, (double)a[192] * x0));
+        y2 = __fma_rn((double)a[512], x2, __fma_rn((double)a[448], x1, (double)a[384] * x0));
+    }
+    for (int cs = c0 + 1; cs < c1; cs += 8) {
         int m[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) m[u] = (cs + u < c1) ? S.scol[(cs + u) * 64 + lane] : n;
@@ -450,7 +460,8 @@ __global__ __launch_bounds__(TPB) void k_scale_prec(int N, const int *__restrict
                                                     const int *__restrict__ slice_ptr, const int *__restrict__ scol,
                                                     const int *__restrict__ diag_slot, double *__restrict__ bv,
                                                     const double *__restrict__ scale, double *__restrict__ minv,
-                                                    T *__restrict__ bv_t, T *__restrict__ minv_t)
+                                                    T *__restrict__ bv_t, T *__restrict__ minv_t,
+                                                    unsigned char *__restrict__ diag_id)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
@@ -475,6 +486,7 @@ __global__ __launch_bounds__(TPB) void k_scale_prec(int N, const int *__restrict
     double c02 = a[3] * a[7] - a[4] * a[6];
     double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
     double m[9];
+    diag_id[n] = fabs(det) > 1e-300 ? 1 : 0; /* the fold below then makes the diagonal block I */
     if (fabs(det) > 1e-300) {
         double id = 1.0 / det;
         m[0] = c00 * id;
@@ -518,11 +530,12 @@ extern "C" int adhk_launch_scale_prec(adhk_ctx *c, int mixed)
     adhk_kt_begin(c, ADHK_KT_PREP);
     if (mixed)
         hipLaunchKernelGGL(k_scale_prec<float>, dim3((c->N + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->N, c->brow,
-                           c->slice_ptr, c->scol, c->diag_slot, c->bvals, c->scale, c->minv, c->bvals32, c->minv32);
+                           c->slice_ptr, c->scol, c->diag_slot, c->bvals, c->scale, c->minv, c->bvals32, c->minv32,
+                           c->diag_id);
     else
         hipLaunchKernelGGL(k_scale_prec<double>, dim3((c->N + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->N,
                            c->brow, c->slice_ptr, c->scol, c->diag_slot, c->bvals, c->scale, c->minv, (double *)NULL,
-                           (double *)NULL);
+                           (double *)NULL, c->diag_id);
     CHECK_LAUNCH();
     adhk_kt_end(c);
     return 0;
@@ -687,18 +700,16 @@ __global__ __launch_bounds__(TPB) void k_upd(BLK_ARGS, T *__restrict__ x, T *__r
         if (si->done) continue;
         const double alpha = si->alpha, omega = si->omega, beta = si->beta;
         double vmax[1] = {0.0};
-        const int n = nb0 + threadIdx.x;
-        if (n < nb1) {
-            ColP<T> col{s, As, p_old, Ap_old, omega, beta};
-            for (int k = 0; k < 3; k++) {
-                const int i = 3 * n + k;
-                T rr, pp;
-                col.one(i, rr, pp);
-                x[i] = (T)__fma_rn(omega, (double)s[i], __fma_rn(alpha, (double)p_old[i], (double)x[i]));
-                r[i] = rr;
-                p_new[i] = pp;
-                vmax[0] = fmax(vmax[0], fabs((double)rr));
-            }
+        /* element-wise over the block's dofs, consecutive dofs on consecutive lanes (every load and store
+         * of a wave is one contiguous 512-byte run) */
+        ColP<T> col{s, As, p_old, Ap_old, omega, beta};
+        for (int i = 3 * nb0 + threadIdx.x; i < 3 * nb1; i += TPB) {
+            T rr, pp;
+            col.one(i, rr, pp);
+            x[i] = (T)__fma_rn(omega, (double)s[i], __fma_rn(alpha, (double)p_old[i], (double)x[i]));
+            r[i] = rr;
+            p_new[i] = pp;
+            vmax[0] = fmax(vmax[0], fabs((double)rr));
         }
         block_partial<1, true>(vmax, partials + PR_A * nblk, nblk, j);
     }
@@ -880,6 +891,7 @@ static Sell sell_of(adhk_ctx *c)
     Sell S;
     S.slice_ptr = c->slice_ptr;
     S.scol = c->scol;
+    S.diag_id = c->diag_id;
     return S;
 }
 

@@ -43,6 +43,7 @@ void adhk_solver_defaults(adhk_solver *s)
     s->precision = ADHK_PREC_FP64;
     s->lin_check_every = 8;
     s->mixed_lin_tol = 1e-6;
+    s->warm_gate = 1;
 }
 
 #define HC(x)                                                                                                     \
@@ -74,7 +75,7 @@ static void ctx_free(adhk_ctx *c)
                     c->minv, c->bvals32, c->minv32, c->erhs, c->emat, c->r, c->pv, c->Ap, c->s, c->As, c->q, c->x0,
                     c->pv2, c->Ap2, c->xk, c->r32, c->p32, c->Ap32, c->s32, c->As32, c->q32, c->xk32, c->p32b,
                     c->Ap32b, c->partials, c->counter, c->sc, c->flags, c->slice_ptr, c->scol, c->sell2blk,
-                    c->blk2sell, c->diag_slot, c->node_inst, c->elem_inst, c->seg_inst, c->blk_node,
+                    c->blk2sell, c->diag_slot, c->diag_id, c->node_inst, c->elem_inst, c->seg_inst, c->blk_node,
                     c->blk_inst, c->inst_blk, c->work};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++)
         if (ptrs[i]) hipFree(ptrs[i]);
@@ -1088,7 +1089,8 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
         LC(residual_dev(c));
         LC(norms(c, hw));
         for (int i = 0; i < K; i++) {
-            ls[i] = !(!hw[i].nanflag && !hs[i].nanflag && hw[i].rmax < hs[i].rmax); /* reject mask */
+            const int better = s->warm_gate == 1 ? hw[i].rl2sum < hs[i].rl2sum : hw[i].rmax < hs[i].rmax;
+            ls[i] = !(!hw[i].nanflag && !hs[i].nanflag && better); /* reject mask */
             if (!ls[i]) {
                 hs[i] = hw[i];
                 c->inst_stats[i].warm_accepted = 1;

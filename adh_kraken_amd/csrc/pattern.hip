@@ -95,7 +95,8 @@ __global__ void k_slice_width(int N, int nslice, const int *brow, int *sw)
     sw[s] = w;
 }
 
-/* one thread per (slice, lane): the node's blocks in column order, then zero padding */
+/* one thread per (slice, lane): the node's diagonal block in slot 0, its other blocks in column order,
+ * then zero padding (the Krylov SpMV takes the folded diagonal block as the identity without reading it) */
 __global__ void k_sell_fill(int N, int nslice, const int *slice_ptr, const int *brow, const int *bcol, int *scol,
                             int *sell2blk, int *blk2sell, int *diag_slot)
 {
@@ -103,20 +104,23 @@ __global__ void k_sell_fill(int N, int nslice, const int *slice_ptr, const int *
     if (t >= nslice * 64) return;
     int s = t >> 6, l = t & 63, n = t;
     int c0 = slice_ptr[s], w = slice_ptr[s + 1] - c0;
-    int b0 = 0, d = 0;
+    int b0 = 0, d = 0, bd = -1;
     if (n < N) {
         b0 = brow[n];
         d = brow[n + 1] - b0;
-        diag_slot[n] = -1;
+        for (int k = 0; k < d; k++)
+            if (bcol[b0 + k] == n) bd = k;
+        diag_slot[n] = bd >= 0 ? 0 : -1;
     }
     for (int k = 0; k < w; k++) {
         int pos = (c0 + k) * 64 + l;
         if (k < d) {
-            int b = b0 + k;
+            /* slot k holds block: the diagonal first, then the others in column order */
+            const int kk = bd < 0 ? k : (k == 0 ? bd : (k <= bd ? k - 1 : k));
+            int b = b0 + kk;
             scol[pos] = bcol[b];
             sell2blk[pos] = b;
             blk2sell[b] = pos;
-            if (bcol[b] == n) diag_slot[n] = k;
         } else {
             scol[pos] = (n < N) ? n : 0;
             sell2blk[pos] = -1;
@@ -149,6 +153,8 @@ static int build_sell(adhk_ctx *c)
     CHECK(hipMalloc((void **)&c->sell2blk, (size_t)c->nslot * 64 * sizeof(int)));
     CHECK(hipMalloc((void **)&c->blk2sell, (size_t)c->nnzb * sizeof(int)));
     CHECK(hipMalloc((void **)&c->diag_slot, (size_t)c->N * sizeof(int)));
+    CHECK(hipMalloc((void **)&c->diag_id, (size_t)(c->N > 0 ? c->N : 1)));
+    CHECK(hipMemsetAsync(c->diag_id, 0, (size_t)(c->N > 0 ? c->N : 1), c->stream));
     hipLaunchKernelGGL(k_sell_fill, dim3((c->nslice * 64 + 255) / 256), dim3(256), 0, c->stream, c->N, c->nslice,
                        c->slice_ptr, c->brow, c->bcol, c->scol, c->sell2blk, c->blk2sell, c->diag_slot);
     CHECK(hipGetLastError());

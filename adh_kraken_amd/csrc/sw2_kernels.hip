@@ -492,3 +492,123 @@ extern "C" int adhk_launch_dry_bc(adhk_ctx *c)
     CHECK_LAUNCH();
     return 0;
 }
+
+/* ---- grid mass (tl_find_grid_mass_elem2d, tl_grid_mass.c:199-262) ----
+ * Water volume of the triangles at the current depth, with the reference's treatment of partially dry
+ * elements: fe_3m2d_dry_wet_factornew (tl_grid_mass.c:442-536) gives the wet area fraction f (a node
+ * is dry when h < 0); one dry node: the full-element integral of h plus -(1 - f) times the integral
+ * of the negative nodal depths; two dry nodes: f times the integral of the clipped depths; otherwise
+ * f times the integral of h.  integrate_triangle_f (fe_integrals_2d.c:25) is c djac (f0+f1+f2)/3.
+ * One workgroup sums 256 elements; the host adds the workgroup sums in index order. */
+__device__ static double gm_wet_factor(const double *xx, const double *yy, const double *h, double djac,
+                                       int &num_dry)
+{
+    int total = 0, pass = 0;
+    for (int i = 0; i < 3; i++) {
+        const int d = h[i] < 0.0 ? 1 : 0;
+        total += d;
+        pass += d << i;
+    }
+    if (total == 3) {
+        num_dry = 3;
+        return 0.0;
+    }
+    if (total == 0) {
+        num_dry = 0;
+        return 1.0;
+    }
+    if (total != 1) pass = 7 - pass;
+    const int only = pass / 2;
+    double nx[3], ny[3];
+    for (int i = 0; i < 3; i++) {
+        if (i != only) {
+            const double xsi = h[i] / (h[i] - h[only]);
+            nx[i] = xx[i] + xsi * (xx[only] - xx[i]); /* FIXED_POS, macro.h:756 */
+            ny[i] = yy[i] + xsi * (yy[only] - yy[i]);
+        } else {
+            nx[i] = xx[i];
+            ny[i] = yy[i];
+        }
+    }
+    const double a = ((nx[1] - nx[0]) * (ny[2] - ny[0]) - (ny[1] - ny[0]) * (nx[2] - nx[0])) / 2.0; /* TRI_AREA */
+    if (a / djac < D_SMALL) {
+        const int i = total % 2;
+        num_dry = i == 1 ? 0 : 3;
+        return (double)i;
+    }
+    double factor = (total >> 1) - (total % 2);
+    num_dry = total;
+    factor *= a / djac;
+    return factor + ((factor < 0) ? 1.0 : 0.0);
+}
+
+__global__ __launch_bounds__(256) void k_grid_mass(MeshView m, double *__restrict__ part)
+{
+    __shared__ double sh[4];
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    double v = 0.0;
+    if (e < m.E) {
+        double xx[3], yy[3], h[3], hneg[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < 3; a++) {
+            const int n = m.tri[a * m.E + e];
+            xx[a] = m.x[n];
+            yy[a] = m.y[n];
+            h[a] = m.sol[3 * n];
+        }
+        const double djac = m.djac[e];
+        int num_dry;
+        const double f = gm_wet_factor(xx, yy, h, djac, num_dry);
+        double f1 = f, f2 = 0.0;
+        if (num_dry == 1) {
+            for (int a = 0; a < 3; a++)
+                if (h[a] < 0.0) hneg[a] = h[a];
+            f1 = 1.0;
+            f2 = -1.0 * (1.0 - f);
+        } else if (num_dry == 2) {
+            for (int a = 0; a < 3; a++)
+                if (h[a] < 0.0) {
+                    hneg[a] = h[a];
+                    h[a] = 0.0;
+                }
+        }
+        v = (f1 * djac * (h[0] + h[1] + h[2]) / 3.0) + (f2 * djac * (hneg[0] + hneg[1] + hneg[2]) / 3.0);
+    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+extern "C" int adhk_sw2_grid_mass(adhk_sw2 *c, double density, double *mass)
+{
+    if (!c || !mass) return -1;
+    const int nb = (c->E + 255) / 256;
+    double *part = NULL, *h = NULL;
+    int rc = -1;
+    if (nb == 0) {
+        *mass = 0.0;
+        return 0;
+    }
+    if (hipMalloc((void **)&part, (size_t)nb * sizeof(double)) != hipSuccess) {
+        adhk_set_error("grid_mass: out of device memory");
+        return -1;
+    }
+    h = (double *)malloc((size_t)nb * sizeof(double));
+    if (h) {
+        hipLaunchKernelGGL(k_grid_mass, dim3(nb), dim3(256), 0, c->stream, make_view(c), part);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(h, part, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) {
+            double s = 0.0;
+            for (int b = 0; b < nb; b++) s += h[b];
+            *mass = s * density;
+            rc = 0;
+        } else {
+            adhk_set_error("grid_mass: %s", hipGetErrorString(e));
+        }
+    }
+    free(h);
+    hipFree(part);
+    return rc;
+}

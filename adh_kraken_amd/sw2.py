@@ -261,6 +261,12 @@ class SModelSuper:
     def get_dt(self):
         return self.lib.adhk_sw2_get_dt(self.h)
 
+    def grid_mass(self, density=1.0):
+        """Water volume (x density) at the current depth (tl_find_grid_mass_elem2d)."""
+        out = C.c_double()
+        L.check(self.lib.adhk_sw2_grid_mass(self.h, float(density), C.byref(out)), "adhk_sw2_grid_mass")
+        return out.value
+
     # time series (SSERIES, src/structs/sseries/sseries.c): boundary values that follow a hydrograph
     def set_series(self, sid, t, v):
         t = np.ascontiguousarray(t, np.float64)

@@ -43,12 +43,13 @@ def log(msg):
 def algorithmic_bytes(kind, N, nslot, ndofs, mixed, split=False):
     """HBM bytes of one launch with every operand moved once.
 
-    matrix: the left-preconditioned M^-1 A in SELL-64, nslot*64 slots x (9 values + 1 column index) + slice
-    pointers; vectors: ndofs values each.
+    matrix: the left-preconditioned M^-1 A in SELL-64 without its diagonal blocks (the identity, slot 0 of
+    every slice, not read): (nslot - nslice)*64 slots x (9 values + 1 column index), slice pointers, and a
+    byte per node that marks the identity diagonals; vectors: ndofs values each.
     """
     vb = 4 if mixed else 8
     nslice = (N + 63) // 64
-    mat = nslot * 64 * (9 * vb + 4) + (nslice + 1) * 4
+    mat = (nslot - nslice) * 64 * (9 * vb + 4) + (nslice + 1) * 4 + N
     if kind == "spmv_p" and split:  # k_spmv_p: read p q, write Ap
         return mat + 3 * ndofs * vb
     if kind == "spmv_p":  # k_iter_a: read x s As p Ap q, write x r p' Ap'
@@ -330,7 +331,7 @@ def main():
         out["mixed_learned"] = {
             "value": round(vml, 4), "unit": "timesteps/s", "speedup_vs_fp64": round(vml / value, 3),
             "variant": "mixed precision + learned per-node MLP warm start (12-64-64-3 GELU, BF16 weights); the "
-                       "start with the smaller max-norm residual of {learned, previous level} is kept",
+                       "start with the smaller l2-norm residual of {learned, previous level} is kept",
             "warm_accepted_steps": ml["stats"]["warm_accepted"], "newton_its": ml["stats"]["newton_its"],
             "bicgstab_its": ml["stats"]["lin_its_total"],
             "depth_rel_err_vs_fp64": float(np.abs(hml - h64).max() / np.abs(h64).max()),

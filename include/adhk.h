@@ -67,6 +67,8 @@ typedef struct {
     int lin_check_every;     /* BiCGStab iterations launched between convergence read-backs (>=1) */
     double mixed_lin_tol;    /* inner (FP32) BiCGStab tolerance of ADHK_PREC_MIXED; the FP64 Newton
                                 residual test at tol_nonlin is unchanged (iterative refinement) */
+    int warm_gate;           /* learned start kept when its residual is below the previous level's in
+                                1: l2 norm (default), 0: max norm (no counterpart in the reference) */
 } adhk_solver;
 
 typedef struct {
@@ -248,6 +250,11 @@ int adhk_flume_build(const adhk_flume_spec *f, double *x, double *y, double *z, 
  * surface at the tailwater level of the bump-free bed; ic_flat_wse = 0 -> h = tailwater everywhere */
 int adhk_flume_initial_state(const adhk_flume_spec *f, int nnodes, const double *z, double *sol);
 
+/* ---- diagnostics.  tl_find_grid_mass_elem2d (tl_grid_mass.c:199-262): the water volume of the
+ *      context's triangles at the current depth, times density, with the reference's weighting of
+ *      partially dry elements; summed over all instances of an ensemble */
+int adhk_sw2_grid_mass(adhk_sw2 *c, double density, double *mass);
+
 /* ---- time-series boundary values (sseries.c, time_loop.c:74-76).  A series is a piecewise-linear
  *      table of (time, value); each step of adhk_time_loop sets every segment bound to a series to
  *      scale * (the series' average over the step [t, t + dt]), as sseries_set_values_average
@@ -316,7 +323,8 @@ double adhk_channel_section_weight(const adhk_channel_spec *s, double x, double 
  *      area-weighted over the node's triangles.
  *      It is trained on the GPU on (features, change) pairs collected from solver trajectories.
  *      With a model attached, fe_newton evaluates the residual at the previous level and at the
- *      predicted iterate and starts from the smaller (max norm); iteration and stop test are unchanged. */
+ *      predicted iterate and starts from the smaller (l2 norm by default, adhk_solver.warm_gate);
+ *      iteration and stop test are unchanged. */
 #define ADHK_MLP_NIN 12
 #define ADHK_MLP_HID 64
 #define ADHK_MLP_NOUT 3

@@ -69,6 +69,7 @@ def load():
         "or_sw2_body_resid": (None, [P, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]),
         "or_sw2_seg_resid": (None, [P, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]),
         "or_wdflag": (None, [P, ip]),
+        "or_grid_mass": (C.c_double, [P, C.c_double]),
         "or_sw2_init": (None, [P]),
         "or_assemble_residual": (None, [P, dp]),
         "or_pattern": (None, [P, ip, ip]),
@@ -155,6 +156,10 @@ class Oracle:
 
     def set_initial(self, sol):
         self.set_state(sol, sol, sol)
+
+    def grid_mass(self, density=1.0):
+        """Water volume (x density) of the current depth, tl_find_grid_mass_elem2d."""
+        return self.lib.or_grid_mass(self.h, float(density))
 
     def set_seg_values(self, val):
         """Boundary values (S, 2) for the following calls (the context reads this array in place)."""

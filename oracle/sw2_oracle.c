@@ -1426,3 +1426,67 @@ int or_time_loop(or_ctx *c, const or_solver *sv, int nsteps, or_newton_stats *ac
     if (acc) *acc = tot;
     return ok ? k : -k;
 }
+
+/* fe_3m2d_dry_wet_factornew, tl_grid_mass.c:442-536: wet area fraction, a node dry when h < 0 */
+static double gm_wet_factor(const double *xx, const double *yy, const double *h, double djac, int *num_dry)
+{
+    int total = 0, pass = 0;
+    for (int i = 0; i < 3; i++) {
+        int d = (h[i] < 0.0) ? 1 : 0;
+        total += d;
+        pass += (d << i);
+    }
+    if (total == 3) { *num_dry = 3; return 0.0; }
+    if (total == 0) { *num_dry = 0; return 1.0; }
+    if (total != 1) pass = 7 - pass;
+    int only = pass / 2;
+    double nx[3], ny[3];
+    for (int i = 0; i < 3; i++) {
+        if (i != only) {
+            double xsi = h[i] / (h[i] - h[only]);
+            nx[i] = xx[i] + xsi * (xx[only] - xx[i]);
+            ny[i] = yy[i] + xsi * (yy[only] - yy[i]);
+        } else {
+            nx[i] = xx[i];
+            ny[i] = yy[i];
+        }
+    }
+    double a = ((nx[1] - nx[0]) * (ny[2] - ny[0]) - (ny[1] - ny[0]) * (nx[2] - nx[0])) / 2.0;
+    if (a / djac < SMALL) {
+        int i = total % 2;
+        *num_dry = (i == 1) ? 0 : 3;
+        return (double)i;
+    }
+    double factor = (total >> 1) - (total % 2);
+    *num_dry = total;
+    factor *= a / djac;
+    return factor + ((factor < 0) ? 1.0 : 0.0);
+}
+
+/* tl_find_grid_mass_elem2d, tl_grid_mass.c:199-262 (triangles), on the current depth */
+double or_grid_mass(const or_ctx *c, double density)
+{
+    double mass = 0.0;
+    for (int e = 0; e < c->m.nelems2d; e++) {
+        const int *t = c->m.tri + 3 * e;
+        double xx[3], yy[3], h[3], hneg[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < 3; a++) {
+            xx[a] = c->m.x[t[a]];
+            yy[a] = c->m.y[t[a]];
+            h[a] = c->sol[3 * t[a]];
+        }
+        int num_dry;
+        double f = gm_wet_factor(xx, yy, h, c->djac[e], &num_dry), f1 = f, f2 = 0.0;
+        if (num_dry == 1) {
+            for (int a = 0; a < 3; a++)
+                if (h[a] < 0.0) hneg[a] = h[a];
+            f1 = 1.0;
+            f2 = -1.0 * (1.0 - f);
+        } else if (num_dry == 2) {
+            for (int a = 0; a < 3; a++)
+                if (h[a] < 0.0) { hneg[a] = h[a]; h[a] = 0.0; }
+        }
+        mass += (f1 * c->djac[e] * (h[0] + h[1] + h[2]) / 3.0) + (f2 * c->djac[e] * (hneg[0] + hneg[1] + hneg[2]) / 3.0);
+    }
+    return mass * density;
+}

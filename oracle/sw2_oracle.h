@@ -108,6 +108,7 @@ void or_sw2_seg_resid(const or_ctx *c, int ie, double perturbation, int pnode, i
 
 /* wet/dry flags, fe_sw2_wdflag_legacy (sw2_wdflag_legacy.c:34-139) */
 void or_wdflag(or_ctx *c, int *wd_flag_out /* may be NULL */);
+double or_grid_mass(const or_ctx *c, double density); /* tl_find_grid_mass_elem2d on the current depth */
 /* fe_sw2_init (fe_sw2_init.c:28-53): wd flags + dry nodes -> u,v Dirichlet */
 void or_sw2_init(or_ctx *c);
 

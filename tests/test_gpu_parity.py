@@ -286,3 +286,14 @@ def test_c2_full_size_pattern_and_residual(gpu):
     idx = rng.choice(mesh.nelems2d, 2000, replace=False)
     eo = np.array([orc.body_resid(int(e), s) for e in idx])
     assert rel_err(eg[idx], eo) <= 1e-12
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_grid_mass_matches_oracle(gpu, c1, seed):
+    mesh, sol0 = c1
+    rng = np.random.default_rng(seed)
+    s = sol0.copy()
+    if seed:
+        s[0::3] = rng.uniform(-0.5, 1.0, mesh.nnodes)  # every dry-node count occurs
+    sm, orc = make_pair(mesh, s)
+    assert abs(sm.grid_mass(1000.0) - orc.grid_mass(1000.0)) <= 1e-12 * abs(orc.grid_mass(1000.0))

@@ -460,3 +460,38 @@ def test_newton_converged_state_satisfies_tolerance(c1):
     sol, sol_old, _ = orc.state()
     orc.set_state(sol, sol_old, sol_old)
     assert np.abs(orc.residual()).max() <= max(1e-8, 1e3 * st["inc_max_norm"])
+
+
+def test_grid_mass():
+    """tl_find_grid_mass_elem2d: all wet, the exact volume sum(area * mean h); with dry nodes, the
+    reference's weights with the wet area fraction taken from the clipped wet polygon (an independent
+    construction of the fraction the reference interpolates)."""
+    from sw2_quadrature import wet_pieces
+    spec, (mesh, sol0) = small_flume(nbumps=4, flat=1)
+    t = mesh.tri
+    area = 0.5 * ((mesh.x[t[:, 1]] - mesh.x[t[:, 0]]) * (mesh.y[t[:, 2]] - mesh.y[t[:, 0]]) -
+                  (mesh.x[t[:, 2]] - mesh.x[t[:, 0]]) * (mesh.y[t[:, 1]] - mesh.y[t[:, 0]]))
+    orc = O.Oracle(mesh, dt=1.0)
+    orc.set_initial(sol0)
+    h = sol0[0::3][t]
+    assert abs(orc.grid_mass(1000.0) - 1000.0 * np.sum(area * h.mean(1))) <= 1e-12 * 1000.0 * np.sum(area * h.mean(1))
+    rng = np.random.default_rng(12)
+    s = sol0.copy()
+    s[0::3] = rng.uniform(-0.6, 1.0, mesh.nnodes)
+    orc.set_initial(s)
+    expect = 0.0
+    counts = np.zeros(4, int)
+    for e in range(mesh.nelems2d):
+        he = s[0::3][t[e]]
+        nd = int((he < 0).sum())
+        counts[nd] += 1
+        pieces, _ = wet_pieces(he)
+        f = sum(abs(np.linalg.det(V)) for V in pieces)
+        if nd == 0:
+            expect += area[e] * he.sum() / 3
+        elif nd == 1:
+            expect += area[e] * he.sum() / 3 - (1 - f) * area[e] * he[he < 0].sum() / 3
+        elif nd == 2:
+            expect += f * area[e] * he[he > 0].sum() / 3
+    assert counts.min() > 0  # every case occurs
+    assert abs(orc.grid_mass() - expect) <= 1e-12 * np.sum(area)
