@@ -297,3 +297,20 @@ def test_grid_mass_matches_oracle(gpu, c1, seed):
         s[0::3] = rng.uniform(-0.5, 1.0, mesh.nnodes)  # every dry-node count occurs
     sm, orc = make_pair(mesh, s)
     assert abs(sm.grid_mass(1000.0) - orc.grid_mass(1000.0)) <= 1e-12 * abs(orc.grid_mass(1000.0))
+
+
+def test_c2_full_size_time_steps_depth(gpu):
+    """The benchmark workload itself: 3 time steps of the 1.2M-dof flume against the oracle (16 threads),
+    depth 1e-8 and velocity 1e-6 relative, same Newton iteration counts."""
+    mesh, sol0 = A.flume_mesh(A.FLUME_C2)
+    sm = A.SModelSuper(mesh, dt=2.0)
+    sm.set_initial(sol0)
+    orc = O.Oracle(mesh, dt=2.0, threads=16)
+    orc.set_initial(sol0)
+    ng, sg = A.time_loop(sm, 3)
+    no, so = orc.time_loop(3)
+    assert ng == no == 3
+    assert sg["newton_its"] == so["newton_its"]
+    gs, os_ = sm.sol, orc.state()[0]
+    assert rel_err(gs[0::3], os_[0::3]) <= 1e-8
+    assert rel_err(gs[1::3], os_[1::3]) <= 1e-6 and rel_err(gs[2::3], os_[2::3]) <= 1e-6
