@@ -740,14 +740,17 @@ int adhk_elem_jacobians(adhk_sw2 *c, double *out)
     if (!out) return 0;
     tmp = malloc(n * sizeof(double));
     HC(hipMemcpy(tmp, c->emat, n * sizeof(double), hipMemcpyDeviceToHost));
-    /* device layout [elem][col][row] -> row-major [elem][row][col] */
+    /* device layout by 3x3 node block (entry (3a+r, 3b+c) at 9 (3a+b) + 3r + c; segments 9 (2a+b) + 3r + c)
+     * -> row-major [elem][row][col] */
     for (size_t e = 0; e < (size_t)c->E; e++)
         for (int r = 0; r < 9; r++)
-            for (int cc = 0; cc < 9; cc++) out[81 * e + 9 * r + cc] = tmp[81 * e + 9 * cc + r];
+            for (int cc = 0; cc < 9; cc++)
+                out[81 * e + 9 * r + cc] = tmp[81 * e + 9 * (3 * (r / 3) + cc / 3) + 3 * (r % 3) + cc % 3];
     size_t off = 81 * (size_t)c->E;
     for (size_t e = 0; e < (size_t)c->S; e++)
         for (int r = 0; r < 6; r++)
-            for (int cc = 0; cc < 6; cc++) out[off + 36 * e + 6 * r + cc] = tmp[off + 36 * e + 6 * cc + r];
+            for (int cc = 0; cc < 6; cc++)
+                out[off + 36 * e + 6 * r + cc] = tmp[off + 36 * e + 9 * (2 * (r / 3) + cc / 3) + 3 * (r % 3) + cc % 3];
     free(tmp);
     return 0;
 fail:

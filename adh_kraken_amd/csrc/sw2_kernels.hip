@@ -275,6 +275,18 @@ extern "C" int adhk_launch_resid_gather(adhk_ctx *c)
 }
 
 /* ---------------- FD Jacobian ---------------- */
+/* Elemental matrices are stored by 3x3 node block: entry (row 3a+r, column 3b+c) of triangle e is at
+ * 81 e + 9 (3a + b) + 3r + c (segments: 36 s + 9 (2a + b) + 3r + c), so that the assembly gather reads
+ * each node pair's block as one contiguous 72-byte run. */
+__device__ __forceinline__ size_t emat_tri(int e, int row, int col)
+{
+    return 81 * (size_t)e + 9 * (3 * (row / 3) + col / 3) + 3 * (row % 3) + col % 3;
+}
+__device__ __forceinline__ size_t emat_seg(int s, int row, int col)
+{
+    return 36 * (size_t)s + 9 * (2 * (row / 3) + col / 3) + 3 * (row % 3) + col % 3;
+}
+
 /* one thread per (element, column): column = node i * 3 + var k; two residual
  * evaluations (+eps, -eps), load_elem_mat/perturb_var assemble_jacobian.c:123-363 */
 __global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, double *__restrict__ emat, int *flags,
@@ -305,9 +317,8 @@ __global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, double *
     int pa = sw2_tri_resid(in, P, i, k, (+1) * eps, rp);
     int pb = sw2_tri_resid(in, P, i, k, (-1) * eps, rm);
     if (pa | pb) flags[0] = 1; /* cannot happen after tri_partial_fd; kept as a guard */
-    double *out = emat + 81 * (size_t)e + 9 * col;
 #pragma unroll
-    for (int r = 0; r < 9; r++) out[r] = (rp[r] - rm[r]) / eps2;
+    for (int r = 0; r < 9; r++) emat[emat_tri(e, r, col)] = (rp[r] - rm[r]) / eps2;
 }
 
 /* queued elements: nine columns each, central differences of the general wet/dry residual */
@@ -326,8 +337,7 @@ __global__ __launch_bounds__(128) void k_elem_jac_wd(MeshView m, Sw2Par P, const
         double rp[9], rm[9];
         sw2_tri_resid_general(in, P, i, k, (+1) * eps, rp);
         sw2_tri_resid_general(in, P, i, k, (-1) * eps, rm);
-        double *out = emat + 81 * (size_t)e + 9 * col;
-        for (int r = 0; r < 9; r++) out[r] = (rp[r] - rm[r]) / eps2;
+        for (int r = 0; r < 9; r++) emat[emat_tri(e, r, col)] = (rp[r] - rm[r]) / eps2;
     }
 }
 
@@ -353,8 +363,7 @@ __global__ void k_seg_jac(SegView s, Sw2Par P, double *__restrict__ emat)
                   P, i, k, (+1) * eps, rp);
     sw2_seg_resid(s.kind[e], s.val[2 * e], s.val[2 * e + 1], s.nrml[2 * e], s.nrml[2 * e + 1], s.sdjac[e], h, u, v,
                   P, i, k, (-1) * eps, rm);
-    double *out = emat + 36 * (size_t)e + 6 * col;
-    for (int r = 0; r < 6; r++) out[r] = (rp[r] - rm[r]) / eps2;
+    for (int r = 0; r < 6; r++) emat[emat_seg(e, r, col)] = (rp[r] - rm[r]) / eps2;
 }
 
 extern "C" int adhk_launch_elem_jac(adhk_ctx *c)
@@ -381,7 +390,8 @@ extern "C" int adhk_launch_elem_jac(adhk_ctx *c)
 }
 
 /* one thread per SELL slot: block entry (3r+cc) = sum over the block's contributions (element order)
- * of the elemental entry (row a*3+r, col bb*3+cc); emat is stored [elem][col][row]; padding slots get 0 */
+ * of the elemental entry (row a*3+r, col bb*3+cc), one contiguous 3x3 block of emat each; padding
+ * slots get 0 */
 __global__ __launch_bounds__(256) void k_jac_gather(int nslot64, int E, const int *__restrict__ sell2blk,
                                                     const int *__restrict__ cptr, const int *__restrict__ contrib,
                                                     const double *__restrict__ emat, double *__restrict__ bvals)
@@ -397,25 +407,21 @@ __global__ __launch_bounds__(256) void k_jac_gather(int nslot64, int E, const in
     for (int j = j0; j < j1; j++) {
         int cde = contrib[j];
         const double *em;
-        int a, bb, ld;
+        int a, bb;
         if (cde < 9 * E) {
             int e = cde / 9;
             a = (cde % 9) / 3;
             bb = cde % 3;
-            em = emat + 81L * e;
-            ld = 9;
+            em = emat + 81L * e + 9 * (3 * a + bb);
         } else {
             int t = cde - 9 * E;
             int s = t / 4;
             a = (t % 4) / 2;
             bb = t % 2;
-            em = emat + segbase + 36L * s;
-            ld = 6;
+            em = emat + segbase + 36L * s + 9 * (2 * a + bb);
         }
 #pragma unroll
-        for (int r = 0; r < 3; r++)
-#pragma unroll
-            for (int cc = 0; cc < 3; cc++) acc[3 * r + cc] += em[(3 * bb + cc) * ld + 3 * a + r];
+        for (int k = 0; k < 9; k++) acc[k] += em[k];
     }
     double *out = bvals + (size_t)(pos >> 6) * 9 * 64 + (pos & 63);
 #pragma unroll
