@@ -541,6 +541,146 @@ extern "C" int adhk_launch_scale_prec(adhk_ctx *c, int mixed)
     return 0;
 }
 
+/* ---- the Newton system in two matrix passes ----
+ * apply_Dirichlet_BC + scale_linear_system + the block-Jacobi fold, as k_dirichlet_scale followed by
+ * k_scale_prec compute them (same entries, same scale factors, same inverse), but with the Dirichlet
+ * rows and columns applied on the fly: pass 1 reads the matrix for the row maxima (no write), pass 2
+ * reads it once more and writes the final operator.  Four matrix passes become two (plus the FP32
+ * copy in mixed precision). */
+__device__ __forceinline__ double dirichlet_entry(double v, int n, int m, int r, int cc, const int *brc,
+                                                  const int *bcc)
+{
+    if (brc[r] == 1) return (m == n && cc == r) ? 1.0 : 0.0;
+    if (bcc[cc] == 1) return 0.0;
+    return v;
+}
+
+__global__ __launch_bounds__(TPB) void k_prep_rowmax(int N, const int *__restrict__ brow,
+                                                     const int *__restrict__ slice_ptr,
+                                                     const int *__restrict__ scol, const double *__restrict__ bv,
+                                                     const int *__restrict__ bc, double *__restrict__ res,
+                                                     double *__restrict__ dsol, double *__restrict__ scale)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int lane = n & 63, c0 = slice_ptr[n >> 6], deg = brow[n + 1] - brow[n];
+    const int brc[3] = {bc[3 * n], bc[3 * n + 1], bc[3 * n + 2]};
+    double mx[3] = {0., 0., 0.};
+    for (int k = 0; k < deg; k++) {
+        const int cs = c0 + k;
+        const int m = scol[cs * 64 + lane];
+        const int bcc[3] = {bc[3 * m], bc[3 * m + 1], bc[3 * m + 2]};
+        for (int r = 0; r < 3; r++)
+            for (int cc = 0; cc < 3; cc++) {
+                const double t = fabs(dirichlet_entry(bv[sell_val(cs, 3 * r + cc, lane)], n, m, r, cc, brc, bcc));
+                if (t > mx[r]) mx[r] = t;
+            }
+    }
+    const double small_factor = 1.0 / sqrt(D_SMALL);
+    for (int r = 0; r < 3; r++) {
+        const int i = 3 * n + r;
+        const double sc = (mx[r] < D_SMALL) ? small_factor : 1.0 / sqrt(fabs(mx[r]));
+        scale[i] = sc;
+        const double b = (brc[r] == 1) ? 0.0 : res[i];
+        res[i] = b * sc;
+        dsol[i] = 0.0;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_prep_fold(int N, const int *__restrict__ brow,
+                                                   const int *__restrict__ slice_ptr, const int *__restrict__ scol,
+                                                   const int *__restrict__ bc, double *__restrict__ bv,
+                                                   const double *__restrict__ scale, double *__restrict__ minv,
+                                                   T *__restrict__ bv_t, T *__restrict__ minv_t,
+                                                   unsigned char *__restrict__ diag_id)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int lane = n & 63, c0 = slice_ptr[n >> 6], w = slice_ptr[(n >> 6) + 1] - c0, deg = brow[n + 1] - brow[n];
+    const int brc[3] = {bc[3 * n], bc[3 * n + 1], bc[3 * n + 2]};
+    const double si[3] = {scale[3 * n], scale[3 * n + 1], scale[3 * n + 2]};
+    /* the scaled diagonal block (slot 0) and its inverse */
+    double a[9];
+    for (int r = 0; r < 3; r++)
+        for (int cc = 0; cc < 3; cc++)
+            a[3 * r + cc] = (deg > 0) ? dirichlet_entry(bv[sell_val(c0, 3 * r + cc, lane)], n, n, r, cc, brc, brc) *
+                                            (si[r] * si[cc])
+                                      : 0.0;
+    double c00 = a[4] * a[8] - a[5] * a[7];
+    double c01 = a[5] * a[6] - a[3] * a[8];
+    double c02 = a[3] * a[7] - a[4] * a[6];
+    double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    double m[9];
+    diag_id[n] = fabs(det) > 1e-300 ? 1 : 0;
+    if (fabs(det) > 1e-300) {
+        double id = 1.0 / det;
+        m[0] = c00 * id;
+        m[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+        m[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+        m[3] = c01 * id;
+        m[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+        m[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+        m[6] = c02 * id;
+        m[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+        m[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+    } else {
+        for (int k = 0; k < 9; k++) m[k] = 0.0;
+        for (int k = 0; k < 3; k++) m[4 * k] = (a[4 * k] != 0.0) ? 1.0 / a[4 * k] : 1.0;
+    }
+    double *mo = minv + (size_t)(n >> 6) * 9 * 64 + lane;
+    for (int k = 0; k < 9; k++) {
+        mo[k * 64] = m[k];
+        if (minv_t) minv_t[(size_t)(n >> 6) * 9 * 64 + lane + k * 64] = (T)m[k];
+    }
+    /* every slot: Dirichlet, scaling, fold; FP64: M^-1 A in place; mixed: the scaled matrix stays in
+     * FP64 and M^-1 A goes to the FP32 copy */
+    for (int k = 0; k < w; k++) {
+        const int cs = c0 + k;
+        double b[9];
+        if (k < deg) {
+            const int mm = scol[cs * 64 + lane];
+            const int bcc[3] = {bc[3 * mm], bc[3 * mm + 1], bc[3 * mm + 2]};
+            const double sj[3] = {scale[3 * mm], scale[3 * mm + 1], scale[3 * mm + 2]};
+            for (int r = 0; r < 3; r++)
+                for (int cc = 0; cc < 3; cc++)
+                    b[3 * r + cc] =
+                        dirichlet_entry(bv[sell_val(cs, 3 * r + cc, lane)], n, mm, r, cc, brc, bcc) * (si[r] * sj[cc]);
+        } else {
+            for (int e = 0; e < 9; e++) b[e] = 0.0;
+        }
+        for (int r = 0; r < 3; r++)
+            for (int cc = 0; cc < 3; cc++) {
+                const double f = __fma_rn(m[3 * r + 2], b[6 + cc], __fma_rn(m[3 * r + 1], b[3 + cc], m[3 * r] * b[cc]));
+                const size_t ix = sell_val(cs, 3 * r + cc, lane);
+                if (bv_t) {
+                    bv_t[ix] = (T)f;
+                    if (k < deg) bv[ix] = b[3 * r + cc];
+                } else if (k < deg) {
+                    bv[ix] = f;
+                }
+            }
+    }
+}
+
+extern "C" int adhk_launch_prep(adhk_ctx *c, int mixed)
+{
+    adhk_kt_begin(c, ADHK_KT_PREP);
+    const dim3 g((c->N + TPB - 1) / TPB);
+    hipLaunchKernelGGL(k_prep_rowmax, g, dim3(TPB), 0, c->stream, c->N, c->brow, c->slice_ptr, c->scol, c->bvals,
+                       c->bc_mask, c->residual, c->dsol, c->scale);
+    CHECK_LAUNCH();
+    if (mixed)
+        hipLaunchKernelGGL(k_prep_fold<float>, g, dim3(TPB), 0, c->stream, c->N, c->brow, c->slice_ptr, c->scol,
+                           c->bc_mask, c->bvals, c->scale, c->minv, c->bvals32, c->minv32, c->diag_id);
+    else
+        hipLaunchKernelGGL(k_prep_fold<double>, g, dim3(TPB), 0, c->stream, c->N, c->brow, c->slice_ptr, c->scol,
+                           c->bc_mask, c->bvals, c->scale, c->minv, (double *)NULL, (double *)NULL, c->diag_id);
+    CHECK_LAUNCH();
+    adhk_kt_end(c);
+    return 0;
+}
+
 /* ---------------- BiCGStab (bcgstab_solver.c:65-335) ----------------
  * Kernel sequence: init1, init2, A<first>, S, then {A, S} per iteration.  A of iteration k+1
  * finishes iteration k (x and r updates and the ||r||_inf test) and starts k+1 (new direction p and

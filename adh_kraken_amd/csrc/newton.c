@@ -974,8 +974,7 @@ int adhk_sw2_autotune(adhk_sw2 *c, const adhk_solver *s, int ninner, adhk_tuning
     c->tune.elem_tpb = best.elem_tpb;
     /* 2. the Newton system of the current state */
     LC(jacobian_dev(c));
-    LC(adhk_launch_dirichlet(c, 1));
-    LC(adhk_launch_scale_prec(c, mixed));
+    LC(adhk_launch_prep(c, mixed));
     /* 3. BiCGStab schedules: complete solves, best of ninner */
     for (int sp = 0; sp <= 1; sp++)
     for (int xm = 1; xm >= 0; xm--)
@@ -1121,8 +1120,7 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
         LC(set_work(c, act));
         LC(jacobian_dev(c));
         LC(adhk_launch_check_diag(c));
-        LC(adhk_launch_dirichlet(c, 1));
-        LC(adhk_launch_scale_prec(c, mixed));
+        LC(adhk_launch_prep(c, mixed));
         LC(krylov(c, s, mixed, hw, lits));
         LC(adhk_launch_increment(c));
         for (int i = 0; i < K; i++)
