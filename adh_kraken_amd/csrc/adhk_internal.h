@@ -87,6 +87,7 @@ struct adhk_sw2 {
     float *bvals32, *minv32;
     /* elemental buffers */
     double *erhs, *emat;
+    float *emat32; /* FP32 elemental Jacobians of the mixed-precision Newton iteration */
     /* Krylov vectors (FP64 and FP32 sets); p and Ap are double-buffered: one BiCGStab kernel
      * reads the previous direction while it writes the next */
     double *r, *pv, *Ap, *s, *As, *q, *x0, *pv2, *Ap2, *xk;
@@ -135,6 +136,9 @@ int adhk_launch_elem_resid(adhk_ctx *c);
 int adhk_launch_resid_gather(adhk_ctx *c);
 int adhk_launch_elem_jac(adhk_ctx *c);
 int adhk_launch_jac_gather(adhk_ctx *c);
+int adhk_launch_elem_jac_p(adhk_ctx *c, int f32);   /* f32: FP32 elemental matrices (emat32) */
+int adhk_launch_jac_gather_p(adhk_ctx *c, int f32); /* f32: emat32 -> the FP32 matrix bvals32 */
+int adhk_launch_check_diag_p(adhk_ctx *c, int f32);
 
 /* ---- launchers (pattern.hip) ---- */
 int adhk_build_pattern(adhk_ctx *c);

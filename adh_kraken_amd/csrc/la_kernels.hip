@@ -369,8 +369,9 @@ extern "C" int adhk_launch_norms(adhk_ctx *c)
 }
 
 /* check_diag, matrix_utilities.c:120-152 */
+template <typename TB>
 __global__ void k_check_diag(int N, const int *__restrict__ slice_ptr, const int *__restrict__ diag_slot,
-                             const double *__restrict__ bv, int *__restrict__ bc, const int *__restrict__ node_inst,
+                             const TB *__restrict__ bv, int *__restrict__ bc, const int *__restrict__ node_inst,
                              const int *__restrict__ work)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -378,20 +379,26 @@ __global__ void k_check_diag(int N, const int *__restrict__ slice_ptr, const int
     int d = diag_slot[n];
     int cs = slice_ptr[n >> 6] + d;
     for (int r = 0; r < 3; r++) {
-        double a = (d >= 0) ? bv[sell_val(cs, 4 * r, n & 63)] : 0.0;
+        double a = (d >= 0) ? (double)bv[sell_val(cs, 4 * r, n & 63)] : 0.0;
         if (fabs(a) < D_SMALL) bc[3 * n + r] = 1;
     }
 }
 
-extern "C" int adhk_launch_check_diag(adhk_ctx *c)
+/* f32: the matrix is the FP32 one the mixed-precision Newton iteration assembles */
+extern "C" int adhk_launch_check_diag_p(adhk_ctx *c, int f32)
 {
     adhk_kt_begin(c, ADHK_KT_PREP);
-    hipLaunchKernelGGL(k_check_diag, dim3((c->N + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->N, c->slice_ptr,
-                       c->diag_slot, c->bvals, c->bc_mask, c->node_inst, c->work);
+    if (f32)
+        hipLaunchKernelGGL(k_check_diag<float>, dim3((c->N + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->N,
+                           c->slice_ptr, c->diag_slot, c->bvals32, c->bc_mask, c->node_inst, c->work);
+    else
+        hipLaunchKernelGGL(k_check_diag<double>, dim3((c->N + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->N,
+                           c->slice_ptr, c->diag_slot, c->bvals, c->bc_mask, c->node_inst, c->work);
     CHECK_LAUNCH();
     adhk_kt_end(c);
     return 0;
 }
+extern "C" int adhk_launch_check_diag(adhk_ctx *c) { return adhk_launch_check_diag_p(c, 0); }
 
 /* apply_Dirichlet_BC (matrix_utilities.c:42-105): Dirichlet columns -> 0, Dirichlet rows -> identity,
  * residual -> 0; then (do_scale) the row maxima of scale_linear_system (scale_linear_system.c:31-121):
@@ -545,8 +552,8 @@ extern "C" int adhk_launch_scale_prec(adhk_ctx *c, int mixed)
  * apply_Dirichlet_BC + scale_linear_system + the block-Jacobi fold, as k_dirichlet_scale followed by
  * k_scale_prec compute them (same entries, same scale factors, same inverse), but with the Dirichlet
  * rows and columns applied on the fly: pass 1 reads the matrix for the row maxima (no write), pass 2
- * reads it once more and writes the final operator.  Four matrix passes become two (plus the FP32
- * copy in mixed precision). */
+ * reads it once more and writes the final operator in place.  Four matrix passes become two.  In mixed
+ * precision both passes work on the FP32 matrix the assembly wrote, with FP64 arithmetic. */
 __device__ __forceinline__ double dirichlet_entry(double v, int n, int m, int r, int cc, const int *brc,
                                                   const int *bcc)
 {
@@ -555,9 +562,10 @@ __device__ __forceinline__ double dirichlet_entry(double v, int n, int m, int r,
     return v;
 }
 
+template <typename TB>
 __global__ __launch_bounds__(TPB) void k_prep_rowmax(int N, const int *__restrict__ brow,
                                                      const int *__restrict__ slice_ptr,
-                                                     const int *__restrict__ scol, const double *__restrict__ bv,
+                                                     const int *__restrict__ scol, const TB *__restrict__ bv,
                                                      const int *__restrict__ bc, double *__restrict__ res,
                                                      double *__restrict__ dsol, double *__restrict__ scale)
 {
@@ -572,7 +580,8 @@ __global__ __launch_bounds__(TPB) void k_prep_rowmax(int N, const int *__restric
         const int bcc[3] = {bc[3 * m], bc[3 * m + 1], bc[3 * m + 2]};
         for (int r = 0; r < 3; r++)
             for (int cc = 0; cc < 3; cc++) {
-                const double t = fabs(dirichlet_entry(bv[sell_val(cs, 3 * r + cc, lane)], n, m, r, cc, brc, bcc));
+                const double t =
+                    fabs(dirichlet_entry((double)bv[sell_val(cs, 3 * r + cc, lane)], n, m, r, cc, brc, bcc));
                 if (t > mx[r]) mx[r] = t;
             }
     }
@@ -587,12 +596,12 @@ __global__ __launch_bounds__(TPB) void k_prep_rowmax(int N, const int *__restric
     }
 }
 
-template <typename T>
+/* TB: the matrix storage, folded in place (FP64, or FP32 in mixed precision); minv in the same type */
+template <typename TB>
 __global__ __launch_bounds__(TPB) void k_prep_fold(int N, const int *__restrict__ brow,
                                                    const int *__restrict__ slice_ptr, const int *__restrict__ scol,
-                                                   const int *__restrict__ bc, double *__restrict__ bv,
-                                                   const double *__restrict__ scale, double *__restrict__ minv,
-                                                   T *__restrict__ bv_t, T *__restrict__ minv_t,
+                                                   const int *__restrict__ bc, TB *__restrict__ bv,
+                                                   const double *__restrict__ scale, TB *__restrict__ minv,
                                                    unsigned char *__restrict__ diag_id)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -604,9 +613,10 @@ __global__ __launch_bounds__(TPB) void k_prep_fold(int N, const int *__restrict_
     double a[9];
     for (int r = 0; r < 3; r++)
         for (int cc = 0; cc < 3; cc++)
-            a[3 * r + cc] = (deg > 0) ? dirichlet_entry(bv[sell_val(c0, 3 * r + cc, lane)], n, n, r, cc, brc, brc) *
-                                            (si[r] * si[cc])
-                                      : 0.0;
+            a[3 * r + cc] =
+                (deg > 0) ? dirichlet_entry((double)bv[sell_val(c0, 3 * r + cc, lane)], n, n, r, cc, brc, brc) *
+                                (si[r] * si[cc])
+                          : 0.0;
     double c00 = a[4] * a[8] - a[5] * a[7];
     double c01 = a[5] * a[6] - a[3] * a[8];
     double c02 = a[3] * a[7] - a[4] * a[6];
@@ -628,13 +638,9 @@ __global__ __launch_bounds__(TPB) void k_prep_fold(int N, const int *__restrict_
         for (int k = 0; k < 9; k++) m[k] = 0.0;
         for (int k = 0; k < 3; k++) m[4 * k] = (a[4 * k] != 0.0) ? 1.0 / a[4 * k] : 1.0;
     }
-    double *mo = minv + (size_t)(n >> 6) * 9 * 64 + lane;
-    for (int k = 0; k < 9; k++) {
-        mo[k * 64] = m[k];
-        if (minv_t) minv_t[(size_t)(n >> 6) * 9 * 64 + lane + k * 64] = (T)m[k];
-    }
-    /* every slot: Dirichlet, scaling, fold; FP64: M^-1 A in place; mixed: the scaled matrix stays in
-     * FP64 and M^-1 A goes to the FP32 copy */
+    TB *mo = minv + (size_t)(n >> 6) * 9 * 64 + lane;
+    for (int k = 0; k < 9; k++) mo[k * 64] = (TB)m[k];
+    /* every slot: Dirichlet, scaling and fold, written in place as M^-1 A */
     for (int k = 0; k < w; k++) {
         const int cs = c0 + k;
         double b[9];
@@ -644,21 +650,15 @@ __global__ __launch_bounds__(TPB) void k_prep_fold(int N, const int *__restrict_
             const double sj[3] = {scale[3 * mm], scale[3 * mm + 1], scale[3 * mm + 2]};
             for (int r = 0; r < 3; r++)
                 for (int cc = 0; cc < 3; cc++)
-                    b[3 * r + cc] =
-                        dirichlet_entry(bv[sell_val(cs, 3 * r + cc, lane)], n, mm, r, cc, brc, bcc) * (si[r] * sj[cc]);
+                    b[3 * r + cc] = dirichlet_entry((double)bv[sell_val(cs, 3 * r + cc, lane)], n, mm, r, cc, brc, bcc) *
+                                    (si[r] * sj[cc]);
         } else {
             for (int e = 0; e < 9; e++) b[e] = 0.0;
         }
         for (int r = 0; r < 3; r++)
             for (int cc = 0; cc < 3; cc++) {
                 const double f = __fma_rn(m[3 * r + 2], b[6 + cc], __fma_rn(m[3 * r + 1], b[3 + cc], m[3 * r] * b[cc]));
-                const size_t ix = sell_val(cs, 3 * r + cc, lane);
-                if (bv_t) {
-                    bv_t[ix] = (T)f;
-                    if (k < deg) bv[ix] = b[3 * r + cc];
-                } else if (k < deg) {
-                    bv[ix] = f;
-                }
+                bv[sell_val(cs, 3 * r + cc, lane)] = (TB)f;
             }
     }
 }
@@ -667,15 +667,19 @@ extern "C" int adhk_launch_prep(adhk_ctx *c, int mixed)
 {
     adhk_kt_begin(c, ADHK_KT_PREP);
     const dim3 g((c->N + TPB - 1) / TPB);
-    hipLaunchKernelGGL(k_prep_rowmax, g, dim3(TPB), 0, c->stream, c->N, c->brow, c->slice_ptr, c->scol, c->bvals,
-                       c->bc_mask, c->residual, c->dsol, c->scale);
-    CHECK_LAUNCH();
-    if (mixed)
+    if (mixed) {
+        hipLaunchKernelGGL(k_prep_rowmax<float>, g, dim3(TPB), 0, c->stream, c->N, c->brow, c->slice_ptr, c->scol,
+                           c->bvals32, c->bc_mask, c->residual, c->dsol, c->scale);
+        CHECK_LAUNCH();
         hipLaunchKernelGGL(k_prep_fold<float>, g, dim3(TPB), 0, c->stream, c->N, c->brow, c->slice_ptr, c->scol,
-                           c->bc_mask, c->bvals, c->scale, c->minv, c->bvals32, c->minv32, c->diag_id);
-    else
+                           c->bc_mask, c->bvals32, c->scale, c->minv32, c->diag_id);
+    } else {
+        hipLaunchKernelGGL(k_prep_rowmax<double>, g, dim3(TPB), 0, c->stream, c->N, c->brow, c->slice_ptr, c->scol,
+                           c->bvals, c->bc_mask, c->residual, c->dsol, c->scale);
+        CHECK_LAUNCH();
         hipLaunchKernelGGL(k_prep_fold<double>, g, dim3(TPB), 0, c->stream, c->N, c->brow, c->slice_ptr, c->scol,
-                           c->bc_mask, c->bvals, c->scale, c->minv, (double *)NULL, (double *)NULL, c->diag_id);
+                           c->bc_mask, c->bvals, c->scale, c->minv, c->diag_id);
+    }
     CHECK_LAUNCH();
     adhk_kt_end(c);
     return 0;

@@ -74,7 +74,7 @@ static void ctx_free(adhk_ctx *c)
                     c->bc_mask, c->wd_flag, c->wd_tmp, c->wdq, c->brow, c->bcol, c->diag_blk, c->cptr, c->contrib, c->bvals,
                     c->minv, c->bvals32, c->minv32, c->erhs, c->emat, c->r, c->pv, c->Ap, c->s, c->As, c->q, c->x0,
                     c->pv2, c->Ap2, c->xk, c->r32, c->p32, c->Ap32, c->s32, c->As32, c->q32, c->xk32, c->p32b,
-                    c->Ap32b, c->partials, c->counter, c->sc, c->flags, c->slice_ptr, c->scol, c->sell2blk,
+                    c->Ap32b, c->emat32, c->partials, c->counter, c->sc, c->flags, c->slice_ptr, c->scol, c->sell2blk,
                     c->blk2sell, c->diag_slot, c->diag_id, c->node_inst, c->elem_inst, c->seg_inst, c->blk_node,
                     c->blk_inst, c->inst_blk, c->work};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++)
@@ -708,10 +708,15 @@ fail:
     return -1;
 }
 
-static int jacobian_dev(adhk_ctx *c)
+/* the FD Jacobian assembled into the FP64 matrix, or (mixed) into the FP32 one through FP32 elemental
+ * matrices */
+static int jacobian_dev(adhk_ctx *c, int mixed)
 {
-    LC(adhk_launch_elem_jac(c));
-    LC(adhk_launch_jac_gather(c));
+    if (mixed && !c->emat32) {
+        HC(hipMalloc((void **)&c->emat32, (81 * (size_t)c->E + 36 * (size_t)c->S + 1) * sizeof(float)));
+    }
+    LC(adhk_launch_elem_jac_p(c, mixed));
+    LC(adhk_launch_jac_gather_p(c, mixed));
     c->timing.jacobian_calls++;
     return 0;
 fail:
@@ -722,7 +727,7 @@ int adhk_assemble_jacobian(adhk_sw2 *c, double *vals)
 {
     if (!c) return -1;
     HC(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
-    LC(jacobian_dev(c));
+    LC(jacobian_dev(c, 0));
     if (check_partial(c)) return -1;
     return export_vals(c, vals);
 fail:
@@ -973,7 +978,7 @@ int adhk_sw2_autotune(adhk_sw2 *c, const adhk_solver *s, int ninner, adhk_tuning
     }
     c->tune.elem_tpb = best.elem_tpb;
     /* 2. the Newton system of the current state */
-    LC(jacobian_dev(c));
+    LC(jacobian_dev(c, mixed));
     LC(adhk_launch_prep(c, mixed));
     /* 3. BiCGStab schedules: complete solves, best of ninner */
     for (int sp = 0; sp <= 1; sp++)
@@ -1118,8 +1123,8 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
         for (int i = 0; i < K; i++) any |= act[i];
         if (!any) break;
         LC(set_work(c, act));
-        LC(jacobian_dev(c));
-        LC(adhk_launch_check_diag(c));
+        LC(jacobian_dev(c, mixed));
+        LC(adhk_launch_check_diag_p(c, mixed));
         LC(adhk_launch_prep(c, mixed));
         LC(krylov(c, s, mixed, hw, lits));
         LC(adhk_launch_increment(c));

@@ -289,8 +289,8 @@ __device__ __forceinline__ size_t emat_seg(int s, int row, int col)
 
 /* one thread per (element, column): column = node i * 3 + var k; two residual
  * evaluations (+eps, -eps), load_elem_mat/perturb_var assemble_jacobian.c:123-363 */
-__global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, double *__restrict__ emat, int *flags,
-                                                  int *wdq)
+template <typename TE>
+__global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, TE *__restrict__ emat, int *flags, int *wdq)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 9 * m.E) return;
@@ -318,12 +318,13 @@ __global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, double *
     int pb = sw2_tri_resid(in, P, i, k, (-1) * eps, rm);
     if (pa | pb) flags[0] = 1; /* cannot happen after tri_partial_fd; kept as a guard */
 #pragma unroll
-    for (int r = 0; r < 9; r++) emat[emat_tri(e, r, col)] = (rp[r] - rm[r]) / eps2;
+    for (int r = 0; r < 9; r++) emat[emat_tri(e, r, col)] = (TE)((rp[r] - rm[r]) / eps2);
 }
 
 /* queued elements: nine columns each, central differences of the general wet/dry residual */
+template <typename TE>
 __global__ __launch_bounds__(128) void k_elem_jac_wd(MeshView m, Sw2Par P, const int *__restrict__ wdq,
-                                                     double *__restrict__ emat)
+                                                     TE *__restrict__ emat)
 {
     const long n = 9L * wdq[0];
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
@@ -337,11 +338,11 @@ __global__ __launch_bounds__(128) void k_elem_jac_wd(MeshView m, Sw2Par P, const
         double rp[9], rm[9];
         sw2_tri_resid_general(in, P, i, k, (+1) * eps, rp);
         sw2_tri_resid_general(in, P, i, k, (-1) * eps, rm);
-        for (int r = 0; r < 9; r++) emat[emat_tri(e, r, col)] = (rp[r] - rm[r]) / eps2;
+        for (int r = 0; r < 9; r++) emat[emat_tri(e, r, col)] = (TE)((rp[r] - rm[r]) / eps2);
     }
 }
 
-__global__ void k_seg_jac(SegView s, Sw2Par P, double *__restrict__ emat)
+template <typename TE> __global__ void k_seg_jac(SegView s, Sw2Par P, TE *__restrict__ emat)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 6 * s.S) return;
@@ -363,10 +364,10 @@ __global__ void k_seg_jac(SegView s, Sw2Par P, double *__restrict__ emat)
                   P, i, k, (+1) * eps, rp);
     sw2_seg_resid(s.kind[e], s.val[2 * e], s.val[2 * e + 1], s.nrml[2 * e], s.nrml[2 * e + 1], s.sdjac[e], h, u, v,
                   P, i, k, (-1) * eps, rm);
-    for (int r = 0; r < 6; r++) emat[emat_seg(e, r, col)] = (rp[r] - rm[r]) / eps2;
+    for (int r = 0; r < 6; r++) emat[emat_seg(e, r, col)] = (TE)((rp[r] - rm[r]) / eps2);
 }
 
-extern "C" int adhk_launch_elem_jac(adhk_ctx *c)
+template <typename TE> static int launch_elem_jac(adhk_ctx *c, TE *emat)
 {
     adhk_kt_begin(c, ADHK_KT_ELEM_JAC);
     Sw2Par P = make_par(c);
@@ -374,27 +375,35 @@ extern "C" int adhk_launch_elem_jac(adhk_ctx *c)
         long nt = 9L * c->E;
         hipLaunchKernelGGL(k_wdq_reset, dim3(1), dim3(64), 0, c->stream, c->wdq);
         const int tpb = c->tune.elem_tpb;
-        hipLaunchKernelGGL(k_elem_jac, dim3((unsigned)((nt + tpb - 1) / tpb)), dim3(tpb), 0, c->stream, make_view(c), P,
-                           c->emat, c->flags, c->wdq);
-        hipLaunchKernelGGL(k_elem_jac_wd, dim3(wd_grid(nt)), dim3(128), 0, c->stream, make_view(c), P, c->wdq,
-                           c->emat);
+        hipLaunchKernelGGL(k_elem_jac<TE>, dim3((unsigned)((nt + tpb - 1) / tpb)), dim3(tpb), 0, c->stream,
+                           make_view(c), P, emat, c->flags, c->wdq);
+        hipLaunchKernelGGL(k_elem_jac_wd<TE>, dim3(wd_grid(nt)), dim3(128), 0, c->stream, make_view(c), P, c->wdq,
+                           emat);
         CHECK_LAUNCH();
     }
     if (c->S > 0) {
-        hipLaunchKernelGGL(k_seg_jac, dim3((6 * c->S + 127) / 128), dim3(128), 0, c->stream, make_segview(c), P,
-                           c->emat + 81 * (size_t)c->E);
+        hipLaunchKernelGGL(k_seg_jac<TE>, dim3((6 * c->S + 127) / 128), dim3(128), 0, c->stream, make_segview(c), P,
+                           emat + 81 * (size_t)c->E);
         CHECK_LAUNCH();
     }
     adhk_kt_end(c);
     return 0;
 }
 
+/* the elemental Jacobians in FP64 (emat), or rounded to FP32 (emat32) for the mixed-precision solve */
+extern "C" int adhk_launch_elem_jac_p(adhk_ctx *c, int f32)
+{
+    return f32 ? launch_elem_jac<float>(c, c->emat32) : launch_elem_jac<double>(c, c->emat);
+}
+extern "C" int adhk_launch_elem_jac(adhk_ctx *c) { return adhk_launch_elem_jac_p(c, 0); }
+
 /* one thread per SELL slot: block entry (3r+cc) = sum over the block's contributions (element order)
  * of the elemental entry (row a*3+r, col bb*3+cc), one contiguous 3x3 block of emat each; padding
  * slots get 0 */
+template <typename TE, typename TB>
 __global__ __launch_bounds__(256) void k_jac_gather(int nslot64, int E, const int *__restrict__ sell2blk,
                                                     const int *__restrict__ cptr, const int *__restrict__ contrib,
-                                                    const double *__restrict__ emat, double *__restrict__ bvals)
+                                                    const TE *__restrict__ emat, TB *__restrict__ bvals)
 {
     int pos = blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= nslot64) return;
@@ -406,7 +415,7 @@ __global__ __launch_bounds__(256) void k_jac_gather(int nslot64, int E, const in
     const int j0 = (b >= 0) ? cptr[b] : 0, j1 = (b >= 0) ? cptr[b + 1] : 0;
     for (int j = j0; j < j1; j++) {
         int cde = contrib[j];
-        const double *em;
+        const TE *em;
         int a, bb;
         if (cde < 9 * E) {
             int e = cde / 9;
@@ -421,22 +430,29 @@ __global__ __launch_bounds__(256) void k_jac_gather(int nslot64, int E, const in
             em = emat + segbase + 36L * s + 9 * (2 * a + bb);
         }
 #pragma unroll
-        for (int k = 0; k < 9; k++) acc[k] += em[k];
+        for (int k = 0; k < 9; k++) acc[k] += (double)em[k];
     }
-    double *out = bvals + (size_t)(pos >> 6) * 9 * 64 + (pos & 63);
+    TB *out = bvals + (size_t)(pos >> 6) * 9 * 64 + (pos & 63);
 #pragma unroll
-    for (int k = 0; k < 9; k++) out[(size_t)k * 64] = acc[k];
+    for (int k = 0; k < 9; k++) out[(size_t)k * 64] = (TB)acc[k];
 }
 
-extern "C" int adhk_launch_jac_gather(adhk_ctx *c)
+/* block assembly: FP64 values from FP64 elemental matrices, or (mixed precision) FP32 values summed in
+ * FP64 from the FP32 elemental matrices, written straight into the FP32 matrix */
+extern "C" int adhk_launch_jac_gather_p(adhk_ctx *c, int f32)
 {
     adhk_kt_begin(c, ADHK_KT_JAC_GATHER);
-    hipLaunchKernelGGL(k_jac_gather, dim3((c->nslot * 64 + 255) / 256), dim3(256), 0, c->stream, c->nslot * 64, c->E,
-                       c->sell2blk, c->cptr, c->contrib, c->emat, c->bvals);
+    if (f32)
+        hipLaunchKernelGGL((k_jac_gather<float, float>), dim3((c->nslot * 64 + 255) / 256), dim3(256), 0, c->stream,
+                           c->nslot * 64, c->E, c->sell2blk, c->cptr, c->contrib, c->emat32, c->bvals32);
+    else
+        hipLaunchKernelGGL((k_jac_gather<double, double>), dim3((c->nslot * 64 + 255) / 256), dim3(256), 0,
+                           c->stream, c->nslot * 64, c->E, c->sell2blk, c->cptr, c->contrib, c->emat, c->bvals);
     CHECK_LAUNCH();
     adhk_kt_end(c);
     return 0;
 }
+extern "C" int adhk_launch_jac_gather(adhk_ctx *c) { return adhk_launch_jac_gather_p(c, 0); }
 
 /* ---------------- wet/dry flags, fe_sw2_wdflag_legacy sw2_wdflag_legacy.c:34-139 ---------------- */
 __global__ void k_wd_zero(int N, int *tmp)
