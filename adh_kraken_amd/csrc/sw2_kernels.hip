@@ -292,13 +292,21 @@ __device__ __forceinline__ size_t emat_seg(int s, int row, int col)
 template <typename TE>
 __global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, TE *__restrict__ emat, int *flags, int *wdq)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 9 * m.E) return;
-    int e = t / 9, col = t % 9;
+    /* the element data are gathered once per element into LDS and shared by its nine column threads
+     * (fewer live registers: 166 instead of 187 VGPRs, three waves per SIMD instead of two) */
+    __shared__ TriIn sh[256 / 9];
+    const int epb = blockDim.x / 9;
+    const int le = threadIdx.x / 9, col = threadIdx.x % 9;
+    const int e = blockIdx.x * epb + le;
+    if (threadIdx.x < epb) {
+        const int ee = blockIdx.x * epb + threadIdx.x;
+        if (ee < m.E) gather_tri(m, ee, P.alpha != 0.0, sh[threadIdx.x]);
+    }
+    __syncthreads();
+    if (le >= epb || e >= m.E) return;
     if (!m.work[m.elem_inst[e]]) return;
     int i = col / 3, k = col % 3;
-    TriIn in;
-    gather_tri(m, e, P.alpha != 0.0, in);
+    const TriIn &in = sh[le];
     /* the nine threads of an element reach the same decision; column 0 queues it */
     if (tri_partial_fd(in, P.alpha != 0.0)) {
         if (col == 0) wdq_push(wdq, e);
@@ -374,8 +382,8 @@ template <typename TE> static int launch_elem_jac(adhk_ctx *c, TE *emat)
     if (c->E > 0) {
         long nt = 9L * c->E;
         hipLaunchKernelGGL(k_wdq_reset, dim3(1), dim3(64), 0, c->stream, c->wdq);
-        const int tpb = c->tune.elem_tpb;
-        hipLaunchKernelGGL(k_elem_jac<TE>, dim3((unsigned)((nt + tpb - 1) / tpb)), dim3(tpb), 0, c->stream,
+        const int tpb = c->tune.elem_tpb, epb = tpb / 9; /* whole elements per workgroup */
+        hipLaunchKernelGGL(k_elem_jac<TE>, dim3((unsigned)((c->E + epb - 1) / epb)), dim3(tpb), 0, c->stream,
                            make_view(c), P, emat, c->flags, c->wdq);
         hipLaunchKernelGGL(k_elem_jac_wd<TE>, dim3(wd_grid(nt)), dim3(128), 0, c->stream, make_view(c), P, c->wdq,
                            emat);
