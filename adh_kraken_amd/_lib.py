@@ -79,6 +79,7 @@ MLP_NIN, MLP_HID, MLP_NOUT, MLP_NNORM = 12, 64, 3, 36
 # (name, restype, argtypes) of every symbol include/adhk.h declares
 SIGNATURES = [
     ("adhk_last_error", C.c_char_p, []),
+    ("adhk_device_synchronize", C.c_int, []),
     ("adhk_solver_defaults", None, [C.POINTER(Solver)]),
     ("adhk_sw2_create", P, [C.c_int, dp, dp, dp, C.c_int, ip, dp, C.c_int, ip, ip, dp, C.POINTER(Params)]),
     ("adhk_sw2_create_batch", P, [C.c_int, ip, ip, ip, C.c_int, dp, dp, dp, C.c_int, ip, dp, C.c_int, ip, ip, dp,

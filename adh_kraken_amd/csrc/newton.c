@@ -419,6 +419,16 @@ int adhk_sw2_sizes(const adhk_sw2 *c, int *ndofs, int *nnz, int *nnzb)
     return 0;
 }
 
+int adhk_device_synchronize(void)
+{
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        adhk_set_error("hipDeviceSynchronize: %s", hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
+}
+
 int adhk_sw2_layout(const adhk_sw2 *c, int *nslice, int *nslot)
 {
     if (!c) return -1;

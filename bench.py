@@ -105,12 +105,13 @@ def run_variant(A, L, mesh, sol0, dt, precision, steps, warmup, check_every, war
     # event pairs around the first and then every KT_STRIDE-th launch of each kernel class: timing every
     # launch would add two stream records per kernel (about 16% of the step on this workload)
     L.check(lib.adhk_sw2_kernel_timing_mask(sm.h, 0xFF, KT_STRIDE), "kernel_timing")
-    _ = sm.sol  # the state read-back waits for the library's stream
     log(f"timed run: {steps} steps, precision {precision}{', learned warm start' if warm is not None else ''}")
+    L.check(lib.adhk_device_synchronize(), "device_synchronize")  # the device is idle when the clock starts
     t0 = time.perf_counter()
     nsteps, st = A.time_loop(sm, steps, precision=precision)
-    sol_end = sm.sol
+    L.check(lib.adhk_device_synchronize(), "device_synchronize")  # and when it stops
     t1 = time.perf_counter()
+    sol_end = sm.sol
     ms = (C.c_double * 8)()
     cnt = (C.c_longlong * 8)()
     L.check(lib.adhk_sw2_kernel_times(sm.h, ms, cnt), "kernel_times")
@@ -120,6 +121,33 @@ def run_variant(A, L, mesh, sol0, dt, precision, steps, warmup, check_every, war
     kt = {L.KT_NAMES[i]: {"timed_launches": int(cnt[i]), "ms_avg": (ms[i] / cnt[i]) if cnt[i] else None}
           for i in range(8)}
     return {"sm": sm, "elapsed": t1 - t0, "stats": st, "kt": kt, "sol": sol_end, "tuning": tuned}
+
+
+def pcie_leg(A, L, mesh, sol0, dt, steps=10):
+    """The rate a caller sees through the reference-style per-call boundary: every time step copies the
+    state (sol, sol_old, sol_older) in from host arrays, runs one step and copies sol back out, as the
+    fe_newton shim over the C-ABI does; FP64, default schedule."""
+    lib = L.load()
+    sm = A.SModelSuper(mesh, dt=dt)
+    sm.set_initial(sol0)
+    A.time_loop(sm, 1)
+    s, so, soo = sol0.copy(), sol0.copy(), sol0.copy()
+    sm.set_dt(dt)
+    L.check(lib.adhk_device_synchronize(), "device_synchronize")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L.check(lib.adhk_sw2_set_state(sm.h, L.dptr(s), L.dptr(so), L.dptr(soo)), "set_state")
+        n, _ = A.time_loop(sm, 1)
+        if n != 1:
+            raise RuntimeError("time step failed in the PCIe leg")
+        soo, so = so, s
+        s = np.zeros_like(s)
+        L.check(lib.adhk_sw2_get_state(sm.h, L.dptr(s), None, None), "get_state")
+    t1 = time.perf_counter()
+    sm.close()
+    return {"value": round(steps / (t1 - t0), 4), "unit": "timesteps/s", "steps": steps,
+            "note": "state copied host->device (3 x ndofs doubles) before and device->host (ndofs doubles) "
+                    "after every step, FP64, default schedule"}
 
 
 def ensemble_run(A, L, steps):
@@ -341,6 +369,9 @@ def main():
                          "samples": mlp.size, "adam_steps": 2000, "batch": 16384,
                          "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-10:].mean()),
                          "seconds_incl_data_generation": round(t_train, 2)}}
+    if args.config == "c2":
+        log("PCIe-inclusive leg")
+        out["pcie_inclusive"] = pcie_leg(A, L, mesh, sol0, dt)
     if not args.no_ensemble:
         log("ensemble")
         out["ensemble"] = ensemble_run(A, L, args.ensemble_steps)

@@ -112,6 +112,8 @@ typedef struct {
 #define ADHK_KT_NKINDS 8
 
 const char *adhk_last_error(void);
+/* waits for all work of every context on the current device (hipDeviceSynchronize) */
+int adhk_device_synchronize(void);
 void adhk_solver_defaults(adhk_solver *s);
 
 /* ---- context: mesh upload, geometry (selem_2d.c:270-351, selem_1d.c:42-56),
