@@ -111,6 +111,7 @@ SIGNATURES = [
     ("adhk_sparsity", C.c_int, [P, ip, ip]),
     ("adhk_assemble_jacobian", C.c_int, [P, dp]),
     ("adhk_elem_jacobians", C.c_int, [P, dp]),
+    ("adhk_elem_jacobians_mixed", C.c_int, [P, dp]),
     ("adhk_check_diag", C.c_int, [P]),
     ("adhk_apply_dirichlet_bc", C.c_int, [P]),
     ("adhk_get_system", C.c_int, [P, dp, dp]),

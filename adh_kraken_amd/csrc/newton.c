@@ -744,6 +744,21 @@ fail:
     return -1;
 }
 
+/* device layout by 3x3 node block (entry (3a+r, 3b+c) at 9 (3a+b) + 3r + c; segments 9 (2a+b) + 3r + c)
+ * -> row-major [elem][row][col] */
+static void emat_to_rowmajor(const adhk_ctx *c, const double *tmp, double *out)
+{
+    for (size_t e = 0; e < (size_t)c->E; e++)
+        for (int r = 0; r < 9; r++)
+            for (int cc = 0; cc < 9; cc++)
+                out[81 * e + 9 * r + cc] = tmp[81 * e + 9 * (3 * (r / 3) + cc / 3) + 3 * (r % 3) + cc % 3];
+    size_t off = 81 * (size_t)c->E;
+    for (size_t e = 0; e < (size_t)c->S; e++)
+        for (int r = 0; r < 6; r++)
+            for (int cc = 0; cc < 6; cc++)
+                out[off + 36 * e + 6 * r + cc] = tmp[off + 36 * e + 9 * (2 * (r / 3) + cc / 3) + 3 * (r % 3) + cc % 3];
+}
+
 int adhk_elem_jacobians(adhk_sw2 *c, double *out)
 {
     double *tmp = NULL;
@@ -754,21 +769,44 @@ int adhk_elem_jacobians(adhk_sw2 *c, double *out)
     if (check_partial(c)) return -1;
     if (!out) return 0;
     tmp = malloc(n * sizeof(double));
+    if (!tmp) {
+        adhk_set_error("adhk_elem_jacobians: out of host memory");
+        return -1;
+    }
     HC(hipMemcpy(tmp, c->emat, n * sizeof(double), hipMemcpyDeviceToHost));
-    /* device layout by 3x3 node block (entry (3a+r, 3b+c) at 9 (3a+b) + 3r + c; segments 9 (2a+b) + 3r + c)
-     * -> row-major [elem][row][col] */
-    for (size_t e = 0; e < (size_t)c->E; e++)
-        for (int r = 0; r < 9; r++)
-            for (int cc = 0; cc < 9; cc++)
-                out[81 * e + 9 * r + cc] = tmp[81 * e + 9 * (3 * (r / 3) + cc / 3) + 3 * (r % 3) + cc % 3];
-    size_t off = 81 * (size_t)c->E;
-    for (size_t e = 0; e < (size_t)c->S; e++)
-        for (int r = 0; r < 6; r++)
-            for (int cc = 0; cc < 6; cc++)
-                out[off + 36 * e + 6 * r + cc] = tmp[off + 36 * e + 9 * (2 * (r / 3) + cc / 3) + 3 * (r % 3) + cc % 3];
+    emat_to_rowmajor(c, tmp, out);
     free(tmp);
     return 0;
 fail:
+    free(tmp);
+    return -1;
+}
+
+int adhk_elem_jacobians_mixed(adhk_sw2 *c, double *out)
+{
+    float *t32 = NULL;
+    double *tmp = NULL;
+    if (!c) return -1;
+    size_t n = (size_t)c->E * 81 + (size_t)c->S * 36;
+    if (!c->emat32) HC(hipMalloc((void **)&c->emat32, (n + 1) * sizeof(float)));
+    HC(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
+    LC(adhk_launch_elem_jac_p(c, 1));
+    if (check_partial(c)) return -1;
+    if (!out) return 0;
+    t32 = malloc(n * sizeof(float));
+    tmp = malloc(n * sizeof(double));
+    if (!t32 || !tmp) {
+        adhk_set_error("adhk_elem_jacobians_mixed: out of host memory");
+        goto fail;
+    }
+    HC(hipMemcpy(t32, c->emat32, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) tmp[i] = t32[i];
+    emat_to_rowmajor(c, tmp, out);
+    free(t32);
+    free(tmp);
+    return 0;
+fail:
+    free(t32);
     free(tmp);
     return -1;
 }

@@ -346,6 +346,15 @@ def elem_jacobians(sm: SModelSuper):
     return out[:81 * E].reshape(E, 9, 9), out[81 * E:].reshape(S, 6, 6)
 
 
+def elem_jacobians_mixed(sm: SModelSuper):
+    """The elemental matrices of the mixed-precision solve (central differences in FP64, stored in FP32),
+    widened to float64; same layout as elem_jacobians."""
+    E, S = sm.mesh.nelems2d, sm.mesh.nelems1d
+    out = np.zeros(81 * E + 36 * S)
+    L.check(sm.lib.adhk_elem_jacobians_mixed(sm.h, L.dptr(out)), "adhk_elem_jacobians_mixed")
+    return out[:81 * E].reshape(E, 9, 9), out[81 * E:].reshape(S, 6, 6)
+
+
 def check_diag(sm: SModelSuper):
     L.check(sm.lib.adhk_check_diag(sm.h), "adhk_check_diag")
 

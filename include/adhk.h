@@ -192,6 +192,10 @@ int adhk_sparsity(adhk_sw2 *c, int *indptr /* ndofs+1 */, int *cols /* nnz */);
 int adhk_assemble_jacobian(adhk_sw2 *c, double *vals);
 /* elemental FD Jacobians (load_elem_mat, assemble_jacobian.c:123): [nelems2d*81 + nelems1d*36], row-major */
 int adhk_elem_jacobians(adhk_sw2 *c, double *elem_mat);
+/* No counterpart in the reference: the elemental matrices of the mixed-precision solve (the same central
+ * differences, computed in FP64 and stored in FP32), widened to double in the layout of
+ * adhk_elem_jacobians. */
+int adhk_elem_jacobians_mixed(adhk_sw2 *c, double *elem_mat);
 
 /* replace check_diag(SMODEL_SUPER*) and apply_Dirichlet_BC(SMODEL_SUPER*) (la.h, matrix_utilities.c:120, :42)
  * on the context's system */

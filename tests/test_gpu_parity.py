@@ -82,6 +82,25 @@ def test_elem_jacobians(gpu, c1):
     assert rel_err(jsg, jso) <= 1e-7
 
 
+@pytest.mark.parametrize("state", ["perturbed", "wet_dry"])
+def test_elem_jacobians_mixed(gpu, c1, state):
+    """The FP32 elemental matrices of the mixed-precision solve against the oracle's FP64 central
+    differences: the FP64 bar (1e-7) plus the FP32 store, which rounds each entry by up to 2^-24 of
+    itself; a wrong step, sign, column or layout is off by O(1)."""
+    mesh, sol0 = c1
+    s, so = perturbed_state(sol0, seed=5) if state == "perturbed" else wet_dry_state(sol0, seed=32)
+    sm, orc = make_pair(mesh, s, so)
+    jg, jsg = A.elem_jacobians_mixed(sm)
+    idx = np.arange(0, mesh.nelems2d, 7)
+    jo = np.array([orc.elem_jac_tri(int(e)) for e in idx])
+    assert rel_err(jg[idx], jo) <= 3e-7
+    jso = np.array([orc.elem_jac_seg(e) for e in range(mesh.nelems1d)])
+    assert rel_err(jsg, jso) <= 3e-7
+    j64, js64 = A.elem_jacobians(sm)  # and entry by entry: the FP64 matrices, rounded once to FP32
+    assert (np.abs(jg - j64) <= 2.0 ** -23 * np.abs(j64)).all()
+    assert (np.abs(jsg - js64) <= 2.0 ** -23 * np.abs(js64)).all()
+
+
 def test_assembled_jacobian(gpu, c1):
     mesh, sol0 = c1
     s, so = perturbed_state(sol0, seed=6)
