@@ -149,7 +149,32 @@ static adhk_sw2 *create_impl(int ninst, const int *node_off, const int *elem_off
             adhk_set_error("adhk_sw2_create: unknown segment kind %d", seg_kind[i]);
             return NULL;
         }
+    /* node direction check of selem_2d.c:75-80: a triangle whose area (djac) is below 1e-6 is
+     * clockwise or degenerate; the reference stops there, this returns the error.  Segments shorter
+     * than 1e-6 have no normal. */
+    for (int e = 0; e < nelems2d; e++) {
+        const int *t = tri + 3 * e;
+        const double area =
+            0.5 * ((x[t[1]] - x[t[0]]) * (y[t[2]] - y[t[0]]) - (x[t[2]] - x[t[0]]) * (y[t[1]] - y[t[0]]));
+        if (!(area >= 1e-6)) {
+            adhk_set_error("adhk_sw2_create: improperly numbered triangle %d (nodes %d %d %d): area %g", e, t[0], t[1],
+                           t[2], area);
+            return NULL;
+        }
+    }
+    for (int e = 0; e < nelems1d; e++) {
+        const int *sg = seg + 2 * e;
+        const double dx = x[sg[1]] - x[sg[0]], dy = y[sg[1]] - y[sg[0]];
+        if (!(sqrt(dx * dx + dy * dy) >= 1e-6)) {
+            adhk_set_error("adhk_sw2_create: degenerate segment %d (nodes %d %d)", e, sg[0], sg[1]);
+            return NULL;
+        }
+    }
     c = (adhk_ctx *)calloc(1, sizeof(adhk_ctx));
+    if (!c) {
+        adhk_set_error("adhk_sw2_create: out of host memory");
+        return NULL;
+    }
     c->N = nnodes;
     c->E = nelems2d;
     c->S = nelems1d;
@@ -173,15 +198,15 @@ static adhk_sw2 *create_impl(int ninst, const int *node_off, const int *elem_off
         h_tri = malloc((size_t)E * 3 * sizeof(int));
         h_sdjac = malloc((size_t)(S > 0 ? S : 1) * sizeof(double));
         h_snrml = malloc((size_t)(S > 0 ? S : 1) * 2 * sizeof(double));
+        if (!h_djac || !h_grad || !h_ks || !h_tri || !h_sdjac || !h_snrml) {
+            adhk_set_error("adhk_sw2_create: out of host memory");
+            goto fail;
+        }
         for (int e = 0; e < E; e++) {
             const int *t = tri + 3 * e;
             double s1x = x[t[1]] - x[t[0]], s1y = y[t[1]] - y[t[0]];
             double s2x = x[t[2]] - x[t[0]], s2y = y[t[2]] - y[t[0]];
-            double dj = (s1x * s2y) - (s2x * s1y);
-            if (!(fabs(dj) > 2.220446049250313e-16)) {
-                adhk_set_error("adhk_sw2_create: degenerate triangle %d", e);
-                goto fail;
-            }
+            double dj = (s1x * s2y) - (s2x * s1y); /* 2 x area >= 2e-6, checked above */
             double g6[6] = {(s1y - s2y), s2y, -s1y, (s2x - s1x), -s2x, s1x}; /* gx0 gx1 gx2 gy0 gy1 gy2 */
             for (int k = 0; k < 6; k++) h_grad[(size_t)k * E + e] = g6[k] * (1. / dj);
             h_djac[e] = dj * 0.5;
@@ -193,11 +218,7 @@ static adhk_sw2 *create_impl(int ninst, const int *node_off, const int *elem_off
         for (int e = 0; e < S; e++) {
             const int *s = seg + 2 * e;
             double dx = x[s[1]] - x[s[0]], dy = y[s[1]] - y[s[0]];
-            double L = sqrt(dx * dx + dy * dy);
-            if (L < 1e-6) {
-                adhk_set_error("adhk_sw2_create: degenerate segment %d", e);
-                goto fail;
-            }
+            double L = sqrt(dx * dx + dy * dy); /* >= 1e-6, checked above */
             h_sdjac[e] = L;
             h_snrml[2 * e] = (y[s[1]] - y[s[0]]) / L;
             h_snrml[2 * e + 1] = -(x[s[1]] - x[s[0]]) / L;
