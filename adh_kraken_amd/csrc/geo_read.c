@@ -10,7 +10,8 @@
  * The bflag field of each element is returned as written (-1 when absent); materials and boundary
  * strings come from the model's .physics / .cover files, as in smodel_super_read.c:143-159.
  * Nodes are placed by their id; elements keep the file order (the reference numbers them by id in
- * the same order).  Reading is single-pass per call and allocates nothing beyond a line buffer.
+ * the same order).  Reading is single-pass per call; besides a line buffer it allocates a byte per
+ * node, so that a repeated node id (which would leave another id unset) is refused.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -69,6 +70,11 @@ int adhk_geo_read(const char *path, int nnodes, double *x, double *y, double *z,
         if (f) fclose(f);
         return -1;
     }
+    unsigned char *seen = calloc(nnodes > 0 ? (size_t)nnodes : 1, 1);
+    if (!seen) {
+        fclose(f);
+        return -1;
+    }
     char line[LINE_MAX_LEN];
     int it = 0, is = 0, rc = 0, lineno = 0;
     while (fgets(line, sizeof line, f)) {
@@ -77,10 +83,11 @@ int adhk_geo_read(const char *path, int nnodes, double *x, double *y, double *z,
         if (card_is(p, "ND")) {
             int id;
             double xx, yy, zz;
-            if (sscanf(p + 2, "%d %lf %lf %lf", &id, &xx, &yy, &zz) != 4 || id < 1 || id > nnodes) {
+            if (sscanf(p + 2, "%d %lf %lf %lf", &id, &xx, &yy, &zz) != 4 || id < 1 || id > nnodes || seen[id - 1]) {
                 rc = -lineno - 2;
                 break;
             }
+            seen[id - 1] = 1;
             x[id - 1] = xx;
             y[id - 1] = yy;
             z[id - 1] = zz;
@@ -117,6 +124,7 @@ int adhk_geo_read(const char *path, int nnodes, double *x, double *y, double *z,
         }
     }
     fclose(f);
+    free(seen);
     if (rc) return rc;
     return (it == ntri && is == nseg) ? 0 : -1;
 }

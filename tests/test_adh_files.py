@@ -37,6 +37,10 @@ def test_bad_geo_card_is_reported(tmp_path):
     p.write_text("ND 1 0 0 0\nND 2 1 0 0\nND 3 0 1 0\nTRI 1 1 2 7 -1\n")
     with pytest.raises(L.AdhkError, match="line 4"):
         F.read_geo(str(p))
+    # a repeated node id (3 ids up to 3, but node 2 never given) is refused at its line
+    p.write_text("ND 1 0 0 0\nND 3 1 0 0\nND 3 0 1 0\nTRI 1 1 2 3 -1\n")
+    with pytest.raises(L.AdhkError, match="line 3"):
+        F.read_geo(str(p))
 
 
 def test_control_files():
