@@ -60,6 +60,15 @@ typedef struct {
     long long cnt[ADHK_KT_NKINDS];
 } adhk_ktimer;
 
+/* a captured Newton stage and the by-value kernel arguments it was captured with */
+typedef struct {
+    hipGraphExec_t exec;
+    adhk_params p;
+    adhk_tuning tune;
+    double tol;
+    int max_it;
+} adhk_stage_graph;
+
 struct adhk_sw2 {
     int N, E, S, ndofs, nnzb, npairs;
     adhk_params p;
@@ -110,6 +119,10 @@ struct adhk_sw2 {
     adhk_tuning tune;
     hipGraphExec_t gchunk[2]; /* captured BiCGStab chunk (FP64, mixed) and its length */
     int gchunk_len[2];
+    adhk_tuning gchunk_tune[2]; /* the schedule the chunk was captured with */
+    /* captured Newton stages (per precision): the launches between two host decisions, with the
+     * by-value kernel arguments they were captured with */
+    adhk_stage_graph gstage[2][3];
     int *flags; /* [0] = unqueued partially dry element (internal guard), [1] = NaN seen */
     /* time-series boundary values (series.c): tables, segment bindings, time of the old level */
     double t_prev;

@@ -88,8 +88,11 @@ static void ctx_free(adhk_ctx *c)
     free(c->alive);
     free(c->steps_done);
     adhk_series_free(c);
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++) {
         if (c->gchunk[i]) hipGraphExecDestroy(c->gchunk[i]);
+        for (int k = 0; k < 3; k++)
+            if (c->gstage[i][k].exec) hipGraphExecDestroy(c->gstage[i][k].exec);
+    }
     for (int i = 0; i < 8; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
     for (int i = 0; i < ADHK_KT_RING; i++) {
@@ -426,6 +429,7 @@ static int ensure_mixed(adhk_ctx *c)
     DALLOC(c->As32, nd, float);
     DALLOC(c->q32, nd, float);
     DALLOC(c->xk32, nd, float);
+    if (!c->emat32) DALLOC(c->emat32, 81L * c->E + 36L * c->S + 1, float);
     return 0;
 fail:
     return -1;
@@ -850,10 +854,10 @@ int adhk_get_system(adhk_sw2 *c, double *vals, double *residual)
 }
 
 /* residual norms of get_residual_norms (newton_tools.c:303-373), one set per instance */
-static int norms(adhk_ctx *c, adhk_scalars *hs)
+/* read back the per-instance scalars (norms) after the norm kernels, and the wet/dry guard flag */
+static int norms_read(adhk_ctx *c, adhk_scalars *hs)
 {
     int f[4];
-    LC(adhk_launch_norms(c));
     HC(hipMemcpyAsync(hs, c->sc, (size_t)c->ninst * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
     HC(hipMemcpyAsync(f, c->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HC(hipStreamSynchronize(c->stream));
@@ -888,7 +892,22 @@ static void drop_graphs(adhk_ctx *c)
         if (c->gchunk[i]) (void)hipGraphExecDestroy(c->gchunk[i]);
         c->gchunk[i] = NULL;
         c->gchunk_len[i] = 0;
+        for (int k = 0; k < 3; k++) {
+            if (c->gstage[i][k].exec) (void)hipGraphExecDestroy(c->gstage[i][k].exec);
+            c->gstage[i][k].exec = NULL;
+        }
     }
+}
+
+/* the schedule fields a captured launch sequence depends on */
+static int same_schedule(const adhk_tuning *a, const adhk_tuning *b)
+{
+    return a->chunk == b->chunk && a->split == b->split && a->xcd_map == b->xcd_map && a->elem_tpb == b->elem_tpb;
+}
+static int same_params(const adhk_params *a, const adhk_params *b)
+{
+    return a->g == b->g && a->dt == b->dt && a->alpha == b->alpha && a->cf_const == b->cf_const &&
+           a->manning_units == b->manning_units && a->fric_mode == b->fric_mode;
 }
 
 /* capture `chunk` BiCGStab iterations (direction buffer 1 at the start; chunk is even, so also at the
@@ -898,7 +917,8 @@ static int capture_chunk(adhk_ctx *c, int mixed, int chunk)
 {
     hipGraph_t g = NULL;
     int cur = 1, rc = 0;
-    if (c->gchunk[mixed] && c->gchunk_len[mixed] == chunk) return 0;
+    if (c->gchunk[mixed] && c->gchunk_len[mixed] == chunk && same_schedule(&c->gchunk_tune[mixed], &c->tune))
+        return 0;
     if (c->gchunk[mixed]) (void)hipGraphExecDestroy(c->gchunk[mixed]);
     c->gchunk[mixed] = NULL;
     c->gchunk_len[mixed] = 0;
@@ -915,21 +935,21 @@ static int capture_chunk(adhk_ctx *c, int mixed, int chunk)
     HC(hipGraphInstantiate(&c->gchunk[mixed], g, NULL, NULL, 0));
     HC(hipGraphDestroy(g));
     c->gchunk_len[mixed] = chunk;
+    c->gchunk_tune[mixed] = c->tune;
     return 0;
 fail:
     if (g) (void)hipGraphDestroy(g);
     return -1;
 }
 
-/* solve_linear_sys_bcgstab on the device for the instances in the work mask; its[i] = iterations */
-static int krylov(adhk_ctx *c, const adhk_solver *s, int mixed, adhk_scalars *hs, int *its)
+/* BiCGStab iterations from the state adhk_launch_krylov_init left, for the instances in the work mask:
+ * the host reads the stop flags between chunks.  On return hs holds every instance's final scalars. */
+static int krylov_iterate(adhk_ctx *c, const adhk_solver *s, int mixed, adhk_scalars *hs)
 {
     const int K = c->ninst;
-    double tol = mixed ? s->mixed_lin_tol : s->lin_tol;
     int chunk = c->tune.chunk > 0 ? c->tune.chunk : (s->lin_check_every > 0 ? s->lin_check_every : 8), all_done = 1;
     const unsigned krylov_kt = (1u << ADHK_KT_SPMV_P) | (1u << ADHK_KT_SPMV_S) | (1u << ADHK_KT_UPDATE);
     const int use_graph = c->tune.graphs && !(c->kt.on && (c->kt.mask & krylov_kt)) && chunk % 2 == 0;
-    LC(adhk_launch_krylov_init(c, mixed, tol, s->lin_max_it));
     HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
     HC(hipStreamSynchronize(c->stream));
     for (int i = 0; i < K; i++) all_done &= hs[i].done;
@@ -956,15 +976,97 @@ static int krylov(adhk_ctx *c, const adhk_solver *s, int mixed, adhk_scalars *hs
             for (int i = 0; i < K; i++) all_done &= hs[i].done;
             if (all_done) break;
         }
-        if (!all_done) LC(adhk_launch_krylov_last(c, mixed, cur));
-        HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
+        if (!all_done) {
+            LC(adhk_launch_krylov_last(c, mixed, cur));
+            HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
+            HC(hipStreamSynchronize(c->stream));
+        }
     }
+    return 0;
+fail:
+    return -1;
+}
+
+/* solve_linear_sys_bcgstab on the device for the instances in the work mask; its[i] = iterations */
+static int krylov(adhk_ctx *c, const adhk_solver *s, int mixed, adhk_scalars *hs, int *its)
+{
+    LC(adhk_launch_krylov_init(c, mixed, mixed ? s->mixed_lin_tol : s->lin_tol, s->lin_max_it));
+    LC(krylov_iterate(c, s, mixed, hs));
     LC(adhk_launch_krylov_finish(c, mixed));
     HC(hipStreamSynchronize(c->stream));
-    for (int i = 0; i < K; i++) its[i] = c->h_work[i] ? hs[i].it : 0;
+    for (int i = 0; i < c->ninst; i++) its[i] = c->h_work[i] ? hs[i].it : 0;
     c->timing.krylov_calls++;
     return 0;
 fail:
+    return -1;
+}
+
+/* Newton stages: the launches between two host decisions of fe_newton.
+ *   STG_RES  residual and its norms (start of a step, learned start, line search)
+ *   STG_SYS  FD Jacobian, check_diag, Dirichlet/scaling/block-Jacobi fold, Krylov initial state
+ *   STG_UPD  Krylov finish (dsol), Newton increment, residual and its norms
+ * With graphs on (and no kernel timer recording events), each is captured once per precision and
+ * replayed; every kernel reads state, flags and work masks from device memory, so a capture stays valid
+ * until a by-value argument changes (model parameters, schedule, Krylov tolerance), which re-captures. */
+enum { STG_RES, STG_SYS, STG_UPD };
+static int stage_launch(adhk_ctx *c, const adhk_solver *s, int mixed, int which)
+{
+    if (which == STG_SYS) {
+        LC(adhk_launch_elem_jac_p(c, mixed));
+        LC(adhk_launch_jac_gather_p(c, mixed));
+        LC(adhk_launch_check_diag_p(c, mixed));
+        LC(adhk_launch_prep(c, mixed));
+        LC(adhk_launch_krylov_init(c, mixed, mixed ? s->mixed_lin_tol : s->lin_tol, s->lin_max_it));
+        return 0;
+    }
+    if (which == STG_UPD) {
+        LC(adhk_launch_krylov_finish(c, mixed));
+        LC(adhk_launch_increment(c));
+    }
+    LC(adhk_launch_elem_resid(c));
+    LC(adhk_launch_resid_gather(c));
+    LC(adhk_launch_norms(c));
+    return 0;
+fail:
+    return -1;
+}
+
+static int run_stage(adhk_ctx *c, const adhk_solver *s, int mixed, int which)
+{
+    const double tol = mixed ? s->mixed_lin_tol : s->lin_tol;
+    hipGraph_t g = NULL;
+    if (which == STG_SYS) c->timing.jacobian_calls++;
+    if (which == STG_UPD) c->timing.krylov_calls++;
+    if (which != STG_SYS) c->timing.residual_calls++;
+    if (!c->tune.graphs || c->kt.on) return stage_launch(c, s, mixed, which);
+    if (mixed && !c->emat32) LC(ensure_mixed(c)); /* no allocation may happen inside a capture */
+    {
+        adhk_stage_graph *st = &c->gstage[mixed][which];
+        if (!st->exec || !same_params(&st->p, &c->p) || !same_schedule(&st->tune, &c->tune) ||
+            (which == STG_SYS && (st->tol != tol || st->max_it != s->lin_max_it))) {
+            int rc;
+            if (st->exec) (void)hipGraphExecDestroy(st->exec);
+            st->exec = NULL;
+            HC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            rc = stage_launch(c, s, mixed, which);
+            {
+                hipError_t e = hipStreamEndCapture(c->stream, &g);
+                if (rc) goto fail;
+                HC(e);
+            }
+            HC(hipGraphInstantiate(&st->exec, g, NULL, NULL, 0));
+            HC(hipGraphDestroy(g));
+            g = NULL;
+            st->p = c->p;
+            st->tune = c->tune;
+            st->tol = tol;
+            st->max_it = s->lin_max_it;
+        }
+        HC(hipGraphLaunch(st->exec, c->stream));
+    }
+    return 0;
+fail:
+    if (g) (void)hipGraphDestroy(g);
     return -1;
 }
 
@@ -1153,8 +1255,8 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
     HC(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
     /* initialize_system + initialize_dirichlet_bc (newton_tools.c:21-53, 130-142) */
     LC(adhk_launch_init_system(c));
-    LC(residual_dev(c));
-    LC(norms(c, hs));
+    LC(run_stage(c, s, mixed, STG_RES));
+    LC(norms_read(c, hs));
     if (c->warm) {
         /* learned start: an instance keeps it only when its residual is smaller than the previous
          * level's; the previous level's residual is kept (in the idle Krylov vector r) for the fallback */
@@ -1162,8 +1264,8 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
         LC(adhk_launch_copy(c, c->r, c->residual));
         LC(adhk_mlp_predict_into(c->warm, c));
         LC(adhk_launch_dirichlet_sol(c));
-        LC(residual_dev(c));
-        LC(norms(c, hw));
+        LC(run_stage(c, s, mixed, STG_RES));
+        LC(norms_read(c, hw));
         for (int i = 0; i < K; i++) {
             const int better = s->warm_gate == 1 ? hw[i].rl2sum < hs[i].rl2sum : hw[i].rmax < hs[i].rmax;
             ls[i] = !(!hw[i].nanflag && !hs[i].nanflag && better); /* reject mask */
@@ -1192,11 +1294,10 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
         for (int i = 0; i < K; i++) any |= act[i];
         if (!any) break;
         LC(set_work(c, act));
-        LC(jacobian_dev(c, mixed));
-        LC(adhk_launch_check_diag_p(c, mixed));
-        LC(adhk_launch_prep(c, mixed));
-        LC(krylov(c, s, mixed, hw, lits));
-        LC(adhk_launch_increment(c));
+        LC(run_stage(c, s, mixed, STG_SYS));
+        LC(krylov_iterate(c, s, mixed, hw));
+        for (int i = 0; i < K; i++) lits[i] = c->h_work[i] ? hw[i].it : 0;
+        LC(run_stage(c, s, mixed, STG_UPD));
         for (int i = 0; i < K; i++)
             if (act[i]) {
                 c->inst_stats[i].lin_its_total += lits[i];
@@ -1204,8 +1305,7 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
                 old_rl2[i] = rl2[i];
                 cuts[i] = 0;
             }
-        LC(residual_dev(c));
-        LC(norms(c, hs));
+        LC(norms_read(c, hs));
         for (int i = 0; i < K; i++)
             if (act[i]) {
                 const double nd = 3.0 * (c->h_inst_node[i + 1] - c->h_inst_node[i]);
@@ -1225,8 +1325,8 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
             if (!anyls) break;
             LC(set_work(c, ls));
             LC(adhk_launch_linesearch(c));
-            LC(residual_dev(c));
-            LC(norms(c, hs));
+            LC(run_stage(c, s, mixed, STG_RES));
+            LC(norms_read(c, hs));
             for (int i = 0; i < K; i++)
                 if (ls[i]) {
                     const double nd = 3.0 * (c->h_inst_node[i + 1] - c->h_inst_node[i]);

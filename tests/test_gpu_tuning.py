@@ -1,9 +1,10 @@
 """GPU tests of the schedule choices, hipGraph replay and the autotuner.
 
 No schedule changes a computed value, so every run below must reproduce the default schedule bit for
-bit (state and iteration counts).  With graphs on, each BiCGStab chunk is one captured graph replayed
-many times per solve and across solves, with eager kernels and host synchronisations between the
-replays; every replay's results feed the compared state.
+bit (state and iteration counts).  With graphs on, each BiCGStab chunk and each Newton stage (residual
+and norms; Jacobian, fold and Krylov start; Krylov finish, increment and residual) is one captured graph
+replayed many times per solve and across solves, with eager kernels, work-mask uploads and host
+synchronisations between the replays; every replay's results feed the compared state.
 """
 import numpy as np
 import pytest
@@ -69,3 +70,40 @@ def test_autotune_keeps_results(gpu):
     n, st = A.time_loop(sm, 5, precision=L.PREC_MIXED)
     assert n == 5 and st["lin_its_total"] == st0["lin_its_total"]
     assert np.array_equal(sm.sol, s0)
+
+
+def test_stage_graphs_recapture_when_dt_changes(gpu):
+    """dt is a by-value kernel argument of the residual and Jacobian stages: after set_dt the captured
+    stages must be captured again, or the replays would keep integrating with the old step."""
+    mesh, sol0 = A.flume_mesh(A.FLUME_C1)
+    out = []
+    for tuning in (None, {"graphs": 1, "chunk": 8}):
+        sm = A.SModelSuper(mesh, dt=1.0)
+        if tuning:
+            sm.tuning = tuning
+        sm.set_initial(sol0)
+        n1, st1 = A.time_loop(sm, 3, precision=L.PREC_MIXED)
+        sm.set_dt(0.5)
+        n2, st2 = A.time_loop(sm, 3, precision=L.PREC_MIXED)
+        out.append((n1, n2, st1["newton_its"] + st2["newton_its"], st1["lin_its_total"] + st2["lin_its_total"],
+                    sm.sol))
+        sm.close()
+    (a1, a2, an, al, sa), (b1, b2, bn, bl, sb) = out
+    assert a1 == a2 == b1 == b2 == 3 and an == bn and al == bl
+    assert np.array_equal(sa, sb)
+
+
+def test_stage_graphs_with_wetting_and_line_search(gpu):
+    """A flume whose bed bumps rise above the water (wet/dry queue in every residual and Jacobian stage,
+    line-search residuals between the stage replays), FP64 and mixed, graphs against eager launches."""
+    spec = A.FlumeSpec(xmin=0.0, xmax=100.0, ymin=0.0, ymax=10.0, npx=101, npy=11, slope=1e-3, nbumps=6,
+                       bump_amp_max=1.6, bump_sigma_min=2.0, bump_sigma_max=6.0, seed=3, manning_n=0.03,
+                       inflow_q=2.0, tailwater=1.0, ic_flat_wse=1)
+    mesh, sol0 = A.flume_mesh(spec)
+    for precision in (L.PREC_FP64, L.PREC_MIXED):
+        n0, st0, s0, _ = run(mesh, sol0, 8, precision)
+        n1, st1, s1, _ = run(mesh, sol0, 8, precision, {"graphs": 1, "chunk": 4})
+        assert n0 == n1 == 8, precision
+        assert (st0["newton_its"], st0["lin_its_total"], st0["linesearch_cuts_total"]) == \
+            (st1["newton_its"], st1["lin_its_total"], st1["linesearch_cuts_total"]), precision
+        assert np.array_equal(s0, s1), precision
