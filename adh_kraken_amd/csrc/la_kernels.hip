@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include "adhk_internal.h"
+#include "xcd.h"
 
 #define TPB 256
 #define MAX_GRID 2048
@@ -470,7 +471,7 @@ __global__ __launch_bounds__(TPB) void k_scale_prec(int N, const int *__restrict
                                                     T *__restrict__ bv_t, T *__restrict__ minv_t,
                                                     unsigned char *__restrict__ diag_id)
 {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = xcd_block() * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const int lane = n & 63, c0 = slice_ptr[n >> 6], w = slice_ptr[(n >> 6) + 1] - c0, deg = brow[n + 1] - brow[n];
     double si[3] = {scale[3 * n], scale[3 * n + 1], scale[3 * n + 2]};
@@ -569,7 +570,7 @@ __global__ __launch_bounds__(TPB) void k_prep_rowmax(int N, const int *__restric
                                                      const int *__restrict__ bc, double *__restrict__ res,
                                                      double *__restrict__ dsol, double *__restrict__ scale)
 {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = xcd_block() * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const int lane = n & 63, c0 = slice_ptr[n >> 6], deg = brow[n + 1] - brow[n];
     const int brc[3] = {bc[3 * n], bc[3 * n + 1], bc[3 * n + 2]};
@@ -604,7 +605,7 @@ __global__ __launch_bounds__(TPB) void k_prep_fold(int N, const int *__restrict_
                                                    const double *__restrict__ scale, TB *__restrict__ minv,
                                                    unsigned char *__restrict__ diag_id)
 {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = xcd_block() * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const int lane = n & 63, c0 = slice_ptr[n >> 6], w = slice_ptr[(n >> 6) + 1] - c0, deg = brow[n + 1] - brow[n];
     const int brc[3] = {bc[3 * n], bc[3 * n + 1], bc[3 * n + 2]};

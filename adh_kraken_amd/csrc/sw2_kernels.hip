@@ -13,6 +13,7 @@
 #include "adhk_internal.h"
 #include "sw2_device.h"
 #include "sw2_wetdry.h"
+#include "xcd.h"
 
 #define CHECK_LAUNCH()                                                                                            \
     do {                                                                                                          \
@@ -130,7 +131,7 @@ __device__ __forceinline__ void wdq_push(int *wdq, int e)
 __global__ __launch_bounds__(256) void k_elem_resid(MeshView m, Sw2Par P, double *__restrict__ erhs, int *flags,
                                                     int *wdq)
 {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= m.E || !m.work[m.elem_inst[e]]) return;
     TriIn in;
     gather_tri(m, e, P.alpha != 0.0, in);
@@ -238,7 +239,7 @@ __global__ __launch_bounds__(256) void k_resid_gather(int N, int E, const int *_
                                                       const int *__restrict__ cptr, const int *__restrict__ contrib,
                                                       const double *__restrict__ erhs, double *__restrict__ residual)
 {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = xcd_block() * blockDim.x + threadIdx.x;
     if (n >= N) return;
     int b = diag_blk[n];
     double r0 = 0.0, r1 = 0.0, r2 = 0.0;
@@ -295,11 +296,11 @@ __global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, TE *__re
     /* the element data are gathered once per element into LDS and shared by its nine column threads
      * (fewer live registers: 166 instead of 187 VGPRs, three waves per SIMD instead of two) */
     __shared__ TriIn sh[256 / 9];
-    const int epb = blockDim.x / 9;
+    const int epb = blockDim.x / 9, wb = xcd_block();
     const int le = threadIdx.x / 9, col = threadIdx.x % 9;
-    const int e = blockIdx.x * epb + le;
+    const int e = wb * epb + le;
     if (threadIdx.x < epb) {
-        const int ee = blockIdx.x * epb + threadIdx.x;
+        const int ee = wb * epb + threadIdx.x;
         if (ee < m.E) gather_tri(m, ee, P.alpha != 0.0, sh[threadIdx.x]);
     }
     __syncthreads();
@@ -413,7 +414,7 @@ __global__ __launch_bounds__(256) void k_jac_gather(int nslot64, int E, const in
                                                     const int *__restrict__ cptr, const int *__restrict__ contrib,
                                                     const TE *__restrict__ emat, TB *__restrict__ bvals)
 {
-    int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    int pos = xcd_block() * blockDim.x + threadIdx.x;
     if (pos >= nslot64) return;
     int b = sell2blk[pos];
     double acc[9];
