@@ -17,8 +17,10 @@
  *                         slots to block-CSR indices; diag_slot [N] is the diagonal's slot.
  *   block-Jacobi          minv[(s*9 + e)*64 + l]
  *   assembly lists        cptr [nnzb+1], contrib [npairs]: contributions to each
- *                         block in element order (2-D first, then 1-D)
- *   elemental buffers     erhs [E*9 + S*6], emat [E*81 + S*36] stored [elem][col][row]
+ *                         block in element order (2-D first, then 1-D); cpos [npairs]: the list
+ *                         position of each elemental block (code 9e + 3a + b, 9E + 4s + 2a + b)
+ *   elemental buffers     erhs [E*9 + S*6]; emat [npairs*9]: the 3x3 block of list position j at
+ *                         9j (entry 3r + c), so a block's contributions are one contiguous run
  *   wet/dry queue         wdq [E+1]: count, then the partially dry elements of the last launch
  *   instances             node_inst [N], elem_inst [E], seg_inst [S]; 256-node blocks blk_node/blk_inst;
  *                         per-instance scalars sc [ninst] and work mask [ninst]
@@ -96,6 +98,7 @@ struct adhk_sw2 {
     float *bvals32, *minv32;
     /* elemental buffers */
     double *erhs, *emat;
+    int *cpos; /* [npairs]: contribution-list position of each elemental block */
     float *emat32; /* FP32 elemental Jacobians of the mixed-precision Newton iteration */
     /* Krylov vectors (FP64 and FP32 sets); p and Ap are double-buffered: one BiCGStab kernel
      * reads the previous direction while it writes the next */

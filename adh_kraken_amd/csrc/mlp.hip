@@ -18,8 +18,9 @@
  * No counterpart in the reference: there the Newton solve starts from the previous
  * time level (smodel_super_prep_sol.c:2-8, initialize_system newton_tools.c:21).
  * Plain Newton stays the fallback and the judge: fe_newton keeps whichever initial
- * iterate, learned or previous level, has the smaller max-norm residual, and runs
- * the unchanged Newton iteration and stopping test from it.
+ * iterate, learned or previous level, has the smaller residual (l2 norm by default,
+ * max norm with adhk_solver.warm_gate = 0), and runs the unchanged Newton iteration
+ * and stopping test from it.
  *
  * Parameter vector (P = 5187 floats): W1[64][12] b1[64] W2[64][64] b2[64] W3[3][64] b3[3].
  * Normalisation (36 floats): x mean [12], x 1/std [12], y mean [3], y std [3], y 1/std [3].

@@ -74,7 +74,7 @@ static void ctx_free(adhk_ctx *c)
                     c->bc_mask, c->wd_flag, c->wd_tmp, c->wdq, c->brow, c->bcol, c->diag_blk, c->cptr, c->contrib, c->bvals,
                     c->minv, c->bvals32, c->minv32, c->erhs, c->emat, c->r, c->pv, c->Ap, c->s, c->As, c->q, c->x0,
                     c->pv2, c->Ap2, c->xk, c->r32, c->p32, c->Ap32, c->s32, c->As32, c->q32, c->xk32, c->p32b,
-                    c->Ap32b, c->emat32, c->partials, c->counter, c->sc, c->flags, c->slice_ptr, c->scol, c->sell2blk,
+                    c->Ap32b, c->emat32, c->partials, c->counter, c->cpos, c->sc, c->flags, c->slice_ptr, c->scol, c->sell2blk,
                     c->blk2sell, c->diag_slot, c->diag_id, c->node_inst, c->elem_inst, c->seg_inst, c->blk_node,
                     c->blk_inst, c->inst_blk, c->work};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++)
@@ -769,24 +769,31 @@ fail:
     return -1;
 }
 
-/* device layout by 3x3 node block (entry (3a+r, 3b+c) at 9 (3a+b) + 3r + c; segments 9 (2a+b) + 3r + c)
- * -> row-major [elem][row][col] */
-static void emat_to_rowmajor(const adhk_ctx *c, const double *tmp, double *out)
+/* device layout: the 3x3 block (a, b) of triangle e (code 9e + 3a + b; segment s: 9E + 4s + 2a + b) at
+ * 9 cpos[code], entry (r, c) at + 3r + c  ->  row-major [elem][row][col] */
+static void emat_to_rowmajor(const adhk_ctx *c, const int *cpos, const double *tmp, double *out)
 {
     for (size_t e = 0; e < (size_t)c->E; e++)
-        for (int r = 0; r < 9; r++)
-            for (int cc = 0; cc < 9; cc++)
-                out[81 * e + 9 * r + cc] = tmp[81 * e + 9 * (3 * (r / 3) + cc / 3) + 3 * (r % 3) + cc % 3];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                const double *blk = tmp + 9 * (size_t)cpos[9 * e + 3 * a + b];
+                for (int r = 0; r < 3; r++)
+                    for (int cc = 0; cc < 3; cc++) out[81 * e + 9 * (3 * a + r) + 3 * b + cc] = blk[3 * r + cc];
+            }
     size_t off = 81 * (size_t)c->E;
-    for (size_t e = 0; e < (size_t)c->S; e++)
-        for (int r = 0; r < 6; r++)
-            for (int cc = 0; cc < 6; cc++)
-                out[off + 36 * e + 6 * r + cc] = tmp[off + 36 * e + 9 * (2 * (r / 3) + cc / 3) + 3 * (r % 3) + cc % 3];
+    for (size_t s = 0; s < (size_t)c->S; s++)
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) {
+                const double *blk = tmp + 9 * (size_t)cpos[9 * (size_t)c->E + 4 * s + 2 * a + b];
+                for (int r = 0; r < 3; r++)
+                    for (int cc = 0; cc < 3; cc++) out[off + 36 * s + 6 * (3 * a + r) + 3 * b + cc] = blk[3 * r + cc];
+            }
 }
 
 int adhk_elem_jacobians(adhk_sw2 *c, double *out)
 {
     double *tmp = NULL;
+    int *cpos = NULL;
     if (!c) return -1;
     size_t n = (size_t)c->E * 81 + (size_t)c->S * 36;
     HC(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
@@ -794,16 +801,20 @@ int adhk_elem_jacobians(adhk_sw2 *c, double *out)
     if (check_partial(c)) return -1;
     if (!out) return 0;
     tmp = malloc(n * sizeof(double));
-    if (!tmp) {
+    cpos = malloc((size_t)c->npairs * sizeof(int));
+    if (!tmp || !cpos) {
         adhk_set_error("adhk_elem_jacobians: out of host memory");
-        return -1;
+        goto fail;
     }
     HC(hipMemcpy(tmp, c->emat, n * sizeof(double), hipMemcpyDeviceToHost));
-    emat_to_rowmajor(c, tmp, out);
+    HC(hipMemcpy(cpos, c->cpos, (size_t)c->npairs * sizeof(int), hipMemcpyDeviceToHost));
+    emat_to_rowmajor(c, cpos, tmp, out);
     free(tmp);
+    free(cpos);
     return 0;
 fail:
     free(tmp);
+    free(cpos);
     return -1;
 }
 
@@ -811,6 +822,7 @@ int adhk_elem_jacobians_mixed(adhk_sw2 *c, double *out)
 {
     float *t32 = NULL;
     double *tmp = NULL;
+    int *cpos = NULL;
     if (!c) return -1;
     size_t n = (size_t)c->E * 81 + (size_t)c->S * 36;
     if (!c->emat32) HC(hipMalloc((void **)&c->emat32, (n + 1) * sizeof(float)));
@@ -820,19 +832,23 @@ int adhk_elem_jacobians_mixed(adhk_sw2 *c, double *out)
     if (!out) return 0;
     t32 = malloc(n * sizeof(float));
     tmp = malloc(n * sizeof(double));
-    if (!t32 || !tmp) {
+    cpos = malloc((size_t)c->npairs * sizeof(int));
+    if (!t32 || !tmp || !cpos) {
         adhk_set_error("adhk_elem_jacobians_mixed: out of host memory");
         goto fail;
     }
     HC(hipMemcpy(t32, c->emat32, n * sizeof(float), hipMemcpyDeviceToHost));
+    HC(hipMemcpy(cpos, c->cpos, (size_t)c->npairs * sizeof(int), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; i++) tmp[i] = t32[i];
-    emat_to_rowmajor(c, tmp, out);
+    emat_to_rowmajor(c, cpos, tmp, out);
     free(t32);
     free(tmp);
+    free(cpos);
     return 0;
 fail:
     free(t32);
     free(tmp);
+    free(cpos);
     return -1;
 }
 

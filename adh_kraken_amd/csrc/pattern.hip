@@ -43,6 +43,13 @@ __global__ void k_emit_pairs(int N, int E, int S, const int *tri, const int *seg
     }
 }
 
+/* cpos[code] = position of the elemental block `code` in the sorted contribution list */
+__global__ void k_cpos(long n, const int *contrib, int *cpos)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cpos[contrib[i]] = (int)i;
+}
+
 __global__ void k_head_flags(long n, const unsigned long long *keys, int *flags)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -201,6 +208,9 @@ extern "C" int adhk_build_pattern(adhk_ctx *c)
     tmp_b2 = tmp_b;
     CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_b2, keys, keys_s, vals, c->contrib, (int)np, 0, bits,
                                              c->stream));
+    CHECK(hipMalloc((void **)&c->cpos, np * sizeof(int)));
+    hipLaunchKernelGGL(k_cpos, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, np, c->contrib, c->cpos);
+    CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_head_flags, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, np, keys_s, flags);
     CHECK(hipGetLastError());
     tmp_b2 = tmp_b;

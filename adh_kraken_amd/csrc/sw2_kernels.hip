@@ -276,22 +276,18 @@ extern "C" int adhk_launch_resid_gather(adhk_ctx *c)
 }
 
 /* ---------------- FD Jacobian ---------------- */
-/* Elemental matrices are stored by 3x3 node block: entry (row 3a+r, column 3b+c) of triangle e is at
- * 81 e + 9 (3a + b) + 3r + c (segments: 36 s + 9 (2a + b) + 3r + c), so that the assembly gather reads
- * each node pair's block as one contiguous 72-byte run. */
-__device__ __forceinline__ size_t emat_tri(int e, int row, int col)
-{
-    return 81 * (size_t)e + 9 * (3 * (row / 3) + col / 3) + 3 * (row % 3) + col % 3;
-}
-__device__ __forceinline__ size_t emat_seg(int s, int row, int col)
-{
-    return 36 * (size_t)s + 9 * (2 * (row / 3) + col / 3) + 3 * (row % 3) + col % 3;
-}
+/* Elemental matrices are stored by 3x3 node block in the order of the contribution list: the block
+ * (a, b) of triangle e (code 9e + 3a + b; segment s: 9E + 4s + 2a + b) sits at 9 cpos[code], entry
+ * (r, c) at + 3r + c.  The assembly gather then reads each block's contributions as one contiguous
+ * run (72 bytes per contribution in FP64); the element kernels, which are compute-bound, absorb the
+ * scattered block writes. */
+__device__ __forceinline__ size_t emat_at(int pos, int r, int c) { return 9 * (size_t)pos + 3 * r + c; }
 
 /* one thread per (element, column): column = node i * 3 + var k; two residual
  * evaluations (+eps, -eps), load_elem_mat/perturb_var assemble_jacobian.c:123-363 */
 template <typename TE>
-__global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, TE *__restrict__ emat, int *flags, int *wdq)
+__global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, TE *__restrict__ emat,
+                                                  const int *__restrict__ cpos, int *flags, int *wdq)
 {
     /* the element data are gathered once per element into LDS and shared by its nine column threads
      * (fewer live registers: 166 instead of 187 VGPRs, three waves per SIMD instead of two) */
@@ -326,14 +322,19 @@ __global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, TE *__re
     int pa = sw2_tri_resid(in, P, i, k, (+1) * eps, rp);
     int pb = sw2_tri_resid(in, P, i, k, (-1) * eps, rm);
     if (pa | pb) flags[0] = 1; /* cannot happen after tri_partial_fd; kept as a guard */
+    /* column (i, k) lies in the blocks (a, i) of the three row nodes a */
 #pragma unroll
-    for (int r = 0; r < 9; r++) emat[emat_tri(e, r, col)] = (TE)((rp[r] - rm[r]) / eps2);
+    for (int a = 0; a < 3; a++) {
+        const int pos = cpos[9 * e + 3 * a + i];
+#pragma unroll
+        for (int r = 0; r < 3; r++) emat[emat_at(pos, r, k)] = (TE)((rp[3 * a + r] - rm[3 * a + r]) / eps2);
+    }
 }
 
 /* queued elements: nine columns each, central differences of the general wet/dry residual */
 template <typename TE>
 __global__ __launch_bounds__(128) void k_elem_jac_wd(MeshView m, Sw2Par P, const int *__restrict__ wdq,
-                                                     TE *__restrict__ emat)
+                                                     TE *__restrict__ emat, const int *__restrict__ cpos)
 {
     const long n = 9L * wdq[0];
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
@@ -347,11 +348,15 @@ __global__ __launch_bounds__(128) void k_elem_jac_wd(MeshView m, Sw2Par P, const
         double rp[9], rm[9];
         sw2_tri_resid_general(in, P, i, k, (+1) * eps, rp);
         sw2_tri_resid_general(in, P, i, k, (-1) * eps, rm);
-        for (int r = 0; r < 9; r++) emat[emat_tri(e, r, col)] = (TE)((rp[r] - rm[r]) / eps2);
+        for (int a = 0; a < 3; a++) {
+            const int pos = cpos[9 * e + 3 * a + i];
+            for (int r = 0; r < 3; r++) emat[emat_at(pos, r, k)] = (TE)((rp[3 * a + r] - rm[3 * a + r]) / eps2);
+        }
     }
 }
 
-template <typename TE> __global__ void k_seg_jac(SegView s, Sw2Par P, TE *__restrict__ emat)
+template <typename TE>
+__global__ void k_seg_jac(SegView s, Sw2Par P, int E, TE *__restrict__ emat, const int *__restrict__ cpos)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 6 * s.S) return;
@@ -373,7 +378,10 @@ template <typename TE> __global__ void k_seg_jac(SegView s, Sw2Par P, TE *__rest
                   P, i, k, (+1) * eps, rp);
     sw2_seg_resid(s.kind[e], s.val[2 * e], s.val[2 * e + 1], s.nrml[2 * e], s.nrml[2 * e + 1], s.sdjac[e], h, u, v,
                   P, i, k, (-1) * eps, rm);
-    for (int r = 0; r < 6; r++) emat[emat_seg(e, r, col)] = (TE)((rp[r] - rm[r]) / eps2);
+    for (int a = 0; a < 2; a++) {
+        const int pos = cpos[9 * E + 4 * e + 2 * a + i];
+        for (int r = 0; r < 3; r++) emat[emat_at(pos, r, k)] = (TE)((rp[3 * a + r] - rm[3 * a + r]) / eps2);
+    }
 }
 
 template <typename TE> static int launch_elem_jac(adhk_ctx *c, TE *emat)
@@ -385,14 +393,14 @@ template <typename TE> static int launch_elem_jac(adhk_ctx *c, TE *emat)
         hipLaunchKernelGGL(k_wdq_reset, dim3(1), dim3(64), 0, c->stream, c->wdq);
         const int tpb = c->tune.elem_tpb, epb = tpb / 9; /* whole elements per workgroup */
         hipLaunchKernelGGL(k_elem_jac<TE>, dim3((unsigned)((c->E + epb - 1) / epb)), dim3(tpb), 0, c->stream,
-                           make_view(c), P, emat, c->flags, c->wdq);
+                           make_view(c), P, emat, c->cpos, c->flags, c->wdq);
         hipLaunchKernelGGL(k_elem_jac_wd<TE>, dim3(wd_grid(nt)), dim3(128), 0, c->stream, make_view(c), P, c->wdq,
-                           emat);
+                           emat, c->cpos);
         CHECK_LAUNCH();
     }
     if (c->S > 0) {
         hipLaunchKernelGGL(k_seg_jac<TE>, dim3((6 * c->S + 127) / 128), dim3(128), 0, c->stream, make_segview(c), P,
-                           emat + 81 * (size_t)c->E);
+                           c->E, emat, c->cpos);
         CHECK_LAUNCH();
     }
     adhk_kt_end(c);
@@ -407,12 +415,12 @@ extern "C" int adhk_launch_elem_jac_p(adhk_ctx *c, int f32)
 extern "C" int adhk_launch_elem_jac(adhk_ctx *c) { return adhk_launch_elem_jac_p(c, 0); }
 
 /* one thread per SELL slot: block entry (3r+cc) = sum over the block's contributions (element order)
- * of the elemental entry (row a*3+r, col bb*3+cc), one contiguous 3x3 block of emat each; padding
- * slots get 0 */
+ * of their entries (r, cc); the contributions of block b are the contiguous run of 3x3 blocks
+ * [cptr[b], cptr[b+1]) of emat; padding slots get 0 */
 template <typename TE, typename TB>
-__global__ __launch_bounds__(256) void k_jac_gather(int nslot64, int E, const int *__restrict__ sell2blk,
-                                                    const int *__restrict__ cptr, const int *__restrict__ contrib,
-                                                    const TE *__restrict__ emat, TB *__restrict__ bvals)
+__global__ __launch_bounds__(256) void k_jac_gather(int nslot64, const int *__restrict__ sell2blk,
+                                                    const int *__restrict__ cptr, const TE *__restrict__ emat,
+                                                    TB *__restrict__ bvals)
 {
     int pos = xcd_block() * blockDim.x + threadIdx.x;
     if (pos >= nslot64) return;
@@ -420,24 +428,9 @@ __global__ __launch_bounds__(256) void k_jac_gather(int nslot64, int E, const in
     double acc[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) acc[k] = 0.0;
-    const long segbase = 81L * E;
     const int j0 = (b >= 0) ? cptr[b] : 0, j1 = (b >= 0) ? cptr[b + 1] : 0;
     for (int j = j0; j < j1; j++) {
-        int cde = contrib[j];
-        const TE *em;
-        int a, bb;
-        if (cde < 9 * E) {
-            int e = cde / 9;
-            a = (cde % 9) / 3;
-            bb = cde % 3;
-            em = emat + 81L * e + 9 * (3 * a + bb);
-        } else {
-            int t = cde - 9 * E;
-            int s = t / 4;
-            a = (t % 4) / 2;
-            bb = t % 2;
-            em = emat + segbase + 36L * s + 9 * (2 * a + bb);
-        }
+        const TE *em = emat + 9L * j;
 #pragma unroll
         for (int k = 0; k < 9; k++) acc[k] += (double)em[k];
     }
@@ -453,10 +446,10 @@ extern "C" int adhk_launch_jac_gather_p(adhk_ctx *c, int f32)
     adhk_kt_begin(c, ADHK_KT_JAC_GATHER);
     if (f32)
         hipLaunchKernelGGL((k_jac_gather<float, float>), dim3((c->nslot * 64 + 255) / 256), dim3(256), 0, c->stream,
-                           c->nslot * 64, c->E, c->sell2blk, c->cptr, c->contrib, c->emat32, c->bvals32);
+                           c->nslot * 64, c->sell2blk, c->cptr, c->emat32, c->bvals32);
     else
         hipLaunchKernelGGL((k_jac_gather<double, double>), dim3((c->nslot * 64 + 255) / 256), dim3(256), 0,
-                           c->stream, c->nslot * 64, c->E, c->sell2blk, c->cptr, c->contrib, c->emat, c->bvals);
+                           c->stream, c->nslot * 64, c->sell2blk, c->cptr, c->emat, c->bvals);
     CHECK_LAUNCH();
     adhk_kt_end(c);
     return 0;
