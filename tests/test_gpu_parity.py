@@ -235,6 +235,43 @@ def test_time_loop_c1_depth(gpu, c1):
     assert rel_err(gs[1::3], os_[1::3]) <= 1e-6
 
 
+def test_failed_step_cuts_dt_and_restarts_from_the_old_level(gpu, c1):
+    """A Newton solve that does not converge (one iteration allowed from rest) returns FALSE, cuts dt
+    by DT_REDUCE_FACTOR (0.25, tc_scale.c:3-19) and restarts from the old level (fe_newton.c's failure
+    path), as the oracle does; the following steps at the cut dt then match the oracle."""
+    mesh, sol0 = c1
+    sm, orc = make_pair(mesh, sol0)
+    sm.max_nonlin_it = 1
+    ng, sg = A.time_loop(sm, 1)
+    no, so = orc.time_loop(1, O.default_solver(max_nonlin_it=1))
+    assert ng == no == -1 and sg["newton_its"] == so["newton_its"] == 1
+    assert sm.get_dt() == 0.25
+    assert np.array_equal(sm.sol, orc.state()[0])  # both back at the old level
+    sm.max_nonlin_it = 10
+    ng, sg = A.time_loop(sm, 4)
+    no, so = orc.time_loop(4)
+    assert ng == no == 4 and sg["newton_its"] == so["newton_its"]
+    gs, os_ = sm.sol, orc.state()[0]
+    assert rel_err(gs[0::3], os_[0::3]) <= 1e-8
+    assert rel_err(gs[1::3], os_[1::3]) <= 1e-6
+
+
+def test_nan_state_fails_the_step_without_a_fault(gpu, c1):
+    """A NaN in the state makes fe_newton return FALSE (the reference calls exit at newton_tools.c:348);
+    the context stays usable: a valid state set afterwards steps normally."""
+    mesh, sol0 = c1
+    sm = A.SModelSuper(mesh, dt=1.0)
+    bad = sol0.copy()
+    bad[3 * 100] = np.nan
+    sm.set_initial(bad)
+    n, st = A.time_loop(sm, 1)
+    assert n == -1 and st["converged"] == 0
+    sm.set_initial(sol0)
+    sm.set_dt(1.0)
+    n, st = A.time_loop(sm, 2)
+    assert n == 2 and np.isfinite(sm.sol).all()
+
+
 def test_mixed_precision_c1(gpu, c1):
     mesh, sol0 = c1
     a = A.SModelSuper(mesh, dt=1.0)
