@@ -226,7 +226,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--train-steps", type=int, default=None,
-                    help="time steps per training trajectory (default 20; c5 200)")
+                    help="time steps per training trajectory (default 100; c5 200)")
+    ap.add_argument("--adam-steps", type=int, default=4000, help="optimizer steps of the warm-start training")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--check-every", type=int, default=8)
@@ -244,7 +245,7 @@ def main():
     if args.steps is None:
         args.steps = {"c1": 100, "c2": 50, "c3": 20, "c5": 200}[args.config]
     if args.train_steps is None:
-        args.train_steps = 200 if args.config == "c5" else 20
+        args.train_steps = 200 if args.config == "c5" else 100
     prepare = None
     if args.config == "c5":
         t0 = time.perf_counter()
@@ -364,7 +365,8 @@ def main():
             tr_prepare = None
             tr_desc = "2 flumes x %d steps (bump seeds 2, 3)" % args.train_steps
         log("learned warm start: trajectories and training")
-        mlp, hist, gen_its = LR.train_warm_start(cases, nsteps=args.train_steps, steps=2000, batch=16384, lr=2e-3,
+        mlp, hist, gen_its = LR.train_warm_start(cases, nsteps=args.train_steps, steps=args.adam_steps, batch=16384,
+                                                 lr=2e-3,
                                                  seed=5, precision=L.PREC_MIXED, dt=dt, prepare=tr_prepare)
         t_train = time.perf_counter() - t0
         ml = run_variant(A, L, mesh, sol0, dt, L.PREC_MIXED, args.steps, args.warmup, args.check_every, warm=mlp,
@@ -381,7 +383,7 @@ def main():
             "plain_mixed_newton_its": out["mixed"]["newton_its"] if "mixed" in out else None,
             "speedup_vs_plain_mixed": round(vml / out["mixed"]["value"], 3) if "mixed" in out else None,
             "training": {"trajectories": tr_desc,
-                         "samples": mlp.size, "adam_steps": 2000, "batch": 16384,
+                         "samples": mlp.size, "adam_steps": args.adam_steps, "batch": 16384,
                          "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-10:].mean()),
                          "seconds_incl_data_generation": round(t_train, 2)}}
     if args.config == "c2":
