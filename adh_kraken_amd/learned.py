@@ -100,34 +100,45 @@ def set_warm_start(sm: SModelSuper, mlp):
     sm._warm = mlp  # keep the model alive while attached
 
 
-def collect_trajectory(sm: SModelSuper, mlp: WarmStartMLP, nsteps, precision=L.PREC_FP64):
-    """Advance sm by nsteps plain time steps, adding each step's (features, change) of every node."""
+def collect_trajectory(sm: SModelSuper, mlp: WarmStartMLP, nsteps, precision=L.PREC_FP64, strict=True):
+    """Advance sm by up to nsteps plain time steps, adding each converged step's (features, change) of
+    every node.  A step that fails raises (strict) or ends the trajectory there, keeping the steps
+    already collected.  Returns (Newton iterations, steps collected)."""
     its = 0
-    for _ in range(nsteps):
+    for k in range(nsteps):
         n, st = time_loop(sm, 1, precision=precision)
         if n != 1:
-            raise L.AdhkError(f"time step failed while collecting: {st}")
+            if strict:
+                raise L.AdhkError(f"time step failed while collecting: {st}")
+            return its, k
         its += st["newton_its"]
         mlp.collect(sm)
-    return its
+    return its, nsteps
 
 
 def train_warm_start(cases, nsteps, steps=2000, batch=8192, lr=2e-3, seed=1, precision=L.PREC_FP64, dt=1.0,
-                     prepare=None):
+                     prepare=None, strict=True):
     """Train a model on solver trajectories: cases = [(mesh, sol0), ...], nsteps steps each;
     prepare(sm, i), when given, sets up case i's model (e.g. its boundary time series) before it runs.
+    strict=False ends a trajectory at its first failed step instead of raising (the steps of each
+    trajectory are then in model.collected_steps).
 
     Returns (model, loss history, data-generation Newton iterations)."""
     mlp = WarmStartMLP(seed=seed)
     mlp.reserve(sum(m.nnodes for m, _ in cases) * nsteps)
+    mlp.collected_steps = []
     gen_its = 0
     for i, (mesh, sol0) in enumerate(cases):
         sm = SModelSuper(mesh, dt=dt)
         sm.set_initial(sol0)
         if prepare is not None:
             prepare(sm, i)
-        gen_its += collect_trajectory(sm, mlp, nsteps, precision)
+        its, k = collect_trajectory(sm, mlp, nsteps, precision, strict=strict)
+        gen_its += its
+        mlp.collected_steps.append(k)
         sm.close()
+    if mlp.size == 0:
+        raise L.AdhkError("no time step converged while collecting training trajectories")
     mlp.fit_normalization()
     hist = mlp.train(steps, batch=batch, lr=lr, seed=seed + 100)
     return mlp, hist, gen_its

@@ -11,7 +11,7 @@ depth/velocity error against the FP64 result of the same K steps are reported
 beside it.
 
 A third variant adds the learned warm start to the mixed-precision solver: a per-node MLP trained
-(untimed) on the GPU on trajectories of two other flumes of the same family (bump seeds 2 and 3).
+(untimed) on the GPU on 100-step trajectories of four other flumes of the same family (bump seeds 2-5).
 
 The roofline is for the dominant kernel of the plain run: algorithmic bytes of
 one launch divided by its average launch time, measured with hipEvent pairs on
@@ -227,7 +227,10 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--train-steps", type=int, default=None,
                     help="time steps per training trajectory (default 100; c5 200)")
-    ap.add_argument("--adam-steps", type=int, default=4000, help="optimizer steps of the warm-start training")
+    ap.add_argument("--adam-steps", type=int, default=6000, help="optimizer steps of the warm-start training")
+    ap.add_argument("--train-cases", type=int, default=4,
+                    help="training meshes: flumes with bed-bump seeds 2, 3, ... (c1, c2) or channels of the "
+                         "c3 family with seeds 11, 12, ... (c5)")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--check-every", type=int, default=8)
@@ -350,24 +353,29 @@ def main():
         from adh_kraken_amd import learned as LR
         t0 = time.perf_counter()
         if args.config == "c5":
-            # trajectories of two straight channels of the c3 family (2 km x 0.5 km, 100 m channel, seeds 11
-            # and 12) under the same hydrograph per unit channel width (25 -> 300 m3/s), mixed precision
+            # trajectories of straight channels of the c3 family (2 km x 0.5 km, 100 m channel, seeds 11,
+            # 12, ...) under the same hydrograph per unit channel width (25 -> 300 m3/s), mixed precision
             tr_specs = [A.ChannelSpec(length=2000.0, width=500.0, spacing=3.0, chan_width=100.0, chan_depth=3.0,
                                       bank_width=20.0, slope=1e-4, nbumps=10, seed=sd, n_channel=0.025,
-                                      n_floodplain=0.06, inflow_q=25.0, fp_depth_out=1.5) for sd in (11, 12)]
+                                      n_floodplain=0.06, inflow_q=25.0, fp_depth_out=1.5)
+                        for sd in range(11, 11 + args.train_cases)]
             cases = [A.channel_mesh(ts) for ts in tr_specs]
             tq = (hydro[0], hydro[1] * 100.0 / 200.0)
             tr_prepare = lambda sm_, i: A.bind_hydrograph(sm_, *tq)  # noqa: E731
-            tr_desc = "2 channels of the c3 family (2 km x 0.5 km, seeds 11, 12) x %d steps" % args.train_steps
+            tr_desc = "%d channels of the c3 family (2 km x 0.5 km, seeds %s) x %d steps" % (
+                len(tr_specs), ", ".join(str(11 + i) for i in range(len(tr_specs))), args.train_steps)
         else:
-            # trajectories of two other flumes of the same family (bed-bump seeds 2 and 3), mixed precision
-            cases = [A.flume_mesh(A.FlumeSpec(**{**spec.__dict__, "seed": sd})) for sd in (2, 3)]
+            # trajectories of other flumes of the same family (bed-bump seeds 2, 3, ...), mixed precision
+            seeds = tuple(range(2, 2 + args.train_cases))
+            cases = [A.flume_mesh(A.FlumeSpec(**{**spec.__dict__, "seed": sd})) for sd in seeds]
             tr_prepare = None
-            tr_desc = "2 flumes x %d steps (bump seeds 2, 3)" % args.train_steps
+            tr_desc = "%d flumes x %d steps (bump seeds %s)" % (len(seeds), args.train_steps,
+                                                                ", ".join(str(sd) for sd in seeds))
         log("learned warm start: trajectories and training")
         mlp, hist, gen_its = LR.train_warm_start(cases, nsteps=args.train_steps, steps=args.adam_steps, batch=16384,
                                                  lr=2e-3,
-                                                 seed=5, precision=L.PREC_MIXED, dt=dt, prepare=tr_prepare)
+                                                 seed=5, precision=L.PREC_MIXED, dt=dt, prepare=tr_prepare,
+                                                 strict=False)
         t_train = time.perf_counter() - t0
         ml = run_variant(A, L, mesh, sol0, dt, L.PREC_MIXED, args.steps, args.warmup, args.check_every, warm=mlp,
                          schedule=schedule, prepare=prepare)
@@ -382,7 +390,7 @@ def main():
             "depth_rel_err_vs_fp64": float(np.abs(hml - h64).max() / np.abs(h64).max()),
             "plain_mixed_newton_its": out["mixed"]["newton_its"] if "mixed" in out else None,
             "speedup_vs_plain_mixed": round(vml / out["mixed"]["value"], 3) if "mixed" in out else None,
-            "training": {"trajectories": tr_desc,
+            "training": {"trajectories": tr_desc, "steps_collected": mlp.collected_steps,
                          "samples": mlp.size, "adam_steps": args.adam_steps, "batch": 16384,
                          "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-10:].mean()),
                          "seconds_incl_data_generation": round(t_train, 2)}}
