@@ -1236,9 +1236,11 @@ fail:
  * count -- exactly as when it is solved alone: launches cover all instances, and the work mask keeps
  * the updates of an instance that has stopped (or is not in a line search) from being applied.
  * Returns TRUE(1) when every instance converged, FALSE(0) otherwise. */
-int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
+/* One Newton solve of every live instance from its previous level, or from the learned start where that
+ * has the smaller residual (use_warm); per-instance results in inst_stats.  Returns TRUE when every live
+ * instance converged, FALSE otherwise, -1 on error; failures are handled by the caller. */
+static int fe_newton_pass(adhk_ctx *c, const adhk_solver *s, int use_warm)
 {
-    if (!c || !s) return -1;
     const int K = c->ninst;
     const int mixed = (s->precision == ADHK_PREC_MIXED);
     adhk_scalars *hs = NULL, *hw = NULL;
@@ -1273,7 +1275,7 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
     LC(adhk_launch_init_system(c));
     LC(run_stage(c, s, mixed, STG_RES));
     LC(norms_read(c, hs));
-    if (c->warm) {
+    if (c->warm && use_warm) {
         /* learned start: an instance keeps it only when its residual is smaller than the previous
          * level's; the previous level's residual is kept (in the idle Krylov vector r) for the fallback */
         int any_reject = 0;
@@ -1371,27 +1373,71 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
                          (rmax[i] >= 1.0E+20));
         all_ok &= t->converged;
     }
-    if (!all_ok && K == 1) {
-        /* tc_scale (tc_scale.c:3-19, DT_REDUCE_FACTOR define.h:127): cut dt and restart from the old level */
-        act[0] = 1;
-        LC(set_work(c, act));
+    free(hs); free(hw); free(act); free(ls); free(solv); free(its); free(lits); free(cuts);
+    free(rmax); free(rl2); free(incmax); free(old_rl2);
+    return all_ok;
+fail:
+    free(hs); free(hw); free(act); free(ls); free(solv); free(its); free(lits); free(cuts);
+    free(rmax); free(rl2); free(incmax); free(old_rl2);
+    return -1;
+}
+
+/* a failed solve: one mesh cuts dt and restarts from the old level (tc_scale, tc_scale.c:3-19,
+ * DT_REDUCE_FACTOR define.h:127); in an ensemble the failed members restart from their old level and
+ * leave the following steps */
+static int newton_failure(adhk_ctx *c)
+{
+    const int K = c->ninst;
+    int *mask = calloc((size_t)K, sizeof(int));
+    if (!mask) {
+        adhk_set_error("adhk_fe_newton: out of host memory");
+        return -1;
+    }
+    if (K == 1) {
+        mask[0] = 1;
+        LC(set_work(c, mask));
         c->p.dt *= 0.25;
         LC(adhk_launch_init_system(c));
-        HC(hipStreamSynchronize(c->stream));
-    } else if (!all_ok) {
-        /* ensemble: a failed member restarts from its old level and leaves the following steps */
+    } else {
         for (int i = 0; i < K; i++) {
-            ls[i] = c->alive[i] && !c->inst_stats[i].converged;
-            if (ls[i]) c->alive[i] = 0;
+            mask[i] = c->alive[i] && !c->inst_stats[i].converged;
+            if (mask[i]) c->alive[i] = 0;
         }
-        LC(set_work(c, ls));
+        LC(set_work(c, mask));
         LC(adhk_launch_start_from_old_masked(c));
-        HC(hipStreamSynchronize(c->stream));
     }
+    HC(hipStreamSynchronize(c->stream));
+    free(mask);
+    return 0;
+fail:
+    free(mask);
+    return -1;
+}
+
+int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
+{
+    if (!c || !s) return -1;
+    int ok = fe_newton_pass(c, s, 1);
+    if (ok < 0) return -1;
+    if (!ok && c->ninst == 1 && c->warm && c->inst_stats[0].warm_accepted) {
+        /* plain Newton stays the fallback: a solve that fails from the learned start is run again from the
+         * previous level (the pass starts with initialize_system) before dt is cut */
+        const adhk_newton_stats first = c->inst_stats[0];
+        ok = fe_newton_pass(c, s, 0);
+        if (ok < 0) return -1;
+        adhk_newton_stats *t = &c->inst_stats[0];
+        t->newton_its += first.newton_its;
+        t->linesearch_cuts_total += first.linesearch_cuts_total;
+        t->lin_its_total += first.lin_its_total;
+        if (first.lin_its_max > t->lin_its_max) t->lin_its_max = first.lin_its_max;
+        t->warm_accepted = 0;
+        t->warm_retried = 1;
+    }
+    if (!ok && newton_failure(c)) return -1;
     if (st) {
         adhk_newton_stats tot;
         memset(&tot, 0, sizeof tot);
-        for (int i = 0; i < K; i++) {
+        for (int i = 0; i < c->ninst; i++) {
             const adhk_newton_stats *t = &c->inst_stats[i];
             tot.newton_its += t->newton_its;
             tot.linesearch_cuts_total += t->linesearch_cuts_total;
@@ -1400,17 +1446,12 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
             if (t->resid_max_norm > tot.resid_max_norm) tot.resid_max_norm = t->resid_max_norm;
             if (t->inc_max_norm > tot.inc_max_norm) tot.inc_max_norm = t->inc_max_norm;
             tot.warm_accepted += t->warm_accepted;
+            tot.warm_retried += t->warm_retried;
         }
-        tot.converged = all_ok;
+        tot.converged = ok;
         *st = tot;
     }
-    free(hs); free(hw); free(act); free(ls); free(solv); free(its); free(lits); free(cuts);
-    free(rmax); free(rl2); free(incmax); free(old_rl2);
-    return all_ok;
-fail:
-    free(hs); free(hw); free(act); free(ls); free(solv); free(its); free(lits); free(cuts);
-    free(rmax); free(rl2); free(incmax); free(old_rl2);
-    return -1;
+    return ok;
 }
 
 /* time_loop / advance_time (time_loop.c:14-93): prep_sol then fe_newton per step; per-instance
@@ -1455,6 +1496,7 @@ int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_st
             run[i].resid_max_norm = t->resid_max_norm;
             run[i].inc_max_norm = t->inc_max_norm;
             run[i].warm_accepted += t->warm_accepted;
+            run[i].warm_retried += t->warm_retried;
             run[i].converged = c->alive[i] ? t->converged : 0;
             if (c->alive[i] && t->converged) c->steps_done[i]++;
         }
@@ -1465,6 +1507,7 @@ int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_st
         tot.resid_max_norm = st.resid_max_norm;
         tot.inc_max_norm = st.inc_max_norm;
         tot.warm_accepted += st.warm_accepted;
+        tot.warm_retried += st.warm_retried;
     }
     memcpy(c->inst_stats, run, (size_t)c->ninst * sizeof *run);
     free(run);

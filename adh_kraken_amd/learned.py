@@ -117,11 +117,13 @@ def collect_trajectory(sm: SModelSuper, mlp: WarmStartMLP, nsteps, precision=L.P
 
 
 def train_warm_start(cases, nsteps, steps=2000, batch=8192, lr=2e-3, seed=1, precision=L.PREC_FP64, dt=1.0,
-                     prepare=None, strict=True):
+                     prepare=None, strict=True, validate=None, checkpoints=1):
     """Train a model on solver trajectories: cases = [(mesh, sol0), ...], nsteps steps each;
     prepare(sm, i), when given, sets up case i's model (e.g. its boundary time series) before it runs.
     strict=False ends a trajectory at its first failed step instead of raising (the steps of each
-    trajectory are then in model.collected_steps).
+    trajectory are then in model.collected_steps).  With validate (model -> {"cost": ...}, e.g.
+    validation_cost on a case held out of the training set), the steps run in `checkpoints` chunks
+    and the chunk end with the lowest cost is kept (model.validation records the choice).
 
     Returns (model, loss history, data-generation Newton iterations)."""
     mlp = WarmStartMLP(seed=seed)
@@ -140,5 +142,41 @@ def train_warm_start(cases, nsteps, steps=2000, batch=8192, lr=2e-3, seed=1, pre
     if mlp.size == 0:
         raise L.AdhkError("no time step converged while collecting training trajectories")
     mlp.fit_normalization()
-    hist = mlp.train(steps, batch=batch, lr=lr, seed=seed + 100)
-    return mlp, hist, gen_its
+    if validate is None:
+        hist = mlp.train(steps, batch=batch, lr=lr, seed=seed + 100)
+        return mlp, hist, gen_its
+    # training in `checkpoints` chunks, each chunk's end a candidate; the candidate whose warm start costs
+    # the held-out case the least solver work is kept (ties: the earlier one)
+    per = max(1, steps // checkpoints)
+    hist, snaps, costs = [], [], []
+    for c in range(checkpoints):
+        hist.append(mlp.train(per, batch=batch, lr=lr, seed=seed + 100 + c))
+        snaps.append(mlp.get_params())
+        costs.append(validate(mlp))
+    best = min(range(checkpoints), key=lambda c: (costs[c]["cost"], c))
+    mlp.set_params(*snaps[best])
+    mlp.validation = {"checkpoints": [per * (c + 1) for c in range(checkpoints)], "costs": costs, "chosen": best}
+    return mlp, np.concatenate(hist), gen_its
+
+
+def validation_cost(mesh, sol0, nsteps, precision=L.PREC_FP64, dt=1.0, prepare=None, newton_weight=15.0):
+    """A validate() for train_warm_start: the solver work of nsteps time steps of a held-out case with the
+    model as warm start, as BiCGStab iterations + newton_weight x Newton iterations (a Newton iteration
+    also builds and folds the Jacobian, about as much work as newton_weight BiCGStab iterations on the
+    flume and channel workloads).  model=None gives the same case without a warm start."""
+    def cost(mlp):
+        sm = SModelSuper(mesh, dt=dt)
+        sm.set_initial(sol0)
+        if prepare is not None:
+            prepare(sm)
+        if mlp is not None:
+            set_warm_start(sm, mlp)
+        n, st = time_loop(sm, nsteps, precision=precision)
+        if mlp is not None:
+            set_warm_start(sm, None)
+        sm.close()
+        ok = n == nsteps
+        return {"steps": n, "newton_its": st["newton_its"], "bicgstab_its": st["lin_its_total"],
+                "warm_accepted": st["warm_accepted"],
+                "cost": (st["lin_its_total"] + newton_weight * st["newton_its"]) if ok else float("inf")}
+    return cost

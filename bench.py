@@ -228,6 +228,9 @@ def main():
     ap.add_argument("--train-steps", type=int, default=None,
                     help="time steps per training trajectory (default 100; c5 200)")
     ap.add_argument("--adam-steps", type=int, default=6000, help="optimizer steps of the warm-start training")
+    ap.add_argument("--checkpoints", type=int, default=6,
+                    help="training chunks whose ends are validated on a held-out case; the best is kept")
+    ap.add_argument("--val-steps", type=int, default=30, help="time steps of the held-out validation case")
     ap.add_argument("--train-cases", type=int, default=4,
                     help="training meshes: flumes with bed-bump seeds 2, 3, ... (c1, c2) or channels of the "
                          "c3 family with seeds 11, 12, ... (c5)")
@@ -355,27 +358,32 @@ def main():
         if args.config == "c5":
             # trajectories of straight channels of the c3 family (2 km x 0.5 km, 100 m channel, seeds 11,
             # 12, ...) under the same hydrograph per unit channel width (25 -> 300 m3/s), mixed precision
-            tr_specs = [A.ChannelSpec(length=2000.0, width=500.0, spacing=3.0, chan_width=100.0, chan_depth=3.0,
-                                      bank_width=20.0, slope=1e-4, nbumps=10, seed=sd, n_channel=0.025,
-                                      n_floodplain=0.06, inflow_q=25.0, fp_depth_out=1.5)
-                        for sd in range(11, 11 + args.train_cases)]
-            cases = [A.channel_mesh(ts) for ts in tr_specs]
+            specs = [A.ChannelSpec(length=2000.0, width=500.0, spacing=3.0, chan_width=100.0, chan_depth=3.0,
+                                   bank_width=20.0, slope=1e-4, nbumps=10, seed=sd, n_channel=0.025,
+                                   n_floodplain=0.06, inflow_q=25.0, fp_depth_out=1.5)
+                     for sd in range(11, 12 + args.train_cases)]  # the last one is held out for validation
+            cases = [A.channel_mesh(ts) for ts in specs]
             tq = (hydro[0], hydro[1] * 100.0 / 200.0)
             tr_prepare = lambda sm_, i: A.bind_hydrograph(sm_, *tq)  # noqa: E731
+            val_prepare = lambda sm_: A.bind_hydrograph(sm_, *tq)  # noqa: E731
             tr_desc = "%d channels of the c3 family (2 km x 0.5 km, seeds %s) x %d steps" % (
-                len(tr_specs), ", ".join(str(11 + i) for i in range(len(tr_specs))), args.train_steps)
+                args.train_cases, ", ".join(str(11 + i) for i in range(args.train_cases)), args.train_steps)
+            val_desc = "channel seed %d, %d steps" % (11 + args.train_cases, args.val_steps)
         else:
             # trajectories of other flumes of the same family (bed-bump seeds 2, 3, ...), mixed precision
-            seeds = tuple(range(2, 2 + args.train_cases))
+            seeds = tuple(range(2, 3 + args.train_cases))  # the last one is held out for validation
             cases = [A.flume_mesh(A.FlumeSpec(**{**spec.__dict__, "seed": sd})) for sd in seeds]
-            tr_prepare = None
-            tr_desc = "%d flumes x %d steps (bump seeds %s)" % (len(seeds), args.train_steps,
-                                                                ", ".join(str(sd) for sd in seeds))
-        log("learned warm start: trajectories and training")
-        mlp, hist, gen_its = LR.train_warm_start(cases, nsteps=args.train_steps, steps=args.adam_steps, batch=16384,
-                                                 lr=2e-3,
-                                                 seed=5, precision=L.PREC_MIXED, dt=dt, prepare=tr_prepare,
-                                                 strict=False)
+            tr_prepare = val_prepare = None
+            tr_desc = "%d flumes x %d steps (bump seeds %s)" % (args.train_cases, args.train_steps,
+                                                                ", ".join(str(sd) for sd in seeds[:-1]))
+            val_desc = "flume bump seed %d, %d steps" % (seeds[-1], args.val_steps)
+        log("learned warm start: trajectories, training and checkpoint validation")
+        val = LR.validation_cost(*cases[-1], args.val_steps, precision=L.PREC_MIXED, dt=dt, prepare=val_prepare)
+        mlp, hist, gen_its = LR.train_warm_start(cases[:-1], nsteps=args.train_steps, steps=args.adam_steps,
+                                                 batch=16384, lr=2e-3, seed=5, precision=L.PREC_MIXED, dt=dt,
+                                                 prepare=tr_prepare, strict=False, validate=val,
+                                                 checkpoints=args.checkpoints)
+        val_plain = val(None)
         t_train = time.perf_counter() - t0
         ml = run_variant(A, L, mesh, sol0, dt, L.PREC_MIXED, args.steps, args.warmup, args.check_every, warm=mlp,
                          schedule=schedule, prepare=prepare)
@@ -385,7 +393,8 @@ def main():
             "value": round(vml, 4), "unit": "timesteps/s", "speedup_vs_fp64": round(vml / value, 3),
             "variant": "mixed precision + learned per-node MLP warm start (12-64-64-3 GELU, BF16 weights); the "
                        "start with the smaller l2-norm residual of {learned, previous level} is kept",
-            "warm_accepted_steps": ml["stats"]["warm_accepted"], "newton_its": ml["stats"]["newton_its"],
+            "warm_accepted_steps": ml["stats"]["warm_accepted"], "warm_retried_steps": ml["stats"]["warm_retried"],
+            "newton_its": ml["stats"]["newton_its"],
             "bicgstab_its": ml["stats"]["lin_its_total"],
             "depth_rel_err_vs_fp64": float(np.abs(hml - h64).max() / np.abs(h64).max()),
             "plain_mixed_newton_its": out["mixed"]["newton_its"] if "mixed" in out else None,
@@ -393,7 +402,13 @@ def main():
             "training": {"trajectories": tr_desc, "steps_collected": mlp.collected_steps,
                          "samples": mlp.size, "adam_steps": args.adam_steps, "batch": 16384,
                          "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-10:].mean()),
-                         "seconds_incl_data_generation": round(t_train, 2)}}
+                         "seconds_incl_data_generation_and_validation": round(t_train, 2)},
+            "validation": {"case": val_desc, "weight": "BiCGStab its + 15 x Newton its",
+                           "checkpoints": mlp.validation["checkpoints"], "chosen": mlp.validation["chosen"],
+                           "costs": [c["cost"] for c in mlp.validation["costs"]], "cost_without_model": val_plain["cost"],
+                           "chosen_its": {k: mlp.validation["costs"][mlp.validation["chosen"]][k]
+                                          for k in ("newton_its", "bicgstab_its", "warm_accepted")},
+                           "its_without_model": {k: val_plain[k] for k in ("newton_its", "bicgstab_its")}}}
     if args.config == "c2":
         log("PCIe-inclusive leg")
         out["pcie_inclusive"] = pcie_leg(A, L, mesh, sol0, dt)

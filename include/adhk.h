@@ -80,6 +80,8 @@ typedef struct {
     double inc_max_norm;
     int converged;     /* fe_newton return value */
     int warm_accepted; /* Newton solves that started from the learned iterate (adhk_sw2_set_warm_start) */
+    int warm_retried;  /* solves that failed from the learned iterate and were solved again from the
+                          previous level (no counterpart in the reference) */
 } adhk_newton_stats;
 
 /* per-stage device time of the last adhk_time_loop / adhk_fe_newton call, ms */

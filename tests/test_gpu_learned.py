@@ -127,3 +127,37 @@ def test_useless_model_falls_back_to_plain_newton(gpu):
     assert na == nb == 5 and stb["warm_accepted"] == 0
     assert sta["newton_its"] == stb["newton_its"] and sta["lin_its_total"] == stb["lin_its_total"]
     assert np.array_equal(a.sol, b.sol)
+
+
+def test_failed_learned_start_is_solved_again_from_the_previous_level(gpu, trained):
+    """Plain Newton stays the fallback: a solve that does not converge from the accepted learned start
+    (here: one Newton iteration allowed) runs again from the previous level before dt is cut, so the
+    step ends exactly as it would without the model (state restored, dt cut by 0.25).  The flume is
+    advanced step by step until a step whose learned start is accepted."""
+    mlp, _ = trained
+    mesh, sol0 = flume(24, q=10.5)  # held out
+    warm = A.SModelSuper(mesh, dt=1.0)
+    warm.set_initial(sol0)
+    LR.set_warm_start(warm, mlp)
+    plain = A.SModelSuper(mesh, dt=1.0)
+    for _ in range(20):
+        snap = (warm.sol.copy(), warm.sol_old.copy(), warm.sol_older.copy())
+        warm.max_nonlin_it = 1
+        n1, st1 = A.time_loop(warm, 1)
+        if st1["warm_retried"]:
+            break
+        # the start was rejected: back to the snapshot, then one ordinary step
+        warm.sol, warm.sol_old, warm.sol_older = snap
+        warm.set_dt(1.0)
+        warm.max_nonlin_it = 10
+        assert A.time_loop(warm, 1)[0] == 1
+    else:
+        pytest.fail("no learned start accepted in 20 steps")
+    plain.sol, plain.sol_old, plain.sol_older = snap
+    plain.max_nonlin_it = 1
+    n0, st0 = A.time_loop(plain, 1)
+    assert n0 == n1 == -1
+    assert st1["warm_retried"] == 1 and st1["warm_accepted"] == 0
+    assert st1["newton_its"] == st0["newton_its"] + 1
+    assert warm.get_dt() == plain.get_dt() == 0.25
+    assert np.array_equal(warm.sol, plain.sol)
