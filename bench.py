@@ -11,7 +11,9 @@ depth/velocity error against the FP64 result of the same K steps are reported
 beside it.
 
 A third variant adds the learned warm start to the mixed-precision solver: a per-node MLP trained
-(untimed) on the GPU on 100-step trajectories of four other flumes of the same family (bump seeds 2-5).
+(untimed) on the GPU on 100-step trajectories of four other flumes of the same family (bump seeds
+2-5), the training checkpoint kept being the one that costs a fifth held-out flume (seed 6) the
+least solver work.
 
 The roofline is for the dominant kernel of the plain run: algorithmic bytes of
 one launch divided by its average launch time, measured with hipEvent pairs on
