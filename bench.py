@@ -18,7 +18,8 @@ least solver work.
 The roofline is for the dominant kernel of the plain run: algorithmic bytes of
 one launch divided by its average launch time, measured with hipEvent pairs on
 the library's stream during the timed region.  The CPU baseline is the oracle
-on all host threads over a bounded sample (the first five time steps of the same workload).
+on all host threads over a bounded sample (the first five time steps of the same workload, median
+of three runs), with a one-thread sample of the first step beside it.
 """
 import argparse
 import ctypes as C
