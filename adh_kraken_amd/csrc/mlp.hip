@@ -17,7 +17,7 @@
  *
  * No counterpart in the reference: there the Newton solve starts from the previous
  * time level (smodel_super_prep_sol.c:2-8, initialize_system newton_tools.c:21).
- * Plain Newton stays the fallback and the judge: fe_newton keeps whichever initial
+ * Plain Newton stays the fallback and the arbiter: fe_newton keeps whichever initial
  * iterate, learned or previous level, has the smaller residual (l2 norm by default,
  * max norm with adhk_solver.warm_gate = 0), and runs the unchanged Newton iteration
  * and stopping test from it.
