@@ -234,6 +234,16 @@ struct Sell {
     const unsigned char *diag_id; /* [N]: 1 = the folded diagonal block is the identity */
 };
 
+/* a matrix load; NT marks it non-temporal, so that a matrix streamed once per product does not evict
+ * the gathered vector lines from L2 (worth it only when the matrix cannot stay in the 256 MB MALL) */
+template <bool NT, typename T> __device__ __forceinline__ T ldm(const T *p)
+{
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
 /* Slot range [c0, c1) of node n's slice.  UNI: the 64 lanes of the wave hold the 64 nodes of one slice
  * (blocks start at multiples of 64), so the slice index is wave-uniform and the loads are scalar. */
 template <bool UNI>
@@ -253,8 +263,8 @@ __device__ __forceinline__ void slice_bounds(int n, const Sell &S, int &c0, int 
  * products still accumulate in slot order.  Every multiply-add here and in the dot products is an
  * explicit fused multiply-add, so each kernel that uses them rounds identically whatever the compiler
  * contracts. */
-template <typename T, typename Col>
-__device__ __forceinline__ void block_row_cs(int n, int c0, int c1, const Sell &S, const T *__restrict__ bv,
+template <typename T, typename Col, bool NT>
+__device__ __forceinline__ void block_row_nt(int n, int c0, int c1, const Sell &S, const T *__restrict__ bv,
                                              const Col &col, double &y0, double &y1, double &y2)
 {
     const int lane = n & 63;
@@ -262,14 +272,16 @@ __device__ __forceinline__ void block_row_cs(int n, int c0, int c1, const Sell &
     if (!S.diag_id[n]) {
         const T *a = bv + (size_t)c0 * 9 * 64 + lane;
         const double x0 = y0, x1 = y1, x2 = y2;
-        y0 = __fma_rn((double)a[128], x2, __fma_rn((double)a[64], x1, (double)a[0] * x0));
-        y1 = __fma_rn((double)a[320], x2, __fma_rn((double)a[256], x1, (double)a[192] * x0));
-        y2 = __fma_rn((double)a[512], x2, __fma_rn((double)a[448], x1, (double)a[384] * x0));
+        y0 = __fma_rn((double)ldm<NT>(a + 128), x2, __fma_rn((double)ldm<NT>(a + 64), x1, (double)ldm<NT>(a) * x0));
+        y1 = __fma_rn((double)ldm<NT>(a + 320), x2,
+                      __fma_rn((double)ldm<NT>(a + 256), x1, (double)ldm<NT>(a + 192) * x0));
+        y2 = __fma_rn((double)ldm<NT>(a + 512), x2,
+                      __fma_rn((double)ldm<NT>(a + 448), x1, (double)ldm<NT>(a + 384) * x0));
     }
     for (int cs = c0 + 1; cs < c1; cs += 8) {
         int m[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) m[u] = (cs + u < c1) ? S.scol[(cs + u) * 64 + lane] : n;
+        for (int u = 0; u < 8; u++) m[u] = (cs + u < c1) ? ldm<NT>(S.scol + (cs + u) * 64 + lane) : n;
         double xv[8][3];
 #pragma unroll
         for (int u = 0; u < 8; u++) col.get(m[u], xv[u][0], xv[u][1], xv[u][2]);
@@ -278,26 +290,33 @@ __device__ __forceinline__ void block_row_cs(int n, int c0, int c1, const Sell &
             if (cs + u >= c1) break;
             const T *a = bv + (size_t)(cs + u) * 9 * 64 + lane;
             const double x0 = xv[u][0], x1 = xv[u][1], x2 = xv[u][2];
-            y0 = __fma_rn((double)a[0], x0, y0);
-            y0 = __fma_rn((double)a[64], x1, y0);
-            y0 = __fma_rn((double)a[128], x2, y0);
-            y1 = __fma_rn((double)a[192], x0, y1);
-            y1 = __fma_rn((double)a[256], x1, y1);
-            y1 = __fma_rn((double)a[320], x2, y1);
-            y2 = __fma_rn((double)a[384], x0, y2);
-            y2 = __fma_rn((double)a[448], x1, y2);
-            y2 = __fma_rn((double)a[512], x2, y2);
+            y0 = __fma_rn((double)ldm<NT>(a), x0, y0);
+            y0 = __fma_rn((double)ldm<NT>(a + 64), x1, y0);
+            y0 = __fma_rn((double)ldm<NT>(a + 128), x2, y0);
+            y1 = __fma_rn((double)ldm<NT>(a + 192), x0, y1);
+            y1 = __fma_rn((double)ldm<NT>(a + 256), x1, y1);
+            y1 = __fma_rn((double)ldm<NT>(a + 320), x2, y1);
+            y2 = __fma_rn((double)ldm<NT>(a + 384), x0, y2);
+            y2 = __fma_rn((double)ldm<NT>(a + 448), x1, y2);
+            y2 = __fma_rn((double)ldm<NT>(a + 512), x2, y2);
         }
     }
 }
 
-template <typename T, typename Col, bool UNI = false>
+template <typename T, typename Col, bool NT = false>
+__device__ __forceinline__ void block_row_cs(int n, int c0, int c1, const Sell &S, const T *__restrict__ bv,
+                                             const Col &col, double &y0, double &y1, double &y2)
+{
+    block_row_nt<T, Col, NT>(n, c0, c1, S, bv, col, y0, y1, y2);
+}
+
+template <typename T, typename Col, bool UNI = false, bool NT = false>
 __device__ __forceinline__ void block_row(int n, const Sell &S, const T *__restrict__ bv, const Col &col, double &y0,
                                           double &y1, double &y2)
 {
     int c0, c1;
     slice_bounds<UNI>(n, S, c0, c1);
-    block_row_cs<T, Col>(n, c0, c1, S, bv, col, y0, y1, y2);
+    block_row_cs<T, Col, NT>(n, c0, c1, S, bv, col, y0, y1, y2);
 }
 
 template <typename T>
@@ -773,8 +792,8 @@ __global__ __launch_bounds__(TPB) void k_kinit3(int nblk, const int *__restrict_
  * r = s - omega As, ||r||_inf and the stop test (:294-307, :210) -- and form p' = (p - omega Ap) beta + r.
  * Then Ap' = M^-1 A p' and q.Ap' -> alpha = rho / (q.Ap')  (:215-246).  Blocks of converged
  * instances are skipped. */
-template <typename T, bool FIRST>
-__global__ __launch_bounds__(TPB) void k_iter_a(BLK_ARGS, Sell S, const T *__restrict__ bv, T *__restrict__ x,
+template <typename T, bool FIRST, bool NT>
+__device__ __forceinline__ void iter_a_body(BLK_ARGS, Sell S, const T *__restrict__ bv, T *__restrict__ x,
                                                 T *__restrict__ r, const T *__restrict__ s,
                                                 const T *__restrict__ As, const T *__restrict__ p_old,
                                                 const T *__restrict__ Ap_old, T *__restrict__ p_new,
@@ -802,7 +821,7 @@ __global__ __launch_bounds__(TPB) void k_iter_a(BLK_ARGS, Sell S, const T *__res
             const double q0 = (double)q[3 * n], q1 = (double)q[3 * n + 1], q2 = (double)q[3 * n + 2];
             if (FIRST) {
                 for (int k = 0; k < 3; k++) p_new[3 * n + k] = r[3 * n + k];
-                block_row_cs<T>(n, c0, c1, S, bv, ColPlain<T>{r}, y0, y1, y2);
+                block_row_cs<T, ColPlain<T>, NT>(n, c0, c1, S, bv, ColPlain<T>{r}, y0, y1, y2);
             } else {
                 ColP<T> col{s, As, p_old, Ap_old, omega, beta};
                 for (int k = 0; k < 3; k++) {
@@ -814,7 +833,7 @@ __global__ __launch_bounds__(TPB) void k_iter_a(BLK_ARGS, Sell S, const T *__res
                     p_new[i] = pp;
                     vmax[0] = fmax(vmax[0], fabs((double)rr));
                 }
-                block_row_cs<T>(n, c0, c1, S, bv, col, y0, y1, y2);
+                block_row_cs<T, ColP<T>, NT>(n, c0, c1, S, bv, col, y0, y1, y2);
             }
             T a0 = (T)y0, a1 = (T)y1, a2 = (T)y2; /* bv holds M^-1 A */
             Ap_new[3 * n] = a0;
@@ -826,6 +845,30 @@ __global__ __launch_bounds__(TPB) void k_iter_a(BLK_ARGS, Sell S, const T *__res
         block_partial<1, false>(vsum, partials + PR_B * nblk, nblk, j);
     }
 }
+template <typename T, bool FIRST>
+__global__ __launch_bounds__(TPB) void k_iter_a(BLK_ARGS, Sell S, const T *__restrict__ bv, T *__restrict__ x,
+                                                T *__restrict__ r, const T *__restrict__ s,
+                                                const T *__restrict__ As, const T *__restrict__ p_old,
+                                                const T *__restrict__ Ap_old, T *__restrict__ p_new,
+                                                T *__restrict__ Ap_new, const T *__restrict__ q, double *partials,
+                                                const adhk_scalars *__restrict__ sc)
+{
+    iter_a_body<T, FIRST, false>(blk_node, blk_inst, nblk, n_single, xcd_map, S, bv, x, r, s, As, p_old, Ap_old,
+                                 p_new, Ap_new, q, partials, sc);
+}
+
+template <typename T, bool FIRST>
+__global__ __launch_bounds__(TPB) void k_iter_a_nt(BLK_ARGS, Sell S, const T *__restrict__ bv, T *__restrict__ x,
+                                                T *__restrict__ r, const T *__restrict__ s,
+                                                const T *__restrict__ As, const T *__restrict__ p_old,
+                                                const T *__restrict__ Ap_old, T *__restrict__ p_new,
+                                                T *__restrict__ Ap_new, const T *__restrict__ q, double *partials,
+                                                const adhk_scalars *__restrict__ sc)
+{
+    iter_a_body<T, FIRST, true>(blk_node, blk_inst, nblk, n_single, xcd_map, S, bv, x, r, s, As, p_old, Ap_old,
+                                p_new, Ap_new, q, partials, sc);
+}
+
 
 /* Split schedule of the A step: k_upd finishes iteration k and stores the new direction, then k_spmv_p
  * multiplies it, gathering one vector per column instead of the four that k_iter_a recomputes p from.
@@ -860,8 +903,8 @@ __global__ __launch_bounds__(TPB) void k_upd(BLK_ARGS, T *__restrict__ x, T *__r
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(TPB) void k_spmv_p(BLK_ARGS, Sell S, const T *__restrict__ bv, const T *__restrict__ p,
+template <typename T, bool NT>
+__device__ __forceinline__ void spmv_p_body(BLK_ARGS, Sell S, const T *__restrict__ bv, const T *__restrict__ p,
                                                 T *__restrict__ Ap, const T *__restrict__ q, double *partials,
                                                 const adhk_scalars *__restrict__ sc)
 {
@@ -882,7 +925,7 @@ __global__ __launch_bounds__(TPB) void k_spmv_p(BLK_ARGS, Sell S, const T *__res
                 slice_bounds<false>(n, S, c0, c1);
             const double q0 = (double)q[3 * n], q1 = (double)q[3 * n + 1], q2 = (double)q[3 * n + 2];
             double y0, y1, y2;
-            block_row_cs<T>(n, c0, c1, S, bv, ColPlain<T>{p}, y0, y1, y2); /* bv holds M^-1 A */
+            block_row_cs<T, ColPlain<T>, NT>(n, c0, c1, S, bv, ColPlain<T>{p}, y0, y1, y2); /* bv holds M^-1 A */
             T a0 = (T)y0, a1 = (T)y1, a2 = (T)y2;
             Ap[3 * n] = a0;
             Ap[3 * n + 1] = a1;
@@ -892,6 +935,22 @@ __global__ __launch_bounds__(TPB) void k_spmv_p(BLK_ARGS, Sell S, const T *__res
         block_partial<1, false>(vsum, partials + PR_B * nblk, nblk, j);
     }
 }
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_spmv_p(BLK_ARGS, Sell S, const T *__restrict__ bv, const T *__restrict__ p,
+                                                T *__restrict__ Ap, const T *__restrict__ q, double *partials,
+                                                const adhk_scalars *__restrict__ sc)
+{
+    spmv_p_body<T, false>(blk_node, blk_inst, nblk, n_single, xcd_map, S, bv, p, Ap, q, partials, sc);
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_spmv_p_nt(BLK_ARGS, Sell S, const T *__restrict__ bv, const T *__restrict__ p,
+                                                T *__restrict__ Ap, const T *__restrict__ q, double *partials,
+                                                const adhk_scalars *__restrict__ sc)
+{
+    spmv_p_body<T, true>(blk_node, blk_inst, nblk, n_single, xcd_map, S, bv, p, Ap, q, partials, sc);
+}
+
 
 /* finalize A, one workgroup per instance: [not FIRST] ||r||_inf of the completed iteration and the
  * stop test (:210); gamma, alpha */
@@ -918,8 +977,8 @@ __global__ __launch_bounds__(TPB) void k_fin_a(int nblk, const int *__restrict__
 
 /* S: s = r - alpha Ap, As = M^-1 A s, omega = (As.s)/(As.As), rho' = -omega (As.q),
  * beta' = (rho' alpha)/(rho omega), iteration count + 1   (:247-291) */
-template <typename T>
-__global__ __launch_bounds__(TPB) void k_iter_s(BLK_ARGS, Sell S, const T *__restrict__ bv,
+template <typename T, bool NT>
+__device__ __forceinline__ void iter_s_body(BLK_ARGS, Sell S, const T *__restrict__ bv,
                                                 const T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ s,
                                                 T *__restrict__ As, const T *__restrict__ q, double *partials,
                                                 const adhk_scalars *__restrict__ sc)
@@ -943,9 +1002,9 @@ __global__ __launch_bounds__(TPB) void k_iter_s(BLK_ARGS, Sell S, const T *__res
             }
             double y0, y1, y2;
             if (n_single)
-                block_row<T, ColS<T>, true>(n, S, bv, col, y0, y1, y2); /* bv holds M^-1 A */
+                block_row<T, ColS<T>, true, NT>(n, S, bv, col, y0, y1, y2); /* bv holds M^-1 A */
             else
-                block_row<T, ColS<T>, false>(n, S, bv, col, y0, y1, y2);
+                block_row<T, ColS<T>, false, NT>(n, S, bv, col, y0, y1, y2);
             double z[3] = {y0, y1, y2};
             for (int k = 0; k < 3; k++) {
                 int i = 3 * n + k;
@@ -960,6 +1019,24 @@ __global__ __launch_bounds__(TPB) void k_iter_s(BLK_ARGS, Sell S, const T *__res
         block_partial<3, false>(v, partials + PR_A * nblk, nblk, j);
     }
 }
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_iter_s(BLK_ARGS, Sell S, const T *__restrict__ bv,
+                                                const T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ s,
+                                                T *__restrict__ As, const T *__restrict__ q, double *partials,
+                                                const adhk_scalars *__restrict__ sc)
+{
+    iter_s_body<T, false>(blk_node, blk_inst, nblk, n_single, xcd_map, S, bv, r, Ap, s, As, q, partials, sc);
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_iter_s_nt(BLK_ARGS, Sell S, const T *__restrict__ bv,
+                                                const T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ s,
+                                                T *__restrict__ As, const T *__restrict__ q, double *partials,
+                                                const adhk_scalars *__restrict__ sc)
+{
+    iter_s_body<T, true>(blk_node, blk_inst, nblk, n_single, xcd_map, S, bv, r, Ap, s, As, q, partials, sc);
+}
+
 
 /* finalize S, one workgroup per instance: omega = (As.s)/(As.As), rho' = -omega (As.q),
  * beta' = (rho' alpha)/(rho omega), it + 1 */
@@ -1031,6 +1108,14 @@ static KV<float> kv32(adhk_ctx *c)
     return k;
 }
 
+/* non-temporal matrix loads: tune.stream_matrix 0/1, or (-1) when the matrix in the value type of
+ * size vb exceeds the 256 MB MALL that otherwise keeps it between products */
+#define ADHK_MALL_BYTES (256L << 20)
+static int stream_matrix(const adhk_ctx *c, size_t vb)
+{
+    if (c->tune.stream_matrix >= 0) return c->tune.stream_matrix;
+    return (long)c->nslot * 64 * (9 * (long)vb + 4) > ADHK_MALL_BYTES;
+}
 static Sell sell_of(adhk_ctx *c)
 {
     Sell S;
@@ -1059,6 +1144,7 @@ template <typename T, bool FIRST> static int launch_a(adhk_ctx *c, KV<T> k, int 
 {
     /* reads direction buffer cur, writes 1 - cur (FIRST writes buffer 1) */
     int nw = FIRST ? 1 : 1 - cur;
+    const bool nt = stream_matrix(c, sizeof(T));
     if (!FIRST && c->tune.split) {
         adhk_kt_begin(c, ADHK_KT_UPDATE);
         hipLaunchKernelGGL(k_upd<T>, dim3(grid_blk(c->nblk)), dim3(TPB), 0, c->stream, BLK_PASS(c), k.x, k.r, k.s, k.As,
@@ -1066,14 +1152,16 @@ template <typename T, bool FIRST> static int launch_a(adhk_ctx *c, KV<T> k, int 
         adhk_kt_end(c);
         CHECK_LAUNCH();
         adhk_kt_begin(c, ADHK_KT_SPMV_P);
-        hipLaunchKernelGGL(k_spmv_p<T>, dim3(grid_blk(c->nblk)), dim3(TPB), 0, c->stream, BLK_PASS(c), sell_of(c),
+        hipLaunchKernelGGL(nt ? k_spmv_p_nt<T> : k_spmv_p<T>, dim3(grid_blk(c->nblk)), dim3(TPB), 0, c->stream,
+                           BLK_PASS(c), sell_of(c),
                            k.bv, k.p[nw], k.Ap[nw], k.q, c->partials, c->sc);
         adhk_kt_end(c);
         CHECK_LAUNCH();
     } else {
         /* under the split schedule the class times k_spmv_p alone: its first iteration's k_iter_a is not timed */
         if (!c->tune.split) adhk_kt_begin(c, ADHK_KT_SPMV_P);
-        hipLaunchKernelGGL((k_iter_a<T, FIRST>), dim3(grid_blk(c->nblk)), dim3(TPB), 0, c->stream, BLK_PASS(c),
+        hipLaunchKernelGGL((nt ? k_iter_a_nt<T, FIRST> : k_iter_a<T, FIRST>), dim3(grid_blk(c->nblk)), dim3(TPB), 0,
+                           c->stream, BLK_PASS(c),
                            sell_of(c), k.bv, k.x, k.r, k.s, k.As, k.p[cur], k.Ap[cur], k.p[nw], k.Ap[nw], k.q,
                            c->partials, c->sc);
         adhk_kt_end(c);
@@ -1088,7 +1176,8 @@ template <typename T, bool FIRST> static int launch_a(adhk_ctx *c, KV<T> k, int 
 template <typename T> static int launch_s(adhk_ctx *c, KV<T> k, int cur)
 {
     adhk_kt_begin(c, ADHK_KT_SPMV_S);
-    hipLaunchKernelGGL(k_iter_s<T>, dim3(grid_blk(c->nblk)), dim3(TPB), 0, c->stream, BLK_PASS(c), sell_of(c), k.bv,
+    hipLaunchKernelGGL(stream_matrix(c, sizeof(T)) ? k_iter_s_nt<T> : k_iter_s<T>, dim3(grid_blk(c->nblk)),
+                       dim3(TPB), 0, c->stream, BLK_PASS(c), sell_of(c), k.bv,
                        k.r, k.Ap[cur], k.s, k.As, k.q, c->partials, c->sc);
     adhk_kt_end(c);
     CHECK_LAUNCH();

@@ -189,6 +189,7 @@ static adhk_sw2 *create_impl(int ninst, const int *node_off, const int *elem_off
     c->tune.xcd_map = 1;
     c->tune.elem_tpb = 256;
     c->tune.split = 0;
+    c->tune.stream_matrix = -1;
     HC(hipStreamCreate(&c->stream));
     for (int i = 0; i < 8; i++) HC(hipEventCreate(&c->ev[i]));
 
@@ -918,7 +919,8 @@ static void drop_graphs(adhk_ctx *c)
 /* the schedule fields a captured launch sequence depends on */
 static int same_schedule(const adhk_tuning *a, const adhk_tuning *b)
 {
-    return a->chunk == b->chunk && a->split == b->split && a->xcd_map == b->xcd_map && a->elem_tpb == b->elem_tpb;
+    return a->chunk == b->chunk && a->split == b->split && a->xcd_map == b->xcd_map && a->elem_tpb == b->elem_tpb &&
+           a->stream_matrix == b->stream_matrix;
 }
 static int same_params(const adhk_params *a, const adhk_params *b)
 {
@@ -1098,9 +1100,9 @@ int adhk_sw2_set_tuning(adhk_sw2 *c, const adhk_tuning *t)
     if (!c || !t) return -1;
     if (t->chunk < 0 || t->chunk % 2 || t->chunk > 100 || (t->graphs != 0 && t->graphs != 1) ||
         (t->xcd_map != 0 && t->xcd_map != 1) || (t->elem_tpb != 64 && t->elem_tpb != 128 && t->elem_tpb != 256) ||
-        (t->split != 0 && t->split != 1)) {
+        (t->split != 0 && t->split != 1) || t->stream_matrix < -1 || t->stream_matrix > 1) {
         adhk_set_error("adhk_sw2_set_tuning: chunk even in [0, 100], graphs, xcd_map and split 0/1, "
-                       "elem_tpb 64/128/256");
+                       "elem_tpb 64/128/256, stream_matrix -1/0/1");
         return -1;
     }
     drop_graphs(c);

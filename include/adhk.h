@@ -92,7 +92,8 @@ typedef struct {
 
 /* Schedule choices of a context.  None changes a computed value: the iterations, the reduction order
  * and every operand are the same under all of them, only their launch and read-back pattern differs.
- * Defaults: chunk 0 (use adhk_solver.lin_check_every), graphs 0, xcd_map 1, elem_tpb 256, split 0. */
+ * Defaults: chunk 0 (use adhk_solver.lin_check_every), graphs 0, xcd_map 1, elem_tpb 256, split 0,
+ * stream_matrix -1. */
 typedef struct {
     int chunk;    /* BiCGStab iterations launched between "done" read-backs (even; 0 = the solver's) */
     int graphs;   /* 1 = replay each chunk as a captured hipGraph (when per-kernel timers are off) */
@@ -100,6 +101,8 @@ typedef struct {
     int elem_tpb; /* workgroup size of the elemental residual / Jacobian kernels (64, 128 or 256) */
     int split;    /* 1 = BiCGStab's A step as two kernels: store the new direction, then multiply it
                      (one gathered vector per column instead of four) */
+    int stream_matrix; /* matrix loads of the BiCGStab products: 1 non-temporal, 0 cached, -1 (default)
+                          non-temporal when the matrix exceeds the 256 MB MALL */
 } adhk_tuning;
 
 /* kernel classes of the per-kernel device timers */
