@@ -196,6 +196,15 @@ __device__ __forceinline__ T upd_p(double p, double Ap, double r_new, double ome
     return (T)__fma_rn(__fma_rn(-omega, Ap, p), beta, r_new); /* p = (p - omega Ap) beta + r (:219-221) */
 }
 
+/* x += alpha p + omega s (:294-297).  x is touched once per iteration and next read an iteration
+ * later, after both products have streamed the matrix: non-temporal access keeps it from displacing
+ * matrix lines in the MALL. */
+template <typename T> __device__ __forceinline__ void upd_x(T *x, double s, double p, double alpha, double omega)
+{
+    const double xi = (double)__builtin_nontemporal_load(x);
+    __builtin_nontemporal_store((T)__fma_rn(omega, s, __fma_rn(alpha, p, xi)), x);
+}
+
 /* s = r - alpha Ap evaluated on the fly for the columns */
 template <typename T> struct ColS {
     const T *r, *Ap;
@@ -828,7 +837,7 @@ __device__ __forceinline__ void iter_a_body(BLK_ARGS, Sell S, const T *__restric
                     const int i = 3 * n + k;
                     T rr, pp;
                     col.one(i, rr, pp);
-                    x[i] = (T)__fma_rn(omega, (double)s[i], __fma_rn(alpha, (double)p_old[i], (double)x[i]));
+                    upd_x(x + i, (double)s[i], (double)p_old[i], alpha, omega);
                     r[i] = rr;
                     p_new[i] = pp;
                     vmax[0] = fmax(vmax[0], fabs((double)rr));
@@ -894,7 +903,7 @@ __global__ __launch_bounds__(TPB) void k_upd(BLK_ARGS, T *__restrict__ x, T *__r
         for (int i = 3 * nb0 + threadIdx.x; i < 3 * nb1; i += TPB) {
             T rr, pp;
             col.one(i, rr, pp);
-            x[i] = (T)__fma_rn(omega, (double)s[i], __fma_rn(alpha, (double)p_old[i], (double)x[i]));
+            upd_x(x + i, (double)s[i], (double)p_old[i], alpha, omega);
             r[i] = rr;
             p_new[i] = pp;
             vmax[0] = fmax(vmax[0], fabs((double)rr));
