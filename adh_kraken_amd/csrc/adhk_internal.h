@@ -167,6 +167,7 @@ int adhk_launch_check_diag(adhk_ctx *c);
 int adhk_launch_dirichlet(adhk_ctx *c, int do_scale);
 int adhk_launch_scale_prec(adhk_ctx *c, int mixed);
 int adhk_launch_prep(adhk_ctx *c, int mixed); /* Dirichlet + scaling + fold in two matrix passes */
+int adhk_stream_matrix_on(adhk_ctx *c, int mixed); /* the products' matrix loads are non-temporal */
 int adhk_launch_krylov_init(adhk_ctx *c, int mixed, double tol, int max_it);
 int adhk_launch_krylov_first(adhk_ctx *c, int mixed);
 int adhk_launch_krylov_iter(adhk_ctx *c, int mixed, int cur);

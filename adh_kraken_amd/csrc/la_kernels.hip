@@ -1125,6 +1125,11 @@ static int stream_matrix(const adhk_ctx *c, size_t vb)
     if (c->tune.stream_matrix >= 0) return c->tune.stream_matrix;
     return (long)c->nslot * 64 * (9 * (long)vb + 4) > ADHK_MALL_BYTES;
 }
+extern "C" int adhk_stream_matrix_on(adhk_ctx *c, int mixed)
+{
+    return stream_matrix(c, mixed ? sizeof(float) : sizeof(double));
+}
+
 static Sell sell_of(adhk_ctx *c)
 {
     Sell S;

@@ -1203,6 +1203,31 @@ int adhk_sw2_autotune(adhk_sw2 *c, const adhk_solver *s, int ninner, adhk_tuning
                     best.graphs = g;
                 }
             }
+    /* 4. the chosen schedule's matrix loads: the other policy than the size rule gives, kept only when
+     * it wins by 2% (the MALL may still hold part of a matrix larger than itself) */
+    {
+        double tbest = 1e300;
+        c->tune = best;
+        const int eff = adhk_stream_matrix_on(c, mixed);
+        c->tune.stream_matrix = !eff;
+        for (int r = 0; r <= ninner; r++) {
+            double t0 = now_ms();
+            LC(krylov(c, s, mixed, hs, its));
+            double t1 = now_ms();
+            if (r > 0 && t1 - t0 < tbest) tbest = t1 - t0;
+        }
+        for (int i = 0; i < K; i++)
+            if (its[i] != its_ref[i]) {
+                adhk_set_error("adhk_sw2_autotune: schedule changed an iteration count (internal error)");
+                goto fail;
+            }
+        if (tbest < best_k * 0.98) {
+            best_k = tbest;
+            best.stream_matrix = !eff;
+        } else {
+            best.stream_matrix = eff;
+        }
+    }
     drop_graphs(c);
     c->tune = best;
     c->kt.on = saved_kt;
