@@ -245,7 +245,7 @@ def main():
     ap.add_argument("--train-steps", type=int, default=None,
                     help="time steps per training trajectory (default 100; c5 200)")
     ap.add_argument("--adam-steps", type=int, default=6000, help="optimizer steps of the warm-start training")
-    ap.add_argument("--checkpoints", type=int, default=6,
+    ap.add_argument("--checkpoints", type=int, default=12,
                     help="training chunks whose ends are validated on a held-out case; the best is kept")
     ap.add_argument("--val-steps", type=int, default=30, help="time steps of the held-out validation case")
     ap.add_argument("--train-cases", type=int, default=4,
