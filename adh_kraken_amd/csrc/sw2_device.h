@@ -83,7 +83,13 @@ __device__ __forceinline__ void sw2_temporal(const TriIn &in, const double *h, c
     }
 }
 
-/* Returns 1 when a partially dry field is met (r is then not the reference value). */
+/* Returns 1 when a partially dry field is met (r is then not the reference value).
+ * FD = true is the form the FD Jacobian evaluates 18 times per element: divisions by 3 become
+ * products with 1/3, the wet/dry averages skip the djac / djac that cancels, and 1 / (2 g) and
+ * 1 / hden are formed once.  Each of these moves a term by at most a few ulps, about 1e-12 of a
+ * Jacobian entry after the central difference; the residual itself (FD = false) keeps the
+ * reference's operation order. */
+template <bool FD = false>
 __device__ __forceinline__ int sw2_tri_resid(const TriIn &in, const Sw2Par &P, int pnode, int pvar, double pe,
                                              double r[9])
 {
@@ -119,7 +125,12 @@ __device__ __forceinline__ int sw2_tri_resid(const TriIn &in, const Sw2Par &P, i
     }
     /* wet/dry averages, fe_sw2_wd_average fe_wet_dry_integrations.c:40 */
     double hwd = 0., uwd = 0., vwd = 0., factor = 0.;
-    if (dh == 0) {
+    if (dh == 0 && FD) {
+        hwd = (h[0] + h[1] + h[2]) * D_ONE_3;
+        uwd = (u[0] + u[1] + u[2]) * D_ONE_3;
+        vwd = (v[0] + v[1] + v[2]) * D_ONE_3;
+        factor = 1.0;
+    } else if (dh == 0) {
         double a0 = djac * ((h[0] + h[1] + h[2]) / 3.0);
         double a1 = djac * ((u[0] + u[1] + u[2]) / 3.0);
         double a2 = djac * ((v[0] + v[1] + v[2]) / 3.0);
@@ -185,7 +196,7 @@ __device__ __forceinline__ int sw2_tri_resid(const TriIn &in, const Sw2Par &P, i
     if (P.alpha != 0.0) sw2_temporal(in, in.hoo, in.uoo, in.voo, dhoo, factor_older, (P.alpha / 2.), in.wd_flag, r);
 
     /* friction, :520-539 */
-    const double resistance = rough / (2.0 * g);
+    const double resistance = FD ? rough * (0.5 / g) : rough / (2.0 * g);
     {
         double fx[3], fy[3];
 #pragma unroll
@@ -265,13 +276,13 @@ __device__ __forceinline__ int sw2_tri_resid(const TriIn &in, const Sw2Par &P, i
         const double tau = 0.5 * le / sr;
         double dhdt = 0., dudt = 0., dvdt = 0.;
         if (dho == 0) {
-            dhdt = -(djac * ((in.ho[0] + in.ho[1] + in.ho[2]) / 3.0));
-            dudt = -(djac * ((in.uo[0] + in.uo[1] + in.uo[2]) / 3.0));
-            dvdt = -(djac * ((in.vo[0] + in.vo[1] + in.vo[2]) / 3.0));
+            dhdt = -(djac * (FD ? (in.ho[0] + in.ho[1] + in.ho[2]) * D_ONE_3 : (in.ho[0] + in.ho[1] + in.ho[2]) / 3.0));
+            dudt = -(djac * (FD ? (in.uo[0] + in.uo[1] + in.uo[2]) * D_ONE_3 : (in.uo[0] + in.uo[1] + in.uo[2]) / 3.0));
+            dvdt = -(djac * (FD ? (in.vo[0] + in.vo[1] + in.vo[2]) * D_ONE_3 : (in.vo[0] + in.vo[1] + in.vo[2]) / 3.0));
         }
-        dudt += djac * ((u[0] + u[1] + u[2]) / 3.0);
-        dvdt += djac * ((v[0] + v[1] + v[2]) / 3.0);
-        dhdt += djac * ((h[0] + h[1] + h[2]) / 3.0);
+        dudt += djac * (FD ? (u[0] + u[1] + u[2]) * D_ONE_3 : (u[0] + u[1] + u[2]) / 3.0);
+        dvdt += djac * (FD ? (v[0] + v[1] + v[2]) * D_ONE_3 : (v[0] + v[1] + v[2]) / 3.0);
+        dhdt += djac * (FD ? (h[0] + h[1] + h[2]) * D_ONE_3 : (h[0] + h[1] + h[2]) / 3.0);
         /* fe_sw2_wd_gls_convection_triangle, integrations.c:700-722 */
         double havg = D_ONE_3 * (h[0] + h[1] + h[2]);
         double uavg = D_ONE_3 * (u[0] + u[1] + u[2]);
@@ -292,8 +303,9 @@ __device__ __forceinline__ int sw2_tri_resid(const TriIn &in, const Sw2Par &P, i
         double xf = g * bsx * dt * fct;
         double yf = g * bsy * dt * fct;
         double c_eq = dhdt + a0;
-        double x_eq = dudt + a1 + (xp + xf / hden) * djac;
-        double y_eq = dvdt + a2 + (yp + yf / hden) * djac;
+        const double ihden = FD ? 1.0 / hden : 0.0;
+        double x_eq = dudt + a1 + (xp + (FD ? xf * ihden : xf / hden)) * djac;
+        double y_eq = dvdt + a2 + (yp + (FD ? yf * ihden : yf / hden)) * djac;
         const double gf = g;
 #pragma unroll
         for (int i = 0; i < 3; i++) {

@@ -319,8 +319,8 @@ __global__ __launch_bounds__(256) void k_elem_jac(MeshView m, Sw2Par P, TE *__re
     double eps = num_diff_eps(var, 1e-4);
     double eps2 = 2.0 * eps;
     double rp[9], rm[9];
-    int pa = sw2_tri_resid(in, P, i, k, (+1) * eps, rp);
-    int pb = sw2_tri_resid(in, P, i, k, (-1) * eps, rm);
+    int pa = sw2_tri_resid<true>(in, P, i, k, (+1) * eps, rp);
+    int pb = sw2_tri_resid<true>(in, P, i, k, (-1) * eps, rm);
     if (pa | pb) flags[0] = 1; /* cannot happen after tri_partial_fd; kept as a guard */
     /* column (i, k) lies in the blocks (a, i) of the three row nodes a */
 #pragma unroll
