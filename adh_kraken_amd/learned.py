@@ -102,26 +102,40 @@ def set_warm_start(sm: SModelSuper, mlp):
 
 def collect_trajectory(sm: SModelSuper, mlp: WarmStartMLP, nsteps, precision=L.PREC_FP64, strict=True):
     """Advance sm by up to nsteps plain time steps, adding each converged step's (features, change) of
-    every node.  A step that fails raises (strict) or ends the trajectory there, keeping the steps
-    already collected.  Returns (Newton iterations, steps collected)."""
-    its = 0
-    for k in range(nsteps):
+    every node.  A step that fails raises (strict).  Otherwise the failed step, which the time loop
+    leaves at its old level with dt cut to a quarter, is taken as four quarter steps that are not
+    collected, and the trajectory goes on at its own dt; a quarter step that fails ends the
+    trajectory, keeping the steps already collected.  The step after the quarter steps is not
+    collected either: its features would hold a quarter step's change as the previous change.
+    Returns (Newton iterations, steps collected)."""
+    its, k, dt0, resumed = 0, 0, sm.get_dt(), False
+    for _ in range(nsteps):
         n, st = time_loop(sm, 1, precision=precision)
-        if n != 1:
-            if strict:
-                raise L.AdhkError(f"time step failed while collecting: {st}")
-            return its, k
         its += st["newton_its"]
-        mlp.collect(sm)
-    return its, nsteps
+        if n == 1:
+            if not resumed:
+                mlp.collect(sm)
+                k += 1
+            resumed = False
+            continue
+        if strict:
+            raise L.AdhkError(f"time step failed while collecting: {st}")
+        sm.set_dt(dt0 / 4)
+        n, st = time_loop(sm, 4, precision=precision)
+        its += st["newton_its"]
+        if n != 4:
+            break
+        sm.set_dt(dt0)
+        resumed = True
+    return its, k
 
 
 def train_warm_start(cases, nsteps, steps=2000, batch=8192, lr=2e-3, seed=1, precision=L.PREC_FP64, dt=1.0,
                      prepare=None, strict=True, validate=None, checkpoints=1):
     """Train a model on solver trajectories: cases = [(mesh, sol0), ...], nsteps steps each;
     prepare(sm, i), when given, sets up case i's model (e.g. its boundary time series) before it runs.
-    strict=False ends a trajectory at its first failed step instead of raising (the steps of each
-    trajectory are then in model.collected_steps).  With validate (model -> {"cost": ...}, e.g.
+    strict=False steps over a failed step instead of raising, as collect_trajectory describes (the
+    steps collected from each trajectory are in model.collected_steps).  With validate (model -> {"cost": ...}, e.g.
     validation_cost on a case held out of the training set), the steps run in `checkpoints` chunks
     and the chunk end with the lowest cost is kept (model.validation records the choice).
 
