@@ -128,6 +128,8 @@ SIGNATURES = [
      [dp, ip, ip, dp, ip, ip, dp, dp, dp, C.c_int, C.c_int, C.c_int, ip, C.c_int]),
     ("Screen_print_CSR", None, [ip, ip, dp, C.c_int]),
     ("adhk_la_set_block_size", None, [C.c_int]),
+    ("adhk_la_last_status", C.c_int, []),
+    ("adhk_la_last_iterations", C.c_int, []),
     ("adhk_flume_counts", C.c_int, [C.POINTER(FlumeSpec), ip, ip, ip]),
     ("adhk_flume_build", C.c_int, [C.POINTER(FlumeSpec), dp, dp, dp, ip, dp, ip, ip, dp]),
     ("adhk_flume_initial_state", C.c_int, [C.POINTER(FlumeSpec), C.c_int, dp, dp]),

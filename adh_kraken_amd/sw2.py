@@ -5,7 +5,7 @@ Names follow the reference: an ``SModelSuper`` carries ``sol``, ``sol_old``,
 SMODEL_SUPER (src/structs/smodel_super/smodel_super.h:7-113, defaults from
 smodel_super_alloc_init.c:55-100); ``assemble_residual``, ``assemble_jacobian``,
 ``fe_newton`` and ``time_loop`` take it like the reference functions
-(residual.h:5, jacobian.h:5, newton.h:15, time_loop.c:14).  Arrays live on the
+(residual.h:5, jacobian.h:5, newton.h:19, time_loop.c:14).  Arrays live on the
 GPU inside the context; the properties copy to and from host numpy arrays.
 """
 import ctypes as C

@@ -208,7 +208,7 @@ int adhk_check_diag(adhk_sw2 *c);
 int adhk_apply_dirichlet_bc(adhk_sw2 *c);
 int adhk_get_system(adhk_sw2 *c, double *vals /* nnz or NULL */, double *residual /* ndofs or NULL */);
 
-/* replaces fe_newton(SMODEL_SUPER*) (src/newton/newton.h:15, fe_newton.c:44).  Returns TRUE(1)/FALSE(0), -1 on error.
+/* replaces fe_newton(SMODEL_SUPER*) (src/newton/newton.h:19, fe_newton.c:44).  Returns TRUE(1)/FALSE(0), -1 on error.
  * For an ensemble: TRUE when every instance converged; st holds sums over the instances. */
 int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st);
 
@@ -239,6 +239,12 @@ int solve_linear_sys_bcgstab(double *x, int *indptr_diag, int *cols_diag, double
 void Screen_print_CSR(int *indptr, int *indices, double *data, int n); /* la.h:11 (declared, never defined there) */
 /* block size used by prep_umfpack/solve_umfpack (3 = nodal SW2 blocks; 1 = point Jacobi) */
 void adhk_la_set_block_size(int bs);
+/* No counterpart in the reference (whose la.h calls end in tl_error -> exit on failure): the status of this
+ * thread's last la.h call, 0 or -1 (adhk_last_error() then holds the message; the outputs are untouched),
+ * and the BiCGStab iterations of its last solve_linear_sys_bcgstab.  Each host thread owns its own la.h
+ * workspace (device buffers, stream, preconditioner); free_bcgstab() releases the calling thread's. */
+int adhk_la_last_status(void);
+int adhk_la_last_iterations(void);
 
 /* ---- synthetic flume meshes (restates create_rectangular_grid.c:4-445 and adds boundary
  *      segments, seeded Gaussian bed bumps and per-element Manning n) ---- */

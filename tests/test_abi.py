@@ -3,6 +3,8 @@ import ctypes
 import os
 import re
 
+import numpy as np
+
 import adh_kraken_amd._lib as L
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -38,3 +40,25 @@ def test_binding_covers_header():
 def test_load_sets_signatures():
     lib = L.load()
     assert lib.adhk_sw2_create.restype == ctypes.c_void_p
+
+
+def test_la_errors_return_instead_of_aborting():
+    """argument errors of the la.h surface set a status and a message, make no device call and leave the
+    outputs untouched (the reference exits through tl_error)"""
+    lib = L.load()
+    ip_ = np.array([0, 1], np.int32)
+    cols = np.array([0], np.int32)
+    vals = np.array([2.0])
+    x = np.array([1.0])
+    y = np.array([7.0])
+    off = np.array([0, 0], np.int32)
+    lib.split_CSR_mat_vec_mult(L.dptr(y), L.iptr(ip_), L.iptr(cols), L.dptr(vals), L.iptr(off), None, None,
+                               L.dptr(x), 1, None, 0)
+    assert lib.adhk_la_last_status() == -1 and y[0] == 7.0
+    assert b"multi-rank" in lib.adhk_last_error()
+    sc = np.ones(1)
+    lib.free_bcgstab()
+    rc = lib.solve_linear_sys_bcgstab(L.dptr(x), L.iptr(ip_), L.iptr(cols), L.dptr(vals), None, None, None,
+                                      L.dptr(y), L.dptr(sc), 1, 1, 0, None, 0)
+    assert rc == -1 and lib.adhk_la_last_status() == -1 and x[0] == 1.0
+    assert b"prep_umfpack" in lib.adhk_last_error()
