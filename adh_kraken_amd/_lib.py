@@ -130,6 +130,8 @@ SIGNATURES = [
     ("adhk_la_set_block_size", None, [C.c_int]),
     ("adhk_la_last_status", C.c_int, []),
     ("adhk_la_last_iterations", C.c_int, []),
+    ("adhk_bcgstab_device_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int)]),
     ("adhk_flume_counts", C.c_int, [C.POINTER(FlumeSpec), ip, ip, ip]),
     ("adhk_flume_build", C.c_int, [C.POINTER(FlumeSpec), dp, dp, dp, ip, dp, ip, ip, dp]),
     ("adhk_flume_initial_state", C.c_int, [C.POINTER(FlumeSpec), C.c_int, dp, dp]),

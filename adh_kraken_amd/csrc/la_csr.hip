@@ -1,6 +1,7 @@
 /*
  * la_csr.hip -- the reference's plain-pointer linear-algebra surface
- * (src/la/la.h) with its exact names and signatures, over host CSR arrays.
+ * (src/la/la.h) with its exact names and signatures, over host CSR arrays,
+ * and a variant of its BiCGStab solve on device arrays that allocates nothing.
  *
  * Single rank: the split-CSR off-diagonal block must be absent
  * (indptr_off_diag == NULL), as it is when the reference runs on one process
@@ -28,6 +29,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -753,4 +755,85 @@ extern "C" int solve_linear_sys_bcgstab(double *x, int *indptr_diag, int *cols_d
     }
     done_status(rc);
     return rc;
+}
+
+/* ---------------------------------------------------- BiCGStab on device arrays without allocation */
+
+/* Scratch of adhk_bcgstab_device_noalloc, one copy per GPU (the array's host-side name is not a device
+ * address: host code uses the per-device pointer from hipGetSymbolAddress).  Layout for n rows: the
+ * vectors x0 r q p Ap s As (n each), the block inverses (3n), the partial sums (4 * NPART_MAX) and the
+ * scalars. */
+#define NOALLOC_DOUBLES ((size_t)10 * ADHK_NOALLOC_MAX_N + 4 * NPART_MAX + 16)
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+#define ADHK_MAX_DEVICES 64
+static double *g_scratch[ADHK_MAX_DEVICES];
+static pthread_mutex_t g_scratch_mu = PTHREAD_MUTEX_INITIALIZER;
+
+/* this device's copy of scratch, looked up once per device with that device current */
+static double *scratch_of_current_device(void)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= ADHK_MAX_DEVICES) {
+        adhk_set_error("adhk_bcgstab_device_noalloc: no current device");
+        return NULL;
+    }
+    pthread_mutex_lock(&g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *ptr = NULL;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            adhk_set_error("adhk_bcgstab_device_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+        else
+            g_scratch[dev] = (double *)ptr;
+    }
+    double *p = g_scratch[dev];
+    pthread_mutex_unlock(&g_scratch_mu);
+    return p;
+}
+
+extern "C" int adhk_bcgstab_device_noalloc(double *x, const int *indptr, const int *cols, const double *vals,
+                                           double *b, const double *scale_vect, int nrow, int bs, void *stream,
+                                           int *its)
+{
+    if (nrow <= 0 || nrow > ADHK_NOALLOC_MAX_N) {
+        adhk_set_error("adhk_bcgstab_device_noalloc: %d rows (1..%d supported: the scratch is a fixed %zu MB)",
+                       nrow, ADHK_NOALLOC_MAX_N, NOALLOC_DOUBLES * sizeof(double) >> 20);
+        return -1;
+    }
+    if (bs != 1 && bs != 3) bs = 3;
+    if (bs == 3 && nrow % 3 != 0) bs = 1;
+    if (!x || !indptr || !cols || !vals || !b || !scale_vect) {
+        adhk_set_error("adhk_bcgstab_device_noalloc: NULL argument");
+        return -1;
+    }
+    double *scr = scratch_of_current_device();
+    if (!scr) return -1;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n = (size_t)nrow;
+    la_view V;
+    memset(&V, 0, sizeof V);
+    V.n = nrow;
+    V.bs = bs;
+    V.nb = nrow / bs;
+    V.ip = indptr;
+    V.ci = cols;
+    V.v = vals;
+    V.scale = scale_vect;
+    V.b = b;
+    V.x = x;
+    V.x0 = scr;
+    V.r = scr + n;
+    V.q = scr + 2 * n;
+    V.p = scr + 3 * n;
+    V.Ap = scr + 4 * n;
+    V.s = scr + 5 * n;
+    V.As = scr + 6 * n;
+    double *minv = scr + 7 * n; /* 3n */
+    V.minv = minv;
+    V.part = scr + 10 * n;
+    V.sc = (la_sc *)(scr + 10 * n + 4 * NPART_MAX);
+    hipLaunchKernelGGL(kc_bj_prep, dim3(nblk(V.nb)), dim3(TPB), 0, st, V.nb, bs, indptr, cols, vals, minv);
+    LCK(hipGetLastError());
+    return bicg_core(V, st, LA_TOL, LA_MAX_IT, its);
 }

@@ -246,6 +246,20 @@ void adhk_la_set_block_size(int bs);
 int adhk_la_last_status(void);
 int adhk_la_last_iterations(void);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): prep_umfpack + solve_linear_sys_bcgstab
+ * in one call on DEVICE arrays (indptr, cols, vals, b, scale_vect, x of nrow rows), with the same kernels
+ * and results, and no hipMalloc/hipFree or allocating library call.  Its working vectors live in a fixed
+ * scratch array of the library, one per GPU (about 21 MB), which limits nrow to ADHK_NOALLOC_MAX_N; larger
+ * systems return -1.  bs: 3 (nodal 3x3 blocks, when nrow % 3 == 0) or 1.  stream: a hipStream_t, or NULL
+ * for the null stream; the call returns after the solve has finished on it.  x: in x0, out (x + x0) * scale;
+ * b: out M^-1 b, as solve_linear_sys_bcgstab leaves them; its (or NULL): BiCGStab iterations.  Every call
+ * on a GPU uses that GPU's one scratch array: calls that overlap on one GPU (from several threads or
+ * streams) race and must be serialised by the caller. */
+#define ADHK_NOALLOC_MAX_N 262144
+int adhk_bcgstab_device_noalloc(double *x, const int *indptr, const int *cols, const double *vals, double *b,
+                                const double *scale_vect, int nrow, int bs, void *stream, int *its);
+
 /* ---- synthetic flume meshes (restates create_rectangular_grid.c:4-445 and adds boundary
  *      segments, seeded Gaussian bed bumps and per-element Manning n) ---- */
 typedef struct {
