@@ -4,9 +4,12 @@ Pins the closed-form P1 integrals of the oracle (src/fe/fe_integrals_2d.c and
 fe_integrals_1d.c as used by sw2_body_resid.c) against the weak form they
 integrate, evaluated with a 7-point degree-5 triangle rule (exact for the
 cubic integrands h*u*u) and 3-point Gauss-Legendre on edges.  Only the
-all-wet branch is stated.  The SUPG combination (sw2_body_resid.c:921-1142)
-is the reference's algebra on element averages; its integrals (time
-differences and the GLS convection term) are taken by quadrature here too.
+all-wet branch is stated.  The SUPG term (sw2_body_resid.c:921-1142) is stated
+from the shallow-water system itself: its strong residuals (time differences,
+convection, pressure, friction) are integrated by quadrature, and they are
+weighted by the primitive-variable flux Jacobians A_x, A_y of the quasi-linear
+operator, obtained by central differences of that operator (flux_jacobians),
+not typed from the oracle.
 """
 import numpy as np
 
@@ -33,8 +36,48 @@ def quad(area, f_at_points):
     return area * np.dot(WTS, f_at_points)
 
 
-def body_residual(x, y, z, h, u, v, ho, uo, vo, dt, g, cf, dll=1e-6):
-    """9-vector [node*3 + (c, x, y)] of the SW2 body residual for an all-wet triangle."""
+def quasi_linear_operator(q, qx, qy, g):
+    """The shallow-water system in primitive variables q = (h, u, v) without sources and time derivative:
+    continuity h_t + (hu)_x + (hv)_y and momentum u_t + u u_x + v u_y + g h_x, v_t + u v_x + v v_y + g h_y."""
+    h, u, v = q
+    hx, ux, vx = qx
+    hy, uy, vy = qy
+    return np.array([u * hx + h * ux + v * hy + h * vy,
+                     u * ux + v * uy + g * hx,
+                     u * vx + v * vy + g * hy])
+
+
+def flux_jacobians(q, g, delta=1.0):
+    """A_x = dF/dq_x and A_y = dF/dq_y of the quasi-linear operator at q, by central differences
+    (the operator is linear in the gradients, so the differences are exact up to rounding)."""
+    q = np.asarray(q, float)
+    Ax, Ay = np.zeros((3, 3)), np.zeros((3, 3))
+    z = np.zeros(3)
+    for j in range(3):
+        e = np.zeros(3)
+        e[j] = delta
+        Ax[:, j] = (quasi_linear_operator(q, e, z, g) - quasi_linear_operator(q, -e, z, g)) / (2 * delta)
+        Ay[:, j] = (quasi_linear_operator(q, z, e, g) - quasi_linear_operator(q, z, -e, g)) / (2 * delta)
+    return Ax, Ay
+
+
+def supg_term(tau, gx, gy, hb, ub, vb, g, R):
+    """Streamline-upwind test-function term of node i: tau * sum_k dphi_i/dk * diag(1, h, h) A_k(h, u, v) R,
+    where R = (c, x, y) are the element-integrated strong residuals of continuity and of the two momentum
+    equations in primitive variables; the diag(1, h, h) weights the momentum rows by the depth, as the
+    conservative momentum unknowns h u, h v are tested."""
+    Ax, Ay = flux_jacobians((hb, ub, vb), g)
+    D = np.diag([1.0, hb, hb])
+    wx, wy = D @ Ax @ R, D @ Ay @ R
+    out = np.zeros(9)
+    for i in range(3):
+        out[3 * i:3 * i + 3] = tau * (gx[i] * wx + gy[i] * wy)
+    return out
+
+
+def body_residual(x, y, z, h, u, v, ho, uo, vo, dt, g, cf, dll=1e-6, supg=True):
+    """9-vector [node*3 + (c, x, y)] of the SW2 body residual for an all-wet triangle (supg=False leaves
+    out the streamline-upwind term)."""
     A, gx, gy = p1_grad(x, y)
     phi = BARY  # (7 points, 3 basis)
     hq, uq, vq = phi @ h, phi @ u, phi @ v
@@ -84,10 +127,8 @@ def body_residual(x, y, z, h, u, v, ho, uo, vo, dt, g, cf, dll=1e-6):
         c_eq = dhdt + a0
         x_eq = dudt + a1 + (xp + xf / hden) * A
         y_eq = dvdt + a2 + (yp + yf / hden) * A
-        for i in range(3):
-            r[3 * i] += tau * (gx[i] * (ub * c_eq + hb * x_eq) + gy[i] * (vb * c_eq + hb * y_eq))
-            r[3 * i + 1] += tau * (gx[i] * (g * hb * c_eq + ub * hb * x_eq) + gy[i] * (vb * hb * x_eq))
-            r[3 * i + 2] += tau * (gy[i] * (g * hb * c_eq + vb * hb * y_eq) + gx[i] * (ub * hb * y_eq))
+        if supg:
+            r += supg_term(tau, gx, gy, hb, ub, vb, g, np.array([c_eq, x_eq, y_eq]))
     return r
 
 
@@ -116,7 +157,7 @@ def wet_quad(A, pieces, fn):
     return tot
 
 
-def wet_body_residual(x, y, z, h, u, v, ho, uo, vo, dt, g, cf, dll=1e-6):
+def wet_body_residual(x, y, z, h, u, v, ho, uo, vo, dt, g, cf, dll=1e-6, supg=True):
     """SW2 body residual of a triangle that may be partly dry (h <= 0 at some nodes).
 
     What the wet/dry wrapper (fe_wet_dry_wrapper.c:104-304, with RE_DISTRIBUTE) integrates:
@@ -191,8 +232,6 @@ def wet_body_residual(x, y, z, h, u, v, ho, uo, vo, dt, g, cf, dll=1e-6):
         c_eq = dhdt + a0
         x_eq = dudt + a1 + (xp + xf / hden) * A
         y_eq = dvdt + a2 + (yp + yf / hden) * A
-        for i in range(3):
-            r[3 * i] += tau * (gx[i] * (ub * c_eq + hb * x_eq) + gy[i] * (vb * c_eq + hb * y_eq))
-            r[3 * i + 1] += tau * (gx[i] * (g * hb * c_eq + ub * hb * x_eq) + gy[i] * (vb * hb * x_eq))
-            r[3 * i + 2] += tau * (gy[i] * (g * hb * c_eq + vb * hb * y_eq) + gx[i] * (ub * hb * y_eq))
+        if supg:
+            r += supg_term(tau, gx, gy, hb, ub, vb, g, np.array([c_eq, x_eq, y_eq]))
     return r

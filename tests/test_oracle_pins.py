@@ -7,7 +7,10 @@ mathematics fixes, each chosen so that a plausible slip in the restatement
 (a dropped term, a wrong coefficient, sign or index, a transposed operand)
 fails:
 
-* per-element residual vs an independent quadrature statement of the weak form
+* per-element residual vs an independent quadrature statement of the weak form, whose SUPG
+  weights come from the flux Jacobians of the shallow-water operator
+* rotation and reflection equivariance of the element residual; the SUPG term vanishes on
+  uniform normal flow
 * partially dry elements vs the same weak form over the clipped wet region with
   the original test functions (what the wet/dry wrapper with RE_DISTRIBUTE
   integrates), for every count of dry nodes at the new and the old level
@@ -495,3 +498,114 @@ def test_grid_mass():
             expect += f * area[e] * he[he > 0].sum() / 3
     assert counts.min() > 0  # every case occurs
     assert abs(orc.grid_mass() - expect) <= 1e-12 * np.sum(area)
+
+
+# ---- the streamline-upwind (SUPG) term, sw2_body_resid.c:921-1079 ----------------------------------------
+
+def _rotated_case(mesh, s, so, theta, reflect):
+    """The mesh, state and old state rotated by theta about the origin (and mirrored y -> -y first when
+    reflect): coordinates and velocities turn, depth and bed are unchanged.  A mirror reverses the node
+    order of every triangle, so two vertices swap to keep them counter-clockwise."""
+    import dataclasses
+    x, y = mesh.x.copy(), mesh.y.copy()
+    sol, sol_o = s.copy(), so.copy()
+    tri = mesh.tri.copy()
+    if reflect:
+        y = -y
+        sol[2::3], sol_o[2::3] = -sol[2::3], -sol_o[2::3]
+        tri = tri[:, [0, 2, 1]]
+    c, sn = np.cos(theta), np.sin(theta)
+    xr, yr = c * x - sn * y, sn * x + c * y
+    for a in (sol, sol_o):
+        u, v = a[1::3].copy(), a[2::3].copy()
+        a[1::3], a[2::3] = c * u - sn * v, sn * u + c * v
+    return dataclasses.replace(mesh, x=xr, y=yr, tri=np.ascontiguousarray(tri)), sol, sol_o
+
+
+@pytest.mark.parametrize("theta_deg,reflect", [(30.0, False), (90.0, False), (0.0, True), (30.0, True)])
+@pytest.mark.parametrize("friction", ["manning", "const"])
+def test_body_residual_rotation_and_reflection_equivariance(theta_deg, reflect, friction):
+    """The SW2 equations have no preferred direction: turning (or mirroring) the element, its velocities and
+    its bed leaves the continuity rows unchanged and turns the momentum pairs the same way.  Any x/y slip in
+    a term that is not symmetric in x and y -- an exchanged dphi/dx, dphi/dy in one SUPG cross term, a
+    u where v belongs in the friction or convection -- breaks this."""
+    spec, (mesh, sol0) = small_flume(npx=9, npy=5, nbumps=4, seed=3)
+    s, so = perturbed_state(sol0, seed=11, amp_h=0.1, amp_v=0.6)
+    orc = O.Oracle(mesh, dt=1.3, friction=friction, cf_const=0.004)
+    orc.set_state(s, so, so)
+    ref = np.array([orc.body_resid(e, s) for e in range(mesh.nelems2d)])
+    th = np.deg2rad(theta_deg)
+    mr, sr, sor = _rotated_case(mesh, s, so, th, reflect)
+    orr = O.Oracle(mr, dt=1.3, friction=friction, cf_const=0.004)
+    orr.set_state(sr, sor, sor)
+    got = np.array([orr.body_resid(e, sr) for e in range(mr.nelems2d)])
+    if reflect:  # the swapped vertices swap their rows back
+        got = got.reshape(-1, 3, 3)[:, [0, 2, 1], :].reshape(-1, 9)
+    c, sn = np.cos(th), np.sin(th)
+    want = ref.copy()
+    rx, ry = ref[:, 1::3], ref[:, 2::3]
+    if reflect:
+        ry = -ry
+    want[:, 1::3], want[:, 2::3] = c * rx - sn * ry, sn * rx + c * ry
+    scale = np.abs(ref).max()
+    assert np.abs(got - want).max() <= 1e-12 * scale
+
+
+def _supg_part(orc, mesh, s, so, dt, cf):
+    """the oracle's body residual minus the quadrature statement without the SUPG term, per element"""
+    out = []
+    for e in range(mesh.nelems2d):
+        t = mesh.tri[e]
+        base = body_residual(mesh.x[t], mesh.y[t], mesh.z[t], s[3 * t], s[3 * t + 1], s[3 * t + 2], so[3 * t],
+                             so[3 * t + 1], so[3 * t + 2], dt, G, cf, supg=False)
+        out.append(orc.body_resid(e, s) - base)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("direction_deg", [0.0, 35.0])
+def test_supg_vanishes_on_uniform_normal_flow(direction_deg):
+    """Uniform flow of depth h0 and speed U down a plane bed whose slope balances the drag,
+    g S = (cf / 2) U^2 / h0, with h = h_old: every strong residual of the element is zero, so the SUPG term
+    must be zero while it is not zero on a perturbed state (the term is active)."""
+    import dataclasses
+    spec, (mesh, sol0) = small_flume(npx=9, npy=5, nbumps=0, seed=3)
+    h0, U, cf, dt = 1.3, 0.8, 0.004, 1.1
+    S = cf * U * U / (2 * G * h0)
+    a = np.deg2rad(direction_deg)
+    ex, ey = np.cos(a), np.sin(a)
+    z = -S * (ex * mesh.x + ey * mesh.y) + 0.25
+    m = dataclasses.replace(mesh, z=z)
+    s = np.zeros(3 * m.nnodes)
+    s[0::3], s[1::3], s[2::3] = h0, U * ex, U * ey
+    orc = O.Oracle(m, dt=dt, friction="const", cf_const=cf)
+    orc.set_state(s, s, s)
+    part = _supg_part(orc, m, s, s, dt, cf)
+    full = np.array([orc.body_resid(e, s) for e in range(m.nelems2d)])
+    assert np.abs(part).max() <= 1e-12 * np.abs(full).max()
+    sp_, so_ = perturbed_state(s, seed=4, amp_h=0.05, amp_v=0.1)
+    orc.set_state(sp_, so_, so_)
+    assert np.abs(_supg_part(orc, m, sp_, so_, dt, cf)).max() > 1e-6
+
+
+def test_supg_weights_from_flux_jacobians_catch_a_swapped_cross_term():
+    """The flux-Jacobian statement of the SUPG weights is what pins the oracle's combination; a plausible
+    slip (dphi/dx and dphi/dy exchanged in the y-momentum cross term, or the gravity dropped from a
+    momentum row) moves the term far above the 1e-12 gate."""
+    from sw2_quadrature import supg_term
+    rng = np.random.default_rng(2)
+    gx, gy = rng.normal(size=3), rng.normal(size=3)
+    hb, ub, vb, tau = 1.7, 0.4, -0.3, 0.2
+    R = rng.normal(size=3)
+    good = supg_term(tau, gx, gy, hb, ub, vb, G, R)
+    c_eq, x_eq, y_eq = R
+    slips = []
+    for variant in ("swap", "nog"):
+        r = np.zeros(9)
+        for i in range(3):
+            r[3 * i] = tau * (gx[i] * (ub * c_eq + hb * x_eq) + gy[i] * (vb * c_eq + hb * y_eq))
+            gfac = 0.0 if variant == "nog" else G
+            r[3 * i + 1] = tau * (gx[i] * (gfac * hb * c_eq + ub * hb * x_eq) + gy[i] * (vb * hb * x_eq))
+            cross = gy[i] if variant == "swap" else gx[i]
+            r[3 * i + 2] = tau * (gy[i] * (G * hb * c_eq + vb * hb * y_eq) + cross * (ub * hb * y_eq))
+        slips.append(np.abs(r - good).max() / np.abs(good).max())
+    assert min(slips) > 1e-3
