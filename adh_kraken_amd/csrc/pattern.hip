@@ -2,17 +2,22 @@
  * pattern.hip -- CSR sparsity pattern of the SW2 system built on the GPU
  * (replaces slin_sys_init_sparsity_mono, slin_sys_init_sparsity_mono.c:28-494).
  *
- * Every element emits its node pairs (row node a, col node b) in element order
- * (2-D elements, then 1-D).  A stable radix sort on key = a*N + b gives, in one
- * pass, (1) the sorted, duplicate-free node adjacency of each row -- the
- * reference's per-row sort + unique, at node-block granularity -- and (2) for
- * every 3x3 block the list of elemental contributions in element order, which
- * drives the atomic-free assembly.  The scalar CSR of the reference is the
- * node-block pattern expanded by dof(node, var) = 3*node + var; cols stay
- * ascending because each block row lists node columns in ascending order.
+ * Every element emits its node pairs (row node a, col node b): contribution t is pair
+ * (e*3 + a)*3 + b of triangle e, then 9E + (s*2 + a)*2 + b of segment s, so t runs in
+ * element order (2-D elements, then 1-D).  Each row node gathers the pairs it heads into its
+ * own run of a node-incidence list (counts, an exclusive scan, a fill), and one thread per
+ * node sorts its run by (col node, t): a run holds the pairs of the few elements around one
+ * node (about 20 on the meshes here), so no global sort is needed.  The sorted runs give
+ * (1) the sorted, duplicate-free node adjacency of each row -- the reference's per-row
+ * sort + unique, at node-block granularity -- and (2) for every 3x3 block the list of its
+ * elemental contributions in element order, which drives the atomic-free assembly.  The
+ * scalar CSR of the reference is the node-block pattern expanded by
+ * dof(node, var) = 3*node + var; cols stay ascending because each block row lists node
+ * columns in ascending order.  The integer counts use atomics; the order of the fill they
+ * give is undone by the per-node sort, whose keys are unique, so the result is
+ * deterministic.
  */
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include "adhk_internal.h"
 
@@ -25,22 +30,170 @@
         }                                                                                                         \
     } while (0)
 
-__global__ void k_emit_pairs(int N, int E, int S, const int *tri, const int *seg, unsigned long long *keys, int *vals)
+#define SCAN_TPB 256
+#define SCAN_ITEMS 4 /* elements per thread: one workgroup scans 1024 */
+
+/* ---- exclusive prefix sum of int[n] (n < 2^31), three passes: workgroup scans, a scan of the workgroup
+ *      totals by one workgroup, the carries added back ---- */
+__device__ int wg_exclusive_scan(int v, int *total)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long ntri = 9L * E;
-    if (t < ntri) {
-        int e = (int)(t / 9), a = (int)((t % 9) / 3), b = (int)(t % 3);
-        unsigned long long na = (unsigned long long)tri[a * E + e], nb = (unsigned long long)tri[b * E + e];
-        keys[t] = na * (unsigned long long)N + nb;
-        vals[t] = (int)t;
-    } else if (t < ntri + 4L * S) {
-        long u = t - ntri;
-        int s = (int)(u / 4), a = (int)((u % 4) / 2), b = (int)(u % 2);
-        unsigned long long na = (unsigned long long)seg[2 * s + a], nb = (unsigned long long)seg[2 * s + b];
-        keys[t] = na * (unsigned long long)N + nb;
-        vals[t] = (int)t;
+    __shared__ int sh[SCAN_TPB];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < SCAN_TPB; o <<= 1) {
+        int t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
     }
+    const int incl = sh[threadIdx.x];
+    *total = sh[SCAN_TPB - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(SCAN_TPB) void k_scan_local(long n, const int *in, int *out, int *wsum)
+{
+    const long base = (long)blockIdx.x * SCAN_TPB * SCAN_ITEMS + (long)threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS], s = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = base + k < n ? in[base + k] : 0;
+        s += v[k];
+    }
+    int tot;
+    int run = wg_exclusive_scan(s, &tot);
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) wsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_TPB) void k_scan_sums(int nw, int *wsum)
+{
+    int carry = 0;
+    for (int c0 = 0; c0 < nw; c0 += SCAN_TPB) {
+        const int i = c0 + threadIdx.x;
+        const int v = i < nw ? wsum[i] : 0;
+        int tot;
+        const int ex = wg_exclusive_scan(v, &tot);
+        if (i < nw) wsum[i] = ex + carry;
+        carry += tot;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_TPB) void k_scan_add(long n, int *out, const int *wsum)
+{
+    const long base = (long)blockIdx.x * SCAN_TPB * SCAN_ITEMS + (long)threadIdx.x * SCAN_ITEMS;
+    const int c = wsum[blockIdx.x];
+    for (int k = 0; k < SCAN_ITEMS; k++)
+        if (base + k < n) out[base + k] += c;
+}
+
+/* out = exclusive scan of in (n entries); *total (host) = sum of in */
+static int exclusive_scan(hipStream_t st, long n, const int *in, int *out, int *total)
+{
+    const long per = (long)SCAN_TPB * SCAN_ITEMS;
+    const int nw = (int)((n + per - 1) / per);
+    int *wsum = NULL, last_in = 0, last_out = 0;
+    if (n <= 0) {
+        *total = 0;
+        return 0;
+    }
+    CHECK(hipMalloc((void **)&wsum, (size_t)nw * sizeof(int)));
+    hipLaunchKernelGGL(k_scan_local, dim3(nw), dim3(SCAN_TPB), 0, st, n, in, out, wsum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_TPB), 0, st, nw, wsum);
+    hipLaunchKernelGGL(k_scan_add, dim3(nw), dim3(SCAN_TPB), 0, st, n, out, wsum);
+    CHECK(hipGetLastError());
+    CHECK(hipMemcpyAsync(&last_in, in + (n - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+    CHECK(hipMemcpyAsync(&last_out, out + (n - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+    CHECK(hipStreamSynchronize(st));
+    (void)hipFree(wsum);
+    *total = last_in + last_out;
+    return 0;
+fail:
+    (void)hipFree(wsum);
+    return -1;
+}
+
+/* row and col node of contribution t */
+__device__ __forceinline__ void pair_nodes(long t, int E, const int *tri, const int *seg, int *a, int *b)
+{
+    const long ntri = 9L * E;
+    if (t < ntri) {
+        const int e = (int)(t / 9), la = (int)((t % 9) / 3), lb = (int)(t % 3);
+        *a = tri[la * E + e];
+        *b = tri[lb * E + e];
+    } else {
+        const long u = t - ntri;
+        const int s = (int)(u / 4), la = (int)((u % 4) / 2), lb = (int)(u % 2);
+        *a = seg[2 * s + la];
+        *b = seg[2 * s + lb];
+    }
+}
+
+__global__ void k_row_count(long np, int E, const int *tri, const int *seg, int *cnt)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np) return;
+    int a, b;
+    pair_nodes(t, E, tri, seg, &a, &b);
+    atomicAdd(cnt + a, 1);
+}
+
+/* key = col node << 32 | t, placed in the row node's run (in arrival order) */
+__global__ void k_row_fill(long np, int E, const int *tri, const int *seg, const int *roff, int *fill,
+                           unsigned long long *key)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np) return;
+    int a, b;
+    pair_nodes(t, E, tri, seg, &a, &b);
+    const int pos = roff[a] + atomicAdd(fill + a, 1);
+    key[pos] = ((unsigned long long)(unsigned)b << 32) | (unsigned long long)(unsigned)t;
+}
+
+/* one thread per node: insertion sort of its run by key (col node, then t), then the run's distinct
+ * col nodes (the block row's degree) */
+__global__ void k_row_sort(int N, const int *roff, unsigned long long *key, int *deg)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int lo = roff[n], hi = roff[n + 1];
+    for (int i = lo + 1; i < hi; i++) {
+        const unsigned long long k = key[i];
+        int j = i - 1;
+        while (j >= lo && key[j] > k) {
+            key[j + 1] = key[j];
+            j--;
+        }
+        key[j + 1] = k;
+    }
+    int d = 0;
+    for (int i = lo; i < hi; i++)
+        if (i == lo || (key[i] >> 32) != (key[i - 1] >> 32)) d++;
+    deg[n] = d;
+}
+
+/* contribution list, block columns, block starts and the diagonal block of every row node */
+__global__ void k_row_blocks(int N, const int *roff, const unsigned long long *key, const int *brow, int *contrib,
+                             int *bcol, int *cptr, int *diag_blk)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int lo = roff[n], hi = roff[n + 1];
+    int blk = brow[n] - 1, d = -1;
+    for (int i = lo; i < hi; i++) {
+        const int col = (int)(key[i] >> 32);
+        contrib[i] = (int)(unsigned)(key[i] & 0xffffffffULL);
+        if (i == lo || col != (int)(key[i - 1] >> 32)) {
+            blk++;
+            bcol[blk] = col;
+            cptr[blk] = i;
+            if (col == n) d = blk;
+        }
+    }
+    diag_blk[n] = d;
 }
 
 /* cpos[code] = position of the elemental block `code` in the sorted contribution list */
@@ -48,42 +201,6 @@ __global__ void k_cpos(long n, const int *contrib, int *cpos)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) cpos[contrib[i]] = (int)i;
-}
-
-__global__ void k_head_flags(long n, const unsigned long long *keys, int *flags)
-{
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
-}
-
-__global__ void k_blocks(long n, int N, const unsigned long long *keys, const int *flags, const int *incl,
-                         int *cptr, int *bcol, int *brow_of)
-{
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !flags[i]) return;
-    int b = incl[i] - 1;
-    cptr[b] = (int)i;
-    bcol[b] = (int)(keys[i] % (unsigned long long)N);
-    brow_of[b] = (int)(keys[i] / (unsigned long long)N);
-}
-
-/* brow[n] = first block whose row >= n (lower bound); diag_blk[n] = block (n, n) or -1 */
-__global__ void k_rowptr(int N, int nnzb, const int *brow_of, const int *bcol, int *brow, int *diag_blk)
-{
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n > N) return;
-    int lo = 0, hi = nnzb;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (brow_of[mid] < n) lo = mid + 1; else hi = mid;
-    }
-    brow[n] = lo;
-    if (n < N) {
-        int d = -1;
-        for (int j = lo; j < nnzb && brow_of[j] == n; j++)
-            if (bcol[j] == n) { d = j; break; }
-        diag_blk[n] = d;
-    }
 }
 
 /* SELL-64: slice width = widest block row among the slice's 64 nodes */
@@ -138,8 +255,6 @@ __global__ void k_sell_fill(int N, int nslice, const int *slice_ptr, const int *
 static int build_sell(adhk_ctx *c)
 {
     int *sw = NULL;
-    void *tmp = NULL;
-    size_t tb = 0;
     c->nslice = (c->N + 63) / 64;
     CHECK(hipMalloc((void **)&sw, ((size_t)c->nslice + 1) * sizeof(int)));
     CHECK(hipMemsetAsync(sw, 0, ((size_t)c->nslice + 1) * sizeof(int), c->stream));
@@ -147,11 +262,7 @@ static int build_sell(adhk_ctx *c)
     hipLaunchKernelGGL(k_slice_width, dim3((c->nslice + 255) / 256), dim3(256), 0, c->stream, c->N, c->nslice, c->brow,
                        sw);
     CHECK(hipGetLastError());
-    CHECK(hipcub::DeviceScan::ExclusiveSum(NULL, tb, sw, c->slice_ptr, c->nslice + 1, c->stream));
-    CHECK(hipMalloc(&tmp, tb));
-    CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tb, sw, c->slice_ptr, c->nslice + 1, c->stream));
-    CHECK(hipMemcpyAsync(&c->nslot, c->slice_ptr + c->nslice, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    CHECK(hipStreamSynchronize(c->stream));
+    if (exclusive_scan(c->stream, (long)c->nslice + 1, sw, c->slice_ptr, &c->nslot)) goto fail;
     if ((long)c->nslot * 64 * 9 >= 2147483647L) {
         adhk_set_error("SELL layout too large for 32-bit indices");
         goto fail;
@@ -166,88 +277,70 @@ static int build_sell(adhk_ctx *c)
                        c->slice_ptr, c->brow, c->bcol, c->scol, c->sell2blk, c->blk2sell, c->diag_slot);
     CHECK(hipGetLastError());
     CHECK(hipStreamSynchronize(c->stream));
-    hipFree(sw);
-    hipFree(tmp);
+    (void)hipFree(sw);
     return 0;
 fail:
-    hipFree(sw);
-    hipFree(tmp);
+    (void)hipFree(sw);
     return -1;
 }
 
 extern "C" int adhk_build_pattern(adhk_ctx *c)
 {
-    long np = 9L * c->E + 4L * c->S;
-    unsigned long long *keys = NULL, *keys_s = NULL;
-    int *vals = NULL, *flags = NULL, *incl = NULL, *brow_of = NULL, *hlast = NULL;
-    void *tmp = NULL;
-    size_t tmp_b = 0, tmp_b2 = 0;
-    int nnzb = 0;
-    int bits = 1;
-    unsigned long long maxkey = (unsigned long long)c->N * (unsigned long long)c->N;
-    while (bits < 64 && (1ULL << bits) < maxkey) bits++;
+    const long np = 9L * c->E + 4L * c->S;
+    unsigned long long *key = NULL;
+    int *cnt = NULL, *roff = NULL, *fill = NULL, *deg = NULL;
+    int total = 0, nnzb = 0;
     if (np <= 0 || np >= 2147483647L) {
         adhk_set_error("adhk_build_pattern: unsupported pair count %ld", np);
         return -1;
     }
     c->npairs = (int)np;
-    CHECK(hipMalloc((void **)&keys, np * sizeof(unsigned long long)));
-    CHECK(hipMalloc((void **)&keys_s, np * sizeof(unsigned long long)));
-    CHECK(hipMalloc((void **)&vals, np * sizeof(int)));
-    CHECK(hipMalloc((void **)&c->contrib, np * sizeof(int)));
-    CHECK(hipMalloc((void **)&flags, np * sizeof(int)));
-    CHECK(hipMalloc((void **)&incl, np * sizeof(int)));
-    hipLaunchKernelGGL(k_emit_pairs, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, c->N, c->E, c->S,
-                       c->tri, c->seg, keys, vals);
+    const dim3 gp((unsigned)((np + 255) / 256)), gn((unsigned)((c->N + 255) / 256)), tb(256);
+    CHECK(hipMalloc((void **)&cnt, ((size_t)c->N + 1) * sizeof(int)));
+    CHECK(hipMalloc((void **)&roff, ((size_t)c->N + 1) * sizeof(int)));
+    CHECK(hipMalloc((void **)&fill, (size_t)c->N * sizeof(int)));
+    CHECK(hipMalloc((void **)&deg, ((size_t)c->N + 1) * sizeof(int)));
+    CHECK(hipMalloc((void **)&key, (size_t)np * sizeof(unsigned long long)));
+    CHECK(hipMemsetAsync(cnt, 0, ((size_t)c->N + 1) * sizeof(int), c->stream));
+    CHECK(hipMemsetAsync(fill, 0, (size_t)c->N * sizeof(int), c->stream));
+    CHECK(hipMemsetAsync(deg, 0, ((size_t)c->N + 1) * sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_row_count, gp, tb, 0, c->stream, np, c->E, c->tri, c->seg, cnt);
     CHECK(hipGetLastError());
-    CHECK(hipcub::DeviceRadixSort::SortPairs(NULL, tmp_b, keys, keys_s, vals, c->contrib, (int)np, 0, bits,
-                                             c->stream));
-    CHECK(hipcub::DeviceScan::InclusiveSum(NULL, tmp_b2, flags, incl, (int)np, c->stream));
-    if (tmp_b2 > tmp_b) tmp_b = tmp_b2;
-    CHECK(hipMalloc(&tmp, tmp_b));
-    tmp_b2 = tmp_b;
-    CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_b2, keys, keys_s, vals, c->contrib, (int)np, 0, bits,
-                                             c->stream));
-    CHECK(hipMalloc((void **)&c->cpos, np * sizeof(int)));
-    hipLaunchKernelGGL(k_cpos, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, np, c->contrib, c->cpos);
+    if (exclusive_scan(c->stream, (long)c->N + 1, cnt, roff, &total)) goto fail;
+    if (total != c->npairs) {
+        adhk_set_error("adhk_build_pattern: %d of %d pairs counted (a node id out of range?)", total, c->npairs);
+        goto fail;
+    }
+    hipLaunchKernelGGL(k_row_fill, gp, tb, 0, c->stream, np, c->E, c->tri, c->seg, roff, fill, key);
+    hipLaunchKernelGGL(k_row_sort, gn, tb, 0, c->stream, c->N, roff, key, deg);
     CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_head_flags, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, np, keys_s, flags);
-    CHECK(hipGetLastError());
-    tmp_b2 = tmp_b;
-    CHECK(hipcub::DeviceScan::InclusiveSum(tmp, tmp_b2, flags, incl, (int)np, c->stream));
-    CHECK(hipMemcpyAsync(&nnzb, incl + (np - 1), sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    CHECK(hipStreamSynchronize(c->stream));
+    CHECK(hipMalloc((void **)&c->brow, ((size_t)c->N + 1) * sizeof(int)));
+    if (exclusive_scan(c->stream, (long)c->N + 1, deg, c->brow, &nnzb)) goto fail;
     c->nnzb = nnzb;
+    CHECK(hipMalloc((void **)&c->contrib, (size_t)np * sizeof(int)));
     CHECK(hipMalloc((void **)&c->cptr, ((size_t)nnzb + 1) * sizeof(int)));
     CHECK(hipMalloc((void **)&c->bcol, (size_t)nnzb * sizeof(int)));
-    CHECK(hipMalloc((void **)&brow_of, (size_t)nnzb * sizeof(int)));
-    CHECK(hipMalloc((void **)&c->brow, ((size_t)c->N + 1) * sizeof(int)));
     CHECK(hipMalloc((void **)&c->diag_blk, (size_t)c->N * sizeof(int)));
-    hipLaunchKernelGGL(k_blocks, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, np, c->N, keys_s, flags,
-                       incl, c->cptr, c->bcol, brow_of);
+    hipLaunchKernelGGL(k_row_blocks, gn, tb, 0, c->stream, c->N, roff, key, c->brow, c->contrib, c->bcol, c->cptr,
+                       c->diag_blk);
     CHECK(hipGetLastError());
     CHECK(hipMemcpyAsync(c->cptr + nnzb, &c->npairs, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_rowptr, dim3((c->N + 1 + 255) / 256), dim3(256), 0, c->stream, c->N, nnzb, brow_of, c->bcol,
-                       c->brow, c->diag_blk);
+    CHECK(hipMalloc((void **)&c->cpos, (size_t)np * sizeof(int)));
+    hipLaunchKernelGGL(k_cpos, gp, tb, 0, c->stream, np, c->contrib, c->cpos);
     CHECK(hipGetLastError());
     CHECK(hipStreamSynchronize(c->stream));
-    hipFree(keys);
-    hipFree(keys_s);
-    hipFree(vals);
-    hipFree(flags);
-    hipFree(incl);
-    hipFree(brow_of);
-    hipFree(tmp);
-    (void)hlast;
+    (void)hipFree(key);
+    (void)hipFree(cnt);
+    (void)hipFree(roff);
+    (void)hipFree(fill);
+    (void)hipFree(deg);
     return build_sell(c);
 fail:
-    hipFree(keys);
-    hipFree(keys_s);
-    hipFree(vals);
-    hipFree(flags);
-    hipFree(incl);
-    hipFree(brow_of);
-    hipFree(tmp);
+    (void)hipFree(key);
+    (void)hipFree(cnt);
+    (void)hipFree(roff);
+    (void)hipFree(fill);
+    (void)hipFree(deg);
     return -1;
 }
 
