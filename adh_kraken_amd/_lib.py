@@ -129,6 +129,8 @@ SIGNATURES = [
     ("Screen_print_CSR", None, [ip, ip, dp, C.c_int]),
     ("adhk_la_set_block_size", None, [C.c_int]),
     ("adhk_la_last_status", C.c_int, []),
+    ("adhk_elem_residual_one", C.c_int, [P, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, dp]),
+    ("adhk_sw2_set_seg_values", C.c_int, [P, dp]),
     ("adhk_la_last_iterations", C.c_int, []),
     ("adhk_bcgstab_device_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int)]),

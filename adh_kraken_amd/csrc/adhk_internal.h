@@ -183,6 +183,13 @@ int adhk_launch_copy_masked(adhk_ctx *c, double *dst, const double *src);
 int adhk_launch_start_from_old_masked(adhk_ctx *c);
 int adhk_launch_dirichlet_sol(adhk_ctx *c);
 
+/* ---- vector helpers of the struct-level surface (la_csr.hip): y += x on host arrays through the GPU;
+ *      get_residual_norms' maxima (first dof on ties), l2 sum and NaN count ---- */
+int adhk_la_vec_add(double *y, const double *x, int n);
+int adhk_la_residual_norms(int n, const double *residual, const double *dsol, const int *bc_mask,
+                           const int *include_dof, double *rmax, double *l2sum, double *dmax, int *irmax, int *idmax,
+                           int *nnan);
+
 /* ---- learned warm start (mlp.hip) ---- */
 int adhk_mlp_predict_into(adhk_mlp *m, adhk_ctx *c);
 

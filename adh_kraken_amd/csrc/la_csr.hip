@@ -757,6 +757,117 @@ extern "C" int solve_linear_sys_bcgstab(double *x, int *indptr_diag, int *cols_d
     return rc;
 }
 
+/* ------------------------------------------- vector helpers of the struct-level surface (ref_shim.c) */
+
+__global__ void kc_add(int n, double *y, const double *x)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += x[i]; /* increment_function, newton_tools.c:261 */
+}
+
+/* get_residual_norms, newton_tools.c:303-373: per workgroup the max |r| and max |dsol| with their first
+ * dof, the sum of r^2 over the dofs not in bc_mask (nor outside include_dof), and a NaN count */
+__global__ __launch_bounds__(TPB) void kc_norms(int n, const double *r, const double *d, const int *bc,
+                                                const int *inc, double *part, int *ipart)
+{
+    __shared__ double sv[3][TPB];
+    __shared__ int si[3][TPB];
+    double rmax = 0.0, dmax = 0.0, l2 = 0.0;
+    int ir = 0x7fffffff, id = 0x7fffffff, nan = 0;
+    for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) {
+        const int use = !(inc && inc[i] != 1) && !(bc && bc[i] == 1);
+        const double a = fabs(r[i]), b = fabs(d[i]);
+        if (r[i] != r[i]) nan++;
+        if (use && b > dmax) { dmax = b; id = i; }
+        if (use && a > rmax) { rmax = a; ir = i; }
+        if (use) l2 = __fma_rn(r[i], r[i], l2);
+    }
+    sv[0][threadIdx.x] = rmax; si[0][threadIdx.x] = ir;
+    sv[1][threadIdx.x] = dmax; si[1][threadIdx.x] = id;
+    sv[2][threadIdx.x] = l2; si[2][threadIdx.x] = nan;
+    __syncthreads();
+    for (int o = TPB / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            for (int c = 0; c < 2; c++) { /* the larger value; on a tie the first dof, as the sequential scan */
+                const double a = sv[c][threadIdx.x], b = sv[c][threadIdx.x + o];
+                const int ia = si[c][threadIdx.x], ib = si[c][threadIdx.x + o];
+                if (b > a || (b == a && ib < ia)) { sv[c][threadIdx.x] = b; si[c][threadIdx.x] = ib; }
+            }
+            sv[2][threadIdx.x] += sv[2][threadIdx.x + o];
+            si[2][threadIdx.x] += si[2][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; c++) {
+            part[c * NPART_MAX + blockIdx.x] = sv[c][0];
+            ipart[c * NPART_MAX + blockIdx.x] = si[c][0];
+        }
+}
+
+extern "C" int adhk_la_vec_add(double *y, const double *x, int n)
+{
+    if (n <= 0 || !y || !x) {
+        adhk_set_error("increment: bad arguments");
+        return -1;
+    }
+    la_ws *w = ws_get(n, 1);
+    if (!w || up(w, w->x, y, n) || up(w, w->s, x, n)) return -1;
+    hipLaunchKernelGGL(kc_add, dim3(nblk(n)), dim3(TPB), 0, w->st, n, w->x, w->s);
+    LCK(hipGetLastError());
+    return down(w, y, w->x, n);
+}
+
+extern "C" int adhk_la_residual_norms(int n, const double *residual, const double *dsol, const int *bc_mask,
+                                      const int *include_dof, double *rmax, double *l2sum, double *dmax, int *irmax,
+                                      int *idmax, int *nnan)
+{
+    if (n <= 0 || !residual || !dsol) {
+        adhk_set_error("get_residual_norms: bad arguments");
+        return -1;
+    }
+    /* room for the vectors, the two integer masks (in the int buffer of the CSR columns) and the integer
+     * partials (in the scratch vector tmp) */
+    const int nws = n > 2 * NPART_MAX ? n : 2 * NPART_MAX;
+    la_ws *w = ws_get(nws, 2 * n);
+    if (!w || up(w, w->r, residual, n) || up(w, w->s, dsol, n)) return -1;
+    int *dbc = NULL, *dinc = NULL, *ipart = (int *)w->tmp;
+    if (bc_mask) {
+        dbc = w->ci;
+        LCK(hipMemcpyAsync(dbc, bc_mask, (size_t)n * sizeof(int), hipMemcpyHostToDevice, w->st));
+    }
+    if (include_dof) {
+        dinc = w->ci + n;
+        LCK(hipMemcpyAsync(dinc, include_dof, (size_t)n * sizeof(int), hipMemcpyHostToDevice, w->st));
+    }
+    const int g = nblk(n) < NPART_MAX ? nblk(n) : NPART_MAX;
+    hipLaunchKernelGGL(kc_norms, dim3(g), dim3(TPB), 0, w->st, n, w->r, w->s, dbc, dinc, w->part, ipart);
+    LCK(hipGetLastError());
+    double hp[3 * NPART_MAX];
+    int hi[3 * NPART_MAX];
+    LCK(hipMemcpyAsync(hp, w->part, 3 * NPART_MAX * sizeof(double), hipMemcpyDeviceToHost, w->st));
+    LCK(hipMemcpyAsync(hi, ipart, 3 * NPART_MAX * sizeof(int), hipMemcpyDeviceToHost, w->st));
+    LCK(hipStreamSynchronize(w->st));
+    double rm = 0.0, dm = 0.0, s2 = 0.0;
+    int ir = 0, id = 0, nn = 0;
+    for (int b = 0; b < g; b++) {
+        if (hp[b] > rm || (hp[b] == rm && hp[b] > 0.0 && hi[b] < ir)) { rm = hp[b]; ir = hi[b]; }
+        if (hp[NPART_MAX + b] > dm || (hp[NPART_MAX + b] == dm && dm > 0.0 && hi[NPART_MAX + b] < id)) {
+            dm = hp[NPART_MAX + b];
+            id = hi[NPART_MAX + b];
+        }
+        s2 += hp[2 * NPART_MAX + b];
+        nn += hi[2 * NPART_MAX + b];
+    }
+    *rmax = rm;
+    *dmax = dm;
+    *l2sum = s2;
+    *irmax = rm > 0.0 ? ir : 0;
+    *idmax = dm > 0.0 ? id : 0;
+    *nnan = nn;
+    return 0;
+}
+
 /* ---------------------------------------------------- BiCGStab on device arrays without allocation */
 
 /* Scratch of adhk_bcgstab_device_noalloc, one copy per GPU (the array's host-side name is not a device

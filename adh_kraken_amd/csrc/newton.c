@@ -507,6 +507,20 @@ int adhk_sw2_set_bc(adhk_sw2 *c, const int *bc_mask, const double *dirichlet)
 fail:
     return -1;
 }
+int adhk_sw2_set_seg_values(adhk_sw2 *c, const double *seg_val)
+{
+    if (!c || (c->S > 0 && !seg_val)) {
+        adhk_set_error("adhk_sw2_set_seg_values: bad arguments");
+        return -1;
+    }
+    if (c->S == 0) return 0;
+    HC(hipMemcpyAsync(c->seg_val, seg_val, (size_t)c->S * 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    if (c->h_seg_val) memcpy(c->h_seg_val, seg_val, (size_t)c->S * 2 * sizeof(double));
+    return 0;
+fail:
+    return -1;
+}
 int adhk_sw2_get_bc(adhk_sw2 *c, int *bc_mask)
 {
     if (!c || !bc_mask) return -1;

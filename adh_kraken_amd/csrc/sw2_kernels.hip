@@ -234,6 +234,59 @@ extern "C" int adhk_launch_elem_resid(adhk_ctx *c)
     return 0;
 }
 
+/* One element's residual, optionally perturbed: the reference's per-element residual call
+ * (pde[k].resid: sw2_body_resid.c:44, sw2_bc_h.c:57, sw2_bc_vel.c:58), one thread.  which 0: triangle
+ * (the general wet/dry form; wd >= 0 replaces the element's wet/dry flag), 1: segment.  The result goes
+ * to the element's own slots of erhs. */
+__global__ void k_elem_resid_one(MeshView m, SegView s, Sw2Par P, int which, int e, int pnode, int pvar, double pe,
+                                 int wd, double *__restrict__ erhs)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (which == 0) {
+        TriIn in;
+        gather_tri(m, e, true, in);
+        if (wd >= 0) in.wd_flag = wd;
+        double r[9];
+        sw2_tri_resid_general(in, P, pnode, pvar, pe, r);
+        for (int k = 0; k < 9; k++) erhs[9 * (size_t)e + k] = r[k];
+        return;
+    }
+    double h[2], u[2], v[2];
+    for (int a = 0; a < 2; a++) {
+        const int n = s.seg[2 * e + a];
+        h[a] = s.sol[3 * n];
+        u[a] = s.sol[3 * n + 1];
+        v[a] = s.sol[3 * n + 2];
+    }
+    double er[6];
+    sw2_seg_resid(s.kind[e], s.val[2 * e], s.val[2 * e + 1], s.nrml[2 * e], s.nrml[2 * e + 1], s.sdjac[e], h, u, v, P,
+                  pnode, pvar, pe, er);
+    for (int k = 0; k < 6; k++) erhs[9 * (size_t)m.E + 6 * (size_t)e + k] = er[k];
+}
+
+extern "C" int adhk_elem_residual_one(adhk_sw2 *c, int which, int ie, double pe, int pnode, int pvar, int wd_flag,
+                                      double *out)
+{
+    if (!c || !out || (which != 0 && which != 1) || ie < 0 || ie >= (which ? c->S : c->E) || pnode < 0 ||
+        pnode > (which ? 1 : 2)) {
+        adhk_set_error("adhk_elem_residual_one: bad arguments (which %d, element %d)", which, ie);
+        return -1;
+    }
+    if (pvar < 0 || pvar > 2) pvar = -1;
+    hipLaunchKernelGGL(k_elem_resid_one, dim3(1), dim3(64), 0, c->stream, make_view(c), make_segview(c), make_par(c),
+                       which, ie, pnode, pvar, pvar >= 0 ? pe : 0.0, wd_flag, c->erhs);
+    CHECK_LAUNCH();
+    const size_t off = which ? 9 * (size_t)c->E + 6 * (size_t)ie : 9 * (size_t)ie;
+    hipError_t e = hipMemcpyAsync(out, c->erhs + off, (which ? 6 : 9) * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        adhk_set_error("adhk_elem_residual_one: %s", hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
+}
+
 /* residual[3n+k] = sum over the diagonal block's contributions (element order) of -erhs */
 __global__ __launch_bounds__(256) void k_resid_gather(int N, int E, const int *__restrict__ diag_blk,
                                                       const int *__restrict__ cptr, const int *__restrict__ contrib,

@@ -153,6 +153,9 @@ int adhk_sw2_get_state(adhk_sw2 *c, double *sol, double *sol_old, double *sol_ol
 int adhk_sw2_set_bc(adhk_sw2 *c, const int *bc_mask, const double *dirichlet_data);
 int adhk_sw2_get_bc(adhk_sw2 *c, int *bc_mask);
 int adhk_sw2_set_dt(adhk_sw2 *c, double dt);
+/* boundary values of every segment, [nelems1d][2] (the meaning of each per ADHK_BC_*); the reference's
+ * boundary residuals read theirs from the time series at each call (sw2_bc_h.c:155-157) */
+int adhk_sw2_set_seg_values(adhk_sw2 *c, const double *seg_val);
 double adhk_sw2_get_dt(const adhk_sw2 *c);
 int adhk_sw2_get_timing(const adhk_sw2 *c, adhk_timing *t);
 /* per-kernel device timers: on = 1 resets and enables (hipEvent pairs around every launch of the
@@ -188,6 +191,13 @@ int adhk_assemble_residual(adhk_sw2 *c, double *residual);
 /* elemental residual for one element (the pde[k].resid pointer, sw2_body_resid.c:44 / sw2_bc_h.c:57 /
  * sw2_bc_vel.c:58) evaluated on the GPU for a batch: elem_rhs [nelems2d*9 + nelems1d*6] */
 int adhk_elem_residuals(adhk_sw2 *c, double *elem_rhs);
+
+/* one element's residual at sol, optionally perturbed (the reference's pde[k].resid call in its element
+ * loops, sw2_body_resid.c:44, sw2_bc_h.c:57, sw2_bc_vel.c:58): which 0 = triangle ie -> out[9], 1 = segment
+ * ie -> out[6]; the variable pvar (0 h, 1 u, 2 v; -1 none) of local node pnode is moved by pe.  wd_flag >= 0
+ * replaces the triangle's wet/dry flag (else the one of the last adhk_sw2_init). */
+int adhk_elem_residual_one(adhk_sw2 *c, int which, int ie, double pe, int pnode, int pvar, int wd_flag,
+                           double *out);
 
 /* replaces slin_sys_init_sparsity_mono (slin_sys_init_sparsity_mono.c:28): CSR pattern built on the GPU */
 int adhk_sparsity(adhk_sw2 *c, int *indptr /* ndofs+1 */, int *cols /* nnz */);

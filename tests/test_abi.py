@@ -10,11 +10,12 @@ import adh_kraken_amd._lib as L
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    txt = open(os.path.join(ROOT, "include", "adhk.h")).read()
+def declared_functions(header="adhk.h"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     names = set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt))
-    return sorted(n for n in names if n not in {"if", "for", "while", "return", "sizeof"} and not n.isupper())
+    skip = {"if", "for", "while", "return", "sizeof", "int", "void", "double", "char"}  # incl. function pointers
+    return sorted(n for n in names if n not in skip and not n.isupper())
 
 
 def test_header_declares_reference_surface():
@@ -29,6 +30,18 @@ def test_header_declares_reference_surface():
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(L.LIB_PATH)
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_library_exports_the_struct_level_surface():
+    lib = ctypes.CDLL(L.LIB_PATH)
+    names = declared_functions("adhk_ref.h")
+    for ref in ["assemble_residual", "assemble_jacobian", "fe_newton", "check_diag", "apply_Dirichlet_BC",
+                "initialize_system", "increment_function", "get_residual_norms", "fe_sw2_body_resid", "fe_sw2_bc_h",
+                "fe_sw2_bc_vel", "fe_sw2_init", "slin_sys_init_sparsity_mono", "load_elem_mat", "perturb_var",
+                "get_cell_dofs_ivars", "load_global_resid", "add_replace_elem_rhs"]:
+        assert ref in names
+    missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
 
 
