@@ -26,13 +26,14 @@ class Solver(C.Structure):
     _fields_ = [("tol_nonlin", C.c_double), ("inc_nonlin", C.c_double), ("max_nonlin_it", C.c_int),
                 ("max_linesearch_cuts", C.c_int), ("lin_tol", C.c_double), ("lin_max_it", C.c_int),
                 ("precision", C.c_int), ("lin_check_every", C.c_int), ("mixed_lin_tol", C.c_double),
-                ("warm_gate", C.c_int)]
+                ("warm_gate", C.c_int), ("substep_depth", C.c_int)]
 
 
 class NewtonStats(C.Structure):
     _fields_ = [("newton_its", C.c_int), ("linesearch_cuts_total", C.c_int), ("lin_its_total", C.c_int),
                 ("lin_its_max", C.c_int), ("resid_max_norm", C.c_double), ("inc_max_norm", C.c_double),
-                ("converged", C.c_int), ("warm_accepted", C.c_int), ("warm_retried", C.c_int)]
+                ("converged", C.c_int), ("warm_accepted", C.c_int), ("warm_retried", C.c_int),
+                ("substeps", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -177,6 +177,7 @@ int adhk_launch_increment(adhk_ctx *c);
 int adhk_launch_linesearch(adhk_ctx *c);
 int adhk_launch_init_system(adhk_ctx *c);
 int adhk_launch_rotate(adhk_ctx *c);
+int adhk_launch_rotate_masked(adhk_ctx *c);
 int adhk_launch_start_from_old(adhk_ctx *c);
 int adhk_launch_copy(adhk_ctx *c, double *dst, const double *src);
 int adhk_launch_copy_masked(adhk_ctx *c, double *dst, const double *src);

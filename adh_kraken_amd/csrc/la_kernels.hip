@@ -1297,15 +1297,21 @@ extern "C" int adhk_launch_linesearch(adhk_ctx *c)
     return 0;
 }
 /* sol <- sol_old; wet/dry flags; dry nodes -> u,v Dirichlet; sol[bc] <- dirichlet */
+__global__ void k_copy_masked(int n, double *dst, const double *src, const int *node_inst, const int *work);
+__global__ void k_dirichlet_sol_masked(int n, double *sol, const int *bc, const double *dir, const int *node_inst,
+                                       const int *work);
+
+/* initialize_system + initialize_dirichlet_bc for the instances in the work mask: instances outside it
+ * (converged ensemble members while others retake their step in sub-steps) keep their state */
 extern "C" int adhk_launch_init_system(adhk_ctx *c)
 {
-    hipLaunchKernelGGL(k_copy, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs, c->sol,
-                       c->sol_old);
+    hipLaunchKernelGGL(k_copy_masked, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs, c->sol,
+                       c->sol_old, c->node_inst, c->work);
     CHECK_LAUNCH();
     if (adhk_launch_wdflag(c)) return -1;
     if (adhk_launch_dry_bc(c)) return -1;
-    hipLaunchKernelGGL(k_apply_dirichlet_sol, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs,
-                       c->sol, c->bc_mask, c->dirichlet);
+    hipLaunchKernelGGL(k_dirichlet_sol_masked, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs,
+                       c->sol, c->bc_mask, c->dirichlet, c->node_inst, c->work);
     CHECK_LAUNCH();
     return 0;
 }
@@ -1358,6 +1364,24 @@ extern "C" int adhk_launch_dirichlet_sol(adhk_ctx *c)
     CHECK_LAUNCH();
     return 0;
 }
+__global__ void k_rotate_masked(int n, double *sol, double *sol_old, double *sol_older, const int *node_inst,
+                                const int *work)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !work[node_inst[i / 3]]) return;
+    sol_older[i] = sol_old[i];
+    sol_old[i] = sol[i];
+}
+
+/* prep_sol for the instances in the work mask only */
+extern "C" int adhk_launch_rotate_masked(adhk_ctx *c)
+{
+    hipLaunchKernelGGL(k_rotate_masked, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs, c->sol,
+                       c->sol_old, c->sol_older, c->node_inst, c->work);
+    CHECK_LAUNCH();
+    return 0;
+}
+
 extern "C" int adhk_launch_rotate(adhk_ctx *c)
 {
     hipLaunchKernelGGL(k_rotate, dim3((c->ndofs + TPB - 1) / TPB), dim3(TPB), 0, c->stream, c->ndofs, c->sol,

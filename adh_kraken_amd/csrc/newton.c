@@ -44,6 +44,7 @@ void adhk_solver_defaults(adhk_solver *s)
     s->lin_check_every = 8;
     s->mixed_lin_tol = 1e-6;
     s->warm_gate = 1;
+    s->substep_depth = 0;
 }
 
 #define HC(x)                                                                                                     \
@@ -1455,6 +1456,119 @@ fail:
     return -1;
 }
 
+static void stats_add(adhk_newton_stats *a, const adhk_newton_stats *b)
+{
+    a->newton_its += b->newton_its;
+    a->linesearch_cuts_total += b->linesearch_cuts_total;
+    a->lin_its_total += b->lin_its_total;
+    if (b->lin_its_max > a->lin_its_max) a->lin_its_max = b->lin_its_max;
+    a->resid_max_norm = b->resid_max_norm;
+    a->inc_max_norm = b->inc_max_norm;
+    a->substeps += b->substeps;
+}
+
+/* The members in mask, each at level t0 (sol = sol_old), advance over [t0, t0 + dt] in four steps of
+ * dt / 4 (DT_REDUCE_FACTOR, define.h:127); a member whose quarter step fails goes back to the quarter's
+ * start and retakes it in four (depth + 1), down to substep_depth.  On return mask[i] is 1 for the members
+ * that reached t0 + dt and 0 for those that failed at the deepest level (left at the last sub-step they
+ * completed).  Their statistics are added to acc. */
+static int substeps_rec(adhk_ctx *c, const adhk_solver *s, int *mask, double t0, double dt, int depth,
+                        adhk_newton_stats *acc)
+{
+    const int K = c->ninst;
+    const double h = 0.25 * dt;
+    int *sub = calloc((size_t)K, sizeof(int)), *save = calloc((size_t)K, sizeof(int)),
+        *again = calloc((size_t)K, sizeof(int));
+    if (!sub || !save || !again) {
+        adhk_set_error("sub-steps: out of host memory");
+        goto fail;
+    }
+    for (int k = 0; k < 4; k++) {
+        int any = 0, anyf = 0;
+        for (int i = 0; i < K; i++) any |= mask[i];
+        if (!any) break;
+        if (k > 0) { /* prep_sol of the members still stepping */
+            LC(set_work(c, mask));
+            LC(adhk_launch_rotate_masked(c));
+        }
+        LC(adhk_series_apply(c, t0 + k * h, t0 + (k + 1) * h));
+        memcpy(save, c->alive, (size_t)K * sizeof(int));
+        memcpy(c->alive, mask, (size_t)K * sizeof(int));
+        c->p.dt = h;
+        const int ok = fe_newton_pass(c, s, 0);
+        memcpy(c->alive, save, (size_t)K * sizeof(int));
+        if (ok < 0) goto fail;
+        for (int i = 0; i < K; i++) {
+            sub[i] = 0;
+            if (!mask[i]) continue;
+            stats_add(&acc[i], &c->inst_stats[i]);
+            acc[i].substeps++;
+            sub[i] = !c->inst_stats[i].converged;
+            anyf |= sub[i];
+        }
+        if (!anyf) continue;
+        LC(set_work(c, sub));
+        LC(adhk_launch_start_from_old_masked(c));
+        memcpy(again, sub, (size_t)K * sizeof(int));
+        if (depth < s->substep_depth && substeps_rec(c, s, again, t0 + k * h, h, depth + 1, acc)) goto fail;
+        for (int i = 0; i < K; i++)
+            if (sub[i] && (depth >= s->substep_depth || !again[i])) mask[i] = 0;
+        c->p.dt = h;
+    }
+    c->p.dt = dt;
+    free(sub);
+    free(save);
+    free(again);
+    return 0;
+fail:
+    c->p.dt = dt;
+    free(sub);
+    free(save);
+    free(again);
+    return -1;
+}
+
+/* after a pass in which some live members failed: each retakes its step in sub-steps (substep_depth > 0).
+ * Returns TRUE when every live member completed the step, FALSE otherwise, -1 on error. */
+static int substep_recovery(adhk_ctx *c, const adhk_solver *s)
+{
+    const int K = c->ninst;
+    const double dt0 = c->p.dt;
+    int rc = -1, all = 1;
+    adhk_newton_stats *first = malloc((size_t)K * sizeof *first), *acc = calloc((size_t)K, sizeof *acc);
+    int *mask = calloc((size_t)K, sizeof(int)), *failed = calloc((size_t)K, sizeof(int));
+    if (!first || !acc || !mask || !failed) {
+        adhk_set_error("sub-steps: out of host memory");
+        goto done;
+    }
+    memcpy(first, c->inst_stats, (size_t)K * sizeof *first);
+    for (int i = 0; i < K; i++) failed[i] = mask[i] = c->alive[i] && !first[i].converged;
+    if (set_work(c, mask) || adhk_launch_start_from_old_masked(c)) goto done;
+    if (substeps_rec(c, s, mask, c->t_prev, dt0, 1, acc)) goto done;
+    c->p.dt = dt0;
+    for (int i = 0; i < K; i++) {
+        c->inst_stats[i] = first[i];
+        if (!failed[i]) continue;
+        stats_add(&c->inst_stats[i], &acc[i]);
+        c->inst_stats[i].converged = mask[i];
+        if (!mask[i]) c->alive[i] = 0;
+    }
+    for (int i = 0; i < K; i++)
+        if (failed[i] || c->alive[i]) all &= c->inst_stats[i].converged;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        adhk_set_error("sub-steps: device error");
+        goto done;
+    }
+    rc = all;
+done:
+    c->p.dt = dt0;
+    free(first);
+    free(acc);
+    free(mask);
+    free(failed);
+    return rc;
+}
+
 int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
 {
     if (!c || !s) return -1;
@@ -1474,7 +1588,12 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
         t->warm_accepted = 0;
         t->warm_retried = 1;
     }
-    if (!ok && newton_failure(c)) return -1;
+    if (!ok && s->substep_depth > 0) {
+        ok = substep_recovery(c, s);
+        if (ok < 0) return -1;
+    } else if (!ok && newton_failure(c)) {
+        return -1;
+    }
     if (st) {
         adhk_newton_stats tot;
         memset(&tot, 0, sizeof tot);
@@ -1488,6 +1607,7 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
             if (t->inc_max_norm > tot.inc_max_norm) tot.inc_max_norm = t->inc_max_norm;
             tot.warm_accepted += t->warm_accepted;
             tot.warm_retried += t->warm_retried;
+            tot.substeps += t->substeps;
         }
         tot.converged = ok;
         *st = tot;
@@ -1538,6 +1658,7 @@ int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_st
             run[i].inc_max_norm = t->inc_max_norm;
             run[i].warm_accepted += t->warm_accepted;
             run[i].warm_retried += t->warm_retried;
+            run[i].substeps += t->substeps;
             run[i].converged = c->alive[i] ? t->converged : 0;
             if (c->alive[i] && t->converged) c->steps_done[i]++;
         }
@@ -1549,6 +1670,7 @@ int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_st
         tot.inc_max_norm = st.inc_max_norm;
         tot.warm_accepted += st.warm_accepted;
         tot.warm_retried += st.warm_retried;
+        tot.substeps += st.substeps;
     }
     memcpy(c->inst_stats, run, (size_t)c->ninst * sizeof *run);
     free(run);

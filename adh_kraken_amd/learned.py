@@ -107,6 +107,8 @@ def collect_trajectory(sm: SModelSuper, mlp: WarmStartMLP, nsteps, precision=L.P
     collected, and the trajectory goes on at its own dt; a quarter step that fails ends the
     trajectory, keeping the steps already collected.  The step after the quarter steps is not
     collected either: its features would hold a quarter step's change as the previous change.
+    A step that is not taken because a boundary series does not cover it (the time loop leaves dt as it
+    was and runs no Newton iteration) ends the trajectory.
     Returns (Newton iterations, steps collected)."""
     its, k, dt0, resumed = 0, 0, sm.get_dt(), False
     for _ in range(nsteps):
@@ -118,6 +120,8 @@ def collect_trajectory(sm: SModelSuper, mlp: WarmStartMLP, nsteps, precision=L.P
                 k += 1
             resumed = False
             continue
+        if sm.get_dt() == dt0:  # the forcing stopped the loop before any Newton iteration
+            break
         if strict:
             raise L.AdhkError(f"time step failed while collecting: {st}")
         sm.set_dt(dt0 / 4)

@@ -69,6 +69,12 @@ typedef struct {
                                 residual test at tol_nonlin is unchanged (iterative refinement) */
     int warm_gate;           /* learned start kept when its residual is below the previous level's in
                                 1: l2 norm (default), 0: max norm (no counterpart in the reference) */
+    int substep_depth;       /* 0 (default): a failed step cuts dt by 0.25 and ends (tc_scale.c:3-19, time_loop.c:
+                                25-35), an ensemble member drops out.  n > 0 (no counterpart in the reference): the
+                                failed model or member retakes the step as four quarter steps from its previous
+                                level, a failed quarter again as four, down to dt / 4^n; the step then completes
+                                at full dt and the next one runs at dt.  A member that fails at dt / 4^n stops at
+                                the last sub-step it completed and drops out. */
 } adhk_solver;
 
 typedef struct {
@@ -82,6 +88,7 @@ typedef struct {
     int warm_accepted; /* Newton solves that started from the learned iterate (adhk_sw2_set_warm_start) */
     int warm_retried;  /* solves that failed from the learned iterate and were solved again from the
                           previous level (no counterpart in the reference) */
+    int substeps;      /* sub-steps taken to recover failed steps (adhk_solver.substep_depth) */
 } adhk_newton_stats;
 
 /* per-stage device time of the last adhk_time_loop / adhk_fe_newton call, ms */

@@ -83,3 +83,95 @@ def test_ensemble_of_the_batched_workload_converges(gpu):
     per = ens.instance_stats()
     assert all(p["converged"] == 1 for p in per)
     assert np.all(np.isfinite(ens.sol))
+
+
+# members of the batched workload (flume_ensemble(64, seed=4)) with a supercritical inflow against a depth
+# condition: at dt = 1 s the first step fails in the oracle as on the GPU; 44 converges at dt / 4 and 49 at
+# dt / 16 (oracle runs of the same members); 14 and 55 converge at dt
+C4_RECOVERED = (44, 49)
+C4_PLAIN = (14, 55)
+
+
+@pytest.fixture(scope="module")
+def c4_cases():
+    return A.flume_ensemble(n=64, seed=4)
+
+
+def test_failed_members_retake_their_step_in_sub_steps(gpu, c4_cases):
+    """With substep_depth > 0 a member whose step fails goes back to its previous level and retakes the step
+    as quarter steps (a failed quarter again in quarters); it completes the step and stays in the ensemble.
+    Every member equals its standalone run with the same setting, bit for bit, and a recovered member's
+    first step equals four oracle steps of dt / 4."""
+    import oracle as O
+    idx = C4_PLAIN + C4_RECOVERED
+    cases = [c4_cases[i] for i in idx]
+    ens = A.SModelSuper.ensemble([m for m, _, _ in cases], dt=1.0)
+    ens.solver.substep_depth = 3
+    ens.set_initial(np.concatenate([s for _, s, _ in cases]))
+    n, st = A.time_loop(ens, 1)
+    assert n == 1 and st["converged"] == 1
+    alive, done = ens.instance_status()
+    assert list(alive) == [1, 1, 1, 1] and list(done) == [1, 1, 1, 1]
+    per = ens.instance_stats()
+    assert [p["substeps"] > 0 for p in per] == [False, False, True, True]
+    assert per[3]["substeps"] > 4  # member 49 retakes a quarter in sixteenths
+    sol = ens.sol
+    for k, (mesh, sol0, _) in enumerate(cases):
+        sm = A.SModelSuper(mesh, dt=1.0)
+        sm.solver.substep_depth = 3
+        sm.set_initial(sol0)
+        ns, sts = A.time_loop(sm, 1)
+        assert ns == 1 and sts["substeps"] == per[k]["substeps"], k
+        assert np.array_equal(sol[ens.instance_slice(k)], sm.sol), k
+        assert sm.get_dt() == 1.0
+    mesh, sol0, _ = cases[2]
+    orc = O.Oracle(mesh, dt=0.25, threads=8)
+    orc.set_initial(sol0)
+    assert orc.time_loop(4)[0] == 4
+    ho, hg = orc.state()[0][0::3], sol[ens.instance_slice(2)][0::3]
+    assert np.abs(hg - ho).max() <= 1e-8 * np.abs(ho).max()
+
+
+def test_without_sub_steps_a_failed_member_drops_out(gpu, c4_cases):
+    cases = [c4_cases[i] for i in (C4_PLAIN[0], C4_RECOVERED[0])]
+    ens = A.SModelSuper.ensemble([m for m, _, _ in cases], dt=1.0)
+    ens.set_initial(np.concatenate([s for _, s, _ in cases]))
+    A.time_loop(ens, 1)
+    alive, done = ens.instance_status()
+    assert list(alive) == [1, 0] and list(done) == [1, 0]
+
+
+def test_full_batched_workload_two_steps(gpu, c4_cases):
+    """All 64 members (5k-200k triangles, 3.6M in all), mixed precision with sub-step recovery: 62 members
+    complete both steps; sampled members (recovered ones among them) equal their standalone runs bit for
+    bit; the two smallest match the FP64 oracle to the mixed-precision tolerance."""
+    import oracle as O
+    ens = A.SModelSuper.ensemble([m for m, _, _ in c4_cases], dt=1.0)
+    ens.solver.substep_depth = 3
+    ens.set_initial(np.concatenate([s for _, s, _ in c4_cases]))
+    n, st = A.time_loop(ens, 2, precision=L.PREC_MIXED)
+    alive, done = ens.instance_status()
+    # 49 and 51 (inflow Froude numbers 4.6 and 16.5 against a depth condition) complete the first step in
+    # sub-steps and fail shortly after t = 1.1 s at every dt down to 1/64 s, in the oracle as here; they
+    # drop out, every other member completes both steps
+    stuck = [i for i in range(len(c4_cases)) if not alive[i]]
+    assert stuck == [49, 51] and list(done[stuck]) == [1, 1]
+    assert all(done[i] == 2 for i in range(len(c4_cases)) if i not in stuck)
+    assert n == -1  # the slowest member completed one step
+    sol = ens.sol
+    sizes = np.array([m.nelems2d for m, _, _ in c4_cases])
+    small = list(np.argsort(sizes)[:2])
+    for i in sorted(set(small) | {0, 19, 44, int(np.argmax(sizes))}):
+        mesh, sol0, _ = c4_cases[i]
+        sm = A.SModelSuper(mesh, dt=1.0)
+        sm.solver.substep_depth = 3
+        sm.set_initial(sol0)
+        assert A.time_loop(sm, 2, precision=L.PREC_MIXED)[0] == 2
+        assert np.array_equal(sol[ens.instance_slice(i)], sm.sol), i
+    for i in small:
+        mesh, sol0, _ = c4_cases[i]
+        orc = O.Oracle(mesh, dt=1.0, threads=8)
+        orc.set_initial(sol0)
+        assert orc.time_loop(2)[0] == 2
+        ho, hg = orc.state()[0][0::3], sol[ens.instance_slice(i)][0::3]
+        assert np.abs(hg - ho).max() <= 1e-6 * np.abs(ho).max(), i

@@ -13,8 +13,9 @@ from adh_kraken_amd import learned as LR
 
 
 class FakeModel:
-    def __init__(self, dt, fail_at=(), quarter_fails=False):
+    def __init__(self, dt, fail_at=(), quarter_fails=False, forcing_ends_at=None):
         self.dt, self.fail_at, self.quarter_fails = dt, set(fail_at), quarter_fails
+        self.forcing_ends_at = forcing_ends_at
         self.calls = []  # (nsteps asked, dt) per time_loop call
         self.nominal_steps = 0
 
@@ -29,6 +30,8 @@ def fake_time_loop(sm, nsteps, precision=None):
     sm.calls.append((nsteps, sm.dt))
     if nsteps == 1:
         sm.nominal_steps += 1
+        if sm.forcing_ends_at is not None and sm.nominal_steps >= sm.forcing_ends_at:
+            return 0, {"newton_its": 0}  # the series does not cover the step: not taken, dt unchanged
         if sm.nominal_steps in sm.fail_at:
             sm.dt *= 0.25  # the time loop's cut; the old level stays
             return 0, {"newton_its": 7}
@@ -79,3 +82,14 @@ def test_strict_collection_raises_at_the_failed_step():
     with pytest.raises(L.AdhkError, match="time step failed while collecting"):
         LR.collect_trajectory(sm, mlp, 6)
     assert mlp.collected == [1]
+
+
+def test_forcing_stop_ends_the_trajectory_without_quarter_steps():
+    """a step that the boundary series do not cover is not taken and dt is not cut: collection ends there
+    instead of stepping past the series' end in quarter steps"""
+    sm, mlp = FakeModel(2.0, forcing_ends_at=4), FakeMLP()
+    its, k = LR.collect_trajectory(sm, mlp, 10, strict=False)
+    assert (its, k) == (9, 3) and mlp.collected == [1, 2, 3]
+    assert all(c == (1, 2.0) for c in sm.calls) and len(sm.calls) == 4
+    sm2, mlp2 = FakeModel(2.0, forcing_ends_at=2), FakeMLP()
+    assert LR.collect_trajectory(sm2, mlp2, 5, strict=True) == (3, 1)
