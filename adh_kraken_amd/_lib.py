@@ -33,7 +33,7 @@ class NewtonStats(C.Structure):
     _fields_ = [("newton_its", C.c_int), ("linesearch_cuts_total", C.c_int), ("lin_its_total", C.c_int),
                 ("lin_its_max", C.c_int), ("resid_max_norm", C.c_double), ("inc_max_norm", C.c_double),
                 ("converged", C.c_int), ("warm_accepted", C.c_int), ("warm_retried", C.c_int),
-                ("substeps", C.c_int)]
+                ("substeps", C.c_int), ("lin_capped", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

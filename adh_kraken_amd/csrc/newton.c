@@ -1362,6 +1362,7 @@ static int fe_newton_pass(adhk_ctx *c, const adhk_solver *s, int use_warm)
             if (act[i]) {
                 c->inst_stats[i].lin_its_total += lits[i];
                 if (lits[i] > c->inst_stats[i].lin_its_max) c->inst_stats[i].lin_its_max = lits[i];
+                if (lits[i] >= s->lin_max_it) c->inst_stats[i].lin_capped++;
                 old_rl2[i] = rl2[i];
                 cuts[i] = 0;
             }
@@ -1465,6 +1466,7 @@ static void stats_add(adhk_newton_stats *a, const adhk_newton_stats *b)
     a->resid_max_norm = b->resid_max_norm;
     a->inc_max_norm = b->inc_max_norm;
     a->substeps += b->substeps;
+    a->lin_capped += b->lin_capped;
 }
 
 /* The members in mask, each at level t0 (sol = sol_old), advance over [t0, t0 + dt] in four steps of
@@ -1584,6 +1586,7 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
         t->newton_its += first.newton_its;
         t->linesearch_cuts_total += first.linesearch_cuts_total;
         t->lin_its_total += first.lin_its_total;
+        t->lin_capped += first.lin_capped;
         if (first.lin_its_max > t->lin_its_max) t->lin_its_max = first.lin_its_max;
         t->warm_accepted = 0;
         t->warm_retried = 1;
@@ -1608,6 +1611,7 @@ int adhk_fe_newton(adhk_sw2 *c, const adhk_solver *s, adhk_newton_stats *st)
             tot.warm_accepted += t->warm_accepted;
             tot.warm_retried += t->warm_retried;
             tot.substeps += t->substeps;
+            tot.lin_capped += t->lin_capped;
         }
         tot.converged = ok;
         *st = tot;
@@ -1659,6 +1663,7 @@ int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_st
             run[i].warm_accepted += t->warm_accepted;
             run[i].warm_retried += t->warm_retried;
             run[i].substeps += t->substeps;
+            run[i].lin_capped += t->lin_capped;
             run[i].converged = c->alive[i] ? t->converged : 0;
             if (c->alive[i] && t->converged) c->steps_done[i]++;
         }
@@ -1671,6 +1676,7 @@ int adhk_time_loop(adhk_sw2 *c, const adhk_solver *s, int nsteps, adhk_newton_st
         tot.warm_accepted += st.warm_accepted;
         tot.warm_retried += st.warm_retried;
         tot.substeps += st.substeps;
+        tot.lin_capped += st.lin_capped;
     }
     memcpy(c->inst_stats, run, (size_t)c->ninst * sizeof *run);
     free(run);

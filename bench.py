@@ -15,11 +15,19 @@ A third variant adds the learned warm start to the mixed-precision solver: a per
 2-5), the training checkpoint kept being the one that costs a fifth held-out flume (seed 6) the
 least solver work.
 
+A fourth variant runs plain FP64 at the mixed variant's inner (BiCGStab) tolerance of 1e-6, so that the
+mixed speed-up splits into what FP32 storage gives and what the looser inner solve gives.
+
 The roofline is for the dominant kernel of the plain run: algorithmic bytes of
 one launch divided by its average launch time, measured with hipEvent pairs on
 the library's stream during the timed region.  The CPU baseline is the oracle
-on all host threads over a bounded sample (the first five time steps of the same workload, median
-of three runs), with a one-thread sample of the first step beside it.
+on the host threads this job may use (the affinity set, or OMP_NUM_THREADS when that names fewer) over
+a bounded sample (the first five time steps of the same workload, median of three runs), with a
+one-thread sample of the first step beside it; the line states the thread choice, the host's CPU model
+and counts, and the sample's step count against the GPU's.
+
+The ensemble leg runs the 64-member batched workload for its 50 steps, failed steps retaken in
+sub-steps, against the same members solved one after another.
 """
 import argparse
 import ctypes as C
@@ -62,17 +70,38 @@ def algorithmic_bytes(kind, N, nslot, ndofs, mixed, split=False):
     raise ValueError(kind)
 
 
-def cpu_baseline(mesh, sol0, dt, nsteps_sample=5, seg_val=None, serial_steps=0, repeats=3):
-    """The oracle on every host thread (capped at 16) over nsteps_sample steps, the median of `repeats`
+def host_cpu():
+    """The host's processor model, logical CPU count, the affinity set and the OpenMP thread setting."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except Exception:
+        affinity = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": affinity,
+            "omp_num_threads_env": int(omp) if omp and omp.isdigit() else None}
+
+
+def cpu_baseline(mesh, sol0, dt, gpu_steps, nsteps_sample=5, seg_val=None, serial_steps=0, repeats=3):
+    """The oracle over nsteps_sample steps on every host thread this process may use, the median of `repeats`
     runs; with serial_steps, also on one thread (the reference itself is single-threaded) over that many
-    steps."""
+    steps.  The threads are the affinity set, unless OMP_NUM_THREADS names fewer: that is the CPU share a
+    one-GPU job gets on a shared host, and more threads than that would time other jobs' cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the checker doubles as the CPU baseline ("port")
-    try:
-        threads = len(os.sched_getaffinity(0))
-    except Exception:
-        threads = os.cpu_count() or 1
-    threads = min(threads, 16)
+    host = host_cpu()
+    threads = host["affinity"]
+    why = "every CPU of the affinity set"
+    if host["omp_num_threads_env"] and host["omp_num_threads_env"] < threads:
+        threads = host["omp_num_threads_env"]
+        why = f"OMP_NUM_THREADS={threads}, this job's CPU share of the {host['affinity']}-CPU affinity set"
 
     def timed(nthreads, nsteps):
         orc = O.Oracle(mesh, dt=dt, threads=nthreads)
@@ -86,10 +115,12 @@ def cpu_baseline(mesh, sol0, dt, nsteps_sample=5, seg_val=None, serial_steps=0, 
     runs = [timed(threads, nsteps_sample) for _ in range(repeats)]
     dt_all, st = sorted(runs, key=lambda r: r[0])[len(runs) // 2]  # the median run
     out = {"value": round(nsteps_sample / dt_all, 5), "unit": "timesteps/s", "cores": threads, "kind": "port",
-           "sample": f"{nsteps_sample} time steps of the same {3 * mesh.nnodes}-dof workload from the same initial "
-                     f"state, OpenMP C oracle, median of {repeats} runs ({dt_all:.1f} s; runs "
-                     f"{', '.join('%.1f' % r[0] for r in runs)} s; {st['newton_its']} Newton its, "
-                     f"{st['lin_its_total']} BiCGStab its)"}
+           "threads_chosen_as": why, "host": host,
+           "sample": f"the first {nsteps_sample} of the {gpu_steps} time steps the GPU leg times, same "
+                     f"{3 * mesh.nnodes}-dof workload and initial state, OpenMP C oracle, median of {repeats} runs "
+                     f"({dt_all:.1f} s; runs {', '.join('%.1f' % r[0] for r in runs)} s; {st['newton_its']} Newton "
+                     f"its, {st['lin_its_total']} BiCGStab its)",
+           "sample_steps": nsteps_sample, "gpu_steps": gpu_steps}
     if serial_steps:
         dt_one, st1 = timed(1, serial_steps)
         out["serial"] = {"value": round(serial_steps / dt_one, 5), "unit": "timesteps/s", "cores": 1,
@@ -97,10 +128,13 @@ def cpu_baseline(mesh, sol0, dt, nsteps_sample=5, seg_val=None, serial_steps=0, 
     return out
 
 
-def run_variant(A, L, mesh, sol0, dt, precision, steps, warmup, check_every, warm=None, schedule=None, prepare=None):
+def run_variant(A, L, mesh, sol0, dt, precision, steps, warmup, check_every, warm=None, schedule=None, prepare=None,
+                solver=None):
     lib = L.load()
     sm = A.SModelSuper(mesh, dt=dt)
     sm.solver.lin_check_every = check_every
+    for k, v in (solver or {}).items():  # e.g. the inner (BiCGStab) tolerance
+        setattr(sm.solver, k, v)
     if prepare is not None:  # e.g. the inflow hydrograph
         prepare(sm)
     if warm is not None:
@@ -182,51 +216,71 @@ def pcie_leg(A, L, mesh, sol0, dt, steps=10):
                     "the same steps from the same state without the copies"}
 
 
-def ensemble_run(A, L, steps):
-    """64 independent uneven flumes (length 200-4000 m, width 20-200 m, 5k-200k triangles, seeded
-    Manning n, inflow, tailwater, bumps), mixed precision, dt 1 s: one context with every member in
-    each launch, against the same members solved one after another."""
+def ensemble_run(A, L, steps, substep_depth=3):
+    """64 independent uneven flumes (length 200-4000 m, width 20-200 m, 5k-200k triangles, seeded Manning n,
+    inflow, tailwater, bumps), mixed precision, dt 1 s, a failed member's step retaken in sub-steps down to
+    dt / 4^substep_depth: one context with every member in each launch, against the same members solved one
+    after another with the same settings."""
     cases = A.flume_ensemble(n=64, seed=4)
     e0 = np.concatenate([s for _, s, _ in cases])
     ens = A.SModelSuper.ensemble([m for m, _, _ in cases], dt=1.0)
+    ens.solver.substep_depth = substep_depth
     ens.set_initial(e0)
-    tuned, _ = ens.autotune(precision=L.PREC_MIXED, ninner=2)
+    tuned, _ = ens.autotune(precision=L.PREC_MIXED, ninner=2)  # graphs are a choice here: no kernel timers
     ens.set_initial(e0)
     A.time_loop(ens, 1, precision=L.PREC_MIXED)  # warm-up
     ens.set_initial(e0)
     _ = ens.sol
+    L.check(L.load().adhk_device_synchronize(), "device_synchronize")
     t0 = time.perf_counter()
     _, st = A.time_loop(ens, steps, precision=L.PREC_MIXED)
-    sol_b = ens.sol
+    L.check(L.load().adhk_device_synchronize(), "device_synchronize")
     t_batch = time.perf_counter() - t0
+    sol_b = ens.sol
     alive, done = ens.instance_status()
+    per = ens.instance_stats()
     log("ensemble: the same members one after another")
-    t_seq, same, differ = 0.0, 0, []
+    t_seq, same, differ, seq_done = 0.0, 0, [], 0
     for i, (m, s0, _) in enumerate(cases):
         sm = A.SModelSuper(m, dt=1.0)
+        sm.solver.substep_depth = substep_depth
+        sm.tuning = dict(tuned)
         sm.set_initial(s0)
         A.time_loop(sm, 1, precision=L.PREC_MIXED)
         sm.set_initial(s0)
         sm.set_dt(1.0)
         _ = sm.sol
+        L.check(L.load().adhk_device_synchronize(), "device_synchronize")
         t0 = time.perf_counter()
         n_alone, _ = A.time_loop(sm, steps, precision=L.PREC_MIXED)
-        s1 = sm.sol
+        L.check(L.load().adhk_device_synchronize(), "device_synchronize")
         t_seq += time.perf_counter() - t0
-        if np.array_equal(s1, sol_b[ens.instance_slice(i)]):
-            same += 1
-        else:
-            differ.append({"member": i, "ensemble_steps": int(done[i]), "standalone_steps": int(n_alone),
-                           "max_abs_diff": float(np.nanmax(np.abs(s1 - sol_b[ens.instance_slice(i)])))})
+        seq_done += max(n_alone, -n_alone)
+        s1 = sm.sol
+        if done[i] == steps:  # bit-identity is claimed for the members that completed every step
+            if np.array_equal(s1, sol_b[ens.instance_slice(i)]):
+                same += 1
+            else:
+                differ.append({"member": i, "max_abs_diff": float(np.nanmax(np.abs(s1 - sol_b[ens.instance_slice(i)])))})
         sm.close()
+    complete = [i for i in range(len(cases)) if done[i] == steps]
+    member_steps = int(done.sum())
     return {"members": len(cases), "triangles": int(sum(m.nelems2d for m, _, _ in cases)),
-            "dofs": int(ens.ndofs), "steps": steps, "precision": "mixed",
+            "dofs": int(ens.ndofs), "steps": steps, "precision": "mixed", "substep_depth": substep_depth,
             "batched_ensemble_steps_per_s": round(steps / t_batch, 3),
+            "batched_member_steps_per_s": round(member_steps / t_batch, 2),
             "sequential_ensemble_steps_per_s": round(steps / t_seq, 3),
+            "sequential_member_steps_per_s": round(seq_done / t_seq, 2),
             "batched_speedup": round(t_seq / t_batch, 2),
-            "members_completed": int(alive.sum()), "members_failed_at_step_1": int((done == 0).sum()),
-            "members_bit_identical_to_standalone": same, "members_differing": differ, "schedule": tuned,
-            "newton_its": st["newton_its"], "bicgstab_its": st["lin_its_total"]}
+            "members_completed_all_steps": len(complete),
+            "members_recovered_by_substeps": [i for i in complete if per[i]["substeps"] > 0],
+            "members_dropped": [{"member": i, "steps_done": int(done[i])} for i in range(len(cases)) if done[i] < steps],
+            "bit_identical_to_standalone_of_completed": f"{same} of {len(complete)}",
+            "members_differing": differ, "schedule": tuned, "newton_its": st["newton_its"],
+            "bicgstab_its": st["lin_its_total"], "substeps": st["substeps"],
+            "note": "member-steps/s counts the time steps every member completed; dropped members (a supercritical "
+                    "inflow against a depth condition that fails at every dt down to 1/64 s, in the oracle too) "
+                    "stop contributing after their last completed step"}
 
 
 def main():
@@ -252,7 +306,8 @@ def main():
                     help="training meshes: flumes with bed-bump seeds 2, 3, ... (c1, c2) or channels of the "
                          "c3 family with seeds 11, 12, ... (c5)")
     ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--ensemble-steps", type=int, default=10)
+    ap.add_argument("--ensemble-steps", type=int, default=50, help="time steps of the 64-member ensemble (the "
+                    "workload's 50 by default)")
     ap.add_argument("--check-every", type=int, default=8)
     ap.add_argument("--schedule", default=None,
                     help='fixed solver schedule as JSON, e.g. \'{"split": 1, "chunk": 8}\' (default: autotune)')
@@ -325,10 +380,16 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "sw2_newton_c2_traffic.json")
     if args.config == "c2" and os.path.exists(tpath):  # HBM bytes per launch from the rocprofv3 counter runs
         traffic = json.load(open(tpath))["bytes_per_launch"].get(kname)
+    mat_bytes = algorithmic_bytes(roof_kind, sm.mesh.nnodes, sm.nslot, 0, False, split)
     roofline = {"kernel": kname, "bound": "hbm",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes": ab,
-                "avg_launch_ms": round(avg_ms, 5), "copy_kernel_gbs": round(copy_gbs, 1)}
+                "avg_launch_ms": round(avg_ms, 5), "copy_kernel_gbs": round(copy_gbs, 1),
+                "frac_of_copy_kernel": round(achieved / copy_gbs, 3),
+                "note": (f"the {mat_bytes / 1e6:.0f} MB matrix fits the 256 MB MALL, which serves part of each "
+                         f"product's reads: the rate exceeds the 1 GiB copy kernel's, and is not HBM bandwidth alone"
+                         if mat_bytes < 256e6 else
+                         f"the {mat_bytes / 1e6:.0f} MB matrix exceeds the 256 MB MALL and streams from HBM")}
     value = args.steps / plain["elapsed"]
     out = {
         "metric": ("SW2 timesteps/s @1.2M dofs: FP64 plain vs mixed+learned, equal 1e-8 Newton tol"
@@ -349,6 +410,9 @@ def main():
                    "variant": "plain FP64: FD Jacobian, BiCGStab + 3x3 block-Jacobi folded into the matrix (tol 1e-11, max 100)",
                    "newton_tol": sm.solver.tol_nonlin, "inc_nonlin": sm.solver.inc_nonlin},
         "newton_its": plain["stats"]["newton_its"], "bicgstab_its": plain["stats"]["lin_its_total"],
+        "bicgstab_its_per_solve": round(plain["stats"]["lin_its_total"] / max(plain["stats"]["newton_its"], 1), 1),
+        "bicgstab_its_max": plain["stats"]["lin_its_max"],
+        "solves_at_iteration_cap": f"{plain['stats']['lin_capped']} of {plain['stats']['newton_its']} (cap 100)",
         "final_resid_max": plain["stats"]["resid_max_norm"],
         "kernel_avg_us": {k: round(1e3 * v["ms_avg"], 2) for k, v in kt.items() if v["ms_avg"]},
         "kernel_timing": f"hipEvent pairs on the first and every {KT_STRIDE}th launch of each kernel class",
@@ -368,7 +432,19 @@ def main():
                         "depth_rel_err_vs_fp64": float(np.abs(hmx - h64).max() / np.abs(h64).max()),
                         "speed_rel_err_vs_fp64": float(np.abs(umx - u64).max() / max(np.abs(u64).max(), 1e-300)),
                         "newton_its": mx["stats"]["newton_its"], "bicgstab_its": mx["stats"]["lin_its_total"],
+                        "bicgstab_its_max": mx["stats"]["lin_its_max"], "solves_at_iteration_cap": mx["stats"]["lin_capped"],
                         "kernel_avg_us": {k: round(1e3 * v["ms_avg"], 2) for k, v in mx["kt"].items() if v["ms_avg"]}}
+        # FP64 throughout at the mixed variant's inner tolerance: splits the mixed speed-up into what the FP32
+        # storage gives and what the looser inner solve gives
+        f6 = run_variant(A, L, mesh, sol0, dt, L.PREC_FP64, args.steps, args.warmup, args.check_every,
+                         schedule=schedule, prepare=prepare, solver={"lin_tol": 1e-6})
+        v6 = args.steps / f6["elapsed"]
+        out["fp64_inner_tol_1e-6"] = {
+            "value": round(v6, 4), "unit": "timesteps/s", "speedup_vs_fp64": round(v6 / value, 3),
+            "mixed_speedup_over_this": round(vmx / v6, 3),
+            "variant": "plain FP64 with the inner BiCGStab tolerance of the mixed variant (1e-6); Newton tolerance 1e-8",
+            "depth_rel_err_vs_fp64": float(np.abs(f6["sol"][0::3] - h64).max() / np.abs(h64).max()),
+            "newton_its": f6["stats"]["newton_its"], "bicgstab_its": f6["stats"]["lin_its_total"]}
     if not args.no_learned:
         from adh_kraken_amd import learned as LR
         t0 = time.perf_counter()
@@ -416,6 +492,10 @@ def main():
             "depth_rel_err_vs_fp64": float(np.abs(hml - h64).max() / np.abs(h64).max()),
             "plain_mixed_newton_its": out["mixed"]["newton_its"] if "mixed" in out else None,
             "speedup_vs_plain_mixed": round(vml / out["mixed"]["value"], 3) if "mixed" in out else None,
+            # steps after which the untimed data generation, training and validation are paid back by the
+            # time each step saves against plain mixed precision
+            "break_even_steps": (round(t_train / (1.0 / out["mixed"]["value"] - 1.0 / vml))
+                                 if "mixed" in out and vml > out["mixed"]["value"] else None),
             "training": {"trajectories": tr_desc, "steps_collected": mlp.collected_steps,
                          "samples": mlp.size, "adam_steps": args.adam_steps, "batch": 16384,
                          "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-10:].mean()),
@@ -440,7 +520,7 @@ def main():
             seg_val[idx, 0] = -A.series_average(hydro[0], hydro[1], 0.0, dt) / w_in
         log("CPU baseline (oracle)")
         big = args.config in ("c3", "c5")
-        cb = cpu_baseline(mesh, sol0, dt, nsteps_sample=1 if big else 5, seg_val=seg_val,
+        cb = cpu_baseline(mesh, sol0, dt, args.steps, nsteps_sample=1 if big else 5, seg_val=seg_val,
                           serial_steps=0 if big else 1, repeats=1 if big else 3)
         cb["gpu_fp64_speedup"] = round(value / cb["value"], 1)
         if "serial" in cb:

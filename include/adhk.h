@@ -89,6 +89,7 @@ typedef struct {
     int warm_retried;  /* solves that failed from the learned iterate and were solved again from the
                           previous level (no counterpart in the reference) */
     int substeps;      /* sub-steps taken to recover failed steps (adhk_solver.substep_depth) */
+    int lin_capped;    /* linear solves that stopped at lin_max_it iterations (bcgstab_solver.c:22) */
 } adhk_newton_stats;
 
 /* per-stage device time of the last adhk_time_loop / adhk_fe_newton call, ms */
