@@ -89,9 +89,22 @@ __device__ __forceinline__ void sw2_temporal(const TriIn &in, const double *h, c
  * 1 / hden are formed once.  Each of these moves a term by at most a few ulps, about 1e-12 of a
  * Jacobian entry after the central difference; the residual itself (FD = false) keeps the
  * reference's operation order. */
+/* Values of an all-wet element's FD residual that a perturbation of one variable leaves unchanged, formed
+ * once per element node thread and reused by its +-eps evaluations (the same operations on the same
+ * operands; the compiler's contraction of multiply-adds may round a shared value one ulp apart from an
+ * inline evaluation, far below the central difference's own error):
+ *   le        sqrt(djac factor_old)                                   -- no perturbation changes it
+ *   rough0    drag coefficient, ihden0 = 1 / hden                     -- unchanged when u or v is perturbed
+ *   vmag0     |(u_wd, v_wd)|, t1_0[a] = |(u_a, v_a)|                  -- unchanged when h is perturbed
+ *                                                                        (t1_0 of the other nodes also when
+ *                                                                        u or v of one node is) */
+struct FdShare {
+    double le, rough0, ihden0, vmag0, t1_0[3];
+};
+
 template <bool FD = false>
 __device__ __forceinline__ int sw2_tri_resid(const TriIn &in, const Sw2Par &P, int pnode, int pvar, double pe,
-                                             double r[9])
+                                             double r[9], const FdShare *sh = nullptr)
 {
     /* perturbation, sw2_body_resid.c:107-120 (selects instead of a dynamic index keep the
      * element in registers; an unperturbed entry gets + 0.0, which leaves it unchanged) */
@@ -142,10 +155,13 @@ __device__ __forceinline__ int sw2_tri_resid(const TriIn &in, const Sw2Par &P, i
     }
     const double factor_old = (dho == 0) ? 1.0 : 0.0;
     const double factor_older = (dhoo == 0) ? 1.0 : 0.0;
-    const double vmag = sqrt(uwd * uwd + vwd * vwd);
+    const bool hsame = FD && sh && pvar != 0, vsame = FD && sh && pvar == 0; /* h / (u, v) unperturbed */
+    const double vmag = vsame ? sh->vmag0 : sqrt(uwd * uwd + vwd * vwd);
     /* friction coefficient, sw2_body_resid.c:238-241 / sw2_get_roughness.c:40-45 */
     double rough;
-    if (P.manning) {
+    if (hsame) {
+        rough = sh->rough0;
+    } else if (P.manning) {
         double hf = hwd > D_SMALL ? hwd : D_SMALL;
         rough = (in.ks > D_SMALL) ? drag_coef(hf, in.ks) : 0.0;
     } else {
@@ -199,11 +215,28 @@ __device__ __forceinline__ int sw2_tri_resid(const TriIn &in, const Sw2Par &P, i
     const double resistance = FD ? rough * (0.5 / g) : rough / (2.0 * g);
     {
         double fx[3], fy[3];
+        if (FD && sh) {
+            /* the perturbed node's speed is the one magnitude a u or v perturbation changes */
+            double up = u[0], vp = v[0];
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            double t1 = sqrt(u[i] * u[i] + v[i] * v[i]);
-            fx[i] = u[i] * t1;
-            fy[i] = v[i] * t1;
+            for (int i = 1; i < 3; i++) {
+                up = (i == pnode) ? u[i] : up;
+                vp = (i == pnode) ? v[i] : vp;
+            }
+            const double tp = vsame ? 0.0 : sqrt(up * up + vp * vp);
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const double t1 = (!vsame && i == pnode) ? tp : sh->t1_0[i];
+                fx[i] = u[i] * t1;
+                fy[i] = v[i] * t1;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                double t1 = sqrt(u[i] * u[i] + v[i] * v[i]);
+                fx[i] = u[i] * t1;
+                fy[i] = v[i] * t1;
+            }
         }
         double t1 = dt * g * resistance * factor;
         double tt = djac * t1 * D_ONE_12;
@@ -270,7 +303,7 @@ __device__ __forceinline__ int sw2_tri_resid(const TriIn &in, const Sw2Par &P, i
     /* SUPG, :921-1142 (triangle branch; tau from get_supg_tau.c:43-80, le = sqrt(area), :173) */
     if (!(fabs(hwd) < D_SMALL)) {
         const double hden = hwd > D_NQS ? hwd : D_NQS;
-        const double le = sqrt(djac * factor_old);
+        const double le = (FD && sh) ? sh->le : sqrt(djac * factor_old);
         const double cel2 = g * hden;
         const double sr = sqrt(uwd * uwd + vwd * vwd + cel2 + D_SMALL);
         const double tau = 0.5 * le / sr;
@@ -303,7 +336,7 @@ __device__ __forceinline__ int sw2_tri_resid(const TriIn &in, const Sw2Par &P, i
         double xf = g * bsx * dt * fct;
         double yf = g * bsy * dt * fct;
         double c_eq = dhdt + a0;
-        const double ihden = FD ? 1.0 / hden : 0.0;
+        const double ihden = FD ? (hsame ? sh->ihden0 : 1.0 / hden) : 0.0;
         double x_eq = dudt + a1 + (xp + (FD ? xf * ihden : xf / hden)) * djac;
         double y_eq = dvdt + a2 + (yp + (FD ? yf * ihden : yf / hden)) * djac;
         const double gf = g;
@@ -315,6 +348,29 @@ __device__ __forceinline__ int sw2_tri_resid(const TriIn &in, const Sw2Par &P, i
         }
     }
     return 0;
+}
+
+/* the shared values of an all-wet element (the FD residual's unperturbed operands, formed the same way) */
+__device__ __forceinline__ void fd_share(const TriIn &in, const Sw2Par &P, FdShare &sh)
+{
+    const double *h = in.h, *u = in.u, *v = in.v;
+    const double hwd = (h[0] + h[1] + h[2]) * D_ONE_3;
+    const double uwd = (u[0] + u[1] + u[2]) * D_ONE_3;
+    const double vwd = (v[0] + v[1] + v[2]) * D_ONE_3;
+    const int dho = n_dry3(in.ho[0], in.ho[1], in.ho[2]);
+    const double factor_old = (dho == 0) ? 1.0 : 0.0;
+    sh.le = sqrt(in.djac * factor_old);
+    sh.vmag0 = sqrt(uwd * uwd + vwd * vwd);
+    if (P.manning) {
+        double hf = hwd > D_SMALL ? hwd : D_SMALL;
+        sh.rough0 = (in.ks > D_SMALL) ? drag_coef(hf, in.ks) : 0.0;
+    } else {
+        sh.rough0 = P.cf;
+    }
+    const double hden = hwd > D_NQS ? hwd : D_NQS;
+    sh.ihden0 = 1.0 / hden;
+#pragma unroll
+    for (int i = 0; i < 3; i++) sh.t1_0[i] = sqrt(u[i] * u[i] + v[i] * v[i]);
 }
 
 /* ---- 1-D boundary residuals ---- */
