@@ -236,18 +236,27 @@ extern "C" int adhk_launch_elem_resid(adhk_ctx *c)
 
 /* One element's residual, optionally perturbed: the reference's per-element residual call
  * (pde[k].resid: sw2_body_resid.c:44, sw2_bc_h.c:57, sw2_bc_vel.c:58), one thread.  which 0: triangle
- * (the general wet/dry form; wd >= 0 replaces the element's wet/dry flag), 1: segment.  The result goes
- * to the element's own slots of erhs. */
+ * (the general wet/dry form; wd >= 0 replaces the element's wet/dry flag), 1: segment; for checking the
+ * kernels' own forms of an all-wet triangle, 2: the residual kernel's, 3: the element Jacobian's (FD form
+ * with its shared values).  The result goes to the element's own slots of erhs. */
 __global__ void k_elem_resid_one(MeshView m, SegView s, Sw2Par P, int which, int e, int pnode, int pvar, double pe,
                                  int wd, double *__restrict__ erhs)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (which == 0) {
+    if (which == 0 || which == 2 || which == 3) {
         TriIn in;
         gather_tri(m, e, true, in);
         if (wd >= 0) in.wd_flag = wd;
         double r[9];
-        sw2_tri_resid_general(in, P, pnode, pvar, pe, r);
+        if (which == 0) {
+            sw2_tri_resid_general(in, P, pnode, pvar, pe, r);
+        } else if (which == 2) { /* the residual kernel's all-wet form */
+            if (sw2_tri_resid<false>(in, P, pnode, pvar, pe, r)) sw2_tri_resid_general(in, P, pnode, pvar, pe, r);
+        } else { /* the element Jacobian's FD form, with its shared values */
+            FdShare fs;
+            fd_share(in, P, fs);
+            if (sw2_tri_resid<true>(in, P, pnode, pvar, pe, r, &fs)) sw2_tri_resid_general(in, P, pnode, pvar, pe, r);
+        }
         for (int k = 0; k < 9; k++) erhs[9 * (size_t)e + k] = r[k];
         return;
     }
@@ -267,8 +276,9 @@ __global__ void k_elem_resid_one(MeshView m, SegView s, Sw2Par P, int which, int
 extern "C" int adhk_elem_residual_one(adhk_sw2 *c, int which, int ie, double pe, int pnode, int pvar, int wd_flag,
                                       double *out)
 {
-    if (!c || !out || (which != 0 && which != 1) || ie < 0 || ie >= (which ? c->S : c->E) || pnode < 0 ||
-        pnode > (which ? 1 : 2)) {
+    const int seg = (which == 1);
+    if (!c || !out || which < 0 || which > 3 || ie < 0 || ie >= (seg ? c->S : c->E) || pnode < 0 ||
+        pnode > (seg ? 1 : 2)) {
         adhk_set_error("adhk_elem_residual_one: bad arguments (which %d, element %d)", which, ie);
         return -1;
     }
@@ -276,8 +286,8 @@ extern "C" int adhk_elem_residual_one(adhk_sw2 *c, int which, int ie, double pe,
     hipLaunchKernelGGL(k_elem_resid_one, dim3(1), dim3(64), 0, c->stream, make_view(c), make_segview(c), make_par(c),
                        which, ie, pnode, pvar, pvar >= 0 ? pe : 0.0, wd_flag, c->erhs);
     CHECK_LAUNCH();
-    const size_t off = which ? 9 * (size_t)c->E + 6 * (size_t)ie : 9 * (size_t)ie;
-    hipError_t e = hipMemcpyAsync(out, c->erhs + off, (which ? 6 : 9) * sizeof(double), hipMemcpyDeviceToHost,
+    const size_t off = seg ? 9 * (size_t)c->E + 6 * (size_t)ie : 9 * (size_t)ie;
+    hipError_t e = hipMemcpyAsync(out, c->erhs + off, (seg ? 6 : 9) * sizeof(double), hipMemcpyDeviceToHost,
                                   c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {

@@ -203,7 +203,9 @@ int adhk_elem_residuals(adhk_sw2 *c, double *elem_rhs);
 /* one element's residual at sol, optionally perturbed (the reference's pde[k].resid call in its element
  * loops, sw2_body_resid.c:44, sw2_bc_h.c:57, sw2_bc_vel.c:58): which 0 = triangle ie -> out[9], 1 = segment
  * ie -> out[6]; the variable pvar (0 h, 1 u, 2 v; -1 none) of local node pnode is moved by pe.  wd_flag >= 0
- * replaces the triangle's wet/dry flag (else the one of the last adhk_sw2_init). */
+ * replaces the triangle's wet/dry flag (else the one of the last adhk_sw2_init).  which 2 and 3 evaluate a
+ * triangle with the all-wet forms of the residual kernel and of the element Jacobian kernel (the latter
+ * multiplies by 1/3 and reuses values shared by the +-eps pair), for checking that they agree. */
 int adhk_elem_residual_one(adhk_sw2 *c, int which, int ie, double pe, int pnode, int pvar, int wd_flag,
                            double *out);
 

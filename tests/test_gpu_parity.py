@@ -373,3 +373,29 @@ def test_c2_full_size_time_steps_depth(gpu):
     gs, os_ = sm.sol, orc.state()[0]
     assert rel_err(gs[0::3], os_[0::3]) <= 1e-8
     assert rel_err(gs[1::3], os_[1::3]) <= 1e-6 and rel_err(gs[2::3], os_[2::3]) <= 1e-6
+
+
+def test_fd_form_of_the_residual_agrees_with_the_residual_form(gpu, c1):
+    """The element Jacobian evaluates an all-wet triangle's residual in an FD form (products for the
+    constant divisions, values shared by the +-eps pair); the Newton residual keeps the reference's
+    operation order.  On sampled C1 triangles at perturbed states, for every perturbation the Jacobian
+    makes, the two forms agree entry by entry to 1e-13 of the element's largest entry; partially dry
+    triangles take the general form in both kernels."""
+    mesh, sol0 = c1
+    s, so = perturbed_state(sol0, seed=31)
+    sm = A.SModelSuper(mesh, dt=1.0)
+    sm.sol, sm.sol_old, sm.sol_older = s, so, so
+    lib = L.load()
+    ra, rb = np.zeros(9), np.zeros(9)
+    worst = 0.0
+    for e in range(0, mesh.nelems2d, 97):
+        t = mesh.tri[e]
+        for i in range(3):
+            for k in range(3):
+                var = s[3 * t[i] + k]
+                eps = max(abs(var) * (1 + 1e-4) - abs(var), 1e-4)
+                for pe in (eps, -eps):
+                    assert lib.adhk_elem_residual_one(sm.h, 2, e, pe, i, k, -1, L.dptr(ra)) == 0
+                    assert lib.adhk_elem_residual_one(sm.h, 3, e, pe, i, k, -1, L.dptr(rb)) == 0
+                    worst = max(worst, np.abs(ra - rb).max() / np.abs(ra).max())
+    assert worst <= 1e-13
