@@ -19,8 +19,12 @@
  *   assembly lists        cptr [nnzb+1], contrib [npairs]: contributions to each
  *                         block in element order (2-D first, then 1-D); cpos [npairs]: the list
  *                         position of each elemental block (code 9e + 3a + b, 9E + 4s + 2a + b)
- *   elemental buffers     erhs [E*9 + S*6]; emat [npairs*9]: the 3x3 block of list position j at
- *                         9j (entry 3r + c), so a block's contributions are one contiguous run
+ *   residual lists        rptr [N+1], rpos [3E + 2S]: element node (e, a) -> 3e + a, segment node
+ *                         (s, a) -> 3E + 2s + a; its residual slot rpos[...] lies in node n's run
+ *                         rptr[n] .. rptr[n+1]-1, which lists n's elements in element order
+ *   elemental buffers     erhs [3 (3E + 2S)]: the three residual entries of element node slot j at
+ *                         3j, so a node's contributions are one contiguous run; emat [npairs*9]:
+ *                         the 3x3 block of list position j at 9j (entry 3r + c), likewise
  *   wet/dry queue         wdq [E+1]: count, then the partially dry elements of the last launch
  *   instances             node_inst [N], elem_inst [E], seg_inst [S]; 256-node blocks blk_node/blk_inst;
  *                         per-instance scalars sc [ninst] and work mask [ninst]
@@ -99,6 +103,7 @@ struct adhk_sw2 {
     /* elemental buffers */
     double *erhs, *emat;
     int *cpos; /* [npairs]: contribution-list position of each elemental block */
+    int *rptr, *rpos; /* node runs of the elemental residuals: [N+1], [3E + 2S] */
     float *emat32; /* FP32 elemental Jacobians of the mixed-precision Newton iteration */
     /* Krylov vectors (FP64 and FP32 sets); p and Ap are double-buffered: one BiCGStab kernel
      * reads the previous direction while it writes the next */

@@ -75,7 +75,7 @@ static void ctx_free(adhk_ctx *c)
                     c->bc_mask, c->wd_flag, c->wd_tmp, c->wdq, c->brow, c->bcol, c->diag_blk, c->cptr, c->contrib, c->bvals,
                     c->minv, c->bvals32, c->minv32, c->erhs, c->emat, c->r, c->pv, c->Ap, c->s, c->As, c->q, c->x0,
                     c->pv2, c->Ap2, c->xk, c->r32, c->p32, c->Ap32, c->s32, c->As32, c->q32, c->xk32, c->p32b,
-                    c->Ap32b, c->emat32, c->partials, c->counter, c->cpos, c->sc, c->flags, c->slice_ptr, c->scol, c->sell2blk,
+                    c->Ap32b, c->emat32, c->partials, c->counter, c->cpos, c->rptr, c->rpos, c->sc, c->flags, c->slice_ptr, c->scol, c->sell2blk,
                     c->blk2sell, c->diag_slot, c->diag_id, c->node_inst, c->elem_inst, c->seg_inst, c->blk_node,
                     c->blk_inst, c->inst_blk, c->work};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++)
@@ -713,15 +713,32 @@ fail:
 
 int adhk_elem_residuals(adhk_sw2 *c, double *elem_rhs)
 {
+    size_t nr;
+    double *tmp = NULL;
+    int *rpos = NULL;
     if (!c) return -1;
+    nr = 3 * (size_t)c->E + 2 * (size_t)c->S;
     HC(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
     LC(adhk_launch_elem_resid(c));
     if (check_partial(c)) return -1;
-    if (elem_rhs)
-        HC(hipMemcpy(elem_rhs, c->erhs, ((size_t)c->E * 9 + (size_t)c->S * 6) * sizeof(double),
-                     hipMemcpyDeviceToHost));
+    if (!elem_rhs) return 0;
+    /* the device holds node runs (slot rpos[element node]); the caller gets element order */
+    tmp = malloc(3 * nr * sizeof(double));
+    rpos = malloc(nr * sizeof(int));
+    if (!tmp || !rpos) {
+        adhk_set_error("adhk_elem_residuals: out of host memory");
+        goto fail;
+    }
+    HC(hipMemcpy(tmp, c->erhs, 3 * nr * sizeof(double), hipMemcpyDeviceToHost));
+    HC(hipMemcpy(rpos, c->rpos, nr * sizeof(int), hipMemcpyDeviceToHost));
+    for (size_t t = 0; t < nr; t++)
+        for (int k = 0; k < 3; k++) elem_rhs[3 * t + k] = tmp[3 * (size_t)rpos[t] + k];
+    free(tmp);
+    free(rpos);
     return 0;
 fail:
+    free(tmp);
+    free(rpos);
     return -1;
 }
 

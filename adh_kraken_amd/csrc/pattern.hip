@@ -10,7 +10,8 @@
  * node (about 20 on the meshes here), so no global sort is needed.  The sorted runs give
  * (1) the sorted, duplicate-free node adjacency of each row -- the reference's per-row
  * sort + unique, at node-block granularity -- and (2) for every 3x3 block the list of its
- * elemental contributions in element order, which drives the atomic-free assembly.  The
+ * elemental contributions in element order, which drives the atomic-free assembly; the
+ * diagonal blocks' lists give (3) every node's run of elemental residual slots.  The
  * scalar CSR of the reference is the node-block pattern expanded by
  * dof(node, var) = 3*node + var; cols stay ascending because each block row lists node
  * columns in ascending order.  The integer counts use atomics; the order of the fill they
@@ -203,6 +204,35 @@ __global__ void k_cpos(long n, const int *contrib, int *cpos)
     if (i < n) cpos[contrib[i]] = (int)i;
 }
 
+/* dd[n] = contributions to node n's diagonal block = the elements (2-D and 1-D) that hold n */
+__global__ void k_diag_count(int N, const int *diag_blk, const int *cptr, int *dd)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n > N) return;
+    const int b = (n < N) ? diag_blk[n] : -1;
+    dd[n] = (b >= 0) ? cptr[b + 1] - cptr[b] : 0;
+}
+
+/* rpos of element node (e, a) / (s, a): its rank among node n's diagonal contributions (which run in
+ * element order) placed in n's residual run */
+__global__ void k_rpos(int E, int S, const int *tri, const int *seg, const int *diag_blk, const int *cptr,
+                       const int *cpos, const int *rptr, int *rpos)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3L * E + 2L * S) return;
+    int n, code;
+    if (t < 3L * E) {
+        const int e = (int)(t / 3), a = (int)(t % 3);
+        n = tri[a * (long)E + e];
+        code = 9 * e + 4 * a;
+    } else {
+        const int u = (int)(t - 3L * E), s = u / 2, a = u % 2;
+        n = seg[2 * s + a];
+        code = 9 * E + 4 * s + 3 * a;
+    }
+    rpos[t] = rptr[n] + (cpos[code] - cptr[diag_blk[n]]);
+}
+
 /* SELL-64: slice width = widest block row among the slice's 64 nodes */
 __global__ void k_slice_width(int N, int nslice, const int *brow, int *sw)
 {
@@ -328,6 +358,26 @@ extern "C" int adhk_build_pattern(adhk_ctx *c)
     CHECK(hipMalloc((void **)&c->cpos, (size_t)np * sizeof(int)));
     hipLaunchKernelGGL(k_cpos, gp, tb, 0, c->stream, np, c->contrib, c->cpos);
     CHECK(hipGetLastError());
+    {
+        /* node runs of the elemental residuals: one slot per element node, a node's slots in the
+         * order of its diagonal block's contributions */
+        const long nr = 3L * c->E + 2L * c->S;
+        int rtot = 0;
+        CHECK(hipMalloc((void **)&c->rptr, ((size_t)c->N + 1) * sizeof(int)));
+        hipLaunchKernelGGL(k_diag_count, dim3((unsigned)((c->N + 1 + 255) / 256)), tb, 0, c->stream, c->N, c->diag_blk,
+                           c->cptr, deg);
+        CHECK(hipGetLastError());
+        if (exclusive_scan(c->stream, (long)c->N + 1, deg, c->rptr, &rtot)) goto fail;
+        if (rtot != nr) {
+            adhk_set_error("adhk_build_pattern: %d diagonal contributions for %ld element nodes (an element that "
+                           "repeats a node?)", rtot, nr);
+            goto fail;
+        }
+        CHECK(hipMalloc((void **)&c->rpos, (size_t)nr * sizeof(int)));
+        hipLaunchKernelGGL(k_rpos, dim3((unsigned)((nr + 255) / 256)), tb, 0, c->stream, c->E, c->S, c->tri, c->seg,
+                           c->diag_blk, c->cptr, c->cpos, c->rptr, c->rpos);
+        CHECK(hipGetLastError());
+    }
     CHECK(hipStreamSynchronize(c->stream));
     (void)hipFree(key);
     (void)hipFree(cnt);

@@ -128,8 +128,21 @@ __device__ __forceinline__ void wdq_push(int *wdq, int e)
     wdq[1 + slot] = e;
 }
 
-__global__ __launch_bounds__(256) void k_elem_resid(MeshView m, Sw2Par P, double *__restrict__ erhs, int *flags,
-                                                    int *wdq)
+/* the nine residual entries of triangle e into its three node slots (3 rpos[3e + a]) */
+__device__ __forceinline__ void store_tri_resid(double *__restrict__ erhs, const int *__restrict__ rpos, int e,
+                                                const double r[9])
+{
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double *d = erhs + 3 * (size_t)rpos[3 * (size_t)e + a];
+        d[0] = r[3 * a];
+        d[1] = r[3 * a + 1];
+        d[2] = r[3 * a + 2];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_elem_resid(MeshView m, Sw2Par P, double *__restrict__ erhs,
+                                                    const int *__restrict__ rpos, int *flags, int *wdq)
 {
     int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= m.E || !m.work[m.elem_inst[e]]) return;
@@ -142,8 +155,7 @@ __global__ __launch_bounds__(256) void k_elem_resid(MeshView m, Sw2Par P, double
     double r[9];
     int partial = sw2_tri_resid(in, P, 0, -1, 0.0, r);
     if (partial) flags[0] = 1; /* cannot happen after tri_partial; kept as a guard */
-#pragma unroll
-    for (int k = 0; k < 9; k++) erhs[9 * (size_t)e + k] = r[k];
+    store_tri_resid(erhs, rpos, e, r);
 }
 
 __global__ void k_wdq_reset(int *wdq)
@@ -153,7 +165,7 @@ __global__ void k_wdq_reset(int *wdq)
 
 /* queued (partially dry) elements: the general wet/dry residual, grid-stride over the queue */
 __global__ __launch_bounds__(128) void k_elem_resid_wd(MeshView m, Sw2Par P, const int *__restrict__ wdq,
-                                                       double *__restrict__ erhs)
+                                                       double *__restrict__ erhs, const int *__restrict__ rpos)
 {
     const int n = wdq[0];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -162,7 +174,7 @@ __global__ __launch_bounds__(128) void k_elem_resid_wd(MeshView m, Sw2Par P, con
         gather_tri(m, e, true, in);
         double r[9];
         sw2_tri_resid_general(in, P, 0, -1, 0.0, r);
-        for (int k = 0; k < 9; k++) erhs[9 * (size_t)e + k] = r[k];
+        store_tri_resid(erhs, rpos, e, r);
     }
 }
 
@@ -173,7 +185,8 @@ struct SegView {
     int S;
 };
 
-__global__ void k_seg_resid(SegView s, Sw2Par P, double *__restrict__ erhs)
+/* rpos_seg: the segments' part of rpos (2 slots per segment) */
+__global__ void k_seg_resid(SegView s, Sw2Par P, double *__restrict__ erhs, const int *__restrict__ rpos_seg)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= s.S || !s.work[s.seg_inst[e]]) return;
@@ -187,7 +200,12 @@ __global__ void k_seg_resid(SegView s, Sw2Par P, double *__restrict__ erhs)
     double er[6];
     sw2_seg_resid(s.kind[e], s.val[2 * e], s.val[2 * e + 1], s.nrml[2 * e], s.nrml[2 * e + 1], s.sdjac[e], h, u, v,
                   P, 0, -1, 0.0, er);
-    for (int k = 0; k < 6; k++) erhs[6 * (size_t)e + k] = er[k];
+    for (int a = 0; a < 2; a++) {
+        double *d = erhs + 3 * (size_t)rpos_seg[2 * (size_t)e + a];
+        d[0] = er[3 * a];
+        d[1] = er[3 * a + 1];
+        d[2] = er[3 * a + 2];
+    }
 }
 
 static SegView make_segview(const adhk_ctx *c)
@@ -220,14 +238,14 @@ extern "C" int adhk_launch_elem_resid(adhk_ctx *c)
         hipLaunchKernelGGL(k_wdq_reset, dim3(1), dim3(64), 0, c->stream, c->wdq);
         const int tpb = c->tune.elem_tpb;
         hipLaunchKernelGGL(k_elem_resid, dim3((c->E + tpb - 1) / tpb), dim3(tpb), 0, c->stream, make_view(c), P, c->erhs,
-                           c->flags, c->wdq);
+                           c->rpos, c->flags, c->wdq);
         hipLaunchKernelGGL(k_elem_resid_wd, dim3(wd_grid(c->E)), dim3(128), 0, c->stream, make_view(c), P, c->wdq,
-                           c->erhs);
+                           c->erhs, c->rpos);
         CHECK_LAUNCH();
     }
     if (c->S > 0) {
         hipLaunchKernelGGL(k_seg_resid, dim3((c->S + 127) / 128), dim3(128), 0, c->stream, make_segview(c), P,
-                           c->erhs + 9 * (size_t)c->E);
+                           c->erhs, c->rpos + 3 * (size_t)c->E);
         CHECK_LAUNCH();
     }
     adhk_kt_end(c);
@@ -238,7 +256,8 @@ extern "C" int adhk_launch_elem_resid(adhk_ctx *c)
  * (pde[k].resid: sw2_body_resid.c:44, sw2_bc_h.c:57, sw2_bc_vel.c:58), one thread.  which 0: triangle
  * (the general wet/dry form; wd >= 0 replaces the element's wet/dry flag), 1: segment; for checking the
  * kernels' own forms of an all-wet triangle, 2: the residual kernel's, 3: the element Jacobian's (FD form
- * with its shared values).  The result goes to the element's own slots of erhs. */
+ * with its shared values).  The result goes to the first 9 (triangle) or 6 (segment) entries of erhs, which
+ * the next residual evaluation overwrites. */
 __global__ void k_elem_resid_one(MeshView m, SegView s, Sw2Par P, int which, int e, int pnode, int pvar, double pe,
                                  int wd, double *__restrict__ erhs)
 {
@@ -257,7 +276,7 @@ __global__ void k_elem_resid_one(MeshView m, SegView s, Sw2Par P, int which, int
             fd_share(in, P, fs);
             if (sw2_tri_resid<true>(in, P, pnode, pvar, pe, r, &fs)) sw2_tri_resid_general(in, P, pnode, pvar, pe, r);
         }
-        for (int k = 0; k < 9; k++) erhs[9 * (size_t)e + k] = r[k];
+        for (int k = 0; k < 9; k++) erhs[k] = r[k];
         return;
     }
     double h[2], u[2], v[2];
@@ -270,7 +289,7 @@ __global__ void k_elem_resid_one(MeshView m, SegView s, Sw2Par P, int which, int
     double er[6];
     sw2_seg_resid(s.kind[e], s.val[2 * e], s.val[2 * e + 1], s.nrml[2 * e], s.nrml[2 * e + 1], s.sdjac[e], h, u, v, P,
                   pnode, pvar, pe, er);
-    for (int k = 0; k < 6; k++) erhs[9 * (size_t)m.E + 6 * (size_t)e + k] = er[k];
+    for (int k = 0; k < 6; k++) erhs[k] = er[k];
 }
 
 extern "C" int adhk_elem_residual_one(adhk_sw2 *c, int which, int ie, double pe, int pnode, int pvar, int wd_flag,
@@ -286,8 +305,7 @@ extern "C" int adhk_elem_residual_one(adhk_sw2 *c, int which, int ie, double pe,
     hipLaunchKernelGGL(k_elem_resid_one, dim3(1), dim3(64), 0, c->stream, make_view(c), make_segview(c), make_par(c),
                        which, ie, pnode, pvar, pvar >= 0 ? pe : 0.0, wd_flag, c->erhs);
     CHECK_LAUNCH();
-    const size_t off = seg ? 9 * (size_t)c->E + 6 * (size_t)ie : 9 * (size_t)ie;
-    hipError_t e = hipMemcpyAsync(out, c->erhs + off, (seg ? 6 : 9) * sizeof(double), hipMemcpyDeviceToHost,
+    hipError_t e = hipMemcpyAsync(out, c->erhs, (seg ? 6 : 9) * sizeof(double), hipMemcpyDeviceToHost,
                                   c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
@@ -297,28 +315,17 @@ extern "C" int adhk_elem_residual_one(adhk_sw2 *c, int which, int ie, double pe,
     return 0;
 }
 
-/* residual[3n+k] = sum over the diagonal block's contributions (element order) of -erhs */
-__global__ __launch_bounds__(256) void k_resid_gather(int N, int E, const int *__restrict__ diag_blk,
-                                                      const int *__restrict__ cptr, const int *__restrict__ contrib,
+/* residual[3n+k] = -(sum of node n's elemental residual slots, in element order): the slots of a node
+ * are one contiguous run, so consecutive nodes read consecutive memory */
+__global__ __launch_bounds__(256) void k_resid_gather(int N, const int *__restrict__ rptr,
                                                       const double *__restrict__ erhs, double *__restrict__ residual)
 {
     int n = xcd_block() * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    int b = diag_blk[n];
     double r0 = 0.0, r1 = 0.0, r2 = 0.0;
-    const long segbase = 9L * E;
-    const int j0 = (b >= 0) ? cptr[b] : 0, j1 = (b >= 0) ? cptr[b + 1] : 0; /* a node in no element has no row */
+    const int j0 = rptr[n], j1 = rptr[n + 1]; /* empty for a node in no element */
     for (int j = j0; j < j1; j++) {
-        int cde = contrib[j];
-        const double *er;
-        if (cde < 9 * E) {
-            int e = cde / 9, a = (cde % 9) / 3;
-            er = erhs + 9L * e + 3 * a;
-        } else {
-            int t = cde - 9 * E;
-            int s = t / 4, a = (t % 4) / 2;
-            er = erhs + segbase + 6L * s + 3 * a;
-        }
+        const double *er = erhs + 3 * (size_t)j;
         r0 += -1.0 * er[0];
         r1 += -1.0 * er[1];
         r2 += -1.0 * er[2];
@@ -331,8 +338,8 @@ __global__ __launch_bounds__(256) void k_resid_gather(int N, int E, const int *_
 extern "C" int adhk_launch_resid_gather(adhk_ctx *c)
 {
     adhk_kt_begin(c, ADHK_KT_RESID_GATHER);
-    hipLaunchKernelGGL(k_resid_gather, dim3((c->N + 255) / 256), dim3(256), 0, c->stream, c->N, c->E, c->diag_blk,
-                       c->cptr, c->contrib, c->erhs, c->residual);
+    hipLaunchKernelGGL(k_resid_gather, dim3((c->N + 255) / 256), dim3(256), 0, c->stream, c->N, c->rptr, c->erhs,
+                       c->residual);
     CHECK_LAUNCH();
     adhk_kt_end(c);
     return 0;
