@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libadhk.so")
 BC_DEPTH, BC_FLUX, BC_VEL, BC_DEPTH_NODAL = 0, 1, 2, 3
 FRIC_CONST, FRIC_MANNING = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
+PC_BJ, PC_SGS = 0, 1
 KT_NAMES = ["spmv_p", "spmv_s", "krylov_update", "elem_jac", "jac_gather", "elem_resid", "resid_gather", "prep"]
 
 
@@ -26,7 +27,7 @@ class Solver(C.Structure):
     _fields_ = [("tol_nonlin", C.c_double), ("inc_nonlin", C.c_double), ("max_nonlin_it", C.c_int),
                 ("max_linesearch_cuts", C.c_int), ("lin_tol", C.c_double), ("lin_max_it", C.c_int),
                 ("precision", C.c_int), ("lin_check_every", C.c_int), ("mixed_lin_tol", C.c_double),
-                ("warm_gate", C.c_int), ("substep_depth", C.c_int)]
+                ("warm_gate", C.c_int), ("substep_depth", C.c_int), ("precond", C.c_int)]
 
 
 class NewtonStats(C.Structure):
@@ -106,6 +107,7 @@ SIGNATURES = [
     ("adhk_sw2_get_tuning", C.c_int, [P, C.POINTER(Tuning)]),
     ("adhk_sw2_set_tuning", C.c_int, [P, C.POINTER(Tuning)]),
     ("adhk_sw2_autotune", C.c_int, [P, C.POINTER(Solver), C.c_int, C.POINTER(Tuning), dp]),
+    ("adhk_sw2_colouring", C.c_int, [P, ip, ip, ip]),
     ("adhk_sw2_init", C.c_int, [P, ip]),
     ("adhk_assemble_residual", C.c_int, [P, dp]),
     ("adhk_elem_residuals", C.c_int, [P, dp]),

@@ -46,8 +46,33 @@ typedef struct {
     double rnorm, bnorm, tol;
     int it, max_it, done, pad0;
     double rmax, rl2sum, incmax;
-    int nanflag, pad1;
+    int nanflag, sgs_bad; /* sgs_bad: a folded diagonal block is not the identity (SGS falls back) */
 } adhk_scalars;
+
+/* Multicolour SGS layout (one mesh alone): rows permuted by colour (natural order within a colour), each
+ * colour's run padded to a multiple of 256 rows, so a 256-row reduction block and a 64-row slice hold one
+ * colour; row Np is a zero row that padding slots point at.  The folded matrix's off-diagonal blocks
+ * split into U (neighbour of a higher colour) and L (lower), each its own SELL-64 over the permuted rows
+ * (blocks of a row in ascending natural column order); src maps a slot to the natural SELL position whose
+ * value it takes (-1: padding). */
+#define ADHK_SGS_MAXC 64
+typedef struct {
+    int *ptr, *col, *src; /* device [nslice + 1], [nslot * 64], [nslot * 64] */
+    int nslot;
+    double *val;
+    float *val32;
+} adhk_sgs_part;
+typedef struct {
+    int ready, nc, Np, nblk;
+    int cs[ADHK_SGS_MAXC + 1]; /* host: permuted row range of colour c is [cs[c], cs[c+1]) */
+    int *colour;               /* host [N] */
+    int *prow;                 /* device [Np]: natural node of a permuted row, -1 for padding */
+    adhk_sgs_part U, L;
+    int *inst_blk;             /* device {0, nblk} */
+    double *partials;          /* device [8 * nblk] */
+    double *v64;               /* device 10 vectors of 3 (Np + 1) doubles: r p Ap s As q xt t u bt */
+    float *v32;                /* the same in FP32 */
+} adhk_sgs;
 
 /* contribution encoding: tri: (e*3 + a)*3 + b  (a,b local nodes, row a, col b)
  *                        seg: CONTRIB_SEG_BASE + (s*2 + a)*2 + b                */
@@ -73,6 +98,7 @@ typedef struct {
     adhk_tuning tune;
     double tol;
     int max_it;
+    int pc; /* preconditioner (ADHK_PC_*) */
 } adhk_stage_graph;
 
 struct adhk_sw2 {
@@ -113,6 +139,9 @@ struct adhk_sw2 {
     double *partials;
     unsigned int *counter;
     adhk_scalars *sc; /* [ninst]: BiCGStab scalars and Newton norms of each instance */
+    /* preconditioner of the current solve (ADHK_PC_*) and the multicolour SGS layout */
+    int pc;
+    adhk_sgs sgs;
     /* instances: an ensemble of independent meshes solved as one block-diagonal system (1 = one mesh) */
     int ninst;
     int *h_inst_node, *h_inst_elem, *h_inst_seg, *h_inst_blk; /* host [ninst + 1] offsets */
@@ -128,6 +157,7 @@ struct adhk_sw2 {
     hipGraphExec_t gchunk[2]; /* captured BiCGStab chunk (FP64, mixed) and its length */
     int gchunk_len[2];
     adhk_tuning gchunk_tune[2]; /* the schedule the chunk was captured with */
+    int gchunk_pc[2];           /* and the preconditioner */
     /* captured Newton stages (per precision): the launches between two host decisions, with the
      * by-value kernel arguments they were captured with */
     adhk_stage_graph gstage[2][3];
@@ -178,6 +208,10 @@ int adhk_launch_krylov_first(adhk_ctx *c, int mixed);
 int adhk_launch_krylov_iter(adhk_ctx *c, int mixed, int cur);
 int adhk_launch_krylov_last(adhk_ctx *c, int mixed, int cur);
 int adhk_launch_krylov_finish(adhk_ctx *c, int mixed);
+int adhk_sgs_setup(adhk_ctx *c);              /* colouring and the permuted L/U layout (sgs_setup.c) */
+int adhk_sgs_ensure(adhk_ctx *c, int mixed);  /* the value arrays and vectors of a precision */
+void adhk_sgs_free(adhk_ctx *c);
+int adhk_launch_sgs_gather(adhk_ctx *c, int mixed); /* folded matrix -> L/U values, diagonal check */
 int adhk_launch_increment(adhk_ctx *c);
 int adhk_launch_linesearch(adhk_ctx *c);
 int adhk_launch_init_system(adhk_ctx *c);

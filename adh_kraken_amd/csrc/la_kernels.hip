@@ -710,6 +710,7 @@ extern "C" int adhk_launch_prep(adhk_ctx *c, int mixed)
                            c->bc_mask, c->bvals, c->scale, c->minv, c->diag_id);
     }
     CHECK_LAUNCH();
+    if (c->pc == ADHK_PC_SGS && adhk_launch_sgs_gather(c, mixed)) return -1;
     adhk_kt_end(c);
     return 0;
 }
@@ -1208,13 +1209,19 @@ template <typename T> static int krylov_finish(adhk_ctx *c, KV<T> k)
     return 0;
 }
 
+/* the multicolour SGS solve (defined below) */
+static int sgs_dispatch(adhk_ctx *c, int mixed, int what, double tol, int max_it);
+enum { SGD_INIT, SGD_FIRST, SGD_ITER, SGD_LAST, SGD_FINISH };
+
 extern "C" int adhk_launch_krylov_init(adhk_ctx *c, int mixed, double tol, int max_it)
 {
+    if (c->pc == ADHK_PC_SGS) return sgs_dispatch(c, mixed, SGD_INIT, tol, max_it);
     return mixed ? krylov_init<float>(c, kv32(c), tol, max_it) : krylov_init<double>(c, kv64(c), tol, max_it);
 }
 /* first iteration: A<first> then S; the current direction buffer is then 1 */
 extern "C" int adhk_launch_krylov_first(adhk_ctx *c, int mixed)
 {
+    if (c->pc == ADHK_PC_SGS) return sgs_dispatch(c, mixed, SGD_FIRST, 0.0, 0);
     if (mixed) {
         if (launch_a<float, true>(c, kv32(c), 0)) return -1;
         return launch_s<float>(c, kv32(c), 1);
@@ -1225,6 +1232,7 @@ extern "C" int adhk_launch_krylov_first(adhk_ctx *c, int mixed)
 /* one more iteration from direction buffer cur: A then S; the current buffer becomes 1 - cur */
 extern "C" int adhk_launch_krylov_iter(adhk_ctx *c, int mixed, int cur)
 {
+    if (c->pc == ADHK_PC_SGS) return sgs_dispatch(c, mixed, SGD_ITER, 0.0, 0);
     if (mixed) {
         if (launch_a<float, false>(c, kv32(c), cur)) return -1;
         return launch_s<float>(c, kv32(c), 1 - cur);
@@ -1235,11 +1243,547 @@ extern "C" int adhk_launch_krylov_iter(adhk_ctx *c, int mixed, int cur)
 /* the A kernel that completes the last iteration (x, r updates and the stop test) */
 extern "C" int adhk_launch_krylov_last(adhk_ctx *c, int mixed, int cur)
 {
+    if (c->pc == ADHK_PC_SGS) return sgs_dispatch(c, mixed, SGD_LAST, 0.0, 0);
     return mixed ? launch_a<float, false>(c, kv32(c), cur) : launch_a<double, false>(c, kv64(c), cur);
 }
 extern "C" int adhk_launch_krylov_finish(adhk_ctx *c, int mixed)
 {
+    if (c->pc == ADHK_PC_SGS) return sgs_dispatch(c, mixed, SGD_FINISH, 0.0, 0);
     return mixed ? krylov_finish<float>(c, kv32(c)) : krylov_finish<double>(c, kv64(c));
+}
+
+/* ---------------- multicolour symmetric Gauss-Seidel, split form ----------------
+ * No counterpart in the reference (its preconditioner is an exact LU, bcgstab_solver.c:371-424).  With
+ * F = I + L + U the block-Jacobi-folded matrix in the colour order of sgs_setup.c, BiCGStab runs on
+ * (I+L)^-1 F (I+U)^-1 with right-hand side (I+L)^-1 D^-1 b, and one product z = (I+L)^-1 F (I+U)^-1 v is,
+ * by Eisenstat's identity F = (I+L) + (I+U) - I,
+ *     t = (I+U)^-1 v      backward, colours last to first:  t_i = v_i - sum_U F_ij t_j
+ *     u = (I+L)^-1 (v - t) forward, colours first to last:  u_i = (v_i - t_i) - sum_L F_ij u_j,   z = t + u,
+ * one pass over F in 2 nc - 2 colour phases: B(nc-2) .. B(1); B(0) fused with F(0) (colour 0 has no L
+ * blocks); F(1) .. F(nc-2); the top colour (no U blocks, so t = v there and is never stored).  A row's
+ * first phase forms the product's input v (the A step: the previous iteration's x and r updates and
+ * p = (p - omega Ap) beta + r; the S step: s = r - alpha Ap); the top colour's, whose own phase comes last,
+ * its lower neighbours evaluate on the fly, as the block-Jacobi kernels do for every column.  Vectors are
+ * updated in place: a row's old p, Ap and s are read only by its own first phase and, for the top colour,
+ * by neighbours in phases before the top one.  Scalars, finalizers and the partial-sum layout are those of
+ * the block-Jacobi solver, over the permuted rows' 256-row blocks. */
+struct SPart {
+    const int *__restrict__ ptr;
+    const int *__restrict__ col;
+};
+template <typename T> struct SgVec {
+    T *r, *p, *Ap, *s, *As, *q, *xt, *t, *u, *bt;
+};
+template <typename T> static SgVec<T> sg_vec(T *base, int Np)
+{
+    const size_t n = (size_t)3 * ((size_t)Np + 1);
+    SgVec<T> v;
+    v.r = base;
+    v.p = base + n;
+    v.Ap = base + 2 * n;
+    v.s = base + 3 * n;
+    v.As = base + 4 * n;
+    v.q = base + 5 * n;
+    v.xt = base + 6 * n;
+    v.t = base + 7 * n;
+    v.u = base + 8 * n;
+    v.bt = base + 9 * n;
+    return v;
+}
+
+/* neighbour values of a backward phase: the stored t of a middle colour, the product's input for the top */
+template <typename T, typename Col> struct GetB {
+    const T *t;
+    Col col;
+    int top0;
+    __device__ __forceinline__ void get(int m, double &a, double &b, double &c) const
+    {
+        if (m >= top0) {
+            col.get(m, a, b, c);
+        } else {
+            a = (double)t[3 * m];
+            b = (double)t[3 * m + 1];
+            c = (double)t[3 * m + 2];
+        }
+    }
+};
+
+/* a = sum over the row's slots of F_ij y_j, slot order (ascending natural column); the 64 rows of a wave
+ * are one slice, so the slot range is wave-uniform */
+template <typename T, typename G>
+__device__ __forceinline__ void part_sum(int i, const SPart P, const T *__restrict__ val, const G &g, double &a0,
+                                         double &a1, double &a2)
+{
+    const int lane = i & 63;
+    const int sl = __builtin_amdgcn_readfirstlane(i >> 6);
+    const int c0 = P.ptr[sl], c1 = P.ptr[sl + 1];
+    a0 = 0.0;
+    a1 = 0.0;
+    a2 = 0.0;
+    for (int cs = c0; cs < c1; cs += 8) {
+        int m[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) m[u] = (cs + u < c1) ? P.col[(cs + u) * 64 + lane] : 0;
+        double xv[8][3];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (cs + u < c1) {
+                g.get(m[u], xv[u][0], xv[u][1], xv[u][2]);
+            } else {
+                xv[u][0] = 0.0;
+                xv[u][1] = 0.0;
+                xv[u][2] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (cs + u >= c1) break;
+            const T *a = val + (size_t)(cs + u) * 9 * 64 + lane;
+            const double x0 = xv[u][0], x1 = xv[u][1], x2 = xv[u][2];
+            a0 = __fma_rn((double)a[0], x0, a0);
+            a0 = __fma_rn((double)a[64], x1, a0);
+            a0 = __fma_rn((double)a[128], x2, a0);
+            a1 = __fma_rn((double)a[192], x0, a1);
+            a1 = __fma_rn((double)a[256], x1, a1);
+            a1 = __fma_rn((double)a[320], x2, a1);
+            a2 = __fma_rn((double)a[384], x0, a2);
+            a2 = __fma_rn((double)a[448], x1, a2);
+            a2 = __fma_rn((double)a[512], x2, a2);
+        }
+    }
+}
+
+/* the product: initial residual, A step, S step, A step whose direction k_sgs_upd has stored */
+enum { SG_INIT = 0, SG_A = 1, SG_S = 2, SG_AP = 3 };
+enum { PH_B = 0, PH_B0 = 1, PH_F = 2, PH_TOP = 3 }; /* the phase kinds above */
+
+/* the product's input v, evaluated at any row from the previous state */
+template <typename T, int OP, bool FIRST> struct SgCol;
+template <typename T> struct SgCol<T, SG_INIT, false> {
+    using type = ColPlain<T>;
+    static __device__ __forceinline__ type make(const SgVec<T> &V, double, double, double) { return type{V.xt}; }
+};
+template <typename T> struct SgCol<T, SG_A, true> { /* p = r at the start */
+    using type = ColPlain<T>;
+    static __device__ __forceinline__ type make(const SgVec<T> &V, double, double, double) { return type{V.r}; }
+};
+template <typename T> struct SgCol<T, SG_A, false> {
+    using type = ColP<T>;
+    static __device__ __forceinline__ type make(const SgVec<T> &V, double, double omega, double beta)
+    {
+        return type{V.s, V.As, V.p, V.Ap, omega, beta};
+    }
+};
+template <typename T> struct SgCol<T, SG_AP, false> {
+    using type = ColPlain<T>;
+    static __device__ __forceinline__ type make(const SgVec<T> &V, double, double, double) { return type{V.p}; }
+};
+template <typename T> struct SgCol<T, SG_S, false> {
+    using type = ColS<T>;
+    static __device__ __forceinline__ type make(const SgVec<T> &V, double alpha, double, double)
+    {
+        return type{V.r, V.Ap, alpha};
+    }
+};
+
+/* one colour phase over the permuted rows [r0, r0 + 256 gridDim.x) */
+/* the 256-row block of a phase's workgroup: with xcd_map, XCD k (workgroups w = k mod 8) takes the k-th
+ * contiguous eighth of the colour's blocks, so neighbour gathers stay in that XCD's L2; -1: none */
+__device__ __forceinline__ int sgs_block(int nb, int xcd_map)
+{
+    const int G = gridDim.x, w = blockIdx.x;
+    const int lb = xcd_map ? (w % NXCD) * (G / NXCD) + w / NXCD : w;
+    return lb < nb ? lb : -1;
+}
+
+template <typename T, int OP, int PH, bool FIRST>
+__global__ __launch_bounds__(TPB) void k_sgs_phase(int r0, int nb, int xcd_map, int top0, SPart U, SPart L,
+                                                   const T *__restrict__ uval, const T *__restrict__ lval, SgVec<T> V,
+                                                   double *partials, int nblk, const adhk_scalars *__restrict__ sc)
+{
+    double alpha = 0.0, omega = 0.0, beta = 0.0;
+    if (OP != SG_INIT) {
+        if (sc->done) return;
+        alpha = sc->alpha;
+        omega = sc->omega;
+        beta = sc->beta;
+    }
+    const int lb = sgs_block(nb, xcd_map);
+    if (lb < 0) return;
+    const int i = r0 + lb * TPB + threadIdx.x;
+    const int j = i / TPB; /* reduction block: r0 is a multiple of 256 */
+    const typename SgCol<T, OP, FIRST>::type col = SgCol<T, OP, FIRST>::make(V, alpha, omega, beta);
+    double pa1[1] = {0.0}, pa2[2] = {0.0, 0.0}, pa3[3] = {0.0, 0.0, 0.0}, pb[1] = {0.0};
+    double v[3], t[3];
+    /* the product's input at the row */
+    if constexpr (OP == SG_INIT) {
+        for (int k = 0; k < 3; k++) v[k] = (double)V.xt[3 * i + k];
+    } else if constexpr (OP == SG_AP || PH == PH_F) { /* stored by k_sgs_upd or the row's first phase */
+        const T *vs = (OP == SG_S) ? V.s : V.p;
+        for (int k = 0; k < 3; k++) v[k] = (double)vs[3 * i + k];
+    } else if constexpr (OP == SG_A && FIRST) {
+        for (int k = 0; k < 3; k++) {
+            const T rr = V.r[3 * i + k];
+            V.p[3 * i + k] = rr;
+            v[k] = (double)rr;
+        }
+    } else if constexpr (OP == SG_A) {
+        for (int k = 0; k < 3; k++) {
+            const int ii = 3 * i + k;
+            T rr, pp;
+            col.one(ii, rr, pp); /* reads the old p and Ap */
+            upd_x(V.xt + ii, (double)V.s[ii], (double)V.p[ii], alpha, omega);
+            V.r[ii] = rr;
+            V.p[ii] = pp;
+            v[k] = (double)pp;
+            pa1[0] = fmax(pa1[0], fabs((double)rr));
+        }
+    } else { /* SG_S */
+        for (int k = 0; k < 3; k++) {
+            const T ss = col.one(3 * i + k);
+            V.s[3 * i + k] = ss;
+            v[k] = (double)ss;
+        }
+    }
+    /* backward: t = v - sum_U F t */
+    if constexpr (PH == PH_B || PH == PH_B0) {
+        double a[3];
+        part_sum<T>(i, U, uval, GetB<T, typename SgCol<T, OP, FIRST>::type>{V.t, col, top0}, a[0], a[1], a[2]);
+        for (int k = 0; k < 3; k++) {
+            const T tk = (T)(v[k] - a[k]);
+            V.t[3 * i + k] = tk;
+            t[k] = (double)tk;
+        }
+    } else if constexpr (PH == PH_F) {
+        for (int k = 0; k < 3; k++) t[k] = (double)V.t[3 * i + k];
+    } else {
+        for (int k = 0; k < 3; k++) t[k] = v[k];
+    }
+    if constexpr (PH != PH_B) {
+        /* forward: u = (v - t) - sum_L F u, then z = t + u */
+        double uu[3];
+        if constexpr (PH == PH_B0) {
+            for (int k = 0; k < 3; k++) uu[k] = v[k] - t[k];
+        } else {
+            double a[3];
+            part_sum<T>(i, L, lval, ColPlain<T>{V.u}, a[0], a[1], a[2]);
+            for (int k = 0; k < 3; k++) uu[k] = (v[k] - t[k]) - a[k];
+        }
+        if constexpr (PH != PH_TOP)
+            for (int k = 0; k < 3; k++) {
+                const T uk = (T)uu[k];
+                V.u[3 * i + k] = uk;
+                uu[k] = (double)uk;
+            }
+        for (int k = 0; k < 3; k++) {
+            const int ii = 3 * i + k;
+            if constexpr (OP == SG_INIT) { /* r = b~ - z, q = r */
+                const double bb = (double)V.bt[ii];
+                const T rr = (T)(bb + (-1) * (t[k] + uu[k]));
+                V.r[ii] = rr;
+                V.q[ii] = rr;
+                pa2[0] = fmax(pa2[0], fabs((double)rr));
+                pa2[1] = fmax(pa2[1], fabs(bb));
+                pb[0] = __fma_rn((double)rr, (double)rr, pb[0]);
+            } else if constexpr (OP == SG_A || OP == SG_AP) {
+                const T zk = (T)(t[k] + uu[k]);
+                V.Ap[ii] = zk;
+                pb[0] = __fma_rn((double)V.q[ii], (double)zk, pb[0]);
+            } else {
+                const T zk = (T)(t[k] + uu[k]);
+                V.As[ii] = zk;
+                pa3[0] = __fma_rn((double)zk, v[k], pa3[0]);
+                pa3[1] = __fma_rn((double)zk, (double)zk, pa3[1]);
+                pa3[2] = __fma_rn((double)zk, (double)V.q[ii], pa3[2]);
+            }
+        }
+    }
+    /* partial sums of the row's 256-row block, each written by exactly one phase */
+    if constexpr (OP == SG_INIT) {
+        if constexpr (PH != PH_B) {
+            block_partial<2, true>(pa2, partials + PR_A * nblk, nblk, j);
+            block_partial<1, false>(pb, partials + PR_B * nblk, nblk, j);
+        }
+    } else if constexpr (OP == SG_A || OP == SG_AP) {
+        if constexpr (OP == SG_A && !FIRST && PH != PH_F)
+            block_partial<1, true>(pa1, partials + PR_A * nblk, nblk, j);
+        if constexpr (PH != PH_B) block_partial<1, false>(pb, partials + PR_B * nblk, nblk, j);
+    } else {
+        if constexpr (PH != PH_B) block_partial<3, false>(pa3, partials + PR_A * nblk, nblk, j);
+    }
+}
+
+/* Split schedule of the A step: the previous iteration's x and r updates and the new direction
+ * p = (p - omega Ap) beta + r as one coalesced pass over the permuted rows (max|r| per 256-row block),
+ * then the product reads p as stored (SG_AP).  Same functions, same bits as the fused first phases. */
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_sgs_upd(int nblk, SgVec<T> V, double *partials,
+                                                 const adhk_scalars *__restrict__ sc)
+{
+    if (sc->done) return;
+    const double alpha = sc->alpha, omega = sc->omega, beta = sc->beta;
+    const ColP<T> col{V.s, V.As, V.p, V.Ap, omega, beta};
+    for (int j = blockIdx.x; j < nblk; j += gridDim.x) {
+        double vmax[1] = {0.0};
+        for (int ii = 3 * TPB * j + threadIdx.x; ii < 3 * TPB * (j + 1); ii += TPB) {
+            T rr, pp;
+            col.one(ii, rr, pp);
+            upd_x(V.xt + ii, (double)V.s[ii], (double)V.p[ii], alpha, omega);
+            V.r[ii] = rr;
+            V.p[ii] = pp;
+            vmax[0] = fmax(vmax[0], fabs((double)rr));
+        }
+        block_partial<1, true>(vmax, partials + PR_A * nblk, nblk, j);
+    }
+}
+
+/* b~ = (I+L)^-1 b^ in place over bt, colour phase of rows [r0, ...) (colour >= 1); the initial guess xt = b~ */
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_sgs_fwd(int r0, SPart L, const T *__restrict__ lval, T *bt, T *xt)
+{
+    const int i = r0 + blockIdx.x * TPB + threadIdx.x;
+    double a[3];
+    part_sum<T>(i, L, lval, ColPlain<T>{bt}, a[0], a[1], a[2]);
+    for (int k = 0; k < 3; k++) {
+        const T w = (T)((double)bt[3 * i + k] - a[k]);
+        bt[3 * i + k] = w;
+        xt[3 * i + k] = w;
+    }
+}
+
+/* x = (I+U)^-1 xt, colour phase (colour <= nc - 2) into t; the top colour's x is xt itself */
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_sgs_bwd(int r0, int top0, SPart U, const T *__restrict__ uval,
+                                                 const T *__restrict__ xt, T *t)
+{
+    const int i = r0 + blockIdx.x * TPB + threadIdx.x;
+    double a[3];
+    part_sum<T>(i, U, uval, GetB<T, ColPlain<T>>{t, ColPlain<T>{xt}, top0}, a[0], a[1], a[2]);
+    for (int k = 0; k < 3; k++) t[3 * i + k] = (T)((double)xt[3 * i + k] - a[k]);
+}
+
+/* b^ = D^-1 b permuted (the block-Jacobi step of k_kinit1), x0 = dsol; padding rows zero */
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_sgs_in(int Np, const int *__restrict__ prow, const T *__restrict__ minv,
+                                                const double *__restrict__ b, const double *__restrict__ dsol,
+                                                double *__restrict__ x0, T *__restrict__ bt, T *__restrict__ xt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Np) return;
+    const int n = prow[i];
+    double z[3] = {0.0, 0.0, 0.0};
+    if (n >= 0) {
+        apply_minv<T>(minv, n, b[3 * n], b[3 * n + 1], b[3 * n + 2], z[0], z[1], z[2]);
+        for (int k = 0; k < 3; k++) x0[3 * n + k] = dsol[3 * n + k];
+    }
+    for (int k = 0; k < 3; k++) {
+        bt[3 * i + k] = (T)z[k];
+        xt[3 * i + k] = (T)z[k];
+    }
+}
+
+/* unscale_linear_system (scale_linear_system.c:137-146) from the permuted solution: dsol = (x + x0) s */
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_sgs_out(int Np, int top0, const int *__restrict__ prow,
+                                                 const T *__restrict__ xt, const T *__restrict__ t,
+                                                 const double *__restrict__ x0, const double *__restrict__ scale,
+                                                 double *__restrict__ dsol)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Np) return;
+    const int n = prow[i];
+    if (n < 0) return;
+    const T *xs = (i >= top0) ? xt : t;
+    for (int k = 0; k < 3; k++) {
+        const double v = (double)xs[3 * i + k] + x0[3 * n + k];
+        dsol[3 * n + k] = v * scale[3 * n + k];
+    }
+}
+
+/* the folded matrix's off-diagonal blocks into a part's slots */
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_sgs_gather(int npos, const int *__restrict__ src, const T *__restrict__ bv,
+                                                    T *__restrict__ out)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npos) return;
+    const int sp = src[q], slot = q >> 6, lane = q & 63;
+#pragma unroll
+    for (int e = 0; e < 9; e++)
+        out[((size_t)slot * 9 + e) * 64 + lane] = sp >= 0 ? bv[((size_t)(sp >> 6) * 9 + e) * 64 + (sp & 63)] : (T)0;
+}
+
+/* a folded diagonal block that is not the identity (a singular block's fallback inverse) */
+__global__ void k_sgs_flag_clear(adhk_scalars *sc)
+{
+    if (threadIdx.x == 0) sc->sgs_bad = 0;
+}
+__global__ __launch_bounds__(TPB) void k_sgs_check(int Np, const int *__restrict__ prow,
+                                                   const unsigned char *__restrict__ diag_id, adhk_scalars *sc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Np) return;
+    const int n = prow[i];
+    if (n >= 0 && !diag_id[n]) sc->sgs_bad = 1;
+}
+
+extern "C" int adhk_launch_sgs_gather(adhk_ctx *c, int mixed)
+{
+    const adhk_sgs *G = &c->sgs;
+    const adhk_sgs_part *P[2] = {&G->U, &G->L};
+    hipLaunchKernelGGL(k_sgs_flag_clear, dim3(1), dim3(64), 0, c->stream, c->sc);
+    hipLaunchKernelGGL(k_sgs_check, dim3((G->Np + TPB - 1) / TPB), dim3(TPB), 0, c->stream, G->Np, G->prow, c->diag_id,
+                       c->sc);
+    for (int k = 0; k < 2; k++) {
+        const int npos = P[k]->nslot * 64;
+        if (npos == 0) continue;
+        if (mixed)
+            hipLaunchKernelGGL(k_sgs_gather<float>, dim3((npos + TPB - 1) / TPB), dim3(TPB), 0, c->stream, npos,
+                               P[k]->src, c->bvals32, P[k]->val32);
+        else
+            hipLaunchKernelGGL(k_sgs_gather<double>, dim3((npos + TPB - 1) / TPB), dim3(TPB), 0, c->stream, npos,
+                               P[k]->src, c->bvals, P[k]->val);
+    }
+    CHECK_LAUNCH();
+    return 0;
+}
+
+template <typename T> struct SgM {
+    SgVec<T> V;
+    const T *uval, *lval, *minv;
+};
+static SgM<double> sgm64(adhk_ctx *c)
+{
+    SgM<double> m;
+    m.V = sg_vec<double>(c->sgs.v64, c->sgs.Np);
+    m.uval = c->sgs.U.val;
+    m.lval = c->sgs.L.val;
+    m.minv = c->minv;
+    return m;
+}
+static SgM<float> sgm32(adhk_ctx *c)
+{
+    SgM<float> m;
+    m.V = sg_vec<float>(c->sgs.v32, c->sgs.Np);
+    m.uval = c->sgs.U.val32;
+    m.lval = c->sgs.L.val32;
+    m.minv = c->minv32;
+    return m;
+}
+
+template <typename T, int OP, int PH, bool FIRST>
+static void sgs_phase(adhk_ctx *c, const SgM<T> &M, int colour)
+{
+    const adhk_sgs *G = &c->sgs;
+    const SPart U{G->U.ptr, G->U.col}, L{G->L.ptr, G->L.col};
+    const int r0 = G->cs[colour], nb = (G->cs[colour + 1] - r0) / TPB, xm = c->tune.xcd_map;
+    const int grid = xm ? (nb + NXCD - 1) / NXCD * NXCD : nb;
+    hipLaunchKernelGGL((k_sgs_phase<T, OP, PH, FIRST>), dim3(grid), dim3(TPB), 0, c->stream, r0, nb, xm,
+                       G->cs[G->nc - 1], U, L, M.uval, M.lval, M.V, G->partials, G->nblk, c->sc);
+}
+
+/* one product in 2 nc - 2 phases */
+template <typename T, int OP, bool FIRST> static int sgs_product(adhk_ctx *c, const SgM<T> &M)
+{
+    const int nc = c->sgs.nc;
+    for (int k = nc - 2; k >= 1; k--) sgs_phase<T, OP, PH_B, FIRST>(c, M, k);
+    sgs_phase<T, OP, PH_B0, FIRST>(c, M, 0);
+    for (int k = 1; k <= nc - 2; k++) sgs_phase<T, OP, PH_F, FIRST>(c, M, k);
+    sgs_phase<T, OP, PH_TOP, FIRST>(c, M, nc - 1);
+    CHECK_LAUNCH();
+    return 0;
+}
+
+template <typename T> static int sgs_init(adhk_ctx *c, const SgM<T> &M, double tol, int max_it)
+{
+    const adhk_sgs *G = &c->sgs;
+    const SPart L{G->L.ptr, G->L.col};
+    hipLaunchKernelGGL(k_sgs_in<T>, dim3((G->Np + TPB - 1) / TPB), dim3(TPB), 0, c->stream, G->Np, G->prow, M.minv,
+                       c->residual, c->dsol, c->x0, M.V.bt, M.V.xt);
+    for (int k = 1; k < G->nc; k++)
+        hipLaunchKernelGGL(k_sgs_fwd<T>, dim3((G->cs[k + 1] - G->cs[k]) / TPB), dim3(TPB), 0, c->stream, G->cs[k], L,
+                           M.lval, M.V.bt, M.V.xt);
+    CHECK_LAUNCH();
+    if (sgs_product<T, SG_INIT, false>(c, M)) return -1;
+    hipLaunchKernelGGL(k_kinit3, dim3(1), dim3(TPB), 0, c->stream, G->nblk, G->inst_blk, G->partials, c->sc, c->work,
+                       tol, max_it);
+    CHECK_LAUNCH();
+    return 0;
+}
+
+template <typename T, bool FIRST> static int sgs_a(adhk_ctx *c, const SgM<T> &M)
+{
+    const adhk_sgs *G = &c->sgs;
+    int rc;
+    if (!FIRST && c->tune.split) {
+        adhk_kt_begin(c, ADHK_KT_UPDATE);
+        hipLaunchKernelGGL(k_sgs_upd<T>, dim3(G->nblk < MAX_GRID ? G->nblk : MAX_GRID), dim3(TPB), 0, c->stream,
+                           G->nblk, M.V, G->partials, c->sc);
+        adhk_kt_end(c);
+        CHECK_LAUNCH();
+        adhk_kt_begin(c, ADHK_KT_SPMV_P);
+        rc = sgs_product<T, SG_AP, false>(c, M);
+    } else {
+        adhk_kt_begin(c, ADHK_KT_SPMV_P);
+        rc = sgs_product<T, SG_A, FIRST>(c, M);
+    }
+    adhk_kt_end(c);
+    if (rc) return -1;
+    hipLaunchKernelGGL(k_fin_a<FIRST>, dim3(1), dim3(TPB), 0, c->stream, G->nblk, G->inst_blk, G->partials, c->sc);
+    CHECK_LAUNCH();
+    return 0;
+}
+
+template <typename T> static int sgs_s(adhk_ctx *c, const SgM<T> &M)
+{
+    const adhk_sgs *G = &c->sgs;
+    adhk_kt_begin(c, ADHK_KT_SPMV_S);
+    const int rc = sgs_product<T, SG_S, false>(c, M);
+    adhk_kt_end(c);
+    if (rc) return -1;
+    hipLaunchKernelGGL(k_fin_s, dim3(1), dim3(TPB), 0, c->stream, G->nblk, G->inst_blk, G->partials, c->sc);
+    CHECK_LAUNCH();
+    return 0;
+}
+
+template <typename T> static int sgs_finish(adhk_ctx *c, const SgM<T> &M)
+{
+    const adhk_sgs *G = &c->sgs;
+    const SPart U{G->U.ptr, G->U.col};
+    const int top0 = G->cs[G->nc - 1];
+    for (int k = G->nc - 2; k >= 0; k--)
+        hipLaunchKernelGGL(k_sgs_bwd<T>, dim3((G->cs[k + 1] - G->cs[k]) / TPB), dim3(TPB), 0, c->stream, G->cs[k], top0,
+                           U, M.uval, M.V.xt, M.V.t);
+    hipLaunchKernelGGL(k_sgs_out<T>, dim3((G->Np + TPB - 1) / TPB), dim3(TPB), 0, c->stream, G->Np, top0, G->prow,
+                       M.V.xt, M.V.t, c->x0, c->scale, c->dsol);
+    CHECK_LAUNCH();
+    return 0;
+}
+
+template <typename T> static int sgs_run(adhk_ctx *c, const SgM<T> &M, int what, double tol, int max_it)
+{
+    switch (what) {
+    case SGD_INIT:
+        return sgs_init<T>(c, M, tol, max_it);
+    case SGD_FIRST:
+        if (sgs_a<T, true>(c, M)) return -1;
+        return sgs_s<T>(c, M);
+    case SGD_ITER:
+        if (sgs_a<T, false>(c, M)) return -1;
+        return sgs_s<T>(c, M);
+    case SGD_LAST:
+        return sgs_a<T, false>(c, M);
+    default:
+        return sgs_finish<T>(c, M);
+    }
+}
+static int sgs_dispatch(adhk_ctx *c, int mixed, int what, double tol, int max_it)
+{
+    if (!c->sgs.ready || (mixed ? c->sgs.v32 == NULL : c->sgs.v64 == NULL)) {
+        adhk_set_error("multicolour SGS solve without its layout (adhk_sgs_ensure)");
+        return -1;
+    }
+    return mixed ? sgs_run<float>(c, sgm32(c), what, tol, max_it) : sgs_run<double>(c, sgm64(c), what, tol, max_it);
 }
 
 /* ---------------- Newton updates ---------------- */

@@ -45,6 +45,13 @@ void adhk_solver_defaults(adhk_solver *s)
     s->mixed_lin_tol = 1e-6;
     s->warm_gate = 1;
     s->substep_depth = 0;
+    s->precond = ADHK_PC_BJ;
+}
+
+/* the preconditioner of a solve: multicolour SGS only for one mesh alone */
+static int pc_of(const adhk_ctx *c, const adhk_solver *s)
+{
+    return (s->precond == ADHK_PC_SGS && c->ninst == 1) ? ADHK_PC_SGS : ADHK_PC_BJ;
 }
 
 #define HC(x)                                                                                                     \
@@ -80,6 +87,7 @@ static void ctx_free(adhk_ctx *c)
                     c->blk_inst, c->inst_blk, c->work};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++)
         if (ptrs[i]) hipFree(ptrs[i]);
+    adhk_sgs_free(c);
     free(c->h_inst_node);
     free(c->h_inst_elem);
     free(c->h_inst_seg);
@@ -967,7 +975,8 @@ static int capture_chunk(adhk_ctx *c, int mixed, int chunk)
 {
     hipGraph_t g = NULL;
     int cur = 1, rc = 0;
-    if (c->gchunk[mixed] && c->gchunk_len[mixed] == chunk && same_schedule(&c->gchunk_tune[mixed], &c->tune))
+    if (c->gchunk[mixed] && c->gchunk_len[mixed] == chunk && same_schedule(&c->gchunk_tune[mixed], &c->tune) &&
+        c->gchunk_pc[mixed] == c->pc)
         return 0;
     if (c->gchunk[mixed]) (void)hipGraphExecDestroy(c->gchunk[mixed]);
     c->gchunk[mixed] = NULL;
@@ -986,6 +995,7 @@ static int capture_chunk(adhk_ctx *c, int mixed, int chunk)
     HC(hipGraphDestroy(g));
     c->gchunk_len[mixed] = chunk;
     c->gchunk_tune[mixed] = c->tune;
+    c->gchunk_pc[mixed] = c->pc;
     return 0;
 fail:
     if (g) (void)hipGraphDestroy(g);
@@ -1002,6 +1012,13 @@ static int krylov_iterate(adhk_ctx *c, const adhk_solver *s, int mixed, adhk_sca
     const int use_graph = c->tune.graphs && !(c->kt.on && (c->kt.mask & krylov_kt)) && chunk % 2 == 0;
     HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
     HC(hipStreamSynchronize(c->stream));
+    if (c->pc == ADHK_PC_SGS && hs[0].sgs_bad) {
+        /* a singular diagonal block kept its fallback inverse: this solve takes block-Jacobi */
+        c->pc = ADHK_PC_BJ;
+        LC(adhk_launch_krylov_init(c, mixed, mixed ? s->mixed_lin_tol : s->lin_tol, s->lin_max_it));
+        HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
+        HC(hipStreamSynchronize(c->stream));
+    }
     for (int i = 0; i < K; i++) all_done &= hs[i].done;
     if (!all_done) {
         /* iteration 1, then chunks; the A kernel of iteration k+1 completes iteration k and tests it */
@@ -1092,7 +1109,7 @@ static int run_stage(adhk_ctx *c, const adhk_solver *s, int mixed, int which)
     if (mixed && !c->emat32) LC(ensure_mixed(c)); /* no allocation may happen inside a capture */
     {
         adhk_stage_graph *st = &c->gstage[mixed][which];
-        if (!st->exec || !same_params(&st->p, &c->p) || !same_schedule(&st->tune, &c->tune) ||
+        if (!st->exec || !same_params(&st->p, &c->p) || !same_schedule(&st->tune, &c->tune) || st->pc != c->pc ||
             (which == STG_SYS && (st->tol != tol || st->max_it != s->lin_max_it))) {
             int rc;
             if (st->exec) (void)hipGraphExecDestroy(st->exec);
@@ -1111,6 +1128,7 @@ static int run_stage(adhk_ctx *c, const adhk_solver *s, int mixed, int which)
             st->tune = c->tune;
             st->tol = tol;
             st->max_it = s->lin_max_it;
+            st->pc = c->pc;
         }
         HC(hipGraphLaunch(st->exec, c->stream));
     }
@@ -1199,6 +1217,8 @@ int adhk_sw2_autotune(adhk_sw2 *c, const adhk_solver *s, int ninner, adhk_tuning
     }
     c->tune.elem_tpb = best.elem_tpb;
     /* 2. the Newton system of the current state */
+    c->pc = pc_of(c, s);
+    if (c->pc == ADHK_PC_SGS) LC(adhk_sgs_ensure(c, mixed));
     LC(jacobian_dev(c, mixed));
     LC(adhk_launch_prep(c, mixed));
     /* 3. BiCGStab schedules: complete solves, best of ninner */
@@ -1306,7 +1326,13 @@ static int fe_newton_pass(adhk_ctx *c, const adhk_solver *s, int use_warm)
     int *act = NULL, *ls = NULL, *solv = NULL, *its = NULL, *lits = NULL, *cuts = NULL;
     double *rmax = NULL, *rl2 = NULL, *incmax = NULL, *old_rl2 = NULL;
     int all_ok = 1;
+    const int pc = pc_of(c, s);
+    if (s->precond != ADHK_PC_BJ && s->precond != ADHK_PC_SGS) {
+        adhk_set_error("adhk_fe_newton: precond %d (ADHK_PC_BJ or ADHK_PC_SGS)", s->precond);
+        return -1;
+    }
     if (mixed && ensure_mixed(c)) return -1;
+    if (pc == ADHK_PC_SGS && adhk_sgs_ensure(c, mixed)) return -1; /* no allocation inside a capture */
     hs = calloc((size_t)K, sizeof *hs);
     hw = calloc((size_t)K, sizeof *hw);
     act = calloc((size_t)K, sizeof(int));
@@ -1371,6 +1397,7 @@ static int fe_newton_pass(adhk_ctx *c, const adhk_solver *s, int use_warm)
         for (int i = 0; i < K; i++) any |= act[i];
         if (!any) break;
         LC(set_work(c, act));
+        c->pc = pc;
         LC(run_stage(c, s, mixed, STG_SYS));
         LC(krylov_iterate(c, s, mixed, hw));
         for (int i = 0; i < K; i++) lits[i] = c->h_work[i] ? hw[i].it : 0;

@@ -1,0 +1,255 @@
+/*
+ * sgs_setup.c -- the permuted layout of the multicolour symmetric Gauss-Seidel preconditioner, built once
+ * per context from the node-block pattern (no counterpart in the reference, whose preconditioner is an
+ * exact LU: bcgstab_solver.c:371-424).
+ *
+ * Colouring: greedy in natural node order -- node i takes the smallest colour that none of its block
+ * neighbours j < i holds.  The rows are then permuted by colour (ascending node within a colour) and
+ * every colour's run padded to a multiple of 256 rows, so that each 256-row reduction block and each
+ * 64-row slice belongs to one colour and a colour's rows form whole workgroups.  The folded matrix's
+ * off-diagonal blocks go to U (neighbour of a higher colour) or L (lower colour), each stored as its own
+ * SELL-64 over the permuted rows, blocks of a row in ascending natural column order.  A padding slot
+ * points at the zero row Np, whose vector entries are zero and never written.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "adhk_internal.h"
+
+#define HC(x)                                                                                                     \
+    do {                                                                                                          \
+        hipError_t e_ = (x);                                                                                      \
+        if (e_ != hipSuccess) {                                                                                   \
+            adhk_set_error("%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_));                        \
+            goto fail;                                                                                            \
+        }                                                                                                         \
+    } while (0)
+
+#define SGS_VECS 10
+
+/* greedy colouring over the block pattern; returns the colour count, or -1 beyond ADHK_SGS_MAXC */
+static int colour_greedy(int N, const int *brow, const int *bcol, int *colour)
+{
+    int nc = 0;
+    for (int i = 0; i < N; i++) {
+        unsigned long long used = 0;
+        for (int b = brow[i]; b < brow[i + 1]; b++) {
+            const int j = bcol[b];
+            if (j < i) used |= 1ULL << colour[j];
+        }
+        int c = 0;
+        while (c < ADHK_SGS_MAXC && (used >> c & 1ULL)) c++;
+        if (c >= ADHK_SGS_MAXC) return -1;
+        colour[i] = c;
+        if (c + 1 > nc) nc = c + 1;
+    }
+    return nc;
+}
+
+/* one part (upper: colour(j) > colour(i), else lower) as SELL-64 over the permuted rows */
+static int build_part(adhk_sgs_part *P, int upper, int Np, const int *prow, const int *colour, const int *brow,
+                      const int *bcol, const int *blk2sell, const int *posof)
+{
+    const int nsl = Np / 64;
+    int *ptr = malloc(((size_t)nsl + 1) * sizeof(int)), *col = NULL, *src = NULL;
+    if (!ptr) goto oom;
+    ptr[0] = 0;
+    for (int s = 0; s < nsl; s++) {
+        int w = 0;
+        for (int l = 0; l < 64; l++) {
+            const int n = prow[s * 64 + l];
+            if (n < 0) continue;
+            int k = 0;
+            for (int b = brow[n]; b < brow[n + 1]; b++) {
+                const int j = bcol[b];
+                if (j != n && (upper ? colour[j] > colour[n] : colour[j] < colour[n])) k++;
+            }
+            if (k > w) w = k;
+        }
+        ptr[s + 1] = ptr[s] + w;
+    }
+    P->nslot = ptr[nsl];
+    {
+        const size_t npos = (size_t)(P->nslot > 0 ? P->nslot : 1) * 64;
+        col = malloc(npos * sizeof(int));
+        src = malloc(npos * sizeof(int));
+        if (!col || !src) goto oom;
+        for (size_t q = 0; q < npos; q++) {
+            col[q] = Np; /* the zero row */
+            src[q] = -1;
+        }
+        for (int s = 0; s < nsl; s++)
+            for (int l = 0; l < 64; l++) {
+                const int n = prow[s * 64 + l];
+                if (n < 0) continue;
+                int k = 0;
+                for (int b = brow[n]; b < brow[n + 1]; b++) {
+                    const int j = bcol[b];
+                    if (j == n || !(upper ? colour[j] > colour[n] : colour[j] < colour[n])) continue;
+                    const size_t q = (size_t)(ptr[s] + k) * 64 + l;
+                    col[q] = posof[j];
+                    src[q] = blk2sell[b];
+                    k++;
+                }
+            }
+        HC(hipMalloc((void **)&P->ptr, ((size_t)nsl + 1) * sizeof(int)));
+        HC(hipMalloc((void **)&P->col, npos * sizeof(int)));
+        HC(hipMalloc((void **)&P->src, npos * sizeof(int)));
+        HC(hipMemcpy(P->ptr, ptr, ((size_t)nsl + 1) * sizeof(int), hipMemcpyHostToDevice));
+        HC(hipMemcpy(P->col, col, npos * sizeof(int), hipMemcpyHostToDevice));
+        HC(hipMemcpy(P->src, src, npos * sizeof(int), hipMemcpyHostToDevice));
+    }
+    free(ptr);
+    free(col);
+    free(src);
+    return 0;
+oom:
+    adhk_set_error("adhk_sgs_setup: out of host memory");
+fail:
+    free(ptr);
+    free(col);
+    free(src);
+    return -1;
+}
+
+int adhk_sgs_setup(adhk_ctx *c)
+{
+    adhk_sgs *G = &c->sgs;
+    const int N = c->N, nnzb = c->nnzb;
+    int *brow = NULL, *bcol = NULL, *b2s = NULL, *prow = NULL, *posof = NULL, *cnt = NULL, *fill = NULL;
+    int ib[2];
+    if (G->ready) return 0;
+    if (c->ninst != 1) {
+        adhk_set_error("adhk_sgs_setup: the multicolour preconditioner serves one mesh alone");
+        return -1;
+    }
+    brow = malloc(((size_t)N + 1) * sizeof(int));
+    bcol = malloc((size_t)(nnzb > 0 ? nnzb : 1) * sizeof(int));
+    b2s = malloc((size_t)(nnzb > 0 ? nnzb : 1) * sizeof(int));
+    G->colour = malloc((size_t)(N > 0 ? N : 1) * sizeof(int));
+    posof = malloc((size_t)(N > 0 ? N : 1) * sizeof(int));
+    cnt = calloc(ADHK_SGS_MAXC, sizeof(int));
+    fill = calloc(ADHK_SGS_MAXC, sizeof(int));
+    if (!brow || !bcol || !b2s || !G->colour || !posof || !cnt || !fill) {
+        adhk_set_error("adhk_sgs_setup: out of host memory");
+        goto fail;
+    }
+    HC(hipMemcpy(brow, c->brow, ((size_t)N + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    HC(hipMemcpy(bcol, c->bcol, (size_t)nnzb * sizeof(int), hipMemcpyDeviceToHost));
+    HC(hipMemcpy(b2s, c->blk2sell, (size_t)nnzb * sizeof(int), hipMemcpyDeviceToHost));
+    G->nc = colour_greedy(N, brow, bcol, G->colour);
+    if (G->nc < 2) {
+        adhk_set_error("adhk_sgs_setup: %d colours (more than %d, or a pattern without couplings)", G->nc,
+                       ADHK_SGS_MAXC);
+        goto fail;
+    }
+    for (int i = 0; i < N; i++) cnt[G->colour[i]]++;
+    G->cs[0] = 0;
+    for (int k = 0; k < G->nc; k++) G->cs[k + 1] = G->cs[k] + (cnt[k] + 255) / 256 * 256;
+    G->Np = G->cs[G->nc];
+    G->nblk = G->Np / 256;
+    if ((long)G->Np * 3 + 3 >= 2147483647L / 9) {
+        adhk_set_error("adhk_sgs_setup: too many rows");
+        goto fail;
+    }
+    prow = malloc((size_t)G->Np * sizeof(int));
+    if (!prow) {
+        adhk_set_error("adhk_sgs_setup: out of host memory");
+        goto fail;
+    }
+    for (int i = 0; i < G->Np; i++) prow[i] = -1;
+    for (int n = 0; n < N; n++) {
+        const int k = G->colour[n], i = G->cs[k] + fill[k]++;
+        prow[i] = n;
+        posof[n] = i;
+    }
+    if (build_part(&G->U, 1, G->Np, prow, G->colour, brow, bcol, b2s, posof)) goto fail;
+    if (build_part(&G->L, 0, G->Np, prow, G->colour, brow, bcol, b2s, posof)) goto fail;
+    HC(hipMalloc((void **)&G->prow, (size_t)G->Np * sizeof(int)));
+    HC(hipMemcpy(G->prow, prow, (size_t)G->Np * sizeof(int), hipMemcpyHostToDevice));
+    ib[0] = 0;
+    ib[1] = G->nblk;
+    HC(hipMalloc((void **)&G->inst_blk, 2 * sizeof(int)));
+    HC(hipMemcpy(G->inst_blk, ib, 2 * sizeof(int), hipMemcpyHostToDevice));
+    HC(hipMalloc((void **)&G->partials, 8 * (size_t)G->nblk * sizeof(double)));
+    G->ready = 1;
+    free(brow);
+    free(bcol);
+    free(b2s);
+    free(prow);
+    free(posof);
+    free(cnt);
+    free(fill);
+    return 0;
+fail:
+    free(brow);
+    free(bcol);
+    free(b2s);
+    free(prow);
+    free(posof);
+    free(cnt);
+    free(fill);
+    adhk_sgs_free(c);
+    return -1;
+}
+
+int adhk_sgs_ensure(adhk_ctx *c, int mixed)
+{
+    adhk_sgs *G = &c->sgs;
+    if (adhk_sgs_setup(c)) return -1;
+    {
+        const size_t vb = mixed ? sizeof(float) : sizeof(double);
+        const size_t nvec = (size_t)3 * ((size_t)G->Np + 1);
+        void **vp = mixed ? (void **)&G->v32 : (void **)&G->v64;
+        void **up = mixed ? (void **)&G->U.val32 : (void **)&G->U.val;
+        void **lp = mixed ? (void **)&G->L.val32 : (void **)&G->L.val;
+        if (!*vp) {
+            /* zeroed once: the zero row and the padding rows stay zero through every product */
+            HC(hipMalloc(vp, SGS_VECS * nvec * vb));
+            HC(hipMemset(*vp, 0, SGS_VECS * nvec * vb));
+        }
+        if (!*up) HC(hipMalloc(up, (size_t)(G->U.nslot > 0 ? G->U.nslot : 1) * 64 * 9 * vb));
+        if (!*lp) HC(hipMalloc(lp, (size_t)(G->L.nslot > 0 ? G->L.nslot : 1) * 64 * 9 * vb));
+    }
+    return 0;
+fail:
+    return -1;
+}
+
+static void free_part(adhk_sgs_part *P)
+{
+    if (P->ptr) (void)hipFree(P->ptr);
+    if (P->col) (void)hipFree(P->col);
+    if (P->src) (void)hipFree(P->src);
+    if (P->val) (void)hipFree(P->val);
+    if (P->val32) (void)hipFree(P->val32);
+    memset(P, 0, sizeof *P);
+}
+
+void adhk_sgs_free(adhk_ctx *c)
+{
+    adhk_sgs *G = &c->sgs;
+    free_part(&G->U);
+    free_part(&G->L);
+    if (G->prow) (void)hipFree(G->prow);
+    if (G->inst_blk) (void)hipFree(G->inst_blk);
+    if (G->partials) (void)hipFree(G->partials);
+    if (G->v64) (void)hipFree(G->v64);
+    if (G->v32) (void)hipFree(G->v32);
+    free(G->colour);
+    memset(G, 0, sizeof *G);
+}
+
+int adhk_sw2_colouring(adhk_sw2 *c, int *colour, int *ncolour, int *layout)
+{
+    if (!c) return -1;
+    if (adhk_sgs_setup(c)) return -1;
+    if (colour) memcpy(colour, c->sgs.colour, (size_t)c->N * sizeof(int));
+    if (ncolour) *ncolour = c->sgs.nc;
+    if (layout) {
+        layout[0] = c->sgs.Np;
+        layout[1] = c->sgs.U.nslot;
+        layout[2] = c->sgs.L.nslot;
+    }
+    return 0;
+}

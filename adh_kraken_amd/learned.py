@@ -135,13 +135,15 @@ def collect_trajectory(sm: SModelSuper, mlp: WarmStartMLP, nsteps, precision=L.P
 
 
 def train_warm_start(cases, nsteps, steps=2000, batch=8192, lr=2e-3, seed=1, precision=L.PREC_FP64, dt=1.0,
-                     prepare=None, strict=True, validate=None, checkpoints=1):
+                     prepare=None, strict=True, validate=None, checkpoints=1, precond=L.PC_BJ):
     """Train a model on solver trajectories: cases = [(mesh, sol0), ...], nsteps steps each;
     prepare(sm, i), when given, sets up case i's model (e.g. its boundary time series) before it runs.
     strict=False steps over a failed step instead of raising, as collect_trajectory describes (the
     steps collected from each trajectory are in model.collected_steps).  With validate (model -> {"cost": ...}, e.g.
     validation_cost on a case held out of the training set), the steps run in `checkpoints` chunks
     and the chunk end with the lowest cost is kept (model.validation records the choice).
+
+    precond is the BiCGStab preconditioner of the trajectories (ADHK_PC_*).
 
     Returns (model, loss history, data-generation Newton iterations)."""
     mlp = WarmStartMLP(seed=seed)
@@ -150,6 +152,7 @@ def train_warm_start(cases, nsteps, steps=2000, batch=8192, lr=2e-3, seed=1, pre
     gen_its = 0
     for i, (mesh, sol0) in enumerate(cases):
         sm = SModelSuper(mesh, dt=dt)
+        sm.solver.precond = precond
         sm.set_initial(sol0)
         if prepare is not None:
             prepare(sm, i)
@@ -177,13 +180,19 @@ def train_warm_start(cases, nsteps, steps=2000, batch=8192, lr=2e-3, seed=1, pre
     return mlp, np.concatenate(hist), gen_its
 
 
-def validation_cost(mesh, sol0, nsteps, precision=L.PREC_FP64, dt=1.0, prepare=None, newton_weight=15.0):
+def validation_cost(mesh, sol0, nsteps, precision=L.PREC_FP64, dt=1.0, prepare=None, newton_weight=None,
+                    precond=L.PC_BJ):
     """A validate() for train_warm_start: the solver work of nsteps time steps of a held-out case with the
     model as warm start, as BiCGStab iterations + newton_weight x Newton iterations (a Newton iteration
-    also builds and folds the Jacobian, about as much work as newton_weight BiCGStab iterations on the
-    flume and channel workloads).  model=None gives the same case without a warm start."""
+    also builds and folds the Jacobian, about as much work as 15 block-Jacobi or 5 multicolour-SGS
+    BiCGStab iterations on the flume and channel workloads: the default for precond).  model=None gives
+    the same case without a warm start."""
+    if newton_weight is None:
+        newton_weight = 5.0 if precond == L.PC_SGS else 15.0
+
     def cost(mlp):
         sm = SModelSuper(mesh, dt=dt)
+        sm.solver.precond = precond
         sm.set_initial(sol0)
         if prepare is not None:
             prepare(sm)

@@ -195,6 +195,18 @@ class SModelSuper:
         return t.as_dict(), {"elem_jac_ms": ms[0], "krylov_ms": ms[1], "default_krylov_ms": ms[2],
                              "krylov_its": int(ms[3])}
 
+    def colouring(self, layout=False):
+        """(colour [nnodes], colour count) of the multicolour SGS preconditioner; with layout, also
+        {rows, u_slots, l_slots}: the permuted row count and the SELL-64 slot columns of U and L"""
+        col = np.zeros(self.mesh.nnodes, np.int32)
+        nc = C.c_int()
+        lay = np.zeros(3, np.int32)
+        L.check(self.lib.adhk_sw2_colouring(self.h, col.ctypes.data_as(L.ip), C.byref(nc), lay.ctypes.data_as(L.ip)),
+                "adhk_sw2_colouring")
+        if layout:
+            return col, nc.value, {"rows": int(lay[0]), "u_slots": int(lay[1]), "l_slots": int(lay[2])}
+        return col, nc.value
+
     def instance_status(self):
         """(alive [ninst], steps_done [ninst]) of an ensemble's members"""
         a, d = np.zeros(self.ninst, np.int32), np.zeros(self.ninst, np.int32)

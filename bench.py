@@ -70,6 +70,17 @@ def algorithmic_bytes(kind, N, nslot, ndofs, mixed, split=False):
     raise ValueError(kind)
 
 
+def sgs_product_bytes(layout, mixed):
+    """HBM bytes of one multicolour SGS product (the S step: six colour-phase launches on a 4-colour mesh)
+    with every operand moved once: the U and L parts of the folded matrix in SELL-64 over the permuted rows
+    (slot columns x 64 x (9 values + 1 column index), slice pointers) and seven vectors of 3 x rows values
+    (r, Ap, q read; s, t, u, As written)."""
+    vb = 4 if mixed else 8
+    rows = layout["rows"]
+    mat = (layout["u_slots"] + layout["l_slots"]) * 64 * (9 * vb + 4) + 2 * (rows // 64 + 1) * 4
+    return mat + 7 * 3 * rows * vb
+
+
 def host_cpu():
     """The host's processor model, logical CPU count, the affinity set and the OpenMP thread setting."""
     model = None
@@ -309,6 +320,9 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=50, help="time steps of the 64-member ensemble (the "
                     "workload's 50 by default)")
     ap.add_argument("--check-every", type=int, default=8)
+    ap.add_argument("--precond", choices=["sgs", "bj"], default="sgs",
+                    help="BiCGStab preconditioner of the timed variants: multicolour SGS (default; block-Jacobi "
+                         "is then timed beside the FP64 variant) or block-Jacobi alone")
     ap.add_argument("--schedule", default=None,
                     help='fixed solver schedule as JSON, e.g. \'{"split": 1, "chunk": 8}\' (default: autotune)')
     args = ap.parse_args()
@@ -360,36 +374,61 @@ def main():
 
     log(f"workload ready: {mesh.nnodes} nodes, {mesh.nelems2d} triangles")
     schedule = json.loads(args.schedule) if args.schedule else None
+    pc = L.PC_SGS if args.precond == "sgs" else L.PC_BJ
+    traffic_tab = {}
+    tpath = os.path.join(ROOT, "profiles", "sw2_newton_c2_traffic.json")
+    if args.config == "c2" and os.path.exists(tpath):  # HBM bytes per launch from the rocprofv3 counter runs
+        traffic_tab = json.load(open(tpath))["bytes_per_launch"]
     plain = run_variant(A, L, mesh, sol0, dt, L.PREC_FP64, args.steps, args.warmup, args.check_every,
-                        schedule=schedule, prepare=prepare)
+                        schedule=schedule, prepare=prepare, solver={"precond": pc})
     sm = plain["sm"]
-    kt = plain["kt"]
-    # the dominant kernel: the BiCGStab SpMV with the larger time per iteration (together they take
-    # about three quarters of the step; the per-kernel totals are in the rocprofv3 summary)
-    roof_kind = max(("spmv_p", "spmv_s"), key=lambda k: kt[k]["ms_avg"] or 0.0)
-    split = bool(plain["tuning"] and plain["tuning"].get("split"))
-    ab = algorithmic_bytes(roof_kind, sm.mesh.nnodes, sm.nslot, sm.ndofs, False, split)
-    avg_ms = kt[roof_kind]["ms_avg"]
-    achieved = ab / (avg_ms / 1e3) / 1e9
     probe_ms = C.c_double()
     nb = 1 << 30
     L.check(L.load().adhk_copy_probe(sm.h, nb, 10, C.byref(probe_ms)), "copy_probe")
     copy_gbs = 2 * nb / (probe_ms.value / 1e3) / 1e9
-    kname = {"spmv_p": "k_spmv_p" if split else "k_iter_a", "spmv_s": "k_iter_s"}[roof_kind]
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "sw2_newton_c2_traffic.json")
-    if args.config == "c2" and os.path.exists(tpath):  # HBM bytes per launch from the rocprofv3 counter runs
-        traffic = json.load(open(tpath))["bytes_per_launch"].get(kname)
-    mat_bytes = algorithmic_bytes(roof_kind, sm.mesh.nnodes, sm.nslot, 0, False, split)
-    roofline = {"kernel": kname, "bound": "hbm",
-                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes": ab,
-                "avg_launch_ms": round(avg_ms, 5), "copy_kernel_gbs": round(copy_gbs, 1),
+
+    def mall_note(mat_bytes):
+        return (f"the {mat_bytes / 1e6:.0f} MB matrix fits the 256 MB MALL, which serves part of each product's "
+                f"reads: the rate can exceed the 1 GiB copy kernel's, and is not HBM bandwidth alone"
+                if mat_bytes < 256e6 else
+                f"the {mat_bytes / 1e6:.0f} MB matrix exceeds the 256 MB MALL and streams from HBM")
+
+    def roofline_bj(run):
+        """the block-Jacobi solve's dominant kernel: the BiCGStab SpMV with the larger time per iteration"""
+        kt = run["kt"]
+        kind = max(("spmv_p", "spmv_s"), key=lambda k: kt[k]["ms_avg"] or 0.0)
+        split = bool(run["tuning"] and run["tuning"].get("split"))
+        ab = algorithmic_bytes(kind, sm.mesh.nnodes, sm.nslot, sm.ndofs, False, split)
+        avg_ms = kt[kind]["ms_avg"]
+        achieved = ab / (avg_ms / 1e3) / 1e9
+        kname = {"spmv_p": "k_spmv_p" if split else "k_iter_a", "spmv_s": "k_iter_s"}[kind]
+        return {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_tab.get(kname),
+                "algorithmic_bytes": ab, "avg_launch_ms": round(avg_ms, 5), "copy_kernel_gbs": round(copy_gbs, 1),
                 "frac_of_copy_kernel": round(achieved / copy_gbs, 3),
-                "note": (f"the {mat_bytes / 1e6:.0f} MB matrix fits the 256 MB MALL, which serves part of each "
-                         f"product's reads: the rate exceeds the 1 GiB copy kernel's, and is not HBM bandwidth alone"
-                         if mat_bytes < 256e6 else
-                         f"the {mat_bytes / 1e6:.0f} MB matrix exceeds the 256 MB MALL and streams from HBM")}
+                "note": mall_note(algorithmic_bytes(kind, sm.mesh.nnodes, sm.nslot, 0, False, split))}
+
+    def roofline_sgs(run):
+        """the SGS solve's dominant unit: the S product, one pass over the folded matrix in 2 nc - 2 colour-phase
+        launches of k_sgs_phase, timed by one event pair around the sequence"""
+        _, nc, lay = run["sm"].colouring(layout=True)
+        ab = sgs_product_bytes(lay, False)
+        avg_ms = run["kt"]["spmv_s"]["ms_avg"]
+        achieved = ab / (avg_ms / 1e3) / 1e9
+        mat = ab - 7 * 3 * lay["rows"] * 8
+        return {"kernel": f"k_sgs_phase<double, S> x {2 * nc - 2} (one SGS product)", "bound": "hbm",
+                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_tab.get("sgs_s_product"),
+                "algorithmic_bytes": ab, "avg_launch_ms": round(avg_ms, 5), "copy_kernel_gbs": round(copy_gbs, 1),
+                "frac_of_copy_kernel": round(achieved / copy_gbs, 3), "colours": nc, "launches": 2 * nc - 2,
+                "note": mall_note(mat) + "; the event pair spans the launch gaps between the phases"}
+
+    def pc_desc(p):
+        return ("BiCGStab + multicolour symmetric Gauss-Seidel on the block-Jacobi-folded matrix, split form "
+                "(Eisenstat products)" if p == L.PC_SGS else "BiCGStab + 3x3 block-Jacobi folded into the matrix")
+
+    roofline = roofline_sgs(plain) if pc == L.PC_SGS else roofline_bj(plain)
+    kt = plain["kt"]
     value = args.steps / plain["elapsed"]
     out = {
         "metric": ("SW2 timesteps/s @1.2M dofs: FP64 plain vs mixed+learned, equal 1e-8 Newton tol"
@@ -407,7 +446,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": workload,
-                   "variant": "plain FP64: FD Jacobian, BiCGStab + 3x3 block-Jacobi folded into the matrix (tol 1e-11, max 100)",
+                   "variant": f"plain FP64: FD Jacobian, {pc_desc(pc)} (tol 1e-11, max 100)",
                    "newton_tol": sm.solver.tol_nonlin, "inc_nonlin": sm.solver.inc_nonlin},
         "newton_its": plain["stats"]["newton_its"], "bicgstab_its": plain["stats"]["lin_its_total"],
         "bicgstab_its_per_solve": round(plain["stats"]["lin_its_total"] / max(plain["stats"]["newton_its"], 1), 1),
@@ -419,16 +458,32 @@ def main():
         "schedule": plain["tuning"],
         "roofline": roofline,
     }
+    if pc == L.PC_SGS:
+        # the same FP64 variant with the block-Jacobi preconditioner
+        bj = run_variant(A, L, mesh, sol0, dt, L.PREC_FP64, args.steps, args.warmup, args.check_every,
+                         schedule=schedule, prepare=prepare, solver={"precond": L.PC_BJ})
+        vbj = args.steps / bj["elapsed"]
+        out["fp64_block_jacobi"] = {
+            "value": round(vbj, 4), "unit": "timesteps/s", "ms_per_step": round(1e3 * bj["elapsed"] / args.steps, 3),
+            "sgs_speedup_over_this": round(value / vbj, 3),
+            "variant": f"plain FP64: FD Jacobian, {pc_desc(L.PC_BJ)} (tol 1e-11, max 100)",
+            "depth_rel_err_vs_sgs": float(np.abs(bj["sol"][0::3] - plain["sol"][0::3]).max() /
+                                          np.abs(plain["sol"][0::3]).max()),
+            "newton_its": bj["stats"]["newton_its"], "bicgstab_its": bj["stats"]["lin_its_total"],
+            "bicgstab_its_per_solve": round(bj["stats"]["lin_its_total"] / max(bj["stats"]["newton_its"], 1), 1),
+            "solves_at_iteration_cap": bj["stats"]["lin_capped"],
+            "kernel_avg_us": {k: round(1e3 * v["ms_avg"], 2) for k, v in bj["kt"].items() if v["ms_avg"]},
+            "schedule": bj["tuning"], "roofline": roofline_bj(bj)}
     if not args.no_mixed:
         mx = run_variant(A, L, mesh, sol0, dt, L.PREC_MIXED, args.steps, args.warmup, args.check_every,
-                         schedule=schedule, prepare=prepare)
+                         schedule=schedule, prepare=prepare, solver={"precond": pc})
         h64, hmx = plain["sol"][0::3], mx["sol"][0::3]
         u64 = np.hypot(plain["sol"][1::3], plain["sol"][2::3])
         umx = np.hypot(mx["sol"][1::3], mx["sol"][2::3])
         vmx = args.steps / mx["elapsed"]
         out["mixed"] = {"value": round(vmx, 4), "unit": "timesteps/s", "speedup_vs_fp64": round(vmx / value, 3),
                         "variant": "FP32 Jacobian storage, SpMV and BiCGStab (inner tol 1e-6); FP64 residual, "
-                                   "Newton update and convergence test",
+                                   "Newton update and convergence test; " + pc_desc(pc),
                         "depth_rel_err_vs_fp64": float(np.abs(hmx - h64).max() / np.abs(h64).max()),
                         "speed_rel_err_vs_fp64": float(np.abs(umx - u64).max() / max(np.abs(u64).max(), 1e-300)),
                         "newton_its": mx["stats"]["newton_its"], "bicgstab_its": mx["stats"]["lin_its_total"],
@@ -437,7 +492,7 @@ def main():
         # FP64 throughout at the mixed variant's inner tolerance: splits the mixed speed-up into what the FP32
         # storage gives and what the looser inner solve gives
         f6 = run_variant(A, L, mesh, sol0, dt, L.PREC_FP64, args.steps, args.warmup, args.check_every,
-                         schedule=schedule, prepare=prepare, solver={"lin_tol": 1e-6})
+                         schedule=schedule, prepare=prepare, solver={"lin_tol": 1e-6, "precond": pc})
         v6 = args.steps / f6["elapsed"]
         out["fp64_inner_tol_1e-6"] = {
             "value": round(v6, 4), "unit": "timesteps/s", "speedup_vs_fp64": round(v6 / value, 3),
@@ -471,21 +526,22 @@ def main():
                                                                 ", ".join(str(sd) for sd in seeds[:-1]))
             val_desc = "flume bump seed %d, %d steps" % (seeds[-1], args.val_steps)
         log("learned warm start: trajectories, training and checkpoint validation")
-        val = LR.validation_cost(*cases[-1], args.val_steps, precision=L.PREC_MIXED, dt=dt, prepare=val_prepare)
+        val = LR.validation_cost(*cases[-1], args.val_steps, precision=L.PREC_MIXED, dt=dt, prepare=val_prepare,
+                                 precond=pc)
         mlp, hist, gen_its = LR.train_warm_start(cases[:-1], nsteps=args.train_steps, steps=args.adam_steps,
                                                  batch=16384, lr=2e-3, seed=5, precision=L.PREC_MIXED, dt=dt,
                                                  prepare=tr_prepare, strict=False, validate=val,
-                                                 checkpoints=args.checkpoints)
+                                                 checkpoints=args.checkpoints, precond=pc)
         val_plain = val(None)
         t_train = time.perf_counter() - t0
         ml = run_variant(A, L, mesh, sol0, dt, L.PREC_MIXED, args.steps, args.warmup, args.check_every, warm=mlp,
-                         schedule=schedule, prepare=prepare)
+                         schedule=schedule, prepare=prepare, solver={"precond": pc})
         h64, hml = plain["sol"][0::3], ml["sol"][0::3]
         vml = args.steps / ml["elapsed"]
         out["mixed_learned"] = {
             "value": round(vml, 4), "unit": "timesteps/s", "speedup_vs_fp64": round(vml / value, 3),
             "variant": "mixed precision + learned per-node MLP warm start (12-64-64-3 GELU, BF16 weights); the "
-                       "start with the smaller l2-norm residual of {learned, previous level} is kept",
+                       "start with the smaller l2-norm residual of {learned, previous level} is kept; " + pc_desc(pc),
             "warm_accepted_steps": ml["stats"]["warm_accepted"], "warm_retried_steps": ml["stats"]["warm_retried"],
             "newton_its": ml["stats"]["newton_its"],
             "bicgstab_its": ml["stats"]["lin_its_total"],
@@ -500,7 +556,8 @@ def main():
                          "samples": mlp.size, "adam_steps": args.adam_steps, "batch": 16384,
                          "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-10:].mean()),
                          "seconds_incl_data_generation_and_validation": round(t_train, 2)},
-            "validation": {"case": val_desc, "weight": "BiCGStab its + 15 x Newton its",
+            "validation": {"case": val_desc,
+                           "weight": f"BiCGStab its + {5 if pc == L.PC_SGS else 15} x Newton its",
                            "checkpoints": mlp.validation["checkpoints"], "chosen": mlp.validation["chosen"],
                            "costs": [c["cost"] for c in mlp.validation["costs"]], "cost_without_model": val_plain["cost"],
                            "chosen_its": {k: mlp.validation["costs"][mlp.validation["chosen"]][k]

@@ -75,7 +75,16 @@ typedef struct {
                                 level, a failed quarter again as four, down to dt / 4^n; the step then completes
                                 at full dt and the next one runs at dt.  A member that fails at dt / 4^n stops at
                                 the last sub-step it completed and drops out. */
+    int precond;             /* ADHK_PC_BJ (default) or ADHK_PC_SGS; an ensemble (more than one instance)
+                                always takes ADHK_PC_BJ */
 } adhk_solver;
+
+/* preconditioner of the BiCGStab solve.  The reference's is an exact LU (bcgstab_solver.c:371-424);
+ * neither choice here has a counterpart there. */
+#define ADHK_PC_BJ  0 /* 3x3 nodal block-Jacobi, folded into the matrix (M^-1 A) */
+#define ADHK_PC_SGS 1 /* multicolour symmetric Gauss-Seidel on the block-Jacobi-folded matrix, applied on both
+                         sides through Eisenstat's identity (one pass over the matrix per product); falls back
+                         to ADHK_PC_BJ for a solve whose folded diagonal blocks are not all the identity */
 
 typedef struct {
     int newton_its;
@@ -188,6 +197,13 @@ int adhk_sw2_set_tuning(adhk_sw2 *c, const adhk_tuning *t);
  * The state is restored afterwards.  chosen (or NULL) receives the choice, ms[4] (or NULL) the best
  * times of the element kernels and of one BiCGStab iteration. */
 int adhk_sw2_autotune(adhk_sw2 *c, const adhk_solver *s, int ninner, adhk_tuning *chosen, double *ms);
+
+/* node colouring of the multicolour SGS preconditioner (ADHK_PC_SGS; no counterpart in the reference):
+ * greedy in natural node order, a node takes the smallest colour that none of its lower-numbered block
+ * neighbours holds.  colour [nnodes] (or NULL) and the colour count; layout [3] (or NULL): the permuted
+ * row count (each colour padded to a multiple of 256) and the SELL-64 slot columns of the U and L parts.
+ * One mesh alone. */
+int adhk_sw2_colouring(adhk_sw2 *c, int *colour, int *ncolour, int *layout);
 
 /* replaces fe_sw2_init (src/models/sw2/fe_sw2_init.c:28) incl. fe_sw2_wdflag_legacy (sw2_wdflag_legacy.c:34) */
 int adhk_sw2_init(adhk_sw2 *c, int *wd_flag_out /* [nelems2d] or NULL */);

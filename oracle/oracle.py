@@ -29,7 +29,10 @@ class OrParams(C.Structure):
 
 class OrSolver(C.Structure):
     _fields_ = [("tol_nonlin", C.c_double), ("inc_nonlin", C.c_double), ("max_nonlin_it", C.c_int),
-                ("max_linesearch_cuts", C.c_int), ("lin_tol", C.c_double), ("lin_max_it", C.c_int)]
+                ("max_linesearch_cuts", C.c_int), ("lin_tol", C.c_double), ("lin_max_it", C.c_int),
+                ("precond", C.c_int)]
+
+PREC_BJ, PREC_SGS = 0, 1
 
 
 class OrStats(C.Structure):
@@ -86,6 +89,8 @@ def load():
         "or_block_jacobi_prep": (None, [C.c_int, ip, ip, dp, dp]),
         "or_block_jacobi_apply": (None, [C.c_int, dp, dp, dp]),
         "or_bicgstab": (C.c_int, [C.c_int, ip, ip, dp, dp, dp, dp, dp, C.c_double, C.c_int, dp]),
+        "or_colour_greedy": (C.c_int, [C.c_int, ip, ip, ip]),
+        "or_bicgstab_sgs": (C.c_int, [C.c_int, ip, ip, dp, dp, ip, C.c_int, dp, dp, dp, C.c_double, C.c_int, dp]),
         "or_residual_norms": (None, [C.c_int, dp, dp, ip, dp, dp, dp]),
         "or_fe_newton": (C.c_int, [P, C.POINTER(OrSolver), C.POINTER(OrStats)]),
         "or_time_loop": (C.c_int, [P, C.POINTER(OrSolver), C.c_int, C.POINTER(OrStats)]),
@@ -108,7 +113,7 @@ def _i(a):
 
 def default_solver(**kw):
     s = OrSolver(tol_nonlin=1e-8, inc_nonlin=1e-6, max_nonlin_it=10, max_linesearch_cuts=5, lin_tol=1e-11,
-                 lin_max_it=100)
+                 lin_max_it=100, precond=PREC_BJ)
     for k, v in kw.items():
         setattr(s, k, v)
     return s
@@ -289,3 +294,23 @@ def bicgstab(indptr, cols, vals, minv, b, scale, x0=None, tol=1e-11, max_it=100)
     it = lib.or_bicgstab(n, _i(indptr), _i(cols), _d(vals), _d(minv), _d(x), _d(bb), _d(scale), tol, max_it,
                          C.byref(rn))
     return x, it, rn.value, bb
+
+
+def colour_greedy(indptr, cols):
+    """node colours of the multicolour preconditioner (greedy, natural order) and their count"""
+    lib = load()
+    nn = (len(indptr) - 1) // 3
+    col = np.zeros(nn, np.int32)
+    nc = lib.or_colour_greedy(nn, _i(indptr), _i(cols), _i(col))
+    return col, nc
+
+
+def bicgstab_sgs(indptr, cols, vals, minv, b, scale, colour, ncolour, x0=None, tol=1e-11, max_it=100):
+    """split multicolour-SGS BiCGStab; it = -1 when a diagonal block is singular"""
+    lib = load()
+    n = len(b)
+    x = np.zeros(n) if x0 is None else x0.copy()
+    rn = C.c_double()
+    it = lib.or_bicgstab_sgs(n, _i(indptr), _i(cols), _d(vals), _d(minv), _i(colour), ncolour, _d(x), _d(b),
+                             _d(scale), tol, max_it, C.byref(rn))
+    return x, it, rn.value

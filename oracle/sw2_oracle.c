@@ -53,6 +53,7 @@ struct or_ctx {
     double *vals;
     int *pos_tri, *pos_seg; /* CSR slot of each elemental matrix entry */
     double *ebuf;           /* elemental work buffer */
+    int *colour, ncolour;   /* node colours of the multicolour preconditioner (built on first use) */
 };
 
 void or_set_threads(int n)
@@ -935,6 +936,7 @@ void or_destroy(or_ctx *c)
     free(c->sol); free(c->sol_old); free(c->sol_older); free(c->dirichlet);
     free(c->dsol); free(c->residual); free(c->scale); free(c->bc_mask); free(c->wd_flag);
     free(c->indptr); free(c->cols); free(c->vals); free(c->pos_tri); free(c->pos_seg); free(c->ebuf);
+    free(c->colour);
     free(c);
 }
 
@@ -1312,6 +1314,199 @@ int or_bicgstab(int n, const int *indptr, const int *cols, const double *vals, c
     return it;
 }
 
+/* ---- multicolour symmetric Gauss-Seidel, split form (no counterpart in the reference, whose
+ * preconditioner is an exact LU, bcgstab_solver.c:371-424) ----
+ *
+ * Greedy node colouring in natural order: node i takes the smallest colour that none of its block
+ * neighbours j < i holds.  Returns the number of colours. */
+int or_colour_greedy(int nnodes, const int *indptr, const int *cols, int *colour)
+{
+    int nc = 0;
+    for (int i = 0; i < nnodes; i++) {
+        unsigned long long used = 0;
+        const int r = 3 * i;
+        for (int k = indptr[r]; k < indptr[r + 1]; k += 3) {
+            const int j = cols[k] / 3;
+            if (j < i && colour[j] < 64) used |= 1ULL << colour[j];
+        }
+        int c = 0;
+        while (c < 64 && (used >> c & 1ULL)) c++;
+        colour[i] = c;
+        if (c + 1 > nc) nc = c + 1;
+    }
+    return nc;
+}
+
+/* The block-Jacobi-folded matrix F = D^-1 A (D: the 3x3 diagonal blocks) splits by colour into
+ * F = I + L + U, L holding the blocks (i, j) with colour(j) < colour(i), U those with colour(j) >
+ * colour(i).  The preconditioner M = (I + L)(I + U) is symmetric Gauss-Seidel in colour order, applied
+ * on both sides: BiCGStab solves (I+L)^-1 F (I+U)^-1 y = (I+L)^-1 D^-1 b and x = (I+U)^-1 y.
+ * Eisenstat's identity, F = (I+L) + (I+U) - I, turns one product with the preconditioned operator into
+ * one backward and one forward sweep,
+ *     t = (I+U)^-1 v,   z = t + (I+L)^-1 (v - t),
+ * a single pass over F.  A row's blocks are summed in ascending column order. */
+typedef struct {
+    int nn, nc;
+    const int *colour;
+    int *fptr, *fcol; /* off-diagonal blocks of F per node, ascending column */
+    double *fval;     /* 9 per block, row-major */
+    int *ord, *cstart; /* nodes by colour (ascending within a colour), colour c at [cstart[c], cstart[c+1]) */
+} or_sgs;
+
+static void sgs_free(or_sgs *S)
+{
+    free(S->fptr); free(S->fcol); free(S->fval); free(S->ord); free(S->cstart);
+}
+
+/* builds F from the scaled CSR system and minv; -1 if a diagonal block was singular (its folded
+ * diagonal is not the identity) */
+static int sgs_build(or_sgs *S, int nn, const int *indptr, const int *cols, const double *vals, const double *minv,
+                     const int *colour, int nc)
+{
+    memset(S, 0, sizeof *S);
+    S->nn = nn;
+    S->nc = nc;
+    S->colour = colour;
+    for (int i = 0; i < nn; i++) {
+        double a[9];
+        for (int r = 0; r < 3; r++)
+            for (int cc = 0; cc < 3; cc++) {
+                const int idx = bsearch_part(cols, indptr[3 * i + r], indptr[3 * i + r + 1], 3 * i + cc);
+                a[3 * r + cc] = (idx >= 0) ? vals[idx] : 0.0;
+            }
+        const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) + a[1] * (a[5] * a[6] - a[3] * a[8]) +
+                           a[2] * (a[3] * a[7] - a[4] * a[6]);
+        if (!(fabs(det) > 1e-300)) return -1;
+    }
+    S->fptr = malloc(((size_t)nn + 1) * sizeof(int));
+    S->fptr[0] = 0;
+    for (int i = 0; i < nn; i++) S->fptr[i + 1] = S->fptr[i] + (indptr[3 * i + 1] - indptr[3 * i]) / 3 - 1;
+    const int nb = S->fptr[nn];
+    S->fcol = malloc((size_t)(nb > 0 ? nb : 1) * sizeof(int));
+    S->fval = malloc((size_t)(nb > 0 ? nb : 1) * 9 * sizeof(double));
+    for (int i = 0; i < nn; i++) {
+        const double *m = minv + 9 * (size_t)i;
+        int q = S->fptr[i];
+        for (int k = indptr[3 * i]; k < indptr[3 * i + 1]; k += 3) {
+            const int j = cols[k] / 3;
+            if (j == i) continue;
+            double a[9], *f = S->fval + 9 * (size_t)q;
+            for (int r = 0; r < 3; r++)
+                for (int cc = 0; cc < 3; cc++) a[3 * r + cc] = vals[indptr[3 * i + r] + (k - indptr[3 * i]) + cc];
+            for (int r = 0; r < 3; r++)
+                for (int cc = 0; cc < 3; cc++)
+                    f[3 * r + cc] = m[3 * r] * a[cc] + m[3 * r + 1] * a[3 + cc] + m[3 * r + 2] * a[6 + cc];
+            S->fcol[q++] = j;
+        }
+    }
+    S->cstart = calloc((size_t)nc + 1, sizeof(int));
+    S->ord = malloc((size_t)(nn > 0 ? nn : 1) * sizeof(int));
+    for (int i = 0; i < nn; i++) S->cstart[colour[i] + 1]++;
+    for (int c = 0; c < nc; c++) S->cstart[c + 1] += S->cstart[c];
+    int *fill = calloc((size_t)nc, sizeof(int));
+    for (int i = 0; i < nn; i++) S->ord[S->cstart[colour[i]] + fill[colour[i]]++] = i;
+    free(fill);
+    return 0;
+}
+
+/* out_i = w_i - sum over the row's blocks in part (upper: colour(j) > colour(i), else lower) of F_ij y_j */
+static void sgs_row(const or_sgs *S, int i, int upper, const double *w, const double *y, double *out)
+{
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int ci = S->colour[i];
+    for (int q = S->fptr[i]; q < S->fptr[i + 1]; q++) {
+        const int j = S->fcol[q], cj = S->colour[j];
+        if (upper ? (cj <= ci) : (cj >= ci)) continue;
+        const double *f = S->fval + 9 * (size_t)q, *yj = y + 3 * (size_t)j;
+        for (int r = 0; r < 3; r++) acc[r] += f[3 * r] * yj[0] + f[3 * r + 1] * yj[1] + f[3 * r + 2] * yj[2];
+    }
+    for (int r = 0; r < 3; r++) out[3 * i + r] = w[3 * i + r] - acc[r];
+}
+
+/* y = (I+U)^-1 w: colours from last to first */
+static void sgs_backward(const or_sgs *S, const double *w, double *y)
+{
+    for (int c = S->nc - 1; c >= 0; c--)
+#pragma omp parallel for schedule(static)
+        for (int k = S->cstart[c]; k < S->cstart[c + 1]; k++) sgs_row(S, S->ord[k], 1, w, y, y);
+}
+
+/* y = (I+L)^-1 w: colours from first to last */
+static void sgs_forward(const or_sgs *S, const double *w, double *y)
+{
+    for (int c = 0; c < S->nc; c++)
+#pragma omp parallel for schedule(static)
+        for (int k = S->cstart[c]; k < S->cstart[c + 1]; k++) sgs_row(S, S->ord[k], 0, w, y, y);
+}
+
+/* z = (I+L)^-1 F (I+U)^-1 v by Eisenstat's identity; t, u: work vectors */
+static void sgs_op(const or_sgs *S, int n, const double *v, double *z, double *t, double *u)
+{
+    sgs_backward(S, v, t);
+    for (int i = 0; i < n; i++) z[i] = v[i] - t[i];
+    sgs_forward(S, z, u);
+    for (int i = 0; i < n; i++) z[i] = t[i] + u[i];
+}
+
+/* BiCGStab of bcgstab_solver.c:65-335 on the split-preconditioned system above: the initial guess is
+ * the preconditioned right-hand side (y0 = (I+L)^-1 D^-1 b, as x0 = M^-1 b there), the stop test
+ * |r~|inf <= tol + tol |b~|inf on the split residual r~ = (I+L)^-1 D^-1 (b - A x).  Returns the iteration
+ * count, or -1 when a diagonal block is singular (the caller then takes block-Jacobi).  x holds the
+ * unscaled increment on exit; b is left as it was given. */
+int or_bicgstab_sgs(int n, const int *indptr, const int *cols, const double *vals, const double *minv,
+                    const int *colour, int ncolour, double *x, const double *b, const double *scale, double tol,
+                    int max_it, double *final_rnorm)
+{
+    const int nn = n / 3;
+    or_sgs S;
+    if (sgs_build(&S, nn, indptr, cols, vals, minv, colour, ncolour)) {
+        sgs_free(&S);
+        return -1;
+    }
+    double *r = calloc((size_t)n, sizeof(double)), *p = calloc((size_t)n, sizeof(double));
+    double *Ap = calloc((size_t)n, sizeof(double)), *q = calloc((size_t)n, sizeof(double));
+    double *s = calloc((size_t)n, sizeof(double)), *As = calloc((size_t)n, sizeof(double));
+    double *t = calloc((size_t)n, sizeof(double)), *u = calloc((size_t)n, sizeof(double));
+    double *bt = calloc((size_t)n, sizeof(double)), *y = calloc((size_t)n, sizeof(double));
+    double *x0 = malloc((size_t)n * sizeof(double));
+    memcpy(x0, x, (size_t)n * sizeof(double));
+    or_block_jacobi_apply(nn, minv, b, t);
+    sgs_forward(&S, t, bt);
+    memcpy(y, bt, (size_t)n * sizeof(double));
+    sgs_op(&S, n, y, Ap, t, u);
+    for (int i = 0; i < n; i++) r[i] = bt[i] + (-1) * Ap[i];
+    memset(Ap, 0, (size_t)n * sizeof(double));
+    memcpy(q, r, (size_t)n * sizeof(double));
+    double rnorm = vinf(n, r), bnorm = vinf(n, bt);
+    double rhop = 1.0, alpha = 1.0, omega = 1.0, rho, beta, gamma;
+    int it = 0;
+    rho = vdot(n, r, q);
+    while (rnorm > (tol + bnorm * tol) && it < max_it) {
+        it++;
+        beta = (rho * alpha) / (rhop * omega);
+        for (int i = 0; i < n; i++) p[i] = (p[i] + (-omega) * Ap[i]) * beta + r[i];
+        sgs_op(&S, n, p, Ap, t, u);
+        gamma = vdot(n, q, Ap);
+        alpha = rho / gamma;
+        for (int i = 0; i < n; i++) s[i] = r[i] + (-alpha) * Ap[i];
+        sgs_op(&S, n, s, As, t, u);
+        omega = vdot(n, As, s) / vdot(n, As, As);
+        rhop = rho;
+        rho = -1.0 * omega * vdot(n, As, q);
+        for (int i = 0; i < n; i++) {
+            y[i] = (y[i] + alpha * p[i]) + omega * s[i];
+            r[i] = s[i] + (-omega) * As[i];
+        }
+        rnorm = vinf(n, r);
+    }
+    sgs_backward(&S, y, x);
+    or_unscale(n, x, x0, scale);
+    if (final_rnorm) *final_rnorm = rnorm;
+    free(r); free(p); free(Ap); free(q); free(s); free(As); free(t); free(u); free(bt); free(y); free(x0);
+    sgs_free(&S);
+    return it;
+}
+
 /* get_residual_norms, newton_tools.c:303-373 (l2 = sqrt(sum r^2) / ndofs as coded) */
 void or_residual_norms(int n, const double *residual, const double *dsol, const int *bc_mask, double *rmax,
                        double *rl2, double *incmax)
@@ -1364,7 +1559,17 @@ int or_fe_newton(or_ctx *c, const or_solver *sv, or_newton_stats *st)
         memset(c->dsol, 0, (size_t)n * sizeof(double));
         or_scale_linear_system(n, c->indptr, c->cols, c->vals, c->residual, c->dsol, c->scale);
         or_block_jacobi_prep(c->m.nnodes, c->indptr, c->cols, c->vals, minv);
-        int lits = or_bicgstab(n, c->indptr, c->cols, c->vals, minv, c->dsol, c->residual, c->scale, sv->lin_tol,
+        int lits = -1;
+        if (sv->precond == OR_PREC_SGS) {
+            if (!c->colour) {
+                c->colour = malloc((size_t)(c->m.nnodes > 0 ? c->m.nnodes : 1) * sizeof(int));
+                c->ncolour = or_colour_greedy(c->m.nnodes, c->indptr, c->cols, c->colour);
+            }
+            lits = or_bicgstab_sgs(n, c->indptr, c->cols, c->vals, minv, c->colour, c->ncolour, c->dsol, c->residual,
+                                   c->scale, sv->lin_tol, sv->lin_max_it, NULL);
+        }
+        if (lits < 0) /* block-Jacobi, also where a diagonal block is singular */
+            lits = or_bicgstab(n, c->indptr, c->cols, c->vals, minv, c->dsol, c->residual, c->scale, sv->lin_tol,
                                sv->lin_max_it, NULL);
         loc.lin_its_total += lits;
         if (lits > loc.lin_its_max) loc.lin_its_max = lits;

@@ -69,7 +69,11 @@ typedef struct {
     int max_linesearch_cuts;      /* 5     :64 */
     double lin_tol;               /* 1e-11 bcgstab_solver.c:20-21 (abs and rel) */
     int lin_max_it;               /* 100   bcgstab_solver.c:22 */
+    int precond;                  /* OR_PREC_BJ, or OR_PREC_SGS (no counterpart in the reference) */
 } or_solver;
+
+#define OR_PREC_BJ  0 /* 3x3 nodal block-Jacobi in place of the reference's exact LU */
+#define OR_PREC_SGS 1 /* multicolour symmetric Gauss-Seidel on the block-Jacobi-folded matrix, split form */
 
 typedef struct {
     int newton_its;
@@ -132,6 +136,10 @@ void or_block_jacobi_prep(int nnodes, const int *indptr, const int *cols, const 
 void or_block_jacobi_apply(int nnodes, const double *minv, const double *b, double *x);
 int  or_bicgstab(int n, const int *indptr, const int *cols, const double *vals, const double *minv,
                  double *x, double *b, const double *scale, double tol, int max_it, double *final_rnorm);
+int  or_colour_greedy(int nnodes, const int *indptr, const int *cols, int *colour);
+int  or_bicgstab_sgs(int n, const int *indptr, const int *cols, const double *vals, const double *minv,
+                     const int *colour, int ncolour, double *x, const double *b, const double *scale, double tol,
+                     int max_it, double *final_rnorm);
 void or_residual_norms(int n, const double *residual, const double *dsol, const int *bc_mask,
                        double *rmax, double *rl2, double *incmax);
 

@@ -1,0 +1,151 @@
+"""GPU multicolour SGS preconditioner (split form, Eisenstat products) against the oracle's restatement.
+
+The colouring and the permuted layout are integer work: bit-exact.  The Krylov sums run in another
+order on the GPU than in the oracle, so iterates agree to rounding, not bit for bit; the bars are those
+of the block-Jacobi path: Newton iteration counts equal, BiCGStab totals within 3 + 10 %, depth after N
+steps 1e-8 relative (velocity 1e-6), mixed precision depth 1e-6 relative to FP64.  Schedules (graphs,
+chunks) must not change a bit.
+"""
+import numpy as np
+import pytest
+
+import adh_kraken_amd as A
+import oracle as O
+from adh_kraken_amd import _lib as L
+from helpers import perturbed_state, rel_err
+
+pytestmark = pytest.mark.gpu
+
+CHAN = A.ChannelSpec(length=500.0, width=150.0, spacing=3.0, chan_width=40.0, bank_width=10.0, meander_amp=20.0,
+                     meander_wavelength=250.0, nbumps=5, inflow_q=60.0, fp_depth_out=1.0, seed=3)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return A.flume_mesh(A.FLUME_C1)
+
+
+@pytest.fixture(scope="module")
+def chan():
+    return A.channel_mesh(CHAN)
+
+
+def sgs_model(mesh, dt, sol0, tuning=None):
+    sm = A.SModelSuper(mesh, dt=dt)
+    sm.solver.precond = L.PC_SGS
+    if tuning:
+        sm.tuning = tuning
+    sm.set_initial(sol0)
+    return sm
+
+
+def test_colouring_matches_oracle(gpu, c1, chan):
+    for mesh, _ in (c1, chan):
+        sm = A.SModelSuper(mesh, dt=1.0)
+        orc = O.Oracle(mesh, dt=1.0, threads=8)
+        ip_, cols = orc.pattern()
+        co, nco = O.colour_greedy(ip_, cols)
+        cg, ncg = sm.colouring()
+        assert ncg == nco and np.array_equal(cg, co)
+
+
+@pytest.mark.parametrize("which", ["c1", "chan"])
+def test_time_loop_sgs_matches_oracle(gpu, c1, chan, which):
+    mesh, sol0 = c1 if which == "c1" else chan
+    dt, steps = (1.0, 10) if which == "c1" else (5.0, 6)
+    print(f"sgs time loop {which}: {mesh.nnodes} nodes, {steps} steps", flush=True)
+    sm = sgs_model(mesh, dt, sol0)
+    orc = O.Oracle(mesh, dt=dt, threads=8)
+    orc.set_initial(sol0)
+    ng, sg = A.time_loop(sm, steps)
+    no, so = orc.time_loop(steps, O.default_solver(precond=O.PREC_SGS))
+    assert ng == no == steps
+    assert sg["newton_its"] == so["newton_its"]
+    assert abs(sg["lin_its_total"] - so["lin_its_total"]) <= 3 + so["lin_its_total"] // 10
+    gs, os_ = sm.sol, orc.state()[0]
+    assert rel_err(gs[0::3], os_[0::3]) <= 1e-8
+    assert rel_err(gs[1::3], os_[1::3]) <= 1e-6 and rel_err(gs[2::3], os_[2::3]) <= 1e-6
+
+
+def test_sgs_takes_fewer_iterations_to_the_block_jacobi_state(gpu, chan):
+    mesh, sol0 = chan
+    a = A.SModelSuper(mesh, dt=5.0)
+    a.set_initial(sol0)
+    b = sgs_model(mesh, 5.0, sol0)
+    na, sa = A.time_loop(a, 6)
+    nb, sb = A.time_loop(b, 6)
+    assert na == nb == 6
+    assert sb["lin_its_total"] < sa["lin_its_total"]
+    assert rel_err(b.sol[0::3], a.sol[0::3]) <= 1e-8
+
+
+def test_sgs_newton_one_step_from_a_perturbed_state(gpu, c1):
+    mesh, sol0 = c1
+    s, so = perturbed_state(sol0, seed=4, amp_h=0.02, amp_v=0.1)
+    sm = A.SModelSuper(mesh, dt=1.0)
+    sm.solver.precond = L.PC_SGS
+    sm.sol, sm.sol_old, sm.sol_older = s, so, so
+    orc = O.Oracle(mesh, dt=1.0, threads=8)
+    orc.set_state(s, so, so)
+    ok_g, sg = A.fe_newton(sm)
+    ok_o, so_ = orc.fe_newton(O.default_solver(precond=O.PREC_SGS))
+    assert ok_g == ok_o == 1 and sg["newton_its"] == so_["newton_its"]
+    assert rel_err(sm.sol[0::3], orc.state()[0][0::3]) <= 1e-8
+
+
+def test_sgs_mixed_precision(gpu, chan):
+    mesh, sol0 = chan
+    a = sgs_model(mesh, 5.0, sol0)
+    b = sgs_model(mesh, 5.0, sol0)
+    na, _ = A.time_loop(a, 6, precision=L.PREC_FP64)
+    nb, stb = A.time_loop(b, 6, precision=L.PREC_MIXED)
+    assert na == nb == 6
+    assert stb["resid_max_norm"] <= 1e-8 or stb["inc_max_norm"] <= 1e-6
+    assert rel_err(b.sol[0::3], a.sol[0::3]) <= 1e-6
+
+
+@pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
+def test_sgs_schedules_change_no_bit(gpu, c1, precision):
+    mesh, sol0 = c1
+    runs = []
+    for tuning in (None, {"graphs": 1, "chunk": 8}, {"graphs": 0, "chunk": 4, "xcd_map": 0},
+                   {"graphs": 1, "chunk": 16, "split": 1}):
+        sm = sgs_model(mesh, 1.0, sol0, tuning)
+        n, st = A.time_loop(sm, 4, precision=precision)
+        runs.append((n, st["newton_its"], st["lin_its_total"], sm.sol.copy()))
+    for r in runs[1:]:
+        assert r[:3] == runs[0][:3] and np.array_equal(r[3], runs[0][3])
+
+
+def test_ensemble_takes_block_jacobi(gpu, c1):
+    """More than one instance: the solver's SGS request is served by block-Jacobi, bit for bit"""
+    mesh, sol0 = c1
+    out = []
+    for pc in (L.PC_BJ, L.PC_SGS):
+        sm = A.SModelSuper.ensemble([mesh, mesh], dt=1.0)
+        sm.solver.precond = pc
+        sm.set_initial(np.concatenate([sol0, sol0]))
+        n, st = A.time_loop(sm, 2)
+        out.append((n, st["lin_its_total"], sm.sol.copy()))
+    assert out[0][:2] == out[1][:2] and np.array_equal(out[0][2], out[1][2])
+
+
+@pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
+def test_sgs_autotune_keeps_results(gpu, c1, precision):
+    mesh, sol0 = c1
+    ref = sgs_model(mesh, 1.0, sol0)
+    n0, st0 = A.time_loop(ref, 4, precision=precision)
+    sm = sgs_model(mesh, 1.0, sol0)
+    choice, t = sm.autotune(precision=precision, ninner=1)
+    assert t["krylov_its"] > 0 and np.array_equal(sm.sol, sol0)
+    n, st = A.time_loop(sm, 4, precision=precision)
+    assert (n, st["lin_its_total"]) == (n0, st0["lin_its_total"]) and np.array_equal(sm.sol, ref.sol)
+
+
+def test_bad_precond_is_an_error(gpu, c1):
+    mesh, sol0 = c1
+    sm = A.SModelSuper(mesh, dt=1.0)
+    sm.set_initial(sol0)
+    sm.solver.precond = 7
+    with pytest.raises(L.AdhkError):
+        A.time_loop(sm, 1)

@@ -25,6 +25,10 @@ fails:
   the reference's ten test-grid geometries (src/test/run_tests.c:42-47)
 * FD Jacobian vs directional differences of the residual
 * BiCGStab/block-Jacobi solution vs a direct sparse solve; block inverses
+* multicolour SGS: the greedy colouring vs a brute-force first-fit colouring (proper, each node the
+  smallest colour free among its lower-numbered neighbours); the split-preconditioned BiCGStab
+  (Eisenstat form) vs a direct sparse solve -- a slip in either sweep or in the identity solves a
+  different system
 * Newton: the converged state satisfies the residual tolerance
 """
 import numpy as np
@@ -451,6 +455,61 @@ def test_block_jacobi_inverse_and_bicgstab(c1):
     Ar = sp.csr_matrix((vals, cols, ip_), shape=(len(b), len(b)))
     xd = spla.spsolve(Ar.tocsc(), b)
     assert rel_err(x, xd) <= 1e-8
+
+
+def _c1_scaled_system(c1, seed=2):
+    mesh, sol0 = c1
+    s, so = perturbed_state(sol0, seed=seed)
+    orc = O.Oracle(mesh, dt=1.0, threads=8)
+    orc.set_state(s, so, so)
+    orc.residual()
+    orc.jacobian()
+    vals, b = orc.system()
+    ip_, cols = orc.pattern()
+    vs, bs, xs, scale = O.scale_linear_system(ip_, cols, vals, b, np.zeros_like(b))
+    return vals, b, ip_, cols, vs, bs, scale
+
+
+def test_greedy_colouring_is_first_fit(c1):
+    _, _, ip_, cols, _, _, _ = _c1_scaled_system(c1)
+    col, nc = O.colour_greedy(ip_, cols)
+    nn = len(col)
+    assert nc == col.max() + 1 and nc >= 3  # a triangle needs three colours
+    for i in range(nn):
+        nb = cols[ip_[3 * i]:ip_[3 * i + 1]:3] // 3
+        nb = nb[nb != i]
+        assert not np.any(col[nb] == col[i])  # proper
+        lower = set(col[nb[nb < i]].tolist())
+        assert col[i] == min(set(range(nc + 1)) - lower)  # first fit in natural order
+
+
+@pytest.mark.parametrize("seed", [2, 5])
+def test_sgs_split_bicgstab_solves_the_system(c1, seed):
+    vals, b, ip_, cols, vs, bs, scale = _c1_scaled_system(c1, seed)
+    minv = O.block_jacobi(ip_, cols, vs)
+    col, nc = O.colour_greedy(ip_, cols)
+    x, it, rn = O.bicgstab_sgs(ip_, cols, vs, minv, bs, scale, col, nc, tol=1e-14, max_it=400)
+    assert 0 < it < 400
+    Ar = sp.csr_matrix((vals, cols, ip_), shape=(len(b), len(b)))
+    xd = spla.spsolve(Ar.tocsc(), b)
+    assert rel_err(x, xd) <= 1e-8
+    # stronger than block-Jacobi: fewer iterations to the same tolerance
+    _, it_bj, _, _ = O.bicgstab(ip_, cols, vs, minv, bs, scale, tol=1e-14, max_it=400)
+    assert it < it_bj
+
+
+def test_sgs_newton_matches_block_jacobi_newton(c1):
+    mesh, sol0 = c1
+    res = []
+    for pc in (O.PREC_BJ, O.PREC_SGS):
+        orc = O.Oracle(mesh, dt=1.0, threads=8)
+        orc.set_initial(sol0)
+        n, st = orc.time_loop(3, O.default_solver(precond=pc))
+        assert n == 3 and st["converged"] == 1
+        res.append((orc.state()[0], st))
+    (sb, stb), (ss, sts) = res
+    assert sts["lin_its_total"] < stb["lin_its_total"]
+    assert rel_err(ss[0::3], sb[0::3]) <= 1e-8
 
 
 def test_newton_converged_state_satisfies_tolerance(c1):
