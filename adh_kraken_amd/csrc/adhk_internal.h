@@ -49,8 +49,8 @@ typedef struct {
     int nanflag, sgs_bad; /* sgs_bad: a folded diagonal block is not the identity (SGS falls back) */
 } adhk_scalars;
 
-/* Multicolour SGS layout (one mesh alone): rows permuted by colour (natural order within a colour), each
- * colour's run padded to a multiple of 256 rows, so a 256-row reduction block and a 64-row slice hold one
+/* Multicolour SGS layout (one mesh alone): rows permuted by colour (within a colour, sorted by their U and L
+ * widths inside windows of 1024 rows), each colour's run padded to a multiple of 256 rows, so a 256-row reduction block and a 64-row slice hold one
  * colour; row Np is a zero row that padding slots point at.  The folded matrix's off-diagonal blocks
  * split into U (neighbour of a higher colour) and L (lower), each its own SELL-64 over the permuted rows
  * (blocks of a row in ascending natural column order); src maps a slot to the natural SELL position whose

@@ -4,9 +4,10 @@
  * exact LU: bcgstab_solver.c:371-424).
  *
  * Colouring: greedy in natural node order -- node i takes the smallest colour that none of its block
- * neighbours j < i holds.  The rows are then permuted by colour (ascending node within a colour) and
- * every colour's run padded to a multiple of 256 rows, so that each 256-row reduction block and each
- * 64-row slice belongs to one colour and a colour's rows form whole workgroups.  The folded matrix's
+ * neighbours j < i holds.  The rows are then permuted by colour and every colour's run padded to a multiple of 256 rows, so that each 256-row reduction block and each
+ * 64-row slice belongs to one colour and a colour's rows form whole workgroups.  Inside a colour the rows
+ * are sorted by their U and L widths within windows of 1024 (rows of one colour do not depend on each
+ * other, so the order is free), which keeps the SELL padding small on meshes of uneven rows.  The folded matrix's
  * off-diagonal blocks go to U (neighbour of a higher colour) or L (lower colour), each stored as its own
  * SELL-64 over the permuted rows, blocks of a row in ascending natural column order.  A padding slot
  * points at the zero row Np, whose vector entries are zero and never written.
@@ -44,6 +45,31 @@ static int colour_greedy(int N, const int *brow, const int *bcol, int *colour)
         if (c + 1 > nc) nc = c + 1;
     }
     return nc;
+}
+
+/* rows of a colour are sorted by this key within windows of SGS_WINDOW rows: U blocks x 64 + L blocks */
+#define SGS_WINDOW 1024
+#define ROW_KEY_MAX (64 * 64)
+static unsigned row_key(int n, const int *brow, const int *bcol, const int *colour)
+{
+    unsigned nu = 0, nl = 0;
+    for (int b = brow[n]; b < brow[n + 1]; b++) {
+        const int j = bcol[b];
+        if (j == n) continue;
+        if (colour[j] > colour[n])
+            nu++;
+        else
+            nl++;
+    }
+    nu = nu < 63 ? nu : 63;
+    nl = nl < 63 ? nl : 63;
+    return nu * 64 + nl;
+}
+
+static int cmp_u64(const void *a, const void *b)
+{
+    const unsigned long long x = *(const unsigned long long *)a, y = *(const unsigned long long *)b;
+    return (x > y) - (x < y);
 }
 
 /* one part (upper: colour(j) > colour(i), else lower) as SELL-64 over the permuted rows */
@@ -161,8 +187,30 @@ int adhk_sgs_setup(adhk_ctx *c)
     for (int n = 0; n < N; n++) {
         const int k = G->colour[n], i = G->cs[k] + fill[k]++;
         prow[i] = n;
-        posof[n] = i;
     }
+    /* Rows of one colour are independent of each other, so their order within the colour is free: inside
+     * windows of SGS_WINDOW rows, sort them by (U blocks, L blocks), descending, ties by node.  A slice then
+     * holds rows of equal widths and pads little on meshes of uneven rows; the windows keep neighbours
+     * close in memory. */
+    {
+        unsigned long long *kv = malloc(SGS_WINDOW * sizeof *kv);
+        if (!kv) {
+            adhk_set_error("adhk_sgs_setup: out of host memory");
+            goto fail;
+        }
+        for (int k = 0; k < G->nc; k++)
+            for (int w0 = G->cs[k]; w0 < G->cs[k] + cnt[k]; w0 += SGS_WINDOW) {
+                const int w1 = w0 + SGS_WINDOW < G->cs[k] + cnt[k] ? w0 + SGS_WINDOW : G->cs[k] + cnt[k];
+                for (int i = w0; i < w1; i++) /* ascending order of (inverted key, node) */
+                    kv[i - w0] = ((unsigned long long)(ROW_KEY_MAX - row_key(prow[i], brow, bcol, G->colour)) << 32) |
+                                 (unsigned)prow[i];
+                qsort(kv, (size_t)(w1 - w0), sizeof *kv, cmp_u64);
+                for (int i = w0; i < w1; i++) prow[i] = (int)(kv[i - w0] & 0xffffffffULL);
+            }
+        free(kv);
+    }
+    for (int i = 0; i < G->Np; i++)
+        if (prow[i] >= 0) posof[prow[i]] = i;
     if (build_part(&G->U, 1, G->Np, prow, G->colour, brow, bcol, b2s, posof)) goto fail;
     if (build_part(&G->L, 0, G->Np, prow, G->colour, brow, bcol, b2s, posof)) goto fail;
     HC(hipMalloc((void **)&G->prow, (size_t)G->Np * sizeof(int)));
