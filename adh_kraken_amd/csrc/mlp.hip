@@ -1,17 +1,19 @@
 /*
  * mlp.hip -- learned warm start of the Newton iterate (gfx950).
  *
- * A per-node MLP, 12 -> 64 -> 64 -> 3 with GELU (erf form), BF16 weights and
- * FP32 arithmetic, predicts the change of (h, u, v) over the next time step
+ * A per-node MLP, 12 -> 64 -> 64 -> 3 with GELU (erf form) and BF16 weights, its 64 x 64 layer on
+ * the matrix cores (BF16 operands, FP32 sums) and the rest in FP32, predicts the change of (h, u, v)
+ * over the next time step
  * from the node's h, hu, hv at t and t-1, the 1-ring neighbour means of h, hu,
  * hv at t, the bed slope and Manning's n (see node_features for the encoding).  It is trained on trajectories that the solver itself
  * produces, on the GPU, by the kernels below:
  *   k_node_static   per-node bed slope and Manning n (area-weighted over the node's elements)
  *   k_collect       features(t, t-1) and target sol(t+1) - sol(t) of every node -> dataset (SoA)
  *   k_stats         per-workgroup sums of each column and its square (normalisation)
- *   k_tile_grad     one wavefront per 64-sample tile: forward, MSE loss, backward; activations
- *                   staged in LDS; the tile's weight gradient is written as one partial row
- *   k_reduce_adam   sums the tile partials in tile order (deterministic) and applies Adam to the
+ *   k_tile_grad     one wavefront per 64-sample tile: forward, MSE loss, backward; the hidden
+ *                   layer's three 64 x 64 x 64 products on the matrix cores, activations staged in
+ *                   LDS; the tile's weight gradient is written as one partial row
+ *   k_reduce_adam   sums the tile partials in a fixed order (deterministic) and applies Adam to the
  *                   FP32 master weights, refreshing their BF16 copy
  *   k_predict       inference straight into the Newton initial iterate
  *
@@ -243,30 +245,175 @@ __device__ __forceinline__ long long batch_index(const Batch &B, int s)
     return (long long)(h % (unsigned long long)B.n);
 }
 
-__device__ __forceinline__ void load_weights(const unsigned short *__restrict__ wbf, float *W)
+/* ---------------- the 64 x 64 layer on the matrix cores ----------------
+ * v_mfma_f32_32x32x16_bf16 computes D (32x32, FP32) = A (32x16, BF16) B (16x32, BF16) + C on one wavefront.
+ * Lane l holds A[l % 32][8 (l / 32) + v] and B[8 (l / 32) + v][l % 32] (v = 0..7, eight BF16 in 16 bytes) and
+ * D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32] in its r-th accumulator (r = 0..15).  A 64 x 64 x 64 product is
+ * 2 x 2 output tiles x 4 K-steps = 16 instructions.  The layer runs transposed, Z2^T = W2 A1^T: the M
+ * dimension is the hidden unit, N the sample, so lane l ends with 32 hidden units of samples l % 32 and
+ * 32 + l % 32, and the output layer needs one exchange with lane l ^ 32 instead of a pass through LDS. */
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(16))) float f32x16;
+#define A1STR 72 /* BF16 per LDS row of a1: 144 bytes, rows 16-byte aligned */
+#define FSTR 68  /* floats per LDS row of a 64-wide FP32 activation tile */
+
+/* the FP32 weights that the VALU layers read, staged in LDS: W1 [64][12], b1, b2, W3 [3][64], b3 */
+#define CW1 0
+#define CB1 (CW1 + HID * NIN)
+#define CB2 (CB1 + HID)
+#define CW3 (CB2 + HID)
+#define CB3 (CW3 + NOUT * HID)
+#define NCOMPACT (CB3 + NOUT)
+
+__device__ __forceinline__ void load_compact(const unsigned short *__restrict__ wbf, float *Wc)
 {
-    for (int p = threadIdx.x; p < NPARAM; p += blockDim.x) W[p] = bf2f(wbf[p]);
+    for (int p = threadIdx.x; p < NCOMPACT; p += blockDim.x) {
+        int q;
+        if (p < CB1)
+            q = OFF_W1 + p;
+        else if (p < CB2)
+            q = OFF_B1 + (p - CB1);
+        else if (p < CW3)
+            q = OFF_B2 + (p - CB2);
+        else if (p < CB3)
+            q = OFF_W3 + (p - CW3);
+        else
+            q = OFF_B3 + (p - CB3);
+        Wc[p] = bf2f(wbf[q]);
+    }
+}
+
+__device__ __forceinline__ bf16x8 ld_bf16x8(const unsigned short *p)
+{
+    const uint4 u = *(const uint4 *)p;
+    return __builtin_bit_cast(bf16x8, u);
+}
+/* eight FP32 values (any stride) rounded to BF16 as the operand of one instruction */
+__device__ __forceinline__ bf16x8 pack_bf16x8(const float v[8])
+{
+    uint4 u;
+    u.x = (unsigned)f2bf(v[0]) | ((unsigned)f2bf(v[1]) << 16);
+    u.y = (unsigned)f2bf(v[2]) | ((unsigned)f2bf(v[3]) << 16);
+    u.z = (unsigned)f2bf(v[4]) | ((unsigned)f2bf(v[5]) << 16);
+    u.w = (unsigned)f2bf(v[6]) | ((unsigned)f2bf(v[7]) << 16);
+    return __builtin_bit_cast(bf16x8, u);
+}
+__device__ __forceinline__ int d_row(int r, int l) { return 8 * (r / 4) + 4 * (l / 32) + (r % 4); }
+
+/* layer 1 of the lane's sample (VALU): z1 = W1 x + b1; its BF16 GELU into row l of A1s (and z1 into Z1, when given) */
+__device__ __forceinline__ void layer1(const float *Wc, const float x[NIN], unsigned short *A1s, float *Z1)
+{
+    const int l = threadIdx.x & 63;
+    for (int j0 = 0; j0 < HID; j0 += 8) {
+        asm volatile("" ::: "memory"); /* one chunk of units' weights live at a time */
+        unsigned short hb[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int j = j0 + u;
+            float zz = Wc[CB1 + j];
+#pragma unroll
+            for (int k = 0; k < NIN; k++) zz += Wc[CW1 + j * NIN + k] * x[k];
+            if (Z1) Z1[l * FSTR + j] = zz;
+            hb[u] = f2bf(gelu(zz));
+        }
+        uint4 u4;
+        u4.x = (unsigned)hb[0] | ((unsigned)hb[1] << 16);
+        u4.y = (unsigned)hb[2] | ((unsigned)hb[3] << 16);
+        u4.z = (unsigned)hb[4] | ((unsigned)hb[5] << 16);
+        u4.w = (unsigned)hb[6] | ((unsigned)hb[7] << 16);
+        *(uint4 *)(A1s + l * A1STR + j0) = u4;
+    }
+}
+
+/* layer 2 on the matrix cores: acc[mt][nt][r] = z2 of hidden unit 32 mt + d_row(r, l), sample 32 nt + l % 32;
+ * W2 row-major BF16 (global), a1 from A1s */
+__device__ __forceinline__ void layer2(const unsigned short *__restrict__ w2, const unsigned short *A1s, const float *Wc,
+                                       f32x16 acc[2][2])
+{
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const float bb = Wc[CB2 + 32 * mt + d_row(r, l)];
+            acc[mt][0][r] = bb;
+            acc[mt][1][r] = bb;
+        }
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) {
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) a[mt] = ld_bf16x8(w2 + (32 * mt + l % 32) * HID + 16 * ks + 8 * (l / 32));
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++) b[nt] = ld_bf16x8(A1s + (32 * nt + l % 32) * A1STR + 16 * ks + 8 * (l / 32));
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+    }
+}
+
+/* output layer: a2 = GELU(z2) from the accumulators into A2 ([sample][unit], FP32, this wave's tile), then
+ * y of the lane's own sample l from its row; every lane of the workgroup must call it (it synchronises the
+ * workgroup).  Writing a2 as it is formed lets the accumulators die one by one instead of holding them, the
+ * GELU temporaries and the output weights of 32 units at once. */
+__device__ __forceinline__ void layer3(const float *Wc, f32x16 acc[2][2], float y[NOUT], float *A2)
+{
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++)
+                A2[(32 * nt + l % 32) * FSTR + 32 * mt + d_row(r, l)] = gelu(acc[mt][nt][r]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NOUT; k++) y[k] = Wc[CB3 + k];
+    for (int i = 0; i < HID; i++) {
+        const float a2 = A2[l * FSTR + i];
+#pragma unroll
+        for (int k = 0; k < NOUT; k++) y[k] += Wc[CW3 + k * HID + i] * a2;
+    }
+}
+
+/* forward pass of a wave's 64-sample tile (lane l: sample l) up to the normalised outputs.  Work is this
+ * wave's LDS region (TILE x FSTR floats): a1 in BF16 first, then a2 over it (the matrix products have
+ * consumed a1 by then).  Every lane of the workgroup must call it (it synchronises the workgroup). */
+__device__ __forceinline__ void tile_forward(const float *Wc, const unsigned short *__restrict__ w2, float *Work,
+                                             const float x[NIN], float y[NOUT])
+{
+    unsigned short *A1s = (unsigned short *)Work;
+    layer1(Wc, x, A1s, nullptr);
+    __syncthreads();
+    f32x16 acc[2][2];
+    layer2(w2, A1s, Wc, acc);
+    layer3(Wc, acc, y, Work);
 }
 
 /* one wavefront = one tile of TILE samples; partial[tile][NPARAM] = d(loss)/d(params) over the tile,
- * loss = 1/(size*NOUT) * sum_s sum_k (y_sk - t_sk)^2 on normalised outputs.
- * Registers hold one sample's a1 and one accumulator vector; rows of W come from LDS in rolled loops,
- * so every register array is indexed by constants.  z2 is recomputed in the backward pass, which also
- * accumulates W2^T dz2 from the same W2 row. */
+ * loss = 1/(size*NOUT) * sum_s sum_k (y_sk - t_sk)^2 on normalised outputs.  The three 64 x 64 x 64
+ * products of the hidden layer (z2, dW2 = dz2^T a1, da1 = dz2 W2) run on the matrix cores with BF16 a1
+ * and dz2; the 12- and 3-wide layers, the biases and every other sum run in FP32 from LDS. */
 __global__ __launch_bounds__(TILE) void k_tile_grad(Batch B, const unsigned short *__restrict__ wbf,
                                                     const float *__restrict__ norm, float *__restrict__ partial,
                                                     float *__restrict__ lossp)
 {
-    __shared__ float W[NPARAM];
-    __shared__ float A[TILE * APAD];  /* a1 rows, later dz1 rows */
-    __shared__ float Bs[TILE * APAD]; /* a2 rows, later dz2 rows */
+    __shared__ float Wc[NCOMPACT];
+    __shared__ __align__(16) unsigned short W2T[HID * A1STR]; /* W2 transposed: [j][i] */
+    __shared__ __align__(16) unsigned short A1s[TILE * A1STR];
     __shared__ float X[TILE * NIN];
+    __shared__ float Z1[TILE * FSTR];  /* z1 [s][j], later dz1 */
+    __shared__ float A2[TILE * FSTR];  /* a2 [s][i], later dz2 */
     __shared__ float DY[TILE * 4];
     __shared__ float red[TILE];
-    const int t = threadIdx.x, tile = blockIdx.x;
+    const int t = threadIdx.x, tile = blockIdx.x, l = t;
     const int s = tile * TILE + t;
     const bool valid = s < B.size;
-    load_weights(wbf, W);
+    const unsigned short *w2 = wbf + OFF_W2;
+    load_compact(wbf, Wc);
+    for (int p = t; p < HID * HID; p += TILE) W2T[(p % HID) * A1STR + p / HID] = w2[p];
     float x[NIN], tn[NOUT];
     if (valid) {
         long long i = batch_index(B, s);
@@ -282,155 +429,202 @@ __global__ __launch_bounds__(TILE) void k_tile_grad(Batch B, const unsigned shor
     }
 #pragma unroll
     for (int k = 0; k < NIN; k++) X[t * NIN + k] = x[k];
+    __syncthreads(); /* the staged weights */
+    /* forward */
+    layer1(Wc, x, A1s, Z1);
     __syncthreads();
-    /* forward: layer 1 (unrolled, a1 in registers) */
-    float a1[HID];
-#pragma unroll
-    for (int i = 0; i < HID; i++) {
-        float zz = W[OFF_B1 + i];
-#pragma unroll
-        for (int j = 0; j < NIN; j++) zz += W[OFF_W1 + i * NIN + j] * x[j];
-        a1[i] = gelu(zz);
-        A[t * APAD + i] = a1[i];
-    }
-    /* layer 2 and output, one W2 row at a time */
+    f32x16 acc[2][2];
+    layer2(w2, A1s, Wc, acc);
     float y[NOUT];
-#pragma unroll
-    for (int k = 0; k < NOUT; k++) y[k] = W[OFF_B3 + k];
-#pragma unroll 1
-    for (int i = 0; i < HID; i++) {
-        float zz = W[OFF_B2 + i];
-#pragma unroll
-        for (int j = 0; j < HID; j++) zz += W[OFF_W2 + i * HID + j] * a1[j];
-        float a2 = gelu(zz);
-        Bs[t * APAD + i] = a2;
-#pragma unroll
-        for (int k = 0; k < NOUT; k++) y[k] += W[OFF_W3 + k * HID + i] * a2;
-    }
-    float l = 0.0f, dy[NOUT];
+    layer3(Wc, acc, y, A2);
+    float lsum = 0.0f, dy[NOUT];
     const float sc = 2.0f / ((float)B.size * (float)NOUT);
 #pragma unroll
     for (int k = 0; k < NOUT; k++) {
         float d = y[k] - tn[k];
-        l += d * d;
+        lsum += d * d;
         dy[k] = valid ? sc * d : 0.0f;
         DY[t * 4 + k] = dy[k];
     }
-    red[t] = valid ? l : 0.0f;
+    red[t] = valid ? lsum : 0.0f;
     __syncthreads();
     float *P = partial + (size_t)tile * NPARAM;
     /* dW3[k][i] = sum_s dy[s][k] a2[s][i], db3, tile loss */
     for (int e = t; e < NOUT * HID; e += TILE) {
         int k = e / HID, i = e % HID;
-        float acc = 0.0f;
-        for (int q = 0; q < TILE; q++) acc += DY[q * 4 + k] * Bs[q * APAD + i];
-        P[OFF_W3 + e] = acc;
+        float a = 0.0f;
+        for (int q = 0; q < TILE; q++) a += DY[q * 4 + k] * A2[q * FSTR + i];
+        P[OFF_W3 + e] = a;
     }
     if (t < NOUT) {
-        float acc = 0.0f;
-        for (int q = 0; q < TILE; q++) acc += DY[q * 4 + t];
-        P[OFF_B3 + t] = acc;
+        float a = 0.0f;
+        for (int q = 0; q < TILE; q++) a += DY[q * 4 + t];
+        P[OFF_B3 + t] = a;
     }
     if (t == 0) {
-        float acc = 0.0f;
-        for (int q = 0; q < TILE; q++) acc += red[q];
-        lossp[tile] = acc;
+        float a = 0.0f;
+        for (int q = 0; q < TILE; q++) a += red[q];
+        lossp[tile] = a;
     }
+    /* dz2 = (W3^T dy) gelu'(z2) in the accumulator layout */
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int i = 32 * mt + d_row(r, l);
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) {
+                const int q = 32 * nt + l % 32;
+                float g = 0.0f;
+#pragma unroll
+                for (int k = 0; k < NOUT; k++) g += Wc[CW3 + k * HID + i] * DY[q * 4 + k];
+                acc[mt][nt][r] = g * gelu_d(acc[mt][nt][r]);
+            }
+        }
     __syncthreads(); /* every read of the a2 rows is done */
-    /* dz2_i = (W3^T dy)_i gelu'(z2_i), z2 recomputed; da1 += W2[i][:] dz2_i */
-    float da1[HID];
 #pragma unroll
-    for (int j = 0; j < HID; j++) da1[j] = 0.0f;
-#pragma unroll 1
-    for (int i = 0; i < HID; i++) {
-        float zz = W[OFF_B2 + i];
+    for (int mt = 0; mt < 2; mt++)
 #pragma unroll
-        for (int j = 0; j < HID; j++) zz += W[OFF_W2 + i * HID + j] * a1[j];
-        float g = 0.0f;
+        for (int r = 0; r < 16; r++)
 #pragma unroll
-        for (int k = 0; k < NOUT; k++) g += W[OFF_W3 + k * HID + i] * dy[k];
-        float dz = g * gelu_d(zz);
-        Bs[t * APAD + i] = dz;
-#pragma unroll
-        for (int j = 0; j < HID; j++) da1[j] += W[OFF_W2 + i * HID + j] * dz;
-    }
-    /* dz1 = da1 * gelu'(z1), z1 recomputed (held in da1) */
-#pragma unroll
-    for (int j = 0; j < HID; j++) {
-        float zz = W[OFF_B1 + j];
-#pragma unroll
-        for (int k = 0; k < NIN; k++) zz += W[OFF_W1 + j * NIN + k] * x[k];
-        da1[j] *= gelu_d(zz);
-    }
-    __syncthreads();
-    /* dW2[i][j] = sum_s dz2[s][i] a1[s][j]: thread t owns rows 8*(t%8).., columns 8*(t/8).. */
+            for (int nt = 0; nt < 2; nt++) A2[(32 * nt + l % 32) * FSTR + 32 * mt + d_row(r, l)] = acc[mt][nt][r];
+    __syncthreads(); /* dz2 [s][i] complete */
+    /* dW2 = dz2^T a1 (M = unit i, N = input unit j, K = sample): A[i][s] = dz2, B[s][j] = a1, both strided */
     {
-        const int i0 = 8 * (t % 8), j0 = 8 * (t / 8);
-        float acc[8][8];
+        f32x16 g2[2][2];
 #pragma unroll
-        for (int a = 0; a < 8; a++)
+        for (int mt = 0; mt < 2; mt++)
 #pragma unroll
-            for (int b = 0; b < 8; b++) acc[a][b] = 0.0f;
-#pragma unroll 2
-        for (int q = 0; q < TILE; q++) {
-            float dz[8], av[8];
+            for (int nt = 0; nt < 2; nt++)
 #pragma unroll
-            for (int a = 0; a < 8; a++) dz[a] = Bs[q * APAD + i0 + a];
+                for (int r = 0; r < 16; r++) g2[mt][nt][r] = 0.0f;
 #pragma unroll
-            for (int b = 0; b < 8; b++) av[b] = A[q * APAD + j0 + b];
+        for (int ks = 0; ks < 4; ks++) {
+            const int q0 = 16 * ks + 8 * (l / 32);
+            bf16x8 a[2], b[2];
 #pragma unroll
-            for (int a = 0; a < 8; a++)
+            for (int mt = 0; mt < 2; mt++) {
+                float v[8];
 #pragma unroll
-                for (int b = 0; b < 8; b++) acc[a][b] += dz[a] * av[b];
+                for (int u = 0; u < 8; u++) v[u] = A2[(q0 + u) * FSTR + 32 * mt + l % 32];
+                a[mt] = pack_bf16x8(v);
+            }
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) {
+                uint4 u4;
+                const unsigned short *c = A1s + q0 * A1STR + 32 * nt + l % 32;
+                u4.x = (unsigned)c[0] | ((unsigned)c[A1STR] << 16);
+                u4.y = (unsigned)c[2 * A1STR] | ((unsigned)c[3 * A1STR] << 16);
+                u4.z = (unsigned)c[4 * A1STR] | ((unsigned)c[5 * A1STR] << 16);
+                u4.w = (unsigned)c[6 * A1STR] | ((unsigned)c[7 * A1STR] << 16);
+                b[nt] = __builtin_bit_cast(bf16x8, u4);
+            }
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++)
+                    g2[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[mt], b[nt], g2[mt][nt], 0, 0, 0);
         }
 #pragma unroll
-        for (int a = 0; a < 8; a++)
+        for (int mt = 0; mt < 2; mt++)
 #pragma unroll
-            for (int b = 0; b < 8; b++) P[OFF_W2 + (i0 + a) * HID + j0 + b] = acc[a][b];
+            for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+                for (int r = 0; r < 16; r++)
+                    P[OFF_W2 + (32 * mt + d_row(r, l)) * HID + 32 * nt + l % 32] = g2[mt][nt][r];
     }
+    /* db2[i] = sum_s dz2[s][i] (FP32) */
     {
-        float acc = 0.0f;
-        for (int q = 0; q < TILE; q++) acc += Bs[q * APAD + t];
-        P[OFF_B2 + t] = acc;
+        float a = 0.0f;
+        for (int q = 0; q < TILE; q++) a += A2[q * FSTR + t];
+        P[OFF_B2 + t] = a;
     }
-    __syncthreads(); /* every read of the a1 rows is done */
+    /* da1^T = W2^T dz2^T (M = input unit j, N = sample, K = unit i): A[j][i] = W2T rows, B[i][s] = dz2 rows */
+    {
+        f32x16 d1[2][2];
 #pragma unroll
-    for (int j = 0; j < HID; j++) A[t * APAD + j] = da1[j];
-    __syncthreads();
-    /* dW1[i][k] = sum_s dz1[s][i] x[s][k]: 768 entries, 12 per thread; db1 */
+        for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) d1[mt][nt][r] = 0.0f;
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            const int i0 = 16 * ks + 8 * (l / 32);
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++) a[mt] = ld_bf16x8(W2T + (32 * mt + l % 32) * A1STR + i0);
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) {
+                float v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) v[u] = A2[(32 * nt + l % 32) * FSTR + i0 + u];
+                b[nt] = pack_bf16x8(v);
+            }
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++)
+                    d1[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[mt], b[nt], d1[mt][nt], 0, 0, 0);
+        }
+        /* dz1 = da1 gelu'(z1), over z1 in place (each (sample, unit) pair belongs to one lane) */
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    float *z = Z1 + (32 * nt + l % 32) * FSTR + 32 * mt + d_row(r, l);
+                    *z = d1[mt][nt][r] * gelu_d(*z);
+                }
+    }
+    __syncthreads(); /* dz1 [s][j] complete */
+    /* dW1[j][k] = sum_s dz1[s][j] x[s][k]: 768 entries, 12 per thread; db1 */
     for (int e = t; e < HID * NIN; e += TILE) {
-        int i = e / NIN, k = e % NIN;
-        float acc = 0.0f;
-        for (int q = 0; q < TILE; q++) acc += A[q * APAD + i] * X[q * NIN + k];
-        P[OFF_W1 + e] = acc;
+        int j = e / NIN, k = e % NIN;
+        float a = 0.0f;
+        for (int q = 0; q < TILE; q++) a += Z1[q * FSTR + j] * X[q * NIN + k];
+        P[OFF_W1 + e] = a;
     }
     {
-        float acc = 0.0f;
-        for (int q = 0; q < TILE; q++) acc += A[q * APAD + t];
-        P[OFF_B1 + t] = acc;
+        float a = 0.0f;
+        for (int q = 0; q < TILE; q++) a += Z1[q * FSTR + t];
+        P[OFF_B1 + t] = a;
     }
 }
 
-/* grad[p] = sum over tiles in order; adam != 0: Adam step on master, BF16 copy refreshed */
-__global__ __launch_bounds__(256) void k_reduce_adam(int ntiles, const float *__restrict__ partial,
-                                                     const float *__restrict__ lossp, float *grad_out,
-                                                     double *loss_out, int adam, float lr, float b1c, float b2c,
-                                                     float *master, float *m1, float *m2, unsigned short *wbf)
+/* grad[p] = sum over the tiles; adam != 0: Adam step on master, BF16 copy refreshed.  A workgroup takes
+ * RPW parameters and RG threads per parameter: thread g sums tiles g, g + RG, g + 2 RG, ... in that order,
+ * then the RG partial sums are added in g order (deterministic; 32 consecutive parameters per tile row
+ * make each load one 128-byte run).  Block 0 also sums the tile losses. */
+#define RPW 32
+#define RG 8
+__global__ __launch_bounds__(RPW * RG) void k_reduce_adam(int ntiles, const float *__restrict__ partial,
+                                                          const float *__restrict__ lossp, float *grad_out,
+                                                          double *loss_out, int adam, float lr, float b1c, float b2c,
+                                                          float *master, float *m1, float *m2, unsigned short *wbf)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == NPARAM && loss_out) {
+    __shared__ float part[RG][RPW];
+    const int pl = threadIdx.x % RPW, g = threadIdx.x / RPW;
+    const int p = blockIdx.x * RPW + pl;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && loss_out) {
         double l = 0.0;
         for (int q = 0; q < ntiles; q++) l += lossp[q];
         *loss_out = l;
     }
-    if (p >= NPARAM) return;
-    float g = 0.0f;
-    for (int q = 0; q < ntiles; q++) g += partial[(size_t)q * NPARAM + p];
-    if (grad_out) grad_out[p] = g;
+    float a = 0.0f;
+    if (p < NPARAM)
+        for (int q = g; q < ntiles; q += RG) a += partial[(size_t)q * NPARAM + p];
+    part[g][pl] = a;
+    __syncthreads();
+    if (g != 0 || p >= NPARAM) return;
+    float gsum = part[0][pl];
+    for (int k = 1; k < RG; k++) gsum += part[k][pl];
+    if (grad_out) grad_out[p] = gsum;
     if (!adam) return;
     const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
-    float m = beta1 * m1[p] + (1.0f - beta1) * g;
-    float v = beta2 * m2[p] + (1.0f - beta2) * g * g;
+    float m = beta1 * m1[p] + (1.0f - beta1) * gsum;
+    float v = beta2 * m2[p] + (1.0f - beta2) * gsum * gsum;
     m1[p] = m;
     m2[p] = v;
     float w = master[p] - lr * (m / b1c) / (sqrtf(v / b2c) + eps);
@@ -439,70 +633,53 @@ __global__ __launch_bounds__(256) void k_reduce_adam(int ntiles, const float *__
 }
 
 /* ---------------- inference ---------------- */
-__device__ __forceinline__ void mlp_forward(const float *W, const float x[NIN], float y[NOUT])
+/* raw features in, raw outputs (denormalised) out; two 64-sample tiles per workgroup */
+#define INF_TPB 128
+__global__ __launch_bounds__(INF_TPB) void k_forward(int n, const float *__restrict__ xin,
+                                                     const unsigned short *__restrict__ wbf,
+                                                     const float *__restrict__ norm, float *__restrict__ yout)
 {
-    float a1[HID];
-#pragma unroll
-    for (int i = 0; i < HID; i++) {
-        float zz = W[OFF_B1 + i];
-#pragma unroll
-        for (int j = 0; j < NIN; j++) zz += W[OFF_W1 + i * NIN + j] * x[j];
-        a1[i] = gelu(zz);
-    }
-#pragma unroll
-    for (int k = 0; k < NOUT; k++) y[k] = W[OFF_B3 + k];
-#pragma unroll 1
-    for (int i = 0; i < HID; i++) {
-        float zz = W[OFF_B2 + i];
-#pragma unroll
-        for (int j = 0; j < HID; j++) zz += W[OFF_W2 + i * HID + j] * a1[j];
-        float a2 = gelu(zz);
-#pragma unroll
-        for (int k = 0; k < NOUT; k++) y[k] += W[OFF_W3 + k * HID + i] * a2;
-    }
-}
-
-/* raw features in, raw outputs (denormalised) out */
-__global__ __launch_bounds__(256) void k_forward(int n, const float *__restrict__ xin,
-                                                 const unsigned short *__restrict__ wbf,
-                                                 const float *__restrict__ norm, float *__restrict__ yout)
-{
-    __shared__ float W[NPARAM];
-    load_weights(wbf, W);
+    __shared__ float Wc[NCOMPACT];
+    __shared__ __align__(16) float Work[(INF_TPB / 64) * TILE * FSTR];
+    load_compact(wbf, Wc);
     __syncthreads();
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = s < n; /* every lane takes part in the wave's matrix products */
     float x[NIN], y[NOUT];
-    #pragma unroll
-
-    for (int k = 0; k < NIN; k++) x[k] = (xin[(size_t)s * NIN + k] - norm[k]) * norm[NIN + k];
-    mlp_forward(W, x, y);
-    #pragma unroll
-
+#pragma unroll
+    for (int k = 0; k < NIN; k++) x[k] = valid ? (xin[(size_t)s * NIN + k] - norm[k]) * norm[NIN + k] : 0.0f;
+    tile_forward(Wc, wbf + OFF_W2, Work + (threadIdx.x / 64) * TILE * FSTR, x, y);
+    if (!valid) return;
+#pragma unroll
     for (int k = 0; k < NOUT; k++) yout[(size_t)s * NOUT + k] = y[k] * norm[27 + k] + norm[24 + k];
 }
 
 /* sol = sol_old + MLP(features(sol_old, sol_older)) */
-__global__ __launch_bounds__(256) void k_predict(int N, double *__restrict__ sol, const double *__restrict__ st,
-                                                 const double *__restrict__ stm1, const int *__restrict__ brow,
-                                                 const int *__restrict__ bcol, const float *__restrict__ nstat,
-                                                 const unsigned short *__restrict__ wbf,
-                                                 const float *__restrict__ norm, const int *__restrict__ node_inst,
-                                                 const int *__restrict__ work)
+__global__ __launch_bounds__(INF_TPB) void k_predict(int N, double *__restrict__ sol, const double *__restrict__ st,
+                                                     const double *__restrict__ stm1, const int *__restrict__ brow,
+                                                     const int *__restrict__ bcol, const float *__restrict__ nstat,
+                                                     const unsigned short *__restrict__ wbf,
+                                                     const float *__restrict__ norm, const int *__restrict__ node_inst,
+                                                     const int *__restrict__ work)
 {
-    __shared__ float W[NPARAM];
-    load_weights(wbf, W);
+    __shared__ float Wc[NCOMPACT];
+    __shared__ __align__(16) float Work[(INF_TPB / 64) * TILE * FSTR];
+    load_compact(wbf, Wc);
     __syncthreads();
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N || !work[node_inst[n]]) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = n < N && work[node_inst[n]]; /* every lane takes part in the wave's matrix products */
     float f[NIN], y[NOUT];
-    node_features(n, N, st, stm1, brow, bcol, nstat, f);
-    #pragma unroll
-
-    for (int k = 0; k < NIN; k++) f[k] = (f[k] - norm[k]) * norm[NIN + k];
-    mlp_forward(W, f, y);
-    #pragma unroll
-
+    if (active) {
+        node_features(n, N, st, stm1, brow, bcol, nstat, f);
+#pragma unroll
+        for (int k = 0; k < NIN; k++) f[k] = (f[k] - norm[k]) * norm[NIN + k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < NIN; k++) f[k] = 0.0f;
+    }
+    tile_forward(Wc, wbf + OFF_W2, Work + (threadIdx.x / 64) * TILE * FSTR, f, y);
+    if (!active) return;
+#pragma unroll
     for (int k = 0; k < NOUT; k++) sol[3 * n + k] = st[3 * n + k] + (double)(y[k] * norm[27 + k] + norm[24 + k]);
 }
 
@@ -756,7 +933,7 @@ extern "C" int adhk_mlp_train(adhk_mlp *m, int steps, int batch, double lr, unsi
         if (nt < 0) goto fail;
         m->adam_t++;
         float b1c = (float)(1.0 - pow(0.9, (double)m->adam_t)), b2c = (float)(1.0 - pow(0.999, (double)m->adam_t));
-        hipLaunchKernelGGL(k_reduce_adam, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, m->stream, nt, m->partials,
+        hipLaunchKernelGGL(k_reduce_adam, dim3((NPARAM + RPW - 1) / RPW), dim3(RPW * RG), 0, m->stream, nt, m->partials,
                            m->lossp, (float *)NULL, dl ? dl + it : (double *)NULL, 1, (float)lr, b1c, b2c,
                            m->master, m->m1, m->m2, m->wbf);
         HC(hipGetLastError());
@@ -811,7 +988,7 @@ extern "C" int adhk_mlp_loss_grad(adhk_mlp *m, int n, const float *x, const floa
         B.sequential = 1;
         int nt = tile_grad(m, B);
         if (nt < 0) goto fail;
-        hipLaunchKernelGGL(k_reduce_adam, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, m->stream, nt, m->partials,
+        hipLaunchKernelGGL(k_reduce_adam, dim3((NPARAM + RPW - 1) / RPW), dim3(RPW * RG), 0, m->stream, nt, m->partials,
                            m->lossp, dg, dl, 0, 0.0f, 1.0f, 1.0f, (float *)NULL, (float *)NULL, (float *)NULL,
                            (unsigned short *)NULL);
         HC(hipGetLastError());
@@ -848,7 +1025,7 @@ extern "C" int adhk_mlp_forward(adhk_mlp *m, int n, const float *x, float *y)
     HC(hipMalloc((void **)&dx, (size_t)n * NIN * sizeof(float)));
     HC(hipMalloc((void **)&dy, (size_t)n * NOUT * sizeof(float)));
     HC(hipMemcpyAsync(dx, x, (size_t)n * NIN * sizeof(float), hipMemcpyHostToDevice, m->stream));
-    hipLaunchKernelGGL(k_forward, dim3((n + 255) / 256), dim3(256), 0, m->stream, n, dx, m->wbf, m->norm, dy);
+    hipLaunchKernelGGL(k_forward, dim3((n + INF_TPB - 1) / INF_TPB), dim3(INF_TPB), 0, m->stream, n, dx, m->wbf, m->norm, dy);
     HC(hipGetLastError());
     HC(hipMemcpyAsync(y, dy, (size_t)n * NOUT * sizeof(float), hipMemcpyDeviceToHost, m->stream));
     HC(hipStreamSynchronize(m->stream));
@@ -876,7 +1053,8 @@ extern "C" int adhk_mlp_predict_into(adhk_mlp *m, adhk_ctx *c)
     if (!m || !c) return -1;
     if (ensure_static(c)) return -1;
     HC(hipStreamSynchronize(m->stream));
-    hipLaunchKernelGGL(k_predict, dim3((c->N + 255) / 256), dim3(256), 0, c->stream, c->N, c->sol, c->sol_old,
+    hipLaunchKernelGGL(k_predict, dim3((c->N + INF_TPB - 1) / INF_TPB), dim3(INF_TPB), 0, c->stream, c->N, c->sol,
+                       c->sol_old,
                        c->sol_older, c->brow, c->bcol, c->nstat, m->wbf, m->norm, c->node_inst, c->work);
     HC(hipGetLastError());
     return 0;

@@ -14,6 +14,11 @@ defines it:
 * forward() / loss_grad(): 12 -> 64 -> 64 -> 3, GELU (erf form), weights rounded
   to BF16 (round to nearest even), the mean squared error on normalised outputs
   and its gradient by backpropagation;
+* with the BF16 weights come the roundings of the matrix-core products of the
+  64 x 64 layer (mfma=True): the layer's input activations a1 are rounded to BF16
+  where they enter the forward product and the weight-gradient product, and its
+  output gradients dz2 where they enter the weight-gradient and input-gradient
+  products; sums, biases, the other layers and the bias gradients stay unrounded;
 * adam(): the optimiser step.
 
 Pinned by tests/test_mlp_oracle.py: the gradient against central finite
@@ -64,31 +69,38 @@ def normalise_x(x, norm):
     return (np.asarray(x, np.float64) - norm[0:12]) * norm[12:24]
 
 
-def forward_norm(params, xn, bf16=True):
-    """normalised inputs (n, 12) -> normalised outputs (n, 3) and the activations"""
+def forward_norm(params, xn, bf16=True, mfma=None):
+    """normalised inputs (n, 12) -> normalised outputs (n, 3) and the activations; mfma (default: with the
+    BF16 weights) rounds a1 to BF16 where it enters the 64 x 64 product"""
+    mfma = bf16 if mfma is None else mfma
     W1, b1, W2, b2, W3, b3 = unpack(_weights(params, bf16))
     z1 = xn @ W1.T + b1
     a1 = gelu(z1)
+    if mfma:
+        a1 = bf16_round(a1).astype(np.float64)
     z2 = a1 @ W2.T + b2
     a2 = gelu(z2)
     y = a2 @ W3.T + b3
     return y, (z1, a1, z2, a2)
 
 
-def forward(params, norm, x, bf16=True):
+def forward(params, norm, x, bf16=True, mfma=None):
     """raw features (n, 12) -> raw outputs (n, 3): the change of h, u, v"""
     norm = np.asarray(norm, np.float64)
-    y, _ = forward_norm(params, normalise_x(x, norm), bf16)
+    y, _ = forward_norm(params, normalise_x(x, norm), bf16, mfma)
     return y * norm[27:30] + norm[24:27]
 
 
-def loss_grad(params, norm, x, t, bf16=True):
-    """mean squared error over n samples and 3 normalised outputs, and its gradient (NPARAM,)"""
+def loss_grad(params, norm, x, t, bf16=True, mfma=None):
+    """mean squared error over n samples and 3 normalised outputs, and its gradient (NPARAM,); with mfma
+    the products of the 64 x 64 layer see BF16 a1 and dz2 (the gradient of the rounded forward, the
+    roundings taken as the identity)"""
+    mfma = bf16 if mfma is None else mfma
     norm = np.asarray(norm, np.float64)
     xn = normalise_x(x, norm)
     tn = (np.asarray(t, np.float64) - norm[24:27]) * norm[30:33]
     W1, b1, W2, b2, W3, b3 = unpack(_weights(params, bf16))
-    y, (z1, a1, z2, a2) = forward_norm(params, xn, bf16)
+    y, (z1, a1, z2, a2) = forward_norm(params, xn, bf16, mfma)
     n = len(xn)
     d = y - tn
     loss = float(np.sum(d * d) / (n * NOUT))
@@ -97,9 +109,10 @@ def loss_grad(params, norm, x, t, bf16=True):
     g[OFF_W3:OFF_B3] = (dy.T @ a2).ravel()
     g[OFF_B3:NPARAM] = dy.sum(0)
     dz2 = (dy @ W3) * gelu_d(z2)
-    g[OFF_W2:OFF_B2] = (dz2.T @ a1).ravel()
+    dz2m = bf16_round(dz2).astype(np.float64) if mfma else dz2  # a1 is already rounded by forward_norm
+    g[OFF_W2:OFF_B2] = (dz2m.T @ a1).ravel()
     g[OFF_B2:OFF_W3] = dz2.sum(0)
-    dz1 = (dz2 @ W2) * gelu_d(z1)
+    dz1 = (dz2m @ W2) * gelu_d(z1)
     g[OFF_W1:OFF_B1] = (dz1.T @ xn).ravel()
     g[OFF_B1:OFF_W2] = dz1.sum(0)
     return loss, g

@@ -1,8 +1,13 @@
 """GPU tests of the learned Newton warm start (libadhk.so's MLP kernels) against the oracle.
 
 Bars: features to float32 rounding; forward outputs 1e-5 and loss / gradient 1e-4
-relative to the oracle's max (FP32 arithmetic with BF16 weights against float64
-with the same BF16 weights); training lowers the loss; the trained model cuts
+relative to the oracle's max (FP32 arithmetic with BF16 weights, the hidden layer's
+products on the matrix cores with BF16 activations, against float64 that rounds the
+same operands to BF16).  An activation whose FP32 and float64 values fall on two sides
+of a BF16 rounding boundary moves its sample's outputs by up to one BF16 step of that
+activation times its weights: the forward bar holds for 99.9% of the samples, and every
+sample is within 2e-3.  The matrix-core products are also checked against the oracle
+with those roundings switched off, to the size of a BF16 rounding; training lowers the loss; the trained model cuts
 Newton iterations on a held-out flume while the solution still meets the same
 1e-8 Newton tolerance; a useless model is rejected every step and reproduces
 plain Newton bit for bit.
@@ -58,7 +63,12 @@ def test_forward_and_gradient_match_oracle(gpu):
     t = rng.normal(size=(1000, 3)).astype(np.float32)
     y = mlp.forward(x)
     yo = M.forward(p, norm.astype(np.float64), x.astype(np.float64))
-    assert rel_err(y, yo) <= 1e-5
+    per = np.abs(y - yo).max(axis=1) / np.abs(yo).max()
+    assert np.quantile(per, 0.999) <= 1e-5 and per.max() <= 2e-3
+    # the matrix-core rounding is real and of BF16 size: against the unrounded hidden layer the outputs
+    # move by more than FP32 rounding and less than a few BF16 steps
+    yu = M.forward(p, norm.astype(np.float64), x.astype(np.float64), mfma=False)
+    assert 1e-6 < rel_err(y, yu) <= 3e-2
     loss, g = mlp.loss_grad(x, t)
     lo, go = M.loss_grad(p, norm.astype(np.float64), x.astype(np.float64), t.astype(np.float64))
     assert abs(loss - lo) <= 1e-5 * lo

@@ -122,3 +122,30 @@ def test_forward_loss_consistent():
     t = y.copy()  # targets equal to the prediction -> zero loss and zero gradient
     loss, g = M.loss_grad(p, norm, x, t)
     assert loss <= 1e-28 and np.abs(g).max() <= 1e-13
+
+
+def test_matrix_core_rounding_is_bf16_sized():
+    """The BF16 roundings of the hidden layer's products (mfma) move the forward by no more than BF16's
+    relative step on each product's terms, and the rounded gradient stays close to the exact one (the
+    finite-difference-pinned path above) -- a rounding at a wrong point (an output, a weight gradient)
+    or none at all fails one of these."""
+    rng = np.random.default_rng(7)
+    p, norm = random_model(rng)
+    x = rng.normal(size=(64, 12))
+    t = rng.normal(size=(64, 3))
+    xn = M.normalise_x(x, norm)
+    y0, (z1, a1, z2, a2) = M.forward_norm(p, xn, bf16=True, mfma=False)
+    y1, (_, a1r, z2r, _) = M.forward_norm(p, xn, bf16=True, mfma=True)
+    W1, b1, W2, b2, W3, b3 = M.unpack(M.bf16_round(p).astype(np.float64))
+    assert np.array_equal(a1r, M.bf16_round(a1).astype(np.float64))
+    bound = (np.abs(a1) @ np.abs(W2).T) * 2.0 ** -8  # |z2 change| <= sum_j |W2_ij| |a1_j| 2^-8
+    assert np.all(np.abs(z2r - z2) <= bound + 1e-15) and np.abs(z2r - z2).max() > 0
+    l0, g0 = M.loss_grad(p, norm, x, t, bf16=True, mfma=False)
+    l1, g1 = M.loss_grad(p, norm, x, t, bf16=True, mfma=True)
+    assert abs(l1 - l0) <= 2e-2 * l0
+    for a, b in ((M.OFF_W1, M.OFF_B1), (M.OFF_W2, M.OFF_B2), (M.OFF_W3, M.OFF_B3)):
+        assert 0 < np.abs(g1[a:b] - g0[a:b]).max() <= 3e-2 * np.abs(g0[a:b]).max()
+    # the bias gradient of the hidden layer sums the unrounded dz2
+    dy = 2.0 * (y1 - (t - norm[24:27]) * norm[30:33]) / (64 * 3)
+    dz2 = (dy @ W3) * M.gelu_d(z2r)
+    assert np.allclose(g1[M.OFF_B2:M.OFF_W3], dz2.sum(0), rtol=1e-13, atol=1e-16)
