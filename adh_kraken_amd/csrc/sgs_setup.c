@@ -3,8 +3,7 @@
  * per context from the node-block pattern (no counterpart in the reference, whose preconditioner is an
  * exact LU: bcgstab_solver.c:371-424).
  *
- * Colouring: greedy in natural node order -- node i takes the smallest colour that none of its block
- * neighbours j < i holds.  The rows are then permuted by colour and every colour's run padded to a multiple of 256 rows, so that each 256-row reduction block and each
+ * Colouring: DSatur (colour_dsatur below).  The rows are then permuted by colour and every colour's run padded to a multiple of 256 rows, so that each 256-row reduction block and each
  * 64-row slice belongs to one colour and a colour's rows form whole workgroups.  Inside a colour the rows
  * are sorted by their U and L widths within windows of 1024 (rows of one colour do not depend on each
  * other, so the order is free), which keeps the SELL padding small on meshes of uneven rows.  The folded matrix's
@@ -28,22 +27,97 @@
 
 #define SGS_VECS 10
 
-/* greedy colouring over the block pattern; returns the colour count, or -1 beyond ADHK_SGS_MAXC */
-static int colour_greedy(int N, const int *brow, const int *bcol, int *colour)
+/* DSatur colouring (Brelaz, "New methods to color the vertices of a graph", CACM 22(4), 1979) over the block
+ * pattern: repeatedly the uncoloured node with the most distinct colours among its neighbours (saturation),
+ * ties by the most uncoloured neighbours, then the lowest index, takes the smallest colour none of its
+ * neighbours holds.  On a triangle mesh a node with two coloured corners of a triangle is saturated by both,
+ * so the colouring propagates forced colours: a mesh whose interior nodes all have even degree (the flumes)
+ * gets 3 colours, the fewest a triangle allows.  A max-heap with lazy deletion (an entry is current when its
+ * node is uncoloured and its keys match).  Returns the colour count, or -1 beyond ADHK_SGS_MAXC. */
+typedef struct {
+    int sat, udeg, node;
+} ds_ent;
+static int ds_below(ds_ent a, ds_ent b)
 {
-    int nc = 0;
-    for (int i = 0; i < N; i++) {
-        unsigned long long used = 0;
-        for (int b = brow[i]; b < brow[i + 1]; b++) {
-            const int j = bcol[b];
-            if (j < i) used |= 1ULL << colour[j];
-        }
-        int c = 0;
-        while (c < ADHK_SGS_MAXC && (used >> c & 1ULL)) c++;
-        if (c >= ADHK_SGS_MAXC) return -1;
-        colour[i] = c;
-        if (c + 1 > nc) nc = c + 1;
+    if (a.sat != b.sat) return a.sat < b.sat;
+    if (a.udeg != b.udeg) return a.udeg < b.udeg;
+    return a.node > b.node;
+}
+static void heap_push(ds_ent *h, long *n, ds_ent e)
+{
+    long i = (*n)++;
+    h[i] = e;
+    while (i > 0) {
+        const long up = (i - 1) / 2;
+        if (!ds_below(h[up], h[i])) break;
+        const ds_ent t = h[up];
+        h[up] = h[i];
+        h[i] = t;
+        i = up;
     }
+}
+static ds_ent heap_pop(ds_ent *h, long *n)
+{
+    const ds_ent top = h[0];
+    h[0] = h[--(*n)];
+    long i = 0;
+    for (;;) {
+        const long a = 2 * i + 1, b = a + 1;
+        long m = i;
+        if (a < *n && ds_below(h[m], h[a])) m = a;
+        if (b < *n && ds_below(h[m], h[b])) m = b;
+        if (m == i) break;
+        const ds_ent t = h[m];
+        h[m] = h[i];
+        h[i] = t;
+        i = m;
+    }
+    return top;
+}
+static int colour_dsatur(int N, const int *brow, const int *bcol, int *colour)
+{
+    unsigned long long *seen = calloc((size_t)(N > 0 ? N : 1), sizeof *seen); /* neighbour colours, bit set */
+    int *sat = calloc((size_t)(N > 0 ? N : 1), sizeof(int)), *udeg = calloc((size_t)(N > 0 ? N : 1), sizeof(int));
+    ds_ent *h = malloc(((size_t)brow[N] + (size_t)N + 1) * sizeof *h); /* every push: a node once, then per neighbour */
+    long n = 0;
+    int nc = 0;
+    if (!seen || !sat || !udeg || !h) {
+        nc = -2;
+        goto done;
+    }
+    for (int v = 0; v < N; v++) {
+        colour[v] = -1;
+        udeg[v] = brow[v + 1] - brow[v] - 1; /* the block row holds the node itself */
+        heap_push(h, &n, (ds_ent){0, udeg[v], v});
+    }
+    while (n > 0) {
+        const ds_ent e = heap_pop(h, &n);
+        const int v = e.node;
+        if (colour[v] >= 0 || e.sat != sat[v] || e.udeg != udeg[v]) continue;
+        int c = 0;
+        while (c < ADHK_SGS_MAXC && (seen[v] >> c & 1ULL)) c++;
+        if (c >= ADHK_SGS_MAXC) {
+            nc = -1;
+            goto done;
+        }
+        colour[v] = c;
+        if (c + 1 > nc) nc = c + 1;
+        for (int b = brow[v]; b < brow[v + 1]; b++) {
+            const int j = bcol[b];
+            if (j == v || colour[j] >= 0) continue;
+            udeg[j]--;
+            if (!(seen[j] >> c & 1ULL)) {
+                seen[j] |= 1ULL << c;
+                sat[j]++;
+            }
+            heap_push(h, &n, (ds_ent){sat[j], udeg[j], j});
+        }
+    }
+done:
+    free(seen);
+    free(sat);
+    free(udeg);
+    free(h);
     return nc;
 }
 
@@ -163,10 +237,10 @@ int adhk_sgs_setup(adhk_ctx *c)
     HC(hipMemcpy(brow, c->brow, ((size_t)N + 1) * sizeof(int), hipMemcpyDeviceToHost));
     HC(hipMemcpy(bcol, c->bcol, (size_t)nnzb * sizeof(int), hipMemcpyDeviceToHost));
     HC(hipMemcpy(b2s, c->blk2sell, (size_t)nnzb * sizeof(int), hipMemcpyDeviceToHost));
-    G->nc = colour_greedy(N, brow, bcol, G->colour);
+    G->nc = colour_dsatur(N, brow, bcol, G->colour);
     if (G->nc < 2) {
-        adhk_set_error("adhk_sgs_setup: %d colours (more than %d, or a pattern without couplings)", G->nc,
-                       ADHK_SGS_MAXC);
+        adhk_set_error("adhk_sgs_setup: %s", G->nc == -2 ? "out of host memory"
+                                                 : "more than 64 colours, or a pattern without couplings");
         goto fail;
     }
     for (int i = 0; i < N; i++) cnt[G->colour[i]]++;

@@ -89,7 +89,7 @@ def load():
         "or_block_jacobi_prep": (None, [C.c_int, ip, ip, dp, dp]),
         "or_block_jacobi_apply": (None, [C.c_int, dp, dp, dp]),
         "or_bicgstab": (C.c_int, [C.c_int, ip, ip, dp, dp, dp, dp, dp, C.c_double, C.c_int, dp]),
-        "or_colour_greedy": (C.c_int, [C.c_int, ip, ip, ip]),
+        "or_colour_dsatur": (C.c_int, [C.c_int, ip, ip, ip]),
         "or_bicgstab_sgs": (C.c_int, [C.c_int, ip, ip, dp, dp, ip, C.c_int, dp, dp, dp, C.c_double, C.c_int, dp]),
         "or_residual_norms": (None, [C.c_int, dp, dp, ip, dp, dp, dp]),
         "or_fe_newton": (C.c_int, [P, C.POINTER(OrSolver), C.POINTER(OrStats)]),
@@ -296,12 +296,12 @@ def bicgstab(indptr, cols, vals, minv, b, scale, x0=None, tol=1e-11, max_it=100)
     return x, it, rn.value, bb
 
 
-def colour_greedy(indptr, cols):
-    """node colours of the multicolour preconditioner (greedy, natural order) and their count"""
+def colour_dsatur(indptr, cols):
+    """node colours by DSatur (saturation order) and their count"""
     lib = load()
     nn = (len(indptr) - 1) // 3
     col = np.zeros(nn, np.int32)
-    nc = lib.or_colour_greedy(nn, _i(indptr), _i(cols), _i(col))
+    nc = lib.or_colour_dsatur(nn, _i(indptr), _i(cols), _i(col))
     return col, nc
 
 

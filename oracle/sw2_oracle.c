@@ -1315,25 +1315,92 @@ int or_bicgstab(int n, const int *indptr, const int *cols, const double *vals, c
 }
 
 /* ---- multicolour symmetric Gauss-Seidel, split form (no counterpart in the reference, whose
- * preconditioner is an exact LU, bcgstab_solver.c:371-424) ----
- *
- * Greedy node colouring in natural order: node i takes the smallest colour that none of its block
- * neighbours j < i holds.  Returns the number of colours. */
-int or_colour_greedy(int nnodes, const int *indptr, const int *cols, int *colour)
+ * preconditioner is an exact LU, bcgstab_solver.c:371-424) ---- */
+
+/* DSatur colouring (Brelaz, "New methods to color the vertices of a graph", CACM 22(4), 1979): repeatedly
+ * the uncoloured node with the most distinct colours among its neighbours (saturation), ties by the most
+ * uncoloured neighbours, then the lowest index, takes the smallest colour none of its neighbours holds.
+ * On a triangulation a node whose triangle has two coloured corners is saturated by both, so colours are
+ * forced across the mesh as far as the graph allows.  Max-heap with lazy deletion; returns the number of
+ * colours, or -1 beyond 64. */
+typedef struct { int sat, udeg, node; } ds_key;
+static int ds_less(ds_key a, ds_key b) /* a below b in the max-heap */
 {
+    if (a.sat != b.sat) return a.sat < b.sat;
+    if (a.udeg != b.udeg) return a.udeg < b.udeg;
+    return a.node > b.node;
+}
+static void ds_push(ds_key *h, long *n, ds_key k)
+{
+    long i = (*n)++;
+    h[i] = k;
+    while (i > 0) {
+        long p = (i - 1) / 2;
+        if (!ds_less(h[p], h[i])) break;
+        ds_key t = h[p];
+        h[p] = h[i];
+        h[i] = t;
+        i = p;
+    }
+}
+static ds_key ds_pop(ds_key *h, long *n)
+{
+    ds_key top = h[0];
+    h[0] = h[--(*n)];
+    long i = 0;
+    for (;;) {
+        long l = 2 * i + 1, r = l + 1, m = i;
+        if (l < *n && ds_less(h[m], h[l])) m = l;
+        if (r < *n && ds_less(h[m], h[r])) m = r;
+        if (m == i) break;
+        ds_key t = h[m];
+        h[m] = h[i];
+        h[i] = t;
+        i = m;
+    }
+    return top;
+}
+int or_colour_dsatur(int nnodes, const int *indptr, const int *cols, int *colour)
+{
+    unsigned long long *used = calloc((size_t)(nnodes > 0 ? nnodes : 1), sizeof *used);
+    int *sat = calloc((size_t)(nnodes > 0 ? nnodes : 1), sizeof(int));
+    int *udeg = calloc((size_t)(nnodes > 0 ? nnodes : 1), sizeof(int));
+    long cap = 0, n = 0;
+    for (int i = 0; i < nnodes; i++) cap += (indptr[3 * i + 1] - indptr[3 * i]) / 3 + 1;
+    ds_key *h = malloc((size_t)(cap > 0 ? cap : 1) * sizeof *h);
     int nc = 0;
     for (int i = 0; i < nnodes; i++) {
-        unsigned long long used = 0;
-        const int r = 3 * i;
-        for (int k = indptr[r]; k < indptr[r + 1]; k += 3) {
-            const int j = cols[k] / 3;
-            if (j < i && colour[j] < 64) used |= 1ULL << colour[j];
-        }
-        int c = 0;
-        while (c < 64 && (used >> c & 1ULL)) c++;
-        colour[i] = c;
-        if (c + 1 > nc) nc = c + 1;
+        colour[i] = -1;
+        udeg[i] = (indptr[3 * i + 1] - indptr[3 * i]) / 3 - 1; /* block neighbours other than itself */
+        ds_push(h, &n, (ds_key){0, udeg[i], i});
     }
+    while (n > 0) {
+        const ds_key k = ds_pop(h, &n);
+        const int v = k.node;
+        if (colour[v] >= 0 || k.sat != sat[v] || k.udeg != udeg[v]) continue; /* stale entry */
+        int c = 0;
+        while (c < 64 && (used[v] >> c & 1ULL)) c++;
+        if (c >= 64) {
+            nc = -1;
+            break;
+        }
+        colour[v] = c;
+        if (c + 1 > nc) nc = c + 1;
+        for (int q = indptr[3 * v]; q < indptr[3 * v + 1]; q += 3) {
+            const int j = cols[q] / 3;
+            if (j == v || colour[j] >= 0) continue;
+            udeg[j]--;
+            if (!(used[j] >> c & 1ULL)) {
+                used[j] |= 1ULL << c;
+                sat[j]++;
+            }
+            ds_push(h, &n, (ds_key){sat[j], udeg[j], j});
+        }
+    }
+    free(used);
+    free(sat);
+    free(udeg);
+    free(h);
     return nc;
 }
 
@@ -1563,7 +1630,7 @@ int or_fe_newton(or_ctx *c, const or_solver *sv, or_newton_stats *st)
         if (sv->precond == OR_PREC_SGS) {
             if (!c->colour) {
                 c->colour = malloc((size_t)(c->m.nnodes > 0 ? c->m.nnodes : 1) * sizeof(int));
-                c->ncolour = or_colour_greedy(c->m.nnodes, c->indptr, c->cols, c->colour);
+                c->ncolour = or_colour_dsatur(c->m.nnodes, c->indptr, c->cols, c->colour);
             }
             lits = or_bicgstab_sgs(n, c->indptr, c->cols, c->vals, minv, c->colour, c->ncolour, c->dsol, c->residual,
                                    c->scale, sv->lin_tol, sv->lin_max_it, NULL);

@@ -136,7 +136,7 @@ void or_block_jacobi_prep(int nnodes, const int *indptr, const int *cols, const 
 void or_block_jacobi_apply(int nnodes, const double *minv, const double *b, double *x);
 int  or_bicgstab(int n, const int *indptr, const int *cols, const double *vals, const double *minv,
                  double *x, double *b, const double *scale, double tol, int max_it, double *final_rnorm);
-int  or_colour_greedy(int nnodes, const int *indptr, const int *cols, int *colour);
+int  or_colour_dsatur(int nnodes, const int *indptr, const int *cols, int *colour);
 int  or_bicgstab_sgs(int n, const int *indptr, const int *cols, const double *vals, const double *minv,
                      const int *colour, int ncolour, double *x, const double *b, const double *scale, double tol,
                      int max_it, double *final_rnorm);

@@ -44,7 +44,7 @@ def test_colouring_matches_oracle(gpu, c1, chan):
         sm = A.SModelSuper(mesh, dt=1.0)
         orc = O.Oracle(mesh, dt=1.0, threads=8)
         ip_, cols = orc.pattern()
-        co, nco = O.colour_greedy(ip_, cols)
+        co, nco = O.colour_dsatur(ip_, cols)
         cg, ncg = sm.colouring()
         assert ncg == nco and np.array_equal(cg, co)
 

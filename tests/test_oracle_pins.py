@@ -25,8 +25,9 @@ fails:
   the reference's ten test-grid geometries (src/test/run_tests.c:42-47)
 * FD Jacobian vs directional differences of the residual
 * BiCGStab/block-Jacobi solution vs a direct sparse solve; block inverses
-* multicolour SGS: the greedy colouring vs a brute-force first-fit colouring (proper, each node the
-  smallest colour free among its lower-numbered neighbours); the split-preconditioned BiCGStab
+* multicolour SGS: the DSatur colouring is proper and sequential (a node of colour c sees every colour
+  below c among its neighbours), and takes three colours on the flume, whose even interior degrees
+  make it 3-colourable; the split-preconditioned BiCGStab
   (Eisenstat form) vs a direct sparse solve -- a slip in either sweep or in the identity solves a
   different system
 * Newton: the converged state satisfies the residual tolerance
@@ -470,24 +471,39 @@ def _c1_scaled_system(c1, seed=2):
     return vals, b, ip_, cols, vs, bs, scale
 
 
-def test_greedy_colouring_is_first_fit(c1):
-    _, _, ip_, cols, _, _, _ = _c1_scaled_system(c1)
-    col, nc = O.colour_greedy(ip_, cols)
-    nn = len(col)
-    assert nc == col.max() + 1 and nc >= 3  # a triangle needs three colours
-    for i in range(nn):
+def _colouring_properties(ip_, cols, col, nc):
+    """proper, and sequential: a node of colour c has neighbours of every colour below c (it took the
+    smallest colour its already coloured neighbours left free)"""
+    assert nc == col.max() + 1 and col.min() == 0
+    for i in range(len(col)):
         nb = cols[ip_[3 * i]:ip_[3 * i + 1]:3] // 3
         nb = nb[nb != i]
-        assert not np.any(col[nb] == col[i])  # proper
-        lower = set(col[nb[nb < i]].tolist())
-        assert col[i] == min(set(range(nc + 1)) - lower)  # first fit in natural order
+        assert not np.any(col[nb] == col[i])
+        assert set(range(col[i])) <= set(col[nb].tolist())
+
+
+def test_dsatur_colouring(c1):
+    # the flume: every interior node has even degree, so the triangulation is 3-colourable and the
+    # saturation order finds it (three is also the least a triangle allows)
+    _, _, ip_, cols, _, _, _ = _c1_scaled_system(c1)
+    col, nc = O.colour_dsatur(ip_, cols)
+    assert nc == 3
+    _colouring_properties(ip_, cols, col, nc)
+    # an unstructured Delaunay channel: odd degrees, more colours, the same properties
+    mesh, _ = A.channel_mesh(A.ChannelSpec(length=200.0, width=80.0, spacing=4.0, chan_width=20.0, bank_width=6.0,
+                                           nbumps=2, inflow_q=10.0, fp_depth_out=1.0, seed=4))
+    orc = O.Oracle(mesh, dt=1.0, threads=8)
+    ip2, cols2 = orc.pattern()
+    col2, nc2 = O.colour_dsatur(ip2, cols2)
+    assert 4 <= nc2 <= 6
+    _colouring_properties(ip2, cols2, col2, nc2)
 
 
 @pytest.mark.parametrize("seed", [2, 5])
 def test_sgs_split_bicgstab_solves_the_system(c1, seed):
     vals, b, ip_, cols, vs, bs, scale = _c1_scaled_system(c1, seed)
     minv = O.block_jacobi(ip_, cols, vs)
-    col, nc = O.colour_greedy(ip_, cols)
+    col, nc = O.colour_dsatur(ip_, cols)
     x, it, rn = O.bicgstab_sgs(ip_, cols, vs, minv, bs, scale, col, nc, tol=1e-14, max_it=400)
     assert 0 < it < 400
     Ar = sp.csr_matrix((vals, cols, ip_), shape=(len(b), len(b)))
