@@ -158,6 +158,7 @@ struct adhk_sw2 {
     int gchunk_len[2];
     adhk_tuning gchunk_tune[2]; /* the schedule the chunk was captured with */
     int gchunk_pc[2];           /* and the preconditioner */
+    int kry_hist[2][16];        /* BiCGStab iterations of the last step's Newton iteration k (per precision) */
     /* captured Newton stages (per precision): the launches between two host decisions, with the
      * by-value kernel arguments they were captured with */
     adhk_stage_graph gstage[2][3];

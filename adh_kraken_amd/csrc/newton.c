@@ -1003,29 +1003,36 @@ fail:
 }
 
 /* BiCGStab iterations from the state adhk_launch_krylov_init left, for the instances in the work mask:
- * the host reads the stop flags between chunks.  On return hs holds every instance's final scalars. */
-static int krylov_iterate(adhk_ctx *c, const adhk_solver *s, int mixed, adhk_scalars *hs)
+ * the host reads the stop flags between chunks.  On return hs holds every instance's final scalars.
+ * Every kernel exits at once when its instance has converged, so the launch pattern changes no result,
+ * only the launches wasted after convergence and the read-backs, each of which drains the GPU:
+ *   - no read-back after the start: a system already converged there runs one chunk of kernels that exit
+ *     at once, and the SGS fallback (a folded diagonal block that is not the identity) is seen at the
+ *     first read-back, which then restarts the solve with block-Jacobi;
+ *   - the Newton iteration nidx (0, 1, ...; -1: none) of a step usually needs about as many BiCGStab
+ *     iterations as the same Newton iteration of the previous step, so for one mesh alone with eager
+ *     launches the first chunk runs to a little below that count and the rest goes in quarter chunks
+ *     (an ensemble's members differ too much for one count to fit them). */
+#define KRY_HIST 16
+static int krylov_iterate(adhk_ctx *c, const adhk_solver *s, int mixed, adhk_scalars *hs, int nidx)
 {
     const int K = c->ninst;
     int chunk = c->tune.chunk > 0 ? c->tune.chunk : (s->lin_check_every > 0 ? s->lin_check_every : 8), all_done = 1;
     const unsigned krylov_kt = (1u << ADHK_KT_SPMV_P) | (1u << ADHK_KT_SPMV_S) | (1u << ADHK_KT_UPDATE);
     const int use_graph = c->tune.graphs && !(c->kt.on && (c->kt.mask & krylov_kt)) && chunk % 2 == 0;
-    HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
-    HC(hipStreamSynchronize(c->stream));
-    if (c->pc == ADHK_PC_SGS && hs[0].sgs_bad) {
-        /* a singular diagonal block kept its fallback inverse: this solve takes block-Jacobi */
-        c->pc = ADHK_PC_BJ;
-        LC(adhk_launch_krylov_init(c, mixed, mixed ? s->mixed_lin_tol : s->lin_tol, s->lin_max_it));
-        HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
-        HC(hipStreamSynchronize(c->stream));
+    const int hi = nidx < 0 ? -1 : (nidx < KRY_HIST ? nidx : KRY_HIST - 1);
+    int first = chunk, tail = chunk;
+    if (!use_graph && hi >= 0 && K == 1 && c->kry_hist[mixed][hi] > 0) {
+        tail = chunk / 4 > 2 ? chunk / 4 : 2;
+        first = c->kry_hist[mixed][hi] - tail;
+        if (first < tail) first = tail;
     }
-    for (int i = 0; i < K; i++) all_done &= hs[i].done;
-    if (!all_done) {
+    for (;;) {
         /* iteration 1, then chunks; the A kernel of iteration k+1 completes iteration k and tests it */
-        int cur = 1, launched = 1;
+        int cur = 1, launched = 1, restart = 0;
         LC(adhk_launch_krylov_first(c, mixed));
         while (launched < s->lin_max_it) {
-            int k = chunk;
+            int k = launched == 1 ? first : tail;
             if (launched + k > s->lin_max_it) k = s->lin_max_it - launched;
             if (use_graph && k == chunk && cur == 1) {
                 LC(capture_chunk(c, mixed, chunk));
@@ -1039,15 +1046,30 @@ static int krylov_iterate(adhk_ctx *c, const adhk_solver *s, int mixed, adhk_sca
             launched += k;
             HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
             HC(hipStreamSynchronize(c->stream));
+            if (c->pc == ADHK_PC_SGS && hs[0].sgs_bad) {
+                /* a singular diagonal block kept its fallback inverse: this solve takes block-Jacobi */
+                c->pc = ADHK_PC_BJ;
+                LC(adhk_launch_krylov_init(c, mixed, mixed ? s->mixed_lin_tol : s->lin_tol, s->lin_max_it));
+                restart = 1;
+                break;
+            }
             all_done = 1;
             for (int i = 0; i < K; i++) all_done &= hs[i].done;
             if (all_done) break;
         }
+        if (restart) continue;
         if (!all_done) {
             LC(adhk_launch_krylov_last(c, mixed, cur));
             HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
             HC(hipStreamSynchronize(c->stream));
         }
+        break;
+    }
+    if (hi >= 0) {
+        int mx = 0;
+        for (int i = 0; i < K; i++)
+            if (c->h_work[i] && hs[i].it > mx) mx = hs[i].it;
+        c->kry_hist[mixed][hi] = mx;
     }
     return 0;
 fail:
@@ -1058,7 +1080,7 @@ fail:
 static int krylov(adhk_ctx *c, const adhk_solver *s, int mixed, adhk_scalars *hs, int *its)
 {
     LC(adhk_launch_krylov_init(c, mixed, mixed ? s->mixed_lin_tol : s->lin_tol, s->lin_max_it));
-    LC(krylov_iterate(c, s, mixed, hs));
+    LC(krylov_iterate(c, s, mixed, hs, -1));
     LC(adhk_launch_krylov_finish(c, mixed));
     HC(hipStreamSynchronize(c->stream));
     for (int i = 0; i < c->ninst; i++) its[i] = c->h_work[i] ? hs[i].it : 0;
@@ -1325,7 +1347,7 @@ static int fe_newton_pass(adhk_ctx *c, const adhk_solver *s, int use_warm)
     adhk_scalars *hs = NULL, *hw = NULL;
     int *act = NULL, *ls = NULL, *solv = NULL, *its = NULL, *lits = NULL, *cuts = NULL;
     double *rmax = NULL, *rl2 = NULL, *incmax = NULL, *old_rl2 = NULL;
-    int all_ok = 1;
+    int all_ok = 1, newton_it = 0;
     const int pc = pc_of(c, s);
     if (s->precond != ADHK_PC_BJ && s->precond != ADHK_PC_SGS) {
         adhk_set_error("adhk_fe_newton: precond %d (ADHK_PC_BJ or ADHK_PC_SGS)", s->precond);
@@ -1399,7 +1421,7 @@ static int fe_newton_pass(adhk_ctx *c, const adhk_solver *s, int use_warm)
         LC(set_work(c, act));
         c->pc = pc;
         LC(run_stage(c, s, mixed, STG_SYS));
-        LC(krylov_iterate(c, s, mixed, hw));
+        LC(krylov_iterate(c, s, mixed, hw, newton_it++));
         for (int i = 0; i < K; i++) lits[i] = c->h_work[i] ? hw[i].it : 0;
         LC(run_stage(c, s, mixed, STG_UPD));
         for (int i = 0; i < K; i++)
