@@ -187,8 +187,11 @@ def load_sw2_model(design_path, model=1, defaults=None):
     used = sorted(set(int(m) + 1 for m in tri_mat))
     other = {m: prm.models.get(m, "?") for m in used if prm.models.get(m) != "SW2"}
     if other:
+        # The reference registers no residual for DW or TRNS materials: select_resid_func returns NULL
+        # for them (src/structs/sadh_def/sadh_def.c:261-263, :267-269), and load_pdes calls the pointer
+        # (src/residual/assemble_residual.c:20), so its own assembly cannot take such an element either.
         raise NotImplementedError(f"{stem}.params: materials {other} are not SW2; this build runs the SW2 "
-                                  f"Newton step only")
+                                  f"Newton step only (the reference has no residual function for DW or TRNS)")
     # triangles counter-clockwise
     a = (x[tri[:, 1]] - x[tri[:, 0]]) * (y[tri[:, 2]] - y[tri[:, 0]]) - \
         (x[tri[:, 2]] - x[tri[:, 0]]) * (y[tri[:, 1]] - y[tri[:, 0]])
