@@ -70,7 +70,7 @@ typedef struct {
     adhk_sgs_part U, L;
     int *inst_blk;             /* device {0, nblk} */
     double *partials;          /* device [8 * nblk] */
-    double *v64;               /* device 10 vectors of 3 (Np + 1) doubles: r p Ap s As q xt t u bt */
+    double *v64;               /* device 8 vectors of 3 (Np + 1) doubles: r (= s) p Ap As q xt t (= u) bt */
     float *v32;                /* the same in FP32 */
 } adhk_sgs;
 

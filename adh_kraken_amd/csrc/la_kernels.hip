@@ -16,10 +16,13 @@
  *   k_iter_s : s = r - alpha Ap, As = M^-1 A s with As.s, As.As, As.q -> omega, rho, beta
  * M^-1 A is formed once per Newton iteration (k_scale_prec folds each row's 3x3 inverse into the
  * row), so each SpMV pass reads one matrix and no preconditioner.
+ * The multicolour symmetric Gauss-Seidel preconditioner (further below) runs each product in colour
+ * phases over colour-permuted U and L parts of that folded matrix.
  * Scalars live in device memory; every kernel exits at once when the solver
  * has converged, so the host launches iterations in chunks and reads the
  * "done" flag between chunks.  Reductions are deterministic: per-workgroup
- * partials, summed in index order by a one-workgroup finalize kernel.
+ * partials, summed in index order by a one-workgroup finalize kernel (in the
+ * SGS solve, by every workgroup of the kernel that uses the result).
  * The block matrix and vectors are templated on the storage type (double for
  * the FP64 path, float for the mixed-precision path); reductions and scalars
  * are always double.
@@ -1271,6 +1274,15 @@ struct SPart {
     const int *__restrict__ ptr;
     const int *__restrict__ col;
 };
+/* Eight buffers of 3 (Np + 1) values, two of them shared, so that the iteration's working set is as
+ * small as the method allows:
+ *   s shares r: the S step forms s_i = r_i - alpha Ap_i at row i's first phase, and no other row reads
+ *     r_i afterwards (the top colour's s, formed on the fly by its neighbours, is stored by its own
+ *     phase, the product's last); the A step forms r_i = s_i - omega As_i at row i alone.
+ *   u shares t: backward phases read t only of higher colours, which have not yet had their forward
+ *     phase; a forward phase reads its own row's t, then stores u over it, and gathers u only of lower
+ *     colours, whose forward phases are done; colour 0's t is used within its one phase only.
+ * The finishing backward sweep (k_sgs_bwd) rewrites t for every colour but the top before reading it. */
 template <typename T> struct SgVec {
     T *r, *p, *Ap, *s, *As, *q, *xt, *t, *u, *bt;
 };
@@ -1279,15 +1291,15 @@ template <typename T> static SgVec<T> sg_vec(T *base, int Np)
     const size_t n = (size_t)3 * ((size_t)Np + 1);
     SgVec<T> v;
     v.r = base;
+    v.s = base;
     v.p = base + n;
     v.Ap = base + 2 * n;
-    v.s = base + 3 * n;
-    v.As = base + 4 * n;
-    v.q = base + 5 * n;
-    v.xt = base + 6 * n;
-    v.t = base + 7 * n;
-    v.u = base + 8 * n;
-    v.bt = base + 9 * n;
+    v.As = base + 3 * n;
+    v.q = base + 4 * n;
+    v.xt = base + 5 * n;
+    v.t = base + 6 * n;
+    v.u = base + 6 * n;
+    v.bt = base + 7 * n;
     return v;
 }
 
@@ -1386,6 +1398,92 @@ template <typename T> struct SgCol<T, SG_S, false> {
     }
 };
 
+/* The two BiCGStab finalizers of the SGS solve, folded into the kernels that use their results: every
+ * workgroup of the consuming kernel sums the same partials in the same order (the sums of k_fin_a and
+ * k_fin_s, bit for bit) and workgroup 0 stores the scalars for the kernels that follow.  A kernel never
+ * reads a scalar that it stores itself, so no workgroup can see another's store half-way:
+ *   - the first phase of the S product (sgs_fin_a): stop test on the A step's max|r| and alpha =
+ *     rho / (q.Ap); stores rnorm, done, gamma, alpha and rhop = rho (the rho this alpha used);
+ *   - k_sgs_upd (sgs_fin_s): omega = (As.s)/(As.As), rho' = -omega (As.q), beta = (rho' alpha)/(rhop
+ *     omega), it + 1; stores omega, rho, beta, it.
+ * A workgroup that reads a done flag just stored by workgroup 0 of the same launch exits, as its own sum
+ * would have told it.  The S partials sit in rows PR_S.. so that the first S phase, which stores them
+ * when the mesh has two colours, never overwrites the max|r| and q.Ap rows it reads. */
+enum { SGF_NONE = 0, SGF_A_FIRST = 1, SGF_A = 2 };
+/* k_sgs_upd's workgroups (each sums every S partial, then loops over 256-row blocks): two per CU; 2048
+ * cost the 4.1M-dof channel 12 us more per launch in partial sums, and the 1.2M-dof flume is no faster */
+#define SGS_UPD_GRID 512
+#define PR_S (PR_A + 1) /* 3 rows: As.s, As.As, As.q */
+
+/* true when the solve has stopped (the workgroup exits); else the iteration's alpha.  Thread 0 reads the
+ * done flag once for the whole workgroup: it is 1 when an earlier launch stopped the solve, and may
+ * already be 1 when workgroup 0 of this launch has stored this test's result, which this workgroup's own
+ * test then repeats. */
+template <bool FIRST>
+__device__ __forceinline__ bool sgs_fin_a(const double *partials, int nblk, adhk_scalars *sc, double &alpha)
+{
+    __shared__ double bc[2];
+    double vmax[1] = {0.0}, vsum[1];
+    if (!FIRST) sum_blocks<1, true>(partials + PR_A * nblk, nblk, 0, nblk, vmax);
+    sum_blocks<1, false>(partials + PR_B * nblk, nblk, 0, nblk, vsum);
+    if (threadIdx.x == 0) {
+        const int stopped = sc->done;
+        const double rho = sc->rho;
+        const int done = FIRST ? 0 : !(vmax[0] > (sc->tol + sc->bnorm * sc->tol) && sc->it < sc->max_it);
+        const double a = rho / vsum[0];
+        bc[0] = a;
+        bc[1] = (double)(stopped || done);
+        if (blockIdx.x == 0 && !stopped) {
+            if (!FIRST) {
+                sc->rnorm = vmax[0];
+                sc->done = done;
+            }
+            sc->gamma = vsum[0];
+            sc->alpha = a;
+            sc->rhop = rho;
+        }
+    }
+    __syncthreads();
+    alpha = bc[0];
+    return bc[1] != 0.0;
+}
+
+/* every workgroup of a launch that calls this has passed a done test on a flag no workgroup of the launch
+ * stores */
+__device__ __forceinline__ void sgs_fin_s(const double *partials, int nblk, adhk_scalars *sc, double &omega,
+                                          double &beta)
+{
+    __shared__ double bc[2];
+    double v[3];
+    sum_blocks<3, false>(partials + PR_S * nblk, nblk, 0, nblk, v);
+    if (threadIdx.x == 0) {
+        const double om = v[0] / v[1];
+        const double rhop = sc->rhop;
+        const double rho = -1.0 * om * v[2];
+        const double be = (rho * sc->alpha) / (rhop * om);
+        bc[0] = om;
+        bc[1] = be;
+        if (blockIdx.x == 0) {
+            sc->omega = om;
+            sc->rho = rho;
+            sc->beta = be;
+            sc->it = sc->it + 1;
+        }
+    }
+    __syncthreads();
+    omega = bc[0];
+    beta = bc[1];
+}
+
+/* the S step's finalizer as a launch of its own, for the fused A step (whose first phases need omega and
+ * beta in every colour) */
+__global__ __launch_bounds__(TPB) void k_sgs_fin_s(int nblk, const double *partials, adhk_scalars *sc)
+{
+    if (sc->done) return;
+    double omega, beta;
+    sgs_fin_s(partials, nblk, sc, omega, beta);
+}
+
 /* one colour phase over the permuted rows [r0, r0 + 256 gridDim.x) */
 /* the 256-row block of a phase's workgroup: with xcd_map, XCD k (workgroups w = k mod 8) takes the k-th
  * contiguous eighth of the colour's blocks, so neighbour gathers stay in that XCD's L2; -1: none */
@@ -1396,20 +1494,27 @@ __device__ __forceinline__ int sgs_block(int nb, int xcd_map)
     return lb < nb ? lb : -1;
 }
 
-template <typename T, int OP, int PH, bool FIRST>
+template <typename T, int OP, int PH, bool FIRST, int FIN>
 __global__ __launch_bounds__(TPB) void k_sgs_phase(int r0, int nb, int xcd_map, int top0, SPart U, SPart L,
                                                    const T *__restrict__ uval, const T *__restrict__ lval, SgVec<T> V,
-                                                   double *partials, int nblk, const adhk_scalars *__restrict__ sc)
+                                                   double *partials, int nblk, const adhk_scalars *__restrict__ sc,
+                                                   adhk_scalars *scw)
 {
     double alpha = 0.0, omega = 0.0, beta = 0.0;
-    if (OP != SG_INIT) {
+    const int lb = sgs_block(nb, xcd_map);
+    if (lb < 0) return;
+    if constexpr (FIN != SGF_NONE) {
+        /* the first phase of the S product finishes the A step itself, in every workgroup: the stop test
+         * and alpha, as k_fin_a would in a launch of its own.  The scalars, which workgroup 0 stores during
+         * this launch, are read through scw alone, by one thread per workgroup (sgs_fin_a), so that every
+         * wave of a workgroup takes the same exit. */
+        if (sgs_fin_a<FIN == SGF_A_FIRST>(partials, nblk, scw, alpha)) return;
+    } else if constexpr (OP != SG_INIT) {
         if (sc->done) return;
         alpha = sc->alpha;
         omega = sc->omega;
         beta = sc->beta;
     }
-    const int lb = sgs_block(nb, xcd_map);
-    if (lb < 0) return;
     const int i = r0 + lb * TPB + threadIdx.x;
     const int j = i / TPB; /* reduction block: r0 is a multiple of 256 */
     const typename SgCol<T, OP, FIRST>::type col = SgCol<T, OP, FIRST>::make(V, alpha, omega, beta);
@@ -1451,7 +1556,7 @@ __global__ __launch_bounds__(TPB) void k_sgs_phase(int r0, int nb, int xcd_map, 
         part_sum<T>(i, U, uval, GetB<T, typename SgCol<T, OP, FIRST>::type>{V.t, col, top0}, a[0], a[1], a[2]);
         for (int k = 0; k < 3; k++) {
             const T tk = (T)(v[k] - a[k]);
-            V.t[3 * i + k] = tk;
+            if constexpr (PH == PH_B) V.t[3 * i + k] = tk; /* colour 0's t is used in this phase only */
             t[k] = (double)tk;
         }
     } else if constexpr (PH == PH_F) {
@@ -1509,7 +1614,7 @@ __global__ __launch_bounds__(TPB) void k_sgs_phase(int r0, int nb, int xcd_map, 
             block_partial<1, true>(pa1, partials + PR_A * nblk, nblk, j);
         if constexpr (PH != PH_B) block_partial<1, false>(pb, partials + PR_B * nblk, nblk, j);
     } else {
-        if constexpr (PH != PH_B) block_partial<3, false>(pa3, partials + PR_A * nblk, nblk, j);
+        if constexpr (PH != PH_B) block_partial<3, false>(pa3, partials + PR_S * nblk, nblk, j);
     }
 }
 
@@ -1517,11 +1622,12 @@ __global__ __launch_bounds__(TPB) void k_sgs_phase(int r0, int nb, int xcd_map, 
  * p = (p - omega Ap) beta + r as one coalesced pass over the permuted rows (max|r| per 256-row block),
  * then the product reads p as stored (SG_AP).  Same functions, same bits as the fused first phases. */
 template <typename T>
-__global__ __launch_bounds__(TPB) void k_sgs_upd(int nblk, SgVec<T> V, double *partials,
-                                                 const adhk_scalars *__restrict__ sc)
+__global__ __launch_bounds__(TPB) void k_sgs_upd(int nblk, SgVec<T> V, double *partials, adhk_scalars *sc)
 {
     if (sc->done) return;
-    const double alpha = sc->alpha, omega = sc->omega, beta = sc->beta;
+    const double alpha = sc->alpha;
+    double omega, beta;
+    sgs_fin_s(partials, nblk, sc, omega, beta); /* the S step's finalizer, in every workgroup */
     const ColP<T> col{V.s, V.As, V.p, V.Ap, omega, beta};
     for (int j = blockIdx.x; j < nblk; j += gridDim.x) {
         double vmax[1] = {0.0};
@@ -1671,23 +1777,28 @@ static SgM<float> sgm32(adhk_ctx *c)
     return m;
 }
 
-template <typename T, int OP, int PH, bool FIRST>
+template <typename T, int OP, int PH, bool FIRST, int FIN = SGF_NONE>
 static void sgs_phase(adhk_ctx *c, const SgM<T> &M, int colour)
 {
     const adhk_sgs *G = &c->sgs;
     const SPart U{G->U.ptr, G->U.col}, L{G->L.ptr, G->L.col};
     const int r0 = G->cs[colour], nb = (G->cs[colour + 1] - r0) / TPB, xm = c->tune.xcd_map;
     const int grid = xm ? (nb + NXCD - 1) / NXCD * NXCD : nb;
-    hipLaunchKernelGGL((k_sgs_phase<T, OP, PH, FIRST>), dim3(grid), dim3(TPB), 0, c->stream, r0, nb, xm,
-                       G->cs[G->nc - 1], U, L, M.uval, M.lval, M.V, G->partials, G->nblk, c->sc);
+    hipLaunchKernelGGL((k_sgs_phase<T, OP, PH, FIRST, FIN>), dim3(grid), dim3(TPB), 0, c->stream, r0, nb, xm,
+                       G->cs[G->nc - 1], U, L, M.uval, M.lval, M.V, G->partials, G->nblk, c->sc, c->sc);
 }
 
-/* one product in 2 nc - 2 phases */
-template <typename T, int OP, bool FIRST> static int sgs_product(adhk_ctx *c, const SgM<T> &M)
+/* one product in 2 nc - 2 phases; FIN: the first phase also finishes the A step (the S product) */
+template <typename T, int OP, bool FIRST, int FIN = SGF_NONE> static int sgs_product(adhk_ctx *c, const SgM<T> &M)
 {
     const int nc = c->sgs.nc;
-    for (int k = nc - 2; k >= 1; k--) sgs_phase<T, OP, PH_B, FIRST>(c, M, k);
-    sgs_phase<T, OP, PH_B0, FIRST>(c, M, 0);
+    if (nc >= 3) {
+        sgs_phase<T, OP, PH_B, FIRST, FIN>(c, M, nc - 2);
+        for (int k = nc - 3; k >= 1; k--) sgs_phase<T, OP, PH_B, FIRST>(c, M, k);
+        sgs_phase<T, OP, PH_B0, FIRST>(c, M, 0);
+    } else {
+        sgs_phase<T, OP, PH_B0, FIRST, FIN>(c, M, 0);
+    }
     for (int k = 1; k <= nc - 2; k++) sgs_phase<T, OP, PH_F, FIRST>(c, M, k);
     sgs_phase<T, OP, PH_TOP, FIRST>(c, M, nc - 1);
     CHECK_LAUNCH();
@@ -1711,13 +1822,17 @@ template <typename T> static int sgs_init(adhk_ctx *c, const SgM<T> &M, double t
     return 0;
 }
 
-template <typename T, bool FIRST> static int sgs_a(adhk_ctx *c, const SgM<T> &M)
+/* the A step: [split, not FIRST] k_sgs_upd (with the S step's finalizer) and the product of the stored p,
+ * else the product with p formed in each row's first phase.  Its finalizer runs in the next S product's
+ * first phase; `last` (the step that only completes the last iteration) finalizes in a launch of its own. */
+template <typename T, bool FIRST> static int sgs_a(adhk_ctx *c, const SgM<T> &M, bool last = false)
 {
     const adhk_sgs *G = &c->sgs;
     int rc;
     if (!FIRST && c->tune.split) {
         adhk_kt_begin(c, ADHK_KT_UPDATE);
-        hipLaunchKernelGGL(k_sgs_upd<T>, dim3(G->nblk < MAX_GRID ? G->nblk : MAX_GRID), dim3(TPB), 0, c->stream,
+        hipLaunchKernelGGL(k_sgs_upd<T>, dim3(G->nblk < SGS_UPD_GRID ? G->nblk : SGS_UPD_GRID), dim3(TPB), 0,
+                           c->stream,
                            G->nblk, M.V, G->partials, c->sc);
         adhk_kt_end(c);
         CHECK_LAUNCH();
@@ -1729,20 +1844,27 @@ template <typename T, bool FIRST> static int sgs_a(adhk_ctx *c, const SgM<T> &M)
     }
     adhk_kt_end(c);
     if (rc) return -1;
-    hipLaunchKernelGGL(k_fin_a<FIRST>, dim3(1), dim3(TPB), 0, c->stream, G->nblk, G->inst_blk, G->partials, c->sc);
-    CHECK_LAUNCH();
+    if (last) {
+        hipLaunchKernelGGL(k_fin_a<FIRST>, dim3(1), dim3(TPB), 0, c->stream, G->nblk, G->inst_blk, G->partials,
+                           c->sc);
+        CHECK_LAUNCH();
+    }
     return 0;
 }
 
-template <typename T> static int sgs_s(adhk_ctx *c, const SgM<T> &M)
+/* the S step after the A step (FIRST: the first iteration's); the fused schedule's next A step needs omega
+ * and beta in every colour, so there the S step's finalizer is a launch of its own */
+template <typename T, bool FIRST> static int sgs_s(adhk_ctx *c, const SgM<T> &M)
 {
     const adhk_sgs *G = &c->sgs;
     adhk_kt_begin(c, ADHK_KT_SPMV_S);
-    const int rc = sgs_product<T, SG_S, false>(c, M);
+    const int rc = sgs_product<T, SG_S, false, FIRST ? SGF_A_FIRST : SGF_A>(c, M);
     adhk_kt_end(c);
     if (rc) return -1;
-    hipLaunchKernelGGL(k_fin_s, dim3(1), dim3(TPB), 0, c->stream, G->nblk, G->inst_blk, G->partials, c->sc);
-    CHECK_LAUNCH();
+    if (!c->tune.split) {
+        hipLaunchKernelGGL(k_sgs_fin_s, dim3(1), dim3(TPB), 0, c->stream, G->nblk, G->partials, c->sc);
+        CHECK_LAUNCH();
+    }
     return 0;
 }
 
@@ -1767,12 +1889,12 @@ template <typename T> static int sgs_run(adhk_ctx *c, const SgM<T> &M, int what,
         return sgs_init<T>(c, M, tol, max_it);
     case SGD_FIRST:
         if (sgs_a<T, true>(c, M)) return -1;
-        return sgs_s<T>(c, M);
+        return sgs_s<T, true>(c, M);
     case SGD_ITER:
         if (sgs_a<T, false>(c, M)) return -1;
-        return sgs_s<T>(c, M);
+        return sgs_s<T, false>(c, M);
     case SGD_LAST:
-        return sgs_a<T, false>(c, M);
+        return sgs_a<T, false>(c, M, true);
     default:
         return sgs_finish<T>(c, M);
     }

@@ -25,7 +25,7 @@
         }                                                                                                         \
     } while (0)
 
-#define SGS_VECS 10
+#define SGS_VECS 8 /* r (= s) p Ap As q xt t (= u) bt, see sg_vec in la_kernels.hip */
 
 /* DSatur colouring (Brelaz, "New methods to color the vertices of a graph", CACM 22(4), 1979) over the block
  * pattern: repeatedly the uncoloured node with the most distinct colours among its neighbours (saturation),

@@ -117,6 +117,31 @@ def test_sgs_schedules_change_no_bit(gpu, c1, precision):
         assert r[:3] == runs[0][:3] and np.array_equal(r[3], runs[0][3])
 
 
+@pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
+def test_sgs_iteration_cap(gpu, c1, precision):
+    """BiCGStab stopped at a cap of 7 iterations (bcgstab_solver.c:22 caps at 100): the step that only
+    completes the last iteration and tests it, after chunks of 2 and 4; both A-step schedules give the same
+    bits, and FP64 Newton still converges to the oracle's state under the same cap."""
+    mesh, sol0 = c1
+    s, so = perturbed_state(sol0, seed=4, amp_h=0.02, amp_v=0.1)
+    runs = []
+    for tuning in ({"split": 0, "chunk": 4}, {"split": 1, "chunk": 4}, {"split": 1, "chunk": 2, "graphs": 1}):
+        sm = sgs_model(mesh, 1.0, sol0, tuning)
+        sm.solver.lin_max_it = 7
+        sm.sol, sm.sol_old, sm.sol_older = s, so, so
+        ok, st = A.fe_newton(sm, precision=precision)
+        runs.append((ok, st["newton_its"], st["lin_its_total"], st["lin_capped"], sm.sol.copy()))
+    assert runs[0][3] > 0, "no solve reached the cap"
+    for r in runs[1:]:
+        assert r[:4] == runs[0][:4] and np.array_equal(r[4], runs[0][4])
+    if precision == L.PREC_FP64:
+        orc = O.Oracle(mesh, dt=1.0, threads=8)
+        orc.set_state(s, so, so)
+        ok_o, _ = orc.fe_newton(O.default_solver(precond=O.PREC_SGS, lin_max_it=7))
+        assert runs[0][0] == ok_o == 1
+        assert rel_err(runs[0][4][0::3], orc.state()[0][0::3]) <= 1e-8
+
+
 def test_ensemble_takes_block_jacobi(gpu, c1):
     """More than one instance: the solver's SGS request is served by block-Jacobi, bit for bit"""
     mesh, sol0 = c1
