@@ -1404,12 +1404,13 @@ template <typename T> struct SgCol<T, SG_S, false> {
  * reads a scalar that it stores itself, so no workgroup can see another's store half-way:
  *   - the first phase of the S product (sgs_fin_a): stop test on the A step's max|r| and alpha =
  *     rho / (q.Ap); stores rnorm, done, gamma, alpha and rhop = rho (the rho this alpha used);
- *   - k_sgs_upd (sgs_fin_s): omega = (As.s)/(As.As), rho' = -omega (As.q), beta = (rho' alpha)/(rhop
- *     omega), it + 1; stores omega, rho, beta, it.
+ *   - k_sgs_upd, or under the fused schedule the first phase of the A product (sgs_fin_s):
+ *     omega = (As.s)/(As.As), rho' = -omega (As.q), beta = (rho' alpha)/(rhop omega), it + 1; stores
+ *     omega, rho, beta, it (the fused product's later phases read omega and beta from there).
  * A workgroup that reads a done flag just stored by workgroup 0 of the same launch exits, as its own sum
  * would have told it.  The S partials sit in rows PR_S.. so that the first S phase, which stores them
  * when the mesh has two colours, never overwrites the max|r| and q.Ap rows it reads. */
-enum { SGF_NONE = 0, SGF_A_FIRST = 1, SGF_A = 2 };
+enum { SGF_NONE = 0, SGF_A_FIRST = 1, SGF_A = 2, SGF_S = 3 };
 /* k_sgs_upd's workgroups (each sums every S partial, then loops over 256-row blocks): two per CU; 2048
  * cost the 4.1M-dof channel 12 us more per launch in partial sums, and the 1.2M-dof flume is no faster */
 #define SGS_UPD_GRID 512
@@ -1475,15 +1476,6 @@ __device__ __forceinline__ void sgs_fin_s(const double *partials, int nblk, adhk
     beta = bc[1];
 }
 
-/* the S step's finalizer as a launch of its own, for the fused A step (whose first phases need omega and
- * beta in every colour) */
-__global__ __launch_bounds__(TPB) void k_sgs_fin_s(int nblk, const double *partials, adhk_scalars *sc)
-{
-    if (sc->done) return;
-    double omega, beta;
-    sgs_fin_s(partials, nblk, sc, omega, beta);
-}
-
 /* one colour phase over the permuted rows [r0, r0 + 256 gridDim.x) */
 /* the 256-row block of a phase's workgroup: with xcd_map, XCD k (workgroups w = k mod 8) takes the k-th
  * contiguous eighth of the colour's blocks, so neighbour gathers stay in that XCD's L2; -1: none */
@@ -1503,12 +1495,18 @@ __global__ __launch_bounds__(TPB) void k_sgs_phase(int r0, int nb, int xcd_map, 
     double alpha = 0.0, omega = 0.0, beta = 0.0;
     const int lb = sgs_block(nb, xcd_map);
     if (lb < 0) return;
-    if constexpr (FIN != SGF_NONE) {
+    if constexpr (FIN == SGF_A || FIN == SGF_A_FIRST) {
         /* the first phase of the S product finishes the A step itself, in every workgroup: the stop test
          * and alpha, as k_fin_a would in a launch of its own.  The scalars, which workgroup 0 stores during
          * this launch, are read through scw alone, by one thread per workgroup (sgs_fin_a), so that every
          * wave of a workgroup takes the same exit. */
         if (sgs_fin_a<FIN == SGF_A_FIRST>(partials, nblk, scw, alpha)) return;
+    } else if constexpr (FIN == SGF_S) {
+        /* the first phase of the fused A product finishes the S step (omega, beta); the done flag, which
+         * no workgroup of this launch stores, is read as in every other phase */
+        if (scw->done) return;
+        alpha = scw->alpha;
+        sgs_fin_s(partials, nblk, scw, omega, beta);
     } else if constexpr (OP != SG_INIT) {
         if (sc->done) return;
         alpha = sc->alpha;
@@ -1822,9 +1820,10 @@ template <typename T> static int sgs_init(adhk_ctx *c, const SgM<T> &M, double t
     return 0;
 }
 
-/* the A step: [split, not FIRST] k_sgs_upd (with the S step's finalizer) and the product of the stored p,
- * else the product with p formed in each row's first phase.  Its finalizer runs in the next S product's
- * first phase; `last` (the step that only completes the last iteration) finalizes in a launch of its own. */
+/* the A step: [split, not FIRST] k_sgs_upd and the product of the stored p, else the product with p formed
+ * in each row's first phase; the S step's finalizer runs in k_sgs_upd or in the product's first phase.  The
+ * A step's own finalizer runs in the next S product's first phase; `last` (the step that only completes the
+ * last iteration) finalizes in a launch of its own. */
 template <typename T, bool FIRST> static int sgs_a(adhk_ctx *c, const SgM<T> &M, bool last = false)
 {
     const adhk_sgs *G = &c->sgs;
@@ -1840,7 +1839,7 @@ template <typename T, bool FIRST> static int sgs_a(adhk_ctx *c, const SgM<T> &M,
         rc = sgs_product<T, SG_AP, false>(c, M);
     } else {
         adhk_kt_begin(c, ADHK_KT_SPMV_P);
-        rc = sgs_product<T, SG_A, FIRST>(c, M);
+        rc = sgs_product<T, SG_A, FIRST, FIRST ? SGF_NONE : SGF_S>(c, M);
     }
     adhk_kt_end(c);
     if (rc) return -1;
@@ -1852,20 +1851,13 @@ template <typename T, bool FIRST> static int sgs_a(adhk_ctx *c, const SgM<T> &M,
     return 0;
 }
 
-/* the S step after the A step (FIRST: the first iteration's); the fused schedule's next A step needs omega
- * and beta in every colour, so there the S step's finalizer is a launch of its own */
+/* the S step after the A step (FIRST: the first iteration's); its finalizer runs in the next A step */
 template <typename T, bool FIRST> static int sgs_s(adhk_ctx *c, const SgM<T> &M)
 {
-    const adhk_sgs *G = &c->sgs;
     adhk_kt_begin(c, ADHK_KT_SPMV_S);
     const int rc = sgs_product<T, SG_S, false, FIRST ? SGF_A_FIRST : SGF_A>(c, M);
     adhk_kt_end(c);
-    if (rc) return -1;
-    if (!c->tune.split) {
-        hipLaunchKernelGGL(k_sgs_fin_s, dim3(1), dim3(TPB), 0, c->stream, G->nblk, G->partials, c->sc);
-        CHECK_LAUNCH();
-    }
-    return 0;
+    return rc ? -1 : 0;
 }
 
 template <typename T> static int sgs_finish(adhk_ctx *c, const SgM<T> &M)
