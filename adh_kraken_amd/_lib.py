@@ -48,7 +48,7 @@ class Timing(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("chunk", C.c_int), ("graphs", C.c_int), ("xcd_map", C.c_int), ("elem_tpb", C.c_int),
-                ("split", C.c_int), ("stream_matrix", C.c_int)]
+                ("split", C.c_int), ("stream_matrix", C.c_int), ("sgs_wide", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

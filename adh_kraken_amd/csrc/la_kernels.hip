@@ -173,13 +173,26 @@ __device__ __forceinline__ void sum_blocks(const double *partials, int stride, i
 #define PR_B 4 /* 4 rows */
 
 /* ---- SELL-64 block row product y = A_n x (column values from a functor) ---- */
+/* kSelect: in a backward phase's gathers, choose between this functor and the stored t per lane by
+ * selecting addresses (one load either way) rather than by a branch; kGroup: the slot-group size of the
+ * narrow SGS phase kernels (groups of more slots keep more loads in flight, and hold more registers) */
 template <typename T> struct ColPlain {
+    static constexpr bool kSelect = true;
+    static constexpr int kGroup = 6;
     const T *x;
     __device__ __forceinline__ void get(int m, double &a, double &b, double &c) const
     {
         a = (double)x[3 * m];
         b = (double)x[3 * m + 1];
         c = (double)x[3 * m + 2];
+    }
+    /* own ? x[m] : alt[m], without a branch */
+    __device__ __forceinline__ void get_or(int m, bool own, const T *alt, double &a, double &b, double &c) const
+    {
+        const T *y = own ? x : alt;
+        a = (double)y[3 * m];
+        b = (double)y[3 * m + 1];
+        c = (double)y[3 * m + 2];
     }
 };
 
@@ -210,6 +223,8 @@ template <typename T> __device__ __forceinline__ void upd_x(T *x, double s, doub
 
 /* s = r - alpha Ap evaluated on the fly for the columns */
 template <typename T> struct ColS {
+    static constexpr bool kSelect = false;
+    static constexpr int kGroup = 4;
     const T *r, *Ap;
     double alpha;
     __device__ __forceinline__ T one(int i) const { return upd_s<T>((double)r[i], (double)Ap[i], alpha); }
@@ -219,9 +234,24 @@ template <typename T> struct ColS {
         b = (double)one(3 * m + 1);
         c = (double)one(3 * m + 2);
     }
+    __device__ __forceinline__ void get_or(int m, bool own, const T *alt, double &a, double &b, double &c) const
+    {
+        const T *x = own ? r : alt, *y = own ? Ap : alt;
+        double o[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const T xv = x[3 * m + k], yv = y[3 * m + k];
+            o[k] = own ? (double)upd_s<T>((double)xv, (double)yv, alpha) : (double)xv;
+        }
+        a = o[0];
+        b = o[1];
+        c = o[2];
+    }
 };
 /* next direction p' = (p - omega Ap) beta + (s - omega As), evaluated on the fly for the columns */
 template <typename T> struct ColP {
+    static constexpr bool kSelect = false;
+    static constexpr int kGroup = 4;
     const T *s, *As, *p, *Ap;
     double omega, beta;
     __device__ __forceinline__ void one(int i, T &rr, T &pp) const
@@ -238,6 +268,21 @@ template <typename T> struct ColP {
         b = (double)pp;
         one(3 * m + 2, rr, pp);
         c = (double)pp;
+    }
+    __device__ __forceinline__ void get_or(int m, bool own, const T *alt, double &a, double &b, double &c) const
+    {
+        const T *x0 = own ? s : alt, *x1 = own ? As : alt, *x2 = own ? p : alt, *x3 = own ? Ap : alt;
+        double o[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const T sv = x0[3 * m + k], asv = x1[3 * m + k], pv = x2[3 * m + k], apv = x3[3 * m + k];
+            const T rr = upd_r<T>((double)sv, (double)asv, omega);
+            const T pp = upd_p<T>((double)pv, (double)apv, (double)rr, omega, beta);
+            o[k] = own ? (double)pp : (double)sv;
+        }
+        a = o[0];
+        b = o[1];
+        c = o[2];
     }
 };
 
@@ -1303,14 +1348,19 @@ template <typename T> static SgVec<T> sg_vec(T *base, int Np)
     return v;
 }
 
-/* neighbour values of a backward phase: the stored t of a middle colour, the product's input for the top */
+/* neighbour values of a backward phase: the stored t of a middle colour, the product's input for the top.
+ * For a stored input (ColPlain) the lane selects the address; an input formed on the fly from two or four
+ * vectors branches, which costs less than loading them for every lane. */
 template <typename T, typename Col> struct GetB {
+    static constexpr int kGroup = Col::kGroup;
     const T *t;
     Col col;
     int top0;
     __device__ __forceinline__ void get(int m, double &a, double &b, double &c) const
     {
-        if (m >= top0) {
+        if constexpr (Col::kSelect) {
+            col.get_or(m, m >= top0, t, a, b, c);
+        } else if (m >= top0) {
             col.get(m, a, b, c);
         } else {
             a = (double)t[3 * m];
@@ -1322,46 +1372,97 @@ template <typename T, typename Col> struct GetB {
 
 /* a = sum over the row's slots of F_ij y_j, slot order (ascending natural column); the 64 rows of a wave
  * are one slice, so the slot range is wave-uniform */
-template <typename T, typename G>
-__device__ __forceinline__ void part_sum(int i, const SPart P, const T *__restrict__ val, const G &g, double &a0,
-                                         double &a1, double &a2)
+/* the slot range [c0, c1) of row i's slice in a part */
+struct SRange {
+    int c0, c1;
+};
+__device__ __forceinline__ SRange part_range(int i, const SPart P)
+{
+    const int sl = __builtin_amdgcn_readfirstlane(i >> 6);
+    return SRange{P.ptr[sl], P.ptr[sl + 1]};
+}
+
+/* N consecutive slots [cs, cs + N) of the row's slice, in straight-line code that issues every column
+ * index and matrix value, then every gather, before the first is used (a load under a branch is waited for
+ * at the branch's end, and values loaded in the order the sums use them would be waited for one by one) */
+template <int N, typename T, typename G>
+__device__ __forceinline__ void part_slots(int lane, const int *__restrict__ colp, int cs, const T *__restrict__ val,
+                                           const G &g, double &a0, double &a1, double &a2)
+{
+    int m[N];
+    T av[N][9];
+#pragma unroll
+    for (int u = 0; u < N; u++) m[u] = colp[(cs + u) * 64 + lane];
+#pragma unroll
+    for (int u = 0; u < N; u++)
+#pragma unroll
+        for (int e = 0; e < 9; e++) av[u][e] = val[((size_t)(cs + u) * 9 + e) * 64 + lane];
+    double xv[N][3];
+#pragma unroll
+    for (int u = 0; u < N; u++) g.get(m[u], xv[u][0], xv[u][1], xv[u][2]);
+#pragma unroll
+    for (int u = 0; u < N; u++) {
+        const double x0 = xv[u][0], x1 = xv[u][1], x2 = xv[u][2];
+        a0 = __fma_rn((double)av[u][0], x0, a0);
+        a0 = __fma_rn((double)av[u][1], x1, a0);
+        a0 = __fma_rn((double)av[u][2], x2, a0);
+        a1 = __fma_rn((double)av[u][3], x0, a1);
+        a1 = __fma_rn((double)av[u][4], x1, a1);
+        a1 = __fma_rn((double)av[u][5], x2, a1);
+        a2 = __fma_rn((double)av[u][6], x0, a2);
+        a2 = __fma_rn((double)av[u][7], x1, a2);
+        a2 = __fma_rn((double)av[u][8], x2, a2);
+    }
+}
+
+/* a group of min(n, MAXG) slots, the code specialised for each size up to MAXG only */
+template <int MAXG, typename T, typename G>
+__device__ __forceinline__ void part_group(int n, int lane, const int *__restrict__ colp, int cs,
+                                           const T *__restrict__ val, const G &g, double &a0, double &a1, double &a2)
+{
+    if constexpr (MAXG == 1) {
+        part_slots<1, T>(lane, colp, cs, val, g, a0, a1, a2);
+    } else {
+        if (n >= MAXG)
+            part_slots<MAXG, T>(lane, colp, cs, val, g, a0, a1, a2);
+        else
+            part_group<MAXG - 1, T>(n, lane, colp, cs, val, g, a0, a1, a2);
+    }
+}
+
+/* Two register budgets (the tuning field sgs_wide picks one; the sums are the same bits):
+ *   narrow: groups of up to G::kGroup slots (6 for stored inputs, 4 for inputs formed on the fly), code
+ *     for each size up to that only: 120-160 VGPRs, 3-4 waves per SIMD.  On the 1.2M-dof flume a colour
+ *     phase is about 2,100 waves, just over the 2,048 that 2 waves per SIMD can hold, so the occupancy
+ *     decides it;
+ *   wide: groups of up to eight slots for every input, every size compiled in each kernel: about 230
+ *     VGPRs and 2 waves per SIMD, every load of a group in flight at once.  On the 4.1M-dof channel, whose
+ *     colour phases take several rounds of waves anyway, this is 4 % faster than narrow. */
+template <bool WIDE, typename T, typename G>
+__device__ __forceinline__ void part_sum(int i, const SPart P, SRange rg, const T *__restrict__ val, const G &g,
+                                         double &a0, double &a1, double &a2)
 {
     const int lane = i & 63;
-    const int sl = __builtin_amdgcn_readfirstlane(i >> 6);
-    const int c0 = P.ptr[sl], c1 = P.ptr[sl + 1];
     a0 = 0.0;
     a1 = 0.0;
     a2 = 0.0;
-    for (int cs = c0; cs < c1; cs += 8) {
-        int m[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) m[u] = (cs + u < c1) ? P.col[(cs + u) * 64 + lane] : 0;
-        double xv[8][3];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            if (cs + u < c1) {
-                g.get(m[u], xv[u][0], xv[u][1], xv[u][2]);
-            } else {
-                xv[u][0] = 0.0;
-                xv[u][1] = 0.0;
-                xv[u][2] = 0.0;
+    /* groups in slot order; the group size is wave-uniform */
+    if constexpr (WIDE) {
+        for (int cs = rg.c0; cs < rg.c1; cs += 8) {
+            switch (rg.c1 - cs) {
+            case 1: part_slots<1, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
+            case 2: part_slots<2, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
+            case 3: part_slots<3, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
+            case 4: part_slots<4, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
+            case 5: part_slots<5, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
+            case 6: part_slots<6, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
+            case 7: part_slots<7, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
+            default: part_slots<8, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
             }
         }
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            if (cs + u >= c1) break;
-            const T *a = val + (size_t)(cs + u) * 9 * 64 + lane;
-            const double x0 = xv[u][0], x1 = xv[u][1], x2 = xv[u][2];
-            a0 = __fma_rn((double)a[0], x0, a0);
-            a0 = __fma_rn((double)a[64], x1, a0);
-            a0 = __fma_rn((double)a[128], x2, a0);
-            a1 = __fma_rn((double)a[192], x0, a1);
-            a1 = __fma_rn((double)a[256], x1, a1);
-            a1 = __fma_rn((double)a[320], x2, a1);
-            a2 = __fma_rn((double)a[384], x0, a2);
-            a2 = __fma_rn((double)a[448], x1, a2);
-            a2 = __fma_rn((double)a[512], x2, a2);
-        }
+    } else {
+        for (int cs = rg.c0; cs < rg.c1; cs += G::kGroup)
+            part_group<G::kGroup, T>(rg.c1 - cs, lane, P.col, cs, val, g, a0, a1, a2);
     }
 }
 
@@ -1486,7 +1587,7 @@ __device__ __forceinline__ int sgs_block(int nb, int xcd_map)
     return lb < nb ? lb : -1;
 }
 
-template <typename T, int OP, int PH, bool FIRST, int FIN>
+template <typename T, int OP, int PH, bool FIRST, int FIN, bool WIDE>
 __global__ __launch_bounds__(TPB) void k_sgs_phase(int r0, int nb, int xcd_map, int top0, SPart U, SPart L,
                                                    const T *__restrict__ uval, const T *__restrict__ lval, SgVec<T> V,
                                                    double *partials, int nblk, const adhk_scalars *__restrict__ sc,
@@ -1495,6 +1596,12 @@ __global__ __launch_bounds__(TPB) void k_sgs_phase(int r0, int nb, int xcd_map, 
     double alpha = 0.0, omega = 0.0, beta = 0.0;
     const int lb = sgs_block(nb, xcd_map);
     if (lb < 0) return;
+    const int i = r0 + lb * TPB + threadIdx.x;
+    /* the slice's slot ranges, loaded before the scalars are tested, so that the two loads share one round
+     * trip (the ranges enter the test, which keeps the compiler from moving their loads after it) */
+    SRange ru{0, 0}, rl{0, 0};
+    if constexpr (PH == PH_B || PH == PH_B0) ru = part_range(i, U);
+    if constexpr (PH != PH_B && PH != PH_B0) rl = part_range(i, L);
     if constexpr (FIN == SGF_A || FIN == SGF_A_FIRST) {
         /* the first phase of the S product finishes the A step itself, in every workgroup: the stop test
          * and alpha, as k_fin_a would in a launch of its own.  The scalars, which workgroup 0 stores during
@@ -1508,96 +1615,121 @@ __global__ __launch_bounds__(TPB) void k_sgs_phase(int r0, int nb, int xcd_map, 
         alpha = scw->alpha;
         sgs_fin_s(partials, nblk, scw, omega, beta);
     } else if constexpr (OP != SG_INIT) {
-        if (sc->done) return;
+        if (sc->done || ru.c1 < ru.c0 || rl.c1 < rl.c0) return; /* the ranges are never reversed */
         alpha = sc->alpha;
         omega = sc->omega;
         beta = sc->beta;
     }
-    const int i = r0 + lb * TPB + threadIdx.x;
     const int j = i / TPB; /* reduction block: r0 is a multiple of 256 */
     const typename SgCol<T, OP, FIRST>::type col = SgCol<T, OP, FIRST>::make(V, alpha, omega, beta);
     double pa1[1] = {0.0}, pa2[2] = {0.0, 0.0}, pa3[3] = {0.0, 0.0, 0.0}, pb[1] = {0.0};
-    double v[3], t[3];
-    /* the product's input at the row */
+    double v[3], t[3], w[3], uu[3];
+    /* Every load of the row's own values, and the neighbour sums (which read nothing this phase stores),
+     * come before the first store: s shares r's buffer and u shares t's, and the compiler, which cannot tell
+     * the vectors apart, would otherwise hold each load after a store until the value stored had returned.
+     * st0/st1 hold the first phase's stores ([S] s; [A, first iteration] p = r; [A] r and p), xn the new x. */
+    T st0[3], st1[3], xn[3], tt[3], ut[3];
+    constexpr bool FIRST_PH = (OP != SG_INIT && OP != SG_AP && PH != PH_F); /* the row's first phase */
     if constexpr (OP == SG_INIT) {
         for (int k = 0; k < 3; k++) v[k] = (double)V.xt[3 * i + k];
-    } else if constexpr (OP == SG_AP || PH == PH_F) { /* stored by k_sgs_upd or the row's first phase */
+    } else if constexpr (!FIRST_PH) { /* stored by k_sgs_upd or the row's first phase */
         const T *vs = (OP == SG_S) ? V.s : V.p;
         for (int k = 0; k < 3; k++) v[k] = (double)vs[3 * i + k];
     } else if constexpr (OP == SG_A && FIRST) {
         for (int k = 0; k < 3; k++) {
-            const T rr = V.r[3 * i + k];
-            V.p[3 * i + k] = rr;
-            v[k] = (double)rr;
+            st0[k] = V.r[3 * i + k];
+            v[k] = (double)st0[k];
         }
     } else if constexpr (OP == SG_A) {
+        /* the previous iteration's x += alpha p + omega s, r = s - omega As, and p = (p - omega Ap) beta + r */
+        T s_[3], As_[3], p_[3], Ap_[3], x_[3];
         for (int k = 0; k < 3; k++) {
             const int ii = 3 * i + k;
-            T rr, pp;
-            col.one(ii, rr, pp); /* reads the old p and Ap */
-            upd_x(V.xt + ii, (double)V.s[ii], (double)V.p[ii], alpha, omega);
-            V.r[ii] = rr;
-            V.p[ii] = pp;
+            s_[k] = V.s[ii];
+            As_[k] = V.As[ii];
+            p_[k] = V.p[ii];
+            Ap_[k] = V.Ap[ii];
+            x_[k] = __builtin_nontemporal_load(V.xt + ii);
+        }
+        for (int k = 0; k < 3; k++) {
+            const T rr = upd_r<T>((double)s_[k], (double)As_[k], omega);
+            const T pp = upd_p<T>((double)p_[k], (double)Ap_[k], (double)rr, omega, beta);
+            xn[k] = (T)__fma_rn(omega, (double)s_[k], __fma_rn(alpha, (double)p_[k], (double)x_[k]));
+            st0[k] = rr;
+            st1[k] = pp;
             v[k] = (double)pp;
             pa1[0] = fmax(pa1[0], fabs((double)rr));
         }
-    } else { /* SG_S */
+    } else { /* SG_S: s = r - alpha Ap */
+        T r_[3], Ap_[3];
         for (int k = 0; k < 3; k++) {
-            const T ss = col.one(3 * i + k);
-            V.s[3 * i + k] = ss;
-            v[k] = (double)ss;
+            r_[k] = V.r[3 * i + k];
+            Ap_[k] = V.Ap[3 * i + k];
+        }
+        for (int k = 0; k < 3; k++) {
+            st0[k] = upd_s<T>((double)r_[k], (double)Ap_[k], alpha);
+            v[k] = (double)st0[k];
         }
     }
+    if constexpr (PH == PH_F)
+        for (int k = 0; k < 3; k++) t[k] = (double)V.t[3 * i + k];
+    if constexpr (PH != PH_B)
+        for (int k = 0; k < 3; k++) w[k] = (double)(OP == SG_INIT ? V.bt[3 * i + k] : V.q[3 * i + k]);
     /* backward: t = v - sum_U F t */
     if constexpr (PH == PH_B || PH == PH_B0) {
         double a[3];
-        part_sum<T>(i, U, uval, GetB<T, typename SgCol<T, OP, FIRST>::type>{V.t, col, top0}, a[0], a[1], a[2]);
+        part_sum<WIDE, T>(i, U, ru, uval, GetB<T, typename SgCol<T, OP, FIRST>::type>{V.t, col, top0}, a[0],
+                          a[1], a[2]);
         for (int k = 0; k < 3; k++) {
-            const T tk = (T)(v[k] - a[k]);
-            if constexpr (PH == PH_B) V.t[3 * i + k] = tk; /* colour 0's t is used in this phase only */
-            t[k] = (double)tk;
+            tt[k] = (T)(v[k] - a[k]);
+            t[k] = (double)tt[k];
         }
-    } else if constexpr (PH == PH_F) {
-        for (int k = 0; k < 3; k++) t[k] = (double)V.t[3 * i + k];
-    } else {
+    } else if constexpr (PH == PH_TOP) {
         for (int k = 0; k < 3; k++) t[k] = v[k];
     }
-    if constexpr (PH != PH_B) {
-        /* forward: u = (v - t) - sum_L F u, then z = t + u */
-        double uu[3];
-        if constexpr (PH == PH_B0) {
-            for (int k = 0; k < 3; k++) uu[k] = v[k] - t[k];
-        } else {
-            double a[3];
-            part_sum<T>(i, L, lval, ColPlain<T>{V.u}, a[0], a[1], a[2]);
-            for (int k = 0; k < 3; k++) uu[k] = (v[k] - t[k]) - a[k];
-        }
-        if constexpr (PH != PH_TOP)
-            for (int k = 0; k < 3; k++) {
-                const T uk = (T)uu[k];
-                V.u[3 * i + k] = uk;
-                uu[k] = (double)uk;
-            }
+    /* forward: u = (v - t) - sum_L F u, then z = t + u */
+    if constexpr (PH == PH_B0) {
+        for (int k = 0; k < 3; k++) uu[k] = v[k] - t[k];
+    } else if constexpr (PH != PH_B) {
+        double a[3];
+        part_sum<WIDE, T>(i, L, rl, lval, ColPlain<T>{V.u}, a[0], a[1], a[2]);
+        for (int k = 0; k < 3; k++) uu[k] = (v[k] - t[k]) - a[k];
+    }
+    if constexpr (PH == PH_B0 || PH == PH_F)
         for (int k = 0; k < 3; k++) {
-            const int ii = 3 * i + k;
+            ut[k] = (T)uu[k];
+            uu[k] = (double)ut[k];
+        }
+    /* the stores */
+    for (int k = 0; k < 3; k++) {
+        const int ii = 3 * i + k;
+        if constexpr (FIRST_PH && OP == SG_S) V.s[ii] = st0[k];
+        if constexpr (FIRST_PH && OP == SG_A && FIRST) V.p[ii] = st0[k];
+        if constexpr (FIRST_PH && OP == SG_A && !FIRST) {
+            __builtin_nontemporal_store(xn[k], V.xt + ii);
+            V.r[ii] = st0[k];
+            V.p[ii] = st1[k];
+        }
+        if constexpr (PH == PH_B) V.t[ii] = tt[k]; /* colour 0's t is used in its one phase only */
+        if constexpr (PH == PH_B0 || PH == PH_F) V.u[ii] = ut[k];
+        if constexpr (PH != PH_B) {
             if constexpr (OP == SG_INIT) { /* r = b~ - z, q = r */
-                const double bb = (double)V.bt[ii];
-                const T rr = (T)(bb + (-1) * (t[k] + uu[k]));
+                const T rr = (T)(w[k] + (-1) * (t[k] + uu[k]));
                 V.r[ii] = rr;
                 V.q[ii] = rr;
                 pa2[0] = fmax(pa2[0], fabs((double)rr));
-                pa2[1] = fmax(pa2[1], fabs(bb));
+                pa2[1] = fmax(pa2[1], fabs(w[k]));
                 pb[0] = __fma_rn((double)rr, (double)rr, pb[0]);
             } else if constexpr (OP == SG_A || OP == SG_AP) {
                 const T zk = (T)(t[k] + uu[k]);
                 V.Ap[ii] = zk;
-                pb[0] = __fma_rn((double)V.q[ii], (double)zk, pb[0]);
+                pb[0] = __fma_rn(w[k], (double)zk, pb[0]);
             } else {
                 const T zk = (T)(t[k] + uu[k]);
                 V.As[ii] = zk;
                 pa3[0] = __fma_rn((double)zk, v[k], pa3[0]);
                 pa3[1] = __fma_rn((double)zk, (double)zk, pa3[1]);
-                pa3[2] = __fma_rn((double)zk, (double)V.q[ii], pa3[2]);
+                pa3[2] = __fma_rn((double)zk, w[k], pa3[2]);
             }
         }
     }
@@ -1647,7 +1779,7 @@ __global__ __launch_bounds__(TPB) void k_sgs_fwd(int r0, SPart L, const T *__res
 {
     const int i = r0 + blockIdx.x * TPB + threadIdx.x;
     double a[3];
-    part_sum<T>(i, L, lval, ColPlain<T>{bt}, a[0], a[1], a[2]);
+    part_sum<false, T>(i, L, part_range(i, L), lval, ColPlain<T>{bt}, a[0], a[1], a[2]);
     for (int k = 0; k < 3; k++) {
         const T w = (T)((double)bt[3 * i + k] - a[k]);
         bt[3 * i + k] = w;
@@ -1662,7 +1794,8 @@ __global__ __launch_bounds__(TPB) void k_sgs_bwd(int r0, int top0, SPart U, cons
 {
     const int i = r0 + blockIdx.x * TPB + threadIdx.x;
     double a[3];
-    part_sum<T>(i, U, uval, GetB<T, ColPlain<T>>{t, ColPlain<T>{xt}, top0}, a[0], a[1], a[2]);
+    part_sum<false, T>(i, U, part_range(i, U), uval, GetB<T, ColPlain<T>>{t, ColPlain<T>{xt}, top0}, a[0], a[1],
+                       a[2]);
     for (int k = 0; k < 3; k++) t[3 * i + k] = (T)((double)xt[3 * i + k] - a[k]);
 }
 
@@ -1782,8 +1915,12 @@ static void sgs_phase(adhk_ctx *c, const SgM<T> &M, int colour)
     const SPart U{G->U.ptr, G->U.col}, L{G->L.ptr, G->L.col};
     const int r0 = G->cs[colour], nb = (G->cs[colour + 1] - r0) / TPB, xm = c->tune.xcd_map;
     const int grid = xm ? (nb + NXCD - 1) / NXCD * NXCD : nb;
-    hipLaunchKernelGGL((k_sgs_phase<T, OP, PH, FIRST, FIN>), dim3(grid), dim3(TPB), 0, c->stream, r0, nb, xm,
-                       G->cs[G->nc - 1], U, L, M.uval, M.lval, M.V, G->partials, G->nblk, c->sc, c->sc);
+    if (c->tune.sgs_wide)
+        hipLaunchKernelGGL((k_sgs_phase<T, OP, PH, FIRST, FIN, true>), dim3(grid), dim3(TPB), 0, c->stream, r0, nb,
+                           xm, G->cs[G->nc - 1], U, L, M.uval, M.lval, M.V, G->partials, G->nblk, c->sc, c->sc);
+    else
+        hipLaunchKernelGGL((k_sgs_phase<T, OP, PH, FIRST, FIN, false>), dim3(grid), dim3(TPB), 0, c->stream, r0, nb,
+                           xm, G->cs[G->nc - 1], U, L, M.uval, M.lval, M.V, G->partials, G->nblk, c->sc, c->sc);
 }
 
 /* one product in 2 nc - 2 phases; FIN: the first phase also finishes the A step (the S product) */

@@ -199,6 +199,7 @@ static adhk_sw2 *create_impl(int ninst, const int *node_off, const int *elem_off
     c->tune.elem_tpb = 256;
     c->tune.split = 0;
     c->tune.stream_matrix = -1;
+    c->tune.sgs_wide = 0;
     HC(hipStreamCreate(&c->stream));
     for (int i = 0; i < 8; i++) HC(hipEventCreate(&c->ev[i]));
 
@@ -960,7 +961,7 @@ static void drop_graphs(adhk_ctx *c)
 static int same_schedule(const adhk_tuning *a, const adhk_tuning *b)
 {
     return a->chunk == b->chunk && a->split == b->split && a->xcd_map == b->xcd_map && a->elem_tpb == b->elem_tpb &&
-           a->stream_matrix == b->stream_matrix;
+           a->stream_matrix == b->stream_matrix && a->sgs_wide == b->sgs_wide;
 }
 static int same_params(const adhk_params *a, const adhk_params *b)
 {
@@ -1172,8 +1173,9 @@ int adhk_sw2_set_tuning(adhk_sw2 *c, const adhk_tuning *t)
     if (!c || !t) return -1;
     if (t->chunk < 0 || t->chunk % 2 || t->chunk > 100 || (t->graphs != 0 && t->graphs != 1) ||
         (t->xcd_map != 0 && t->xcd_map != 1) || (t->elem_tpb != 64 && t->elem_tpb != 128 && t->elem_tpb != 256) ||
-        (t->split != 0 && t->split != 1) || t->stream_matrix < -1 || t->stream_matrix > 1) {
-        adhk_set_error("adhk_sw2_set_tuning: chunk even in [0, 100], graphs, xcd_map and split 0/1, "
+        (t->split != 0 && t->split != 1) || t->stream_matrix < -1 || t->stream_matrix > 1 ||
+        (t->sgs_wide != 0 && t->sgs_wide != 1)) {
+        adhk_set_error("adhk_sw2_set_tuning: chunk even in [0, 100], graphs, xcd_map, split and sgs_wide 0/1, "
                        "elem_tpb 64/128/256, stream_matrix -1/0/1");
         return -1;
     }
@@ -1300,6 +1302,28 @@ int adhk_sw2_autotune(adhk_sw2 *c, const adhk_solver *s, int ninner, adhk_tuning
             best.stream_matrix = !eff;
         } else {
             best.stream_matrix = eff;
+        }
+    }
+    /* 5. the SGS phase kernels' register budget, the other one than the chosen schedule's, on the same 2%
+     * margin */
+    if (c->pc == ADHK_PC_SGS) {
+        double tbest = 1e300;
+        c->tune = best;
+        c->tune.sgs_wide = !best.sgs_wide;
+        for (int r = 0; r <= ninner; r++) {
+            double t0 = now_ms();
+            LC(krylov(c, s, mixed, hs, its));
+            double t1 = now_ms();
+            if (r > 0 && t1 - t0 < tbest) tbest = t1 - t0;
+        }
+        for (int i = 0; i < K; i++)
+            if (its[i] != its_ref[i]) {
+                adhk_set_error("adhk_sw2_autotune: schedule changed an iteration count (internal error)");
+                goto fail;
+            }
+        if (tbest < best_k * 0.98) {
+            best_k = tbest;
+            best.sgs_wide = !best.sgs_wide;
         }
     }
     drop_graphs(c);

@@ -110,7 +110,7 @@ typedef struct {
 /* Schedule choices of a context.  None changes a computed value: the iterations, the reduction order
  * and every operand are the same under all of them, only their launch and read-back pattern differs.
  * Defaults: chunk 0 (use adhk_solver.lin_check_every), graphs 0, xcd_map 1, elem_tpb 256, split 0,
- * stream_matrix -1. */
+ * stream_matrix -1, sgs_wide 0. */
 typedef struct {
     int chunk;    /* BiCGStab iterations launched between "done" read-backs (even; 0 = the solver's) */
     int graphs;   /* 1 = replay each chunk as a captured hipGraph (when per-kernel timers are off) */
@@ -120,6 +120,8 @@ typedef struct {
                      (one gathered vector per column instead of four) */
     int stream_matrix; /* matrix loads of the BiCGStab products: 1 non-temporal, 0 cached, -1 (default)
                           non-temporal when the matrix exceeds the 256 MB MALL */
+    int sgs_wide; /* multicolour SGS phase kernels: 1 = groups of eight slots with every load in flight
+                     (2 waves per SIMD), 0 = smaller groups at 3-4 waves per SIMD */
 } adhk_tuning;
 
 /* kernel classes of the per-kernel device timers */

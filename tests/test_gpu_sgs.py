@@ -109,7 +109,8 @@ def test_sgs_schedules_change_no_bit(gpu, c1, precision):
     mesh, sol0 = c1
     runs = []
     for tuning in (None, {"graphs": 1, "chunk": 8}, {"graphs": 0, "chunk": 4, "xcd_map": 0},
-                   {"graphs": 1, "chunk": 16, "split": 1}):
+                   {"graphs": 1, "chunk": 16, "split": 1}, {"sgs_wide": 1, "chunk": 8},
+                   {"sgs_wide": 1, "split": 1, "graphs": 1, "chunk": 8}):
         sm = sgs_model(mesh, 1.0, sol0, tuning)
         n, st = A.time_loop(sm, 4, precision=precision)
         runs.append((n, st["newton_its"], st["lin_its_total"], sm.sol.copy()))
@@ -125,7 +126,8 @@ def test_sgs_iteration_cap(gpu, c1, precision):
     mesh, sol0 = c1
     s, so = perturbed_state(sol0, seed=4, amp_h=0.02, amp_v=0.1)
     runs = []
-    for tuning in ({"split": 0, "chunk": 4}, {"split": 1, "chunk": 4}, {"split": 1, "chunk": 2, "graphs": 1}):
+    for tuning in ({"split": 0, "chunk": 4}, {"split": 1, "chunk": 4}, {"split": 1, "chunk": 2, "graphs": 1},
+                   {"split": 1, "chunk": 4, "sgs_wide": 1}):
         sm = sgs_model(mesh, 1.0, sol0, tuning)
         sm.solver.lin_max_it = 7
         sm.sol, sm.sol_old, sm.sol_older = s, so, so
