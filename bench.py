@@ -70,15 +70,17 @@ def algorithmic_bytes(kind, N, nslot, ndofs, mixed, split=False):
     raise ValueError(kind)
 
 
-def sgs_product_bytes(layout, mixed):
-    """HBM bytes of one multicolour SGS product (the S step: six colour-phase launches on a 4-colour mesh)
-    with every operand moved once: the U and L parts of the folded matrix in SELL-64 over the permuted rows
-    (slot columns x 64 x (9 values + 1 column index), slice pointers) and seven vectors of 3 x rows values
-    (r, Ap, q read; s, t, u, As written)."""
+def sgs_product_bytes(layout, counts, mixed):
+    """HBM bytes of one multicolour SGS product (the S step: 2 nc - 2 colour-phase launches) with every operand
+    moved once, and the matrix's share of them: the U and L parts of the folded matrix in SELL-64 over the
+    permuted rows (slot columns x 64 x (9 values + 1 column index), slice pointers); r, Ap and q read and s and
+    As written for every node; t written for the middle colours (1 .. nc - 2) and u for every colour but the
+    top one (colour 0's t and the top colour's t and u stay in registers).  counts: nodes per colour."""
     vb = 4 if mixed else 8
     rows = layout["rows"]
     mat = (layout["u_slots"] + layout["l_slots"]) * 64 * (9 * vb + 4) + 2 * (rows // 64 + 1) * 4
-    return mat + 7 * 3 * rows * vb
+    nodes, mid, nontop = int(sum(counts)), int(sum(counts[1:-1])), int(sum(counts[:-1]))
+    return mat + 3 * vb * (5 * nodes + mid + nontop), mat
 
 
 def host_cpu():
@@ -411,11 +413,10 @@ def main():
     def roofline_sgs(run):
         """the SGS solve's dominant unit: the S product, one pass over the folded matrix in 2 nc - 2 colour-phase
         launches of k_sgs_phase, timed by one event pair around the sequence"""
-        _, nc, lay = run["sm"].colouring(layout=True)
-        ab = sgs_product_bytes(lay, False)
+        col, nc, lay = run["sm"].colouring(layout=True)
+        ab, mat = sgs_product_bytes(lay, np.bincount(col, minlength=nc), False)
         avg_ms = run["kt"]["spmv_s"]["ms_avg"]
         achieved = ab / (avg_ms / 1e3) / 1e9
-        mat = ab - 7 * 3 * lay["rows"] * 8
         return {"kernel": f"k_sgs_phase<double, S> x {2 * nc - 2} (one SGS product)", "bound": "hbm",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_tab.get("sgs_s_product"),
