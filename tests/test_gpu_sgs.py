@@ -119,6 +119,21 @@ def test_sgs_schedules_change_no_bit(gpu, c1, precision):
 
 
 @pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
+def test_sgs_register_budgets_agree_on_a_channel(gpu, chan, precision):
+    """The unstructured channel's rows hold up to eight slots per part, so the wide phase kernels take their
+    largest groups and the narrow ones split rows into several groups: the same bits either way."""
+    mesh, sol0 = chan
+    runs = []
+    for tuning in ({"sgs_wide": 0, "split": 1}, {"sgs_wide": 1, "split": 1}, {"sgs_wide": 1, "split": 0}):
+        sm = sgs_model(mesh, 5.0, sol0, tuning)
+        n, st = A.time_loop(sm, 2, precision=precision)
+        runs.append((n, st["newton_its"], st["lin_its_total"], sm.sol.copy()))
+    assert runs[0][0] == 2
+    for r in runs[1:]:
+        assert r[:3] == runs[0][:3] and np.array_equal(r[3], runs[0][3])
+
+
+@pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
 def test_sgs_iteration_cap(gpu, c1, precision):
     """BiCGStab stopped at a cap of 7 iterations (bcgstab_solver.c:22 caps at 100): the step that only
     completes the last iteration and tests it, after chunks of 2 and 4; both A-step schedules give the same
