@@ -212,8 +212,7 @@ int adhk_launch_krylov_finish(adhk_ctx *c, int mixed);
 int adhk_sgs_setup(adhk_ctx *c);              /* colouring and the permuted L/U layout (sgs_setup.c) */
 int adhk_sgs_ensure(adhk_ctx *c, int mixed);  /* the value arrays and vectors of a precision */
 void adhk_sgs_free(adhk_ctx *c);
-int adhk_launch_sgs_fold(adhk_ctx *c, int mixed); /* fold into the SGS L/U parts, diagonal check */
-int adhk_launch_fold(adhk_ctx *c, int mixed);     /* block-Jacobi fold of the natural SELL matrix in place */
+int adhk_launch_sgs_gather(adhk_ctx *c, int mixed); /* folded matrix -> L/U values, diagonal check */
 int adhk_launch_increment(adhk_ctx *c);
 int adhk_launch_linesearch(adhk_ctx *c);
 int adhk_launch_init_system(adhk_ctx *c);

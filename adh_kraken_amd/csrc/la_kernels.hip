@@ -674,15 +674,19 @@ __global__ __launch_bounds__(TPB) void k_prep_rowmax(int N, const int *__restric
 }
 
 /* TB: the matrix storage, folded in place (FP64, or FP32 in mixed precision); minv in the same type */
-/* node n's scaled, Dirichlet-applied diagonal block (slot 0 of its natural SELL row) and its inverse, stored
- * in minv and returned in m; diag_id[n] = 0 when the block is singular and its diagonal's reciprocals stand
- * in for the inverse */
 template <typename TB>
-__device__ __forceinline__ void fold_minv(int n, int c0, int deg, const int (&brc)[3], const double (&si)[3],
-                                          const TB *__restrict__ bv, TB *__restrict__ minv,
-                                          unsigned char *__restrict__ diag_id, double (&m)[9])
+__global__ __launch_bounds__(TPB) void k_prep_fold(int N, const int *__restrict__ brow,
+                                                   const int *__restrict__ slice_ptr, const int *__restrict__ scol,
+                                                   const int *__restrict__ bc, TB *__restrict__ bv,
+                                                   const double *__restrict__ scale, TB *__restrict__ minv,
+                                                   unsigned char *__restrict__ diag_id)
 {
-    const int lane = n & 63;
+    int n = xcd_block() * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int lane = n & 63, c0 = slice_ptr[n >> 6], w = slice_ptr[(n >> 6) + 1] - c0, deg = brow[n + 1] - brow[n];
+    const int brc[3] = {bc[3 * n], bc[3 * n + 1], bc[3 * n + 2]};
+    const double si[3] = {scale[3 * n], scale[3 * n + 1], scale[3 * n + 2]};
+    /* the scaled diagonal block (slot 0) and its inverse */
     double a[9];
     for (int r = 0; r < 3; r++)
         for (int cc = 0; cc < 3; cc++)
@@ -694,6 +698,7 @@ __device__ __forceinline__ void fold_minv(int n, int c0, int deg, const int (&br
     double c01 = a[5] * a[6] - a[3] * a[8];
     double c02 = a[3] * a[7] - a[4] * a[6];
     double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    double m[9];
     diag_id[n] = fabs(det) > 1e-300 ? 1 : 0;
     if (fabs(det) > 1e-300) {
         double id = 1.0 / det;
@@ -712,128 +717,48 @@ __device__ __forceinline__ void fold_minv(int n, int c0, int deg, const int (&br
     }
     TB *mo = minv + (size_t)(n >> 6) * 9 * 64 + lane;
     for (int k = 0; k < 9; k++) mo[k * 64] = (TB)m[k];
-}
-
-/* the fold of one block: M_n^-1 (S A S)_nm with Dirichlet rows and columns applied (the same operations in
- * the same order wherever a block is folded) */
-template <typename TB, typename TO>
-__device__ __forceinline__ void fold_block(const double (&m)[9], const TB *__restrict__ bv, int cs, int lane, int n,
-                                           int mm, const int (&brc)[3], const double (&si)[3],
-                                           const int *__restrict__ bc, const double *__restrict__ scale, TO *out,
-                                           int ocs, int olane)
-{
-    const int bcc[3] = {bc[3 * mm], bc[3 * mm + 1], bc[3 * mm + 2]};
-    const double sj[3] = {scale[3 * mm], scale[3 * mm + 1], scale[3 * mm + 2]};
-    double b[9];
-    for (int r = 0; r < 3; r++)
-        for (int cc = 0; cc < 3; cc++)
-            b[3 * r + cc] = dirichlet_entry((double)bv[sell_val(cs, 3 * r + cc, lane)], n, mm, r, cc, brc, bcc) *
-                            (si[r] * sj[cc]);
-    for (int r = 0; r < 3; r++)
-        for (int cc = 0; cc < 3; cc++) {
-            const double f = __fma_rn(m[3 * r + 2], b[6 + cc], __fma_rn(m[3 * r + 1], b[3 + cc], m[3 * r] * b[cc]));
-            out[sell_val(ocs, 3 * r + cc, olane)] = (TO)f;
-        }
-}
-
-template <typename TB>
-__global__ __launch_bounds__(TPB) void k_prep_fold(int N, const int *__restrict__ brow,
-                                                   const int *__restrict__ slice_ptr, const int *__restrict__ scol,
-                                                   const int *__restrict__ bc, TB *__restrict__ bv,
-                                                   const double *__restrict__ scale, TB *__restrict__ minv,
-                                                   unsigned char *__restrict__ diag_id)
-{
-    int n = xcd_block() * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    const int lane = n & 63, c0 = slice_ptr[n >> 6], w = slice_ptr[(n >> 6) + 1] - c0, deg = brow[n + 1] - brow[n];
-    const int brc[3] = {bc[3 * n], bc[3 * n + 1], bc[3 * n + 2]};
-    const double si[3] = {scale[3 * n], scale[3 * n + 1], scale[3 * n + 2]};
-    double m[9];
-    fold_minv<TB>(n, c0, deg, brc, si, bv, minv, diag_id, m);
     /* every slot: Dirichlet, scaling and fold, written in place as M^-1 A */
     for (int k = 0; k < w; k++) {
         const int cs = c0 + k;
+        double b[9];
         if (k < deg) {
-            fold_block<TB, TB>(m, bv, cs, lane, n, scol[cs * 64 + lane], brc, si, bc, scale, bv, cs, lane);
+            const int mm = scol[cs * 64 + lane];
+            const int bcc[3] = {bc[3 * mm], bc[3 * mm + 1], bc[3 * mm + 2]};
+            const double sj[3] = {scale[3 * mm], scale[3 * mm + 1], scale[3 * mm + 2]};
+            for (int r = 0; r < 3; r++)
+                for (int cc = 0; cc < 3; cc++)
+                    b[3 * r + cc] = dirichlet_entry((double)bv[sell_val(cs, 3 * r + cc, lane)], n, mm, r, cc, brc, bcc) *
+                                    (si[r] * sj[cc]);
         } else {
-            for (int e = 0; e < 9; e++) bv[sell_val(cs, e, lane)] = (TB)0.0;
+            for (int e = 0; e < 9; e++) b[e] = 0.0;
         }
-    }
-}
-
-/* The multicolour SGS path's fold, straight into its colour-permuted parts, leaving the natural SELL
- * matrix unfolded: one thread per permuted row forms its node's block inverse (stored in minv, as the
- * in-place fold does) and folds, for each of its U and L slots, the natural block the slot maps to (src).
- * The parts hold the bits a fold in place followed by a copy gave, for about half the traffic. */
-template <typename TB>
-__global__ __launch_bounds__(TPB) void k_sgs_fold(int Np, const int *__restrict__ prow, const int *__restrict__ brow,
-                                                  const int *__restrict__ slice_ptr, const int *__restrict__ uptr,
-                                                  const int *__restrict__ usrc, const int *__restrict__ lptr,
-                                                  const int *__restrict__ lsrc, const int *__restrict__ scol,
-                                                  const int *__restrict__ bc, const TB *__restrict__ bv,
-                                                  const double *__restrict__ scale, TB *__restrict__ minv,
-                                                  unsigned char *__restrict__ diag_id, TB *__restrict__ uval,
-                                                  TB *__restrict__ lval)
-{
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= Np) return;
-    const int lane = p & 63, sl = p >> 6, n = prow[p];
-    int brc[3] = {0, 0, 0};
-    double si[3] = {0.0, 0.0, 0.0}, m[9];
-    if (n >= 0) {
-        for (int k = 0; k < 3; k++) {
-            brc[k] = bc[3 * n + k];
-            si[k] = scale[3 * n + k];
-        }
-        fold_minv<TB>(n, slice_ptr[n >> 6], brow[n + 1] - brow[n], brc, si, bv, minv, diag_id, m);
-    }
-    for (int part = 0; part < 2; part++) {
-        const int *ptr = part ? lptr : uptr, *src = part ? lsrc : usrc;
-        TB *out = part ? lval : uval;
-        for (int cs = ptr[sl]; cs < ptr[sl + 1]; cs++) {
-            const int sp = src[cs * 64 + lane];
-            if (n >= 0 && sp >= 0) {
-                fold_block<TB, TB>(m, bv, sp >> 6, sp & 63, n, scol[sp], brc, si, bc, scale, out, cs, lane);
-            } else {
-                for (int e = 0; e < 9; e++) out[sell_val(cs, e, lane)] = (TB)0.0;
+        for (int r = 0; r < 3; r++)
+            for (int cc = 0; cc < 3; cc++) {
+                const double f = __fma_rn(m[3 * r + 2], b[6 + cc], __fma_rn(m[3 * r + 1], b[3 + cc], m[3 * r] * b[cc]));
+                bv[sell_val(cs, 3 * r + cc, lane)] = (TB)f;
             }
-        }
     }
 }
 
-/* the block-Jacobi fold in place (M^-1 A into the natural SELL matrix) */
-extern "C" int adhk_launch_fold(adhk_ctx *c, int mixed)
-{
-    const dim3 g((c->N + TPB - 1) / TPB);
-    if (mixed)
-        hipLaunchKernelGGL(k_prep_fold<float>, g, dim3(TPB), 0, c->stream, c->N, c->brow, c->slice_ptr, c->scol,
-                           c->bc_mask, c->bvals32, c->scale, c->minv32, c->diag_id);
-    else
-        hipLaunchKernelGGL(k_prep_fold<double>, g, dim3(TPB), 0, c->stream, c->N, c->brow, c->slice_ptr, c->scol,
-                           c->bc_mask, c->bvals, c->scale, c->minv, c->diag_id);
-    CHECK_LAUNCH();
-    return 0;
-}
-
-/* Dirichlet, scaling and the preconditioner's fold.  Block-Jacobi folds the natural SELL matrix in place;
- * multicolour SGS folds straight into its colour-permuted parts and leaves the natural matrix as assembled
- * (a solve that falls back to block-Jacobi folds it then, adhk_launch_fold). */
 extern "C" int adhk_launch_prep(adhk_ctx *c, int mixed)
 {
     adhk_kt_begin(c, ADHK_KT_PREP);
     const dim3 g((c->N + TPB - 1) / TPB);
-    if (mixed)
+    if (mixed) {
         hipLaunchKernelGGL(k_prep_rowmax<float>, g, dim3(TPB), 0, c->stream, c->N, c->brow, c->slice_ptr, c->scol,
                            c->bvals32, c->bc_mask, c->residual, c->dsol, c->scale);
-    else
+        CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_prep_fold<float>, g, dim3(TPB), 0, c->stream, c->N, c->brow, c->slice_ptr, c->scol,
+                           c->bc_mask, c->bvals32, c->scale, c->minv32, c->diag_id);
+    } else {
         hipLaunchKernelGGL(k_prep_rowmax<double>, g, dim3(TPB), 0, c->stream, c->N, c->brow, c->slice_ptr, c->scol,
                            c->bvals, c->bc_mask, c->residual, c->dsol, c->scale);
-    CHECK_LAUNCH();
-    if (c->pc == ADHK_PC_SGS) {
-        if (adhk_launch_sgs_fold(c, mixed)) return -1;
-    } else if (adhk_launch_fold(c, mixed)) {
-        return -1;
+        CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_prep_fold<double>, g, dim3(TPB), 0, c->stream, c->N, c->brow, c->slice_ptr, c->scol,
+                           c->bc_mask, c->bvals, c->scale, c->minv, c->diag_id);
     }
+    CHECK_LAUNCH();
+    if (c->pc == ADHK_PC_SGS && adhk_launch_sgs_gather(c, mixed)) return -1;
     adhk_kt_end(c);
     return 0;
 }
@@ -1912,6 +1837,19 @@ __global__ __launch_bounds__(TPB) void k_sgs_out(int Np, int top0, const int *__
     }
 }
 
+/* the folded matrix's off-diagonal blocks into a part's slots */
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_sgs_gather(int npos, const int *__restrict__ src, const T *__restrict__ bv,
+                                                    T *__restrict__ out)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npos) return;
+    const int sp = src[q], slot = q >> 6, lane = q & 63;
+#pragma unroll
+    for (int e = 0; e < 9; e++)
+        out[((size_t)slot * 9 + e) * 64 + lane] = sp >= 0 ? bv[((size_t)(sp >> 6) * 9 + e) * 64 + (sp & 63)] : (T)0;
+}
+
 /* a folded diagonal block that is not the identity (a singular block's fallback inverse) */
 __global__ void k_sgs_flag_clear(adhk_scalars *sc)
 {
@@ -1926,24 +1864,23 @@ __global__ __launch_bounds__(TPB) void k_sgs_check(int Np, const int *__restrict
     if (n >= 0 && !diag_id[n]) sc->sgs_bad = 1;
 }
 
-/* the SGS parts of the folded matrix (k_sgs_fold) and the check that every folded diagonal block is the
- * identity */
-extern "C" int adhk_launch_sgs_fold(adhk_ctx *c, int mixed)
+extern "C" int adhk_launch_sgs_gather(adhk_ctx *c, int mixed)
 {
     const adhk_sgs *G = &c->sgs;
-    const dim3 g((G->Np + TPB - 1) / TPB);
-    if (mixed)
-        hipLaunchKernelGGL(k_sgs_fold<float>, g, dim3(TPB), 0, c->stream, G->Np, G->prow, c->brow, c->slice_ptr,
-                           G->U.ptr, G->U.src, G->L.ptr, G->L.src, c->scol, c->bc_mask, c->bvals32, c->scale,
-                           c->minv32, c->diag_id, G->U.val32, G->L.val32);
-    else
-        hipLaunchKernelGGL(k_sgs_fold<double>, g, dim3(TPB), 0, c->stream, G->Np, G->prow, c->brow, c->slice_ptr,
-                           G->U.ptr, G->U.src, G->L.ptr, G->L.src, c->scol, c->bc_mask, c->bvals, c->scale, c->minv,
-                           c->diag_id, G->U.val, G->L.val);
-    CHECK_LAUNCH();
+    const adhk_sgs_part *P[2] = {&G->U, &G->L};
     hipLaunchKernelGGL(k_sgs_flag_clear, dim3(1), dim3(64), 0, c->stream, c->sc);
     hipLaunchKernelGGL(k_sgs_check, dim3((G->Np + TPB - 1) / TPB), dim3(TPB), 0, c->stream, G->Np, G->prow, c->diag_id,
                        c->sc);
+    for (int k = 0; k < 2; k++) {
+        const int npos = P[k]->nslot * 64;
+        if (npos == 0) continue;
+        if (mixed)
+            hipLaunchKernelGGL(k_sgs_gather<float>, dim3((npos + TPB - 1) / TPB), dim3(TPB), 0, c->stream, npos,
+                               P[k]->src, c->bvals32, P[k]->val32);
+        else
+            hipLaunchKernelGGL(k_sgs_gather<double>, dim3((npos + TPB - 1) / TPB), dim3(TPB), 0, c->stream, npos,
+                               P[k]->src, c->bvals, P[k]->val);
+    }
     CHECK_LAUNCH();
     return 0;
 }

@@ -1048,10 +1048,8 @@ static int krylov_iterate(adhk_ctx *c, const adhk_solver *s, int mixed, adhk_sca
             HC(hipMemcpyAsync(hs, c->sc, (size_t)K * sizeof(adhk_scalars), hipMemcpyDeviceToHost, c->stream));
             HC(hipStreamSynchronize(c->stream));
             if (c->pc == ADHK_PC_SGS && hs[0].sgs_bad) {
-                /* a singular diagonal block kept its fallback inverse: this solve takes block-Jacobi, on the
-                 * natural matrix folded now (the SGS path folded only its own parts) */
+                /* a singular diagonal block kept its fallback inverse: this solve takes block-Jacobi */
                 c->pc = ADHK_PC_BJ;
-                LC(adhk_launch_fold(c, mixed));
                 LC(adhk_launch_krylov_init(c, mixed, mixed ? s->mixed_lin_tol : s->lin_tol, s->lin_max_it));
                 restart = 1;
                 break;
