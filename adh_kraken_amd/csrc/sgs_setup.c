@@ -3,7 +3,8 @@
  * per context from the node-block pattern (no counterpart in the reference, whose preconditioner is an
  * exact LU: bcgstab_solver.c:371-424).
  *
- * Colouring: DSatur (colour_dsatur below).  The rows are then permuted by colour and every colour's run padded to a multiple of 256 rows, so that each 256-row reduction block and each
+ * Colouring: DSatur (colour_dsatur below), then Kempe-chain exchanges that drop colours where they can
+ * (colour_reduce).  The rows are then permuted by colour and every colour's run padded to a multiple of 256 rows, so that each 256-row reduction block and each
  * 64-row slice belongs to one colour and a colour's rows form whole workgroups.  Inside a colour the rows
  * are sorted by their U and L widths within windows of 1024 (rows of one colour do not depend on each
  * other, so the order is free), which keeps the SELL padding small on meshes of uneven rows.  The folded matrix's
@@ -11,6 +12,7 @@
  * SELL-64 over the permuted rows, blocks of a row in ascending natural column order.  A padding slot
  * points at the zero row Np, whose vector entries are zero and never written.
  */
+#include <limits.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -118,6 +120,89 @@ done:
     free(sat);
     free(udeg);
     free(h);
+    return nc;
+}
+
+/* Colour reduction by Kempe chains (Kempe 1879, the interchange of the five-colour proof): the colours a
+ * and b may be exchanged on any connected component of the subgraph the two colours induce, and the
+ * colouring stays proper.  While the top colour t is in use, each node v of colour t in node order moves to
+ * the least colour below t that none of its neighbours holds; failing that, for the pairs a < b < t in
+ * order, it collects the {a, b} components that hold v's neighbours of colour a (depth first, at most
+ * SGS_KEMPE_CAP nodes) and, when none of v's neighbours of colour b lies in them, exchanges a and b there and
+ * takes a.  Once no node holds t the colour is gone; a node that no pair frees restores the colouring from
+ * before the pass and ends the reduction.  The outcome depends only on the components, not on the order of
+ * the search.  Meshes whose DSatur colouring ends with a colour of a few nodes lose it (one such phase
+ * costs a launch, two per product, whatever its size); the SGS iteration counts do not change with it.
+ * Returns the colour count, or -2 when out of host memory. */
+#define SGS_KEMPE_CAP 4096
+static int kempe_free(int v, int a, int b, const int *brow, const int *bcol, int *colour, int *seen, int tag,
+                      int *stack)
+{
+    int top = 0, taken = 0;
+    for (int e = brow[v]; e < brow[v + 1]; e++) {
+        const int j = bcol[e];
+        if (j == v || colour[j] != a || seen[j] == tag) continue;
+        if (taken == SGS_KEMPE_CAP) return 0;
+        seen[j] = tag;
+        stack[taken++] = j;
+    }
+    while (top < taken) { /* the entries below `taken` are the visited set; `top` walks them */
+        const int x = stack[top++];
+        for (int e = brow[x]; e < brow[x + 1]; e++) {
+            const int y = bcol[e];
+            if (seen[y] == tag || (colour[y] != a && colour[y] != b)) continue;
+            if (colour[y] == b) {
+                for (int f = brow[v]; f < brow[v + 1]; f++)
+                    if (bcol[f] == y) return 0; /* the chain reaches a b-neighbour of v: no exchange */
+            }
+            if (taken == SGS_KEMPE_CAP) return 0;
+            seen[y] = tag;
+            stack[taken++] = y;
+        }
+    }
+    for (int k = 0; k < taken; k++) colour[stack[k]] = a + b - colour[stack[k]];
+    colour[v] = a;
+    return 1;
+}
+static int colour_reduce(int N, const int *brow, const int *bcol, int *colour, int nc)
+{
+    int *seen = calloc((size_t)(N > 0 ? N : 1), sizeof(int)), *stack = malloc(SGS_KEMPE_CAP * sizeof(int));
+    int *before = malloc((size_t)(N > 0 ? N : 1) * sizeof(int));
+    int tag = 0;
+    if (!seen || !stack || !before) nc = -2;
+    while (nc > 1) {
+        const int t = nc - 1;
+        int freed = 1;
+        memcpy(before, colour, (size_t)N * sizeof(int));
+        for (int v = 0; v < N && freed; v++) {
+            if (colour[v] != t) continue;
+            unsigned long long held = 0;
+            for (int e = brow[v]; e < brow[v + 1]; e++)
+                if (bcol[e] != v) held |= 1ULL << colour[bcol[e]];
+            const int c = __builtin_ctzll(~held);
+            if (c < t) {
+                colour[v] = c;
+                continue;
+            }
+            freed = 0;
+            for (int a = 0; a < t && !freed; a++)
+                for (int b = a + 1; b < t && !freed; b++) {
+                    if (tag == INT_MAX) { /* a fresh mark per search; start the marks again before they wrap */
+                        memset(seen, 0, (size_t)N * sizeof(int));
+                        tag = 0;
+                    }
+                    freed = kempe_free(v, a, b, brow, bcol, colour, seen, ++tag, stack);
+                }
+        }
+        if (!freed) {
+            memcpy(colour, before, (size_t)N * sizeof(int));
+            break;
+        }
+        nc = t;
+    }
+    free(seen);
+    free(stack);
+    free(before);
     return nc;
 }
 
@@ -238,6 +323,7 @@ int adhk_sgs_setup(adhk_ctx *c)
     HC(hipMemcpy(bcol, c->bcol, (size_t)nnzb * sizeof(int), hipMemcpyDeviceToHost));
     HC(hipMemcpy(b2s, c->blk2sell, (size_t)nnzb * sizeof(int), hipMemcpyDeviceToHost));
     G->nc = colour_dsatur(N, brow, bcol, G->colour);
+    if (G->nc >= 2) G->nc = colour_reduce(N, brow, bcol, G->colour, G->nc);
     if (G->nc < 2) {
         adhk_set_error("adhk_sgs_setup: %s", G->nc == -2 ? "out of host memory"
                                                  : "more than 64 colours, or a pattern without couplings");

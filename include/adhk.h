@@ -203,7 +203,9 @@ int adhk_sw2_autotune(adhk_sw2 *c, const adhk_solver *s, int ninner, adhk_tuning
 /* node colouring of the multicolour SGS preconditioner (ADHK_PC_SGS; no counterpart in the reference):
  * DSatur (Brelaz 1979) over the block pattern -- the uncoloured node with the most distinct neighbour
  * colours first, ties by the most uncoloured neighbours, then the lowest index; it takes the smallest colour
- * none of its neighbours holds.  colour [nnodes] (or NULL) and the colour count; layout [3] (or NULL): the permuted
+ * none of its neighbours holds; then Kempe-chain exchanges drop the top colour while every node of it can be
+ * moved (components of two colours up to 4096 nodes).  colour [nnodes] (or NULL) and the colour count;
+ * layout [3] (or NULL): the permuted
  * row count (each colour padded to a multiple of 256) and the SELL-64 slot columns of the U and L parts.
  * One mesh alone. */
 int adhk_sw2_colouring(adhk_sw2 *c, int *colour, int *ncolour, int *layout);

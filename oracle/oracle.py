@@ -90,6 +90,7 @@ def load():
         "or_block_jacobi_apply": (None, [C.c_int, dp, dp, dp]),
         "or_bicgstab": (C.c_int, [C.c_int, ip, ip, dp, dp, dp, dp, dp, C.c_double, C.c_int, dp]),
         "or_colour_dsatur": (C.c_int, [C.c_int, ip, ip, ip]),
+        "or_colour_reduce": (C.c_int, [C.c_int, ip, ip, ip, C.c_int, C.c_int]),
         "or_bicgstab_sgs": (C.c_int, [C.c_int, ip, ip, dp, dp, ip, C.c_int, dp, dp, dp, C.c_double, C.c_int, dp]),
         "or_residual_norms": (None, [C.c_int, dp, dp, ip, dp, dp, dp]),
         "or_fe_newton": (C.c_int, [P, C.POINTER(OrSolver), C.POINTER(OrStats)]),
@@ -303,6 +304,24 @@ def colour_dsatur(indptr, cols):
     col = np.zeros(nn, np.int32)
     nc = lib.or_colour_dsatur(nn, _i(indptr), _i(cols), _i(col))
     return col, nc
+
+
+KEMPE_CAP = 4096
+
+
+def colour_reduce(indptr, cols, colour, nc, cap=KEMPE_CAP):
+    """a colouring with fewer colours by Kempe-chain exchanges (the input colouring when none frees the top one)"""
+    lib = load()
+    nn = (len(indptr) - 1) // 3
+    col = np.ascontiguousarray(colour, np.int32).copy()
+    nc2 = lib.or_colour_reduce(nn, _i(indptr), _i(cols), _i(col), int(nc), int(cap))
+    return col, nc2
+
+
+def colour_sgs(indptr, cols):
+    """the SGS colouring the Newton solve uses: DSatur, then the Kempe-chain reduction"""
+    col, nc = colour_dsatur(indptr, cols)
+    return colour_reduce(indptr, cols, col, nc)
 
 
 def bicgstab_sgs(indptr, cols, vals, minv, b, scale, colour, ncolour, x0=None, tol=1e-11, max_it=100):

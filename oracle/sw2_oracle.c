@@ -1404,6 +1404,89 @@ int or_colour_dsatur(int nnodes, const int *indptr, const int *cols, int *colour
     return nc;
 }
 
+/* Fewer colours by Kempe-chain interchanges (A. B. Kempe, Amer. J. Math. 2(3), 1879: the exchange behind
+ * the five-colour theorem's proof), applied to the DSatur colouring: while the top colour t is in use, every
+ * node v of colour t, in node order, takes the least colour below t that no neighbour holds; failing that,
+ * for the pairs a < b < t in order, the part of the subgraph of colours {a, b} reached from v's neighbours
+ * of colour a is searched breadth first (neighbours in CSR order, at most cap nodes); when it holds none of
+ * v's neighbours of colour b, a and b are exchanged on it and v takes a.  The top colour is dropped once no
+ * node holds it; a node that no pair frees ends the reduction and undoes the pass over its colour, so the
+ * result is the colouring given or one with fewer colours, proper throughout.  Returns the colour count
+ * (-2: out of memory). */
+int or_colour_reduce(int nnodes, const int *indptr, const int *cols, int *colour, int nc, int cap)
+{
+    int *mark = calloc((size_t)(nnodes > 0 ? nnodes : 1), sizeof(int));
+    int *queue = malloc((size_t)(cap > 0 ? cap : 1) * sizeof(int));
+    int *saved = malloc((size_t)(nnodes > 0 ? nnodes : 1) * sizeof(int));
+    int stamp = 0;
+    if (!mark || !queue || !saved) nc = -2;
+    while (nc > 1) {
+        const int t = nc - 1;
+        int stuck = 0;
+        memcpy(saved, colour, (size_t)nnodes * sizeof(int));
+        for (int v = 0; v < nnodes && !stuck; v++) {
+            if (colour[v] != t) continue;
+            unsigned long long nb = 0;
+            for (int q = indptr[3 * v]; q < indptr[3 * v + 1]; q += 3) {
+                const int j = cols[q] / 3;
+                if (j != v) nb |= 1ULL << colour[j];
+            }
+            int c = 0;
+            while (c < t && (nb >> c & 1ULL)) c++;
+            if (c < t) {
+                colour[v] = c;
+                continue;
+            }
+            int done = 0;
+            for (int a = 0; a < t && !done; a++)
+                for (int b = a + 1; b < t && !done; b++) {
+                    int head = 0, tail = 0, ok = 1;
+                    stamp++;
+                    for (int q = indptr[3 * v]; q < indptr[3 * v + 1] && ok; q += 3) {
+                        const int j = cols[q] / 3;
+                        if (j == v || colour[j] != a || mark[j] == stamp) continue;
+                        if (tail == cap) ok = 0;
+                        else {
+                            mark[j] = stamp;
+                            queue[tail++] = j;
+                        }
+                    }
+                    while (ok && head < tail) {
+                        const int x = queue[head++];
+                        for (int q = indptr[3 * x]; q < indptr[3 * x + 1]; q += 3) {
+                            const int y = cols[q] / 3;
+                            if (y == x || mark[y] == stamp || (colour[y] != a && colour[y] != b)) continue;
+                            if (tail == cap) {
+                                ok = 0;
+                                break;
+                            }
+                            mark[y] = stamp;
+                            queue[tail++] = y;
+                        }
+                    }
+                    for (int q = indptr[3 * v]; q < indptr[3 * v + 1] && ok; q += 3) {
+                        const int j = cols[q] / 3;
+                        if (j != v && colour[j] == b && mark[j] == stamp) ok = 0;
+                    }
+                    if (!ok) continue;
+                    for (int k = 0; k < tail; k++) colour[queue[k]] = (colour[queue[k]] == a) ? b : a;
+                    colour[v] = a;
+                    done = 1;
+                }
+            if (!done) stuck = 1;
+        }
+        if (stuck) {
+            memcpy(colour, saved, (size_t)nnodes * sizeof(int));
+            break;
+        }
+        nc = t;
+    }
+    free(mark);
+    free(queue);
+    free(saved);
+    return nc;
+}
+
 /* The block-Jacobi-folded matrix F = D^-1 A (D: the 3x3 diagonal blocks) splits by colour into
  * F = I + L + U, L holding the blocks (i, j) with colour(j) < colour(i), U those with colour(j) >
  * colour(i).  The preconditioner M = (I + L)(I + U) is symmetric Gauss-Seidel in colour order, applied
@@ -1630,7 +1713,9 @@ int or_fe_newton(or_ctx *c, const or_solver *sv, or_newton_stats *st)
         if (sv->precond == OR_PREC_SGS) {
             if (!c->colour) {
                 c->colour = malloc((size_t)(c->m.nnodes > 0 ? c->m.nnodes : 1) * sizeof(int));
-                c->ncolour = or_colour_dsatur(c->m.nnodes, c->indptr, c->cols, c->colour);
+                c->ncolour = or_colour_reduce(c->m.nnodes, c->indptr, c->cols, c->colour,
+                                              or_colour_dsatur(c->m.nnodes, c->indptr, c->cols, c->colour),
+                                              OR_KEMPE_CAP);
             }
             lits = or_bicgstab_sgs(n, c->indptr, c->cols, c->vals, minv, c->colour, c->ncolour, c->dsol, c->residual,
                                    c->scale, sv->lin_tol, sv->lin_max_it, NULL);

@@ -74,6 +74,7 @@ typedef struct {
 
 #define OR_PREC_BJ  0 /* 3x3 nodal block-Jacobi in place of the reference's exact LU */
 #define OR_PREC_SGS 1 /* multicolour symmetric Gauss-Seidel on the block-Jacobi-folded matrix, split form */
+#define OR_KEMPE_CAP 4096 /* largest two-colour component the SGS colour reduction exchanges */
 
 typedef struct {
     int newton_its;
@@ -137,6 +138,7 @@ void or_block_jacobi_apply(int nnodes, const double *minv, const double *b, doub
 int  or_bicgstab(int n, const int *indptr, const int *cols, const double *vals, const double *minv,
                  double *x, double *b, const double *scale, double tol, int max_it, double *final_rnorm);
 int  or_colour_dsatur(int nnodes, const int *indptr, const int *cols, int *colour);
+int  or_colour_reduce(int nnodes, const int *indptr, const int *cols, int *colour, int nc, int cap);
 int  or_bicgstab_sgs(int n, const int *indptr, const int *cols, const double *vals, const double *minv,
                      const int *colour, int ncolour, double *x, const double *b, const double *scale, double tol,
                      int max_it, double *final_rnorm);

@@ -30,6 +30,14 @@ def chan():
     return A.channel_mesh(CHAN)
 
 
+@pytest.fixture(scope="module")
+def small():
+    """a 700-node channel whose DSatur colouring ends on a colour of eight nodes, which the Kempe-chain
+    reduction frees: four colours instead of five"""
+    return A.channel_mesh(A.ChannelSpec(length=200.0, width=80.0, spacing=4.0, chan_width=20.0, bank_width=6.0,
+                                        nbumps=2, inflow_q=10.0, fp_depth_out=1.0, seed=4))
+
+
 def sgs_model(mesh, dt, sol0, tuning=None):
     sm = A.SModelSuper(mesh, dt=dt)
     sm.solver.precond = L.PC_SGS
@@ -39,19 +47,21 @@ def sgs_model(mesh, dt, sol0, tuning=None):
     return sm
 
 
-def test_colouring_matches_oracle(gpu, c1, chan):
-    for mesh, _ in (c1, chan):
+def test_colouring_matches_oracle(gpu, c1, chan, small):
+    for mesh, _ in (c1, chan, small):
         sm = A.SModelSuper(mesh, dt=1.0)
         orc = O.Oracle(mesh, dt=1.0, threads=8)
         ip_, cols = orc.pattern()
-        co, nco = O.colour_dsatur(ip_, cols)
+        co, nco = O.colour_sgs(ip_, cols)
         cg, ncg = sm.colouring()
         assert ncg == nco and np.array_equal(cg, co)
+        if mesh is small[0]:
+            assert nco < O.colour_dsatur(ip_, cols)[1]
 
 
-@pytest.mark.parametrize("which", ["c1", "chan"])
-def test_time_loop_sgs_matches_oracle(gpu, c1, chan, which):
-    mesh, sol0 = c1 if which == "c1" else chan
+@pytest.mark.parametrize("which", ["c1", "chan", "small"])
+def test_time_loop_sgs_matches_oracle(gpu, c1, chan, small, which):
+    mesh, sol0 = {"c1": c1, "chan": chan, "small": small}[which]
     dt, steps = (1.0, 10) if which == "c1" else (5.0, 6)
     print(f"sgs time loop {which}: {mesh.nnodes} nodes, {steps} steps", flush=True)
     sm = sgs_model(mesh, dt, sol0)

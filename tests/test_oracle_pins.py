@@ -27,11 +27,15 @@ fails:
 * BiCGStab/block-Jacobi solution vs a direct sparse solve; block inverses
 * multicolour SGS: the DSatur colouring is proper and sequential (a node of colour c sees every colour
   below c among its neighbours), and takes three colours on the flume, whose even interior degrees
-  make it 3-colourable; the split-preconditioned BiCGStab
+  make it 3-colourable; the Kempe-chain reduction on hand-worked graphs (an exchange that frees the top
+  colour, one the chain forbids, one beyond the cap, a direct move) and on random graphs against the
+  brute-force chromatic number; the split-preconditioned BiCGStab
   (Eisenstat form) vs a direct sparse solve -- a slip in either sweep or in the identity solves a
   different system
 * Newton: the converged state satisfies the residual tolerance
 """
+import itertools
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -497,6 +501,73 @@ def test_dsatur_colouring(c1):
     col2, nc2 = O.colour_dsatur(ip2, cols2)
     assert 4 <= nc2 <= 6
     _colouring_properties(ip2, cols2, col2, nc2)
+    # DSatur ends on a colour of a few nodes there; the Kempe reduction frees it (and stays proper)
+    col3, nc3 = O.colour_sgs(ip2, cols2)
+    assert nc3 == nc2 - 1 and np.bincount(col2)[-1] < 20
+    _proper(ip2, cols2, col3, nc3)
+
+
+def _proper(ip_, cols, col, nc):
+    assert nc == col.max() + 1 and col.min() == 0
+    for i in range(len(col)):
+        nb = cols[ip_[3 * i]:ip_[3 * i + 1]:3] // 3
+        assert not np.any(col[nb[nb != i]] == col[i])
+
+
+def _block_pattern(adj):
+    """the oracle's scalar pattern (3x3 blocks, the diagonal included) of a graph given by neighbour lists"""
+    ip_, cols = [0], []
+    for i, nb in enumerate(adj):
+        blk = sorted(set(nb) | {i})
+        for _ in range(3):
+            cols += [3 * j + k for j in blk for k in range(3)]
+            ip_.append(len(cols))
+    return np.array(ip_, np.int32), np.array(cols, np.int32)
+
+
+def test_kempe_colour_reduction():
+    # x=0, y=1, z=2, v=3: v sees x (colour 0) and y (1), x sees z (1); no colour below 2 is free at v, but
+    # the {0, 1} chain through x is {x, z} and misses y: exchanged there, v takes 0, two colours remain
+    ip_, cols = _block_pattern([[2, 3], [3], [0], [0, 1]])
+    col, nc = O.colour_reduce(ip_, cols, [0, 1, 1, 2], 3)
+    assert nc == 2 and col.tolist() == [1, 1, 0, 0]
+    # the same chain is longer than a cap of one node: nothing changes
+    col, nc = O.colour_reduce(ip_, cols, [0, 1, 1, 2], 3, cap=1)
+    assert nc == 3 and col.tolist() == [0, 1, 1, 2]
+    # a triangle: the chain from x reaches y, v's neighbour of colour 1, so an exchange would clash
+    ip_, cols = _block_pattern([[1, 2], [0, 2], [0, 1]])
+    col, nc = O.colour_reduce(ip_, cols, [0, 1, 2], 3)
+    assert nc == 3 and col.tolist() == [0, 1, 2]
+    # a path: the end node takes the free colour 0 directly; the middle one then has no second colour to go to
+    ip_, cols = _block_pattern([[1], [0, 2], [1]])
+    col, nc = O.colour_reduce(ip_, cols, [0, 1, 2], 3)
+    assert nc == 2 and col.tolist() == [0, 1, 0]
+    # random graphs from a random proper colouring: proper, never more colours, never fewer than the
+    # chromatic number (brute force), and the input itself whenever the count stays
+    rng = np.random.default_rng(7)
+    for _ in range(40):
+        n = 7
+        adj = [[] for _ in range(n)]
+        for a in range(n):
+            for b in range(a + 1, n):
+                if rng.random() < 0.4:
+                    adj[a].append(b)
+                    adj[b].append(a)
+        col0 = np.full(n, -1)
+        for v in rng.permutation(n):  # greedy in a random order, then one node on a colour of its own
+            used = {col0[j] for j in adj[v]}
+            col0[v] = min(c for c in range(n + 1) if c not in used)
+        col0[rng.integers(n)] = col0.max() + 1
+        nc0 = int(col0.max()) + 1
+        ip_, cols = _block_pattern(adj)
+        _proper(ip_, cols, col0, nc0)
+        col, nc = O.colour_reduce(ip_, cols, col0, nc0)
+        _proper(ip_, cols, col, nc)
+        chi = next(k for k in range(1, n + 1) if any(
+            all(c[a] != c[b] for a in range(n) for b in adj[a]) for c in itertools.product(range(k), repeat=n)))
+        assert chi <= nc <= nc0
+        if nc == nc0:
+            assert np.array_equal(col, col0)
 
 
 @pytest.mark.parametrize("seed", [2, 5])
