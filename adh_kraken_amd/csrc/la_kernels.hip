@@ -1758,13 +1758,27 @@ __global__ __launch_bounds__(TPB) void k_sgs_upd(int nblk, SgVec<T> V, double *p
     const double alpha = sc->alpha;
     double omega, beta;
     sgs_fin_s(partials, nblk, sc, omega, beta); /* the S step's finalizer, in every workgroup */
-    const ColP<T> col{V.s, V.As, V.p, V.Ap, omega, beta};
     for (int j = blockIdx.x; j < nblk; j += gridDim.x) {
+        /* the block's three entries per thread: every load before the first store (r shares s's buffer,
+         * and the compiler would otherwise hold each entry's loads until the previous entry's stores) */
+        T s_[3], As_[3], p_[3], Ap_[3], x_[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int ii = 3 * TPB * j + k * TPB + threadIdx.x;
+            s_[k] = V.s[ii];
+            As_[k] = V.As[ii];
+            p_[k] = V.p[ii];
+            Ap_[k] = V.Ap[ii];
+            x_[k] = __builtin_nontemporal_load(V.xt + ii);
+        }
         double vmax[1] = {0.0};
-        for (int ii = 3 * TPB * j + threadIdx.x; ii < 3 * TPB * (j + 1); ii += TPB) {
-            T rr, pp;
-            col.one(ii, rr, pp);
-            upd_x(V.xt + ii, (double)V.s[ii], (double)V.p[ii], alpha, omega);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int ii = 3 * TPB * j + k * TPB + threadIdx.x;
+            const T rr = upd_r<T>((double)s_[k], (double)As_[k], omega);
+            const T pp = upd_p<T>((double)p_[k], (double)Ap_[k], (double)rr, omega, beta);
+            __builtin_nontemporal_store((T)__fma_rn(omega, (double)s_[k], __fma_rn(alpha, (double)p_[k], (double)x_[k])),
+                                        V.xt + ii);
             V.r[ii] = rr;
             V.p[ii] = pp;
             vmax[0] = fmax(vmax[0], fabs((double)rr));
