@@ -4,6 +4,10 @@
  * of include/adhk.h.  Structure layouts are those of include/adhk_ref.h (checked field by field against
  * the reference's headers).
  *
+ * The time loop (time_loop, advance_time, update_dt, tc_init, smodel_super_prep_sol), the series averages,
+ * the friction laws, the single-rank messg reductions and the dof-map builder run on the host: they move
+ * the caller's arrays and scalars and call fe_newton, which runs on the GPU.
+ *
  * Each SMODEL_SUPER gets a context on its first call, built from sm->grid: node coordinates, triangles,
  * boundary segments and, per segment, the boundary kind named by the residual function its physics
  * material carries (fe_sw2_bc_h: depth, fe_sw2_bc_vel: velocity, fe_sw2_bc_flux: explicit flux).  The
@@ -19,6 +23,7 @@
  * Errors: a message in adhk_last_error() and adhk_ref_status() == -1; fe_newton returns FALSE, the other
  * functions leave their outputs as they were.  Thread-compatible: one thread per model at a time.
  */
+#include <float.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdio.h>
@@ -785,4 +790,243 @@ int global_to_local(int global_col_no, int local_size, int *ghosts, int nghost)
     for (int i = 0; i < nghost; i++)
         if (ghosts[i] == global_col_no) return i + local_size;
     return -1;
+}
+
+/* ------------------------------------------------------------------------------- sivar_position.h */
+
+#define REF_N_IVARS 25 /* adh_def.n_ivars = MAX_IVARS + MAX_TRNS, sadh_def.c:26 and sadh_def.h:5-6 */
+
+/* sivar_position_build_dof_map, sivar_position.c:61-80: rows ip->var[j] of the active variables j of ipNode
+ * get the node's dofs in node-major order, every other entry UNSET_INT */
+int sivar_position_build_dof_map(SIVAR_POSITION *ip, int nnodes, SIVAR_POSITION *ipNode, int ***ivars)
+{
+    if (!ip || !ipNode || !ivars || nnodes < 0 || ip->n < 0 || !ipNode->var || !ip->var) {
+        ref_fail("sivar_position_build_dof_map: NULL or negative argument");
+        return -1;
+    }
+    int **iv = (int **)calloc((size_t)(ip->n > 0 ? ip->n : 1), sizeof(int *));
+    int bad = !iv;
+    for (int r = 0; !bad && r < ip->n; r++) {
+        if (!(iv[r] = (int *)malloc((size_t)(nnodes > 0 ? nnodes : 1) * sizeof(int)))) bad = 1;
+        for (int n = 0; !bad && n < nnodes; n++) iv[r][n] = REF_UNSET_INT;
+    }
+    for (int j = 0; !bad && j < REF_N_IVARS; j++)
+        if (ipNode->var[j] != REF_UNSET_INT && (ip->var[j] < 0 || ip->var[j] >= ip->n)) bad = 2;
+    if (bad) {
+        for (int r = 0; iv && r < ip->n; r++) free(iv[r]);
+        free(iv);
+        ref_fail(bad == 2 ? "sivar_position_build_dof_map: an active variable of ipNode has no row in ip"
+                          : "sivar_position_build_dof_map: out of host memory");
+        return -1;
+    }
+    int ndofs = 0;
+    for (int n = 0; n < nnodes; n++)
+        for (int j = 0; j < REF_N_IVARS; j++)
+            if (ipNode->var[j] != REF_UNSET_INT) iv[ip->var[j]][n] = ndofs++;
+    *ivars = iv;
+    tl_status = 0;
+    return ndofs;
+}
+
+/* ----------------------------------------------------------------------------------------- messg.h */
+
+/* messg_dmax.c:10, messg_dsum.c:9, messg_imin.c:6 and their kin without _MESSG: one rank, the identity */
+double messg_dmax(double x) { return x; }
+double messg_dmin(double x) { return x; }
+int messg_imin(int x) { return x; }
+int messg_imax(int x) { return x; }
+double messg_dsum(double x) { return x; }
+
+/* ------------------------------------------------------------------------------------ friction */
+
+/* fr_bedshstr_drag_coef, fr_bedshstr_drag_coef.c:18-43: c_f = 2 (0.4 b / ((b + 1)(ln(b + 1) - 1) + 1))^2,
+ * b = 29.7 depth / k_s, with depth and k_s clamped at 1e-8 and depth / k_s at 1 / 29.7 from below */
+double fr_bedshstr_drag_coef(double depth, double roughness_height)
+{
+    depth = fmax(depth, 1.e-8);
+    roughness_height = fmax(roughness_height, 1.e-8);
+    const double ratio = roughness_height > 29.7 * depth ? 1.0 / 29.7 : depth / roughness_height;
+    const double b = 29.7 * ratio;
+    const double q = 0.4 * b / ((b + 1.0) * (log(b + 1.0) - 1.0) + 1.0);
+    return 2.0 * q * q;
+}
+
+/* fr_manningsn_to_rheight, fr_manningsn_to_rheight.c:15-32: k_s = (8.25 sqrt(g) n / k_n)^6, 0 below n = 1e-8 */
+double fr_manningsn_to_rheight(double n, double kn, double g)
+{
+    return n < 1.e-8 ? 0.0 : pow(8.25 * sqrt(g) * n / kn, 6.0);
+}
+
+#define REF_UNSET_FLT (-9999999.9) /* UNSET_FLT, define.h:443 */
+
+/* fe_sw2_get_roughness, sw2_get_roughness.c:25-49: the drag coefficient of string id from its roughness
+ * height (the bed-shear law; zero without a height).  The vegetation and ice laws of that function are not
+ * on this path. */
+double fe_sw2_get_roughness(SMODEL_SUPER *mod, double depth, double velocity, int id, int which)
+{
+    (void)velocity;
+    (void)which;
+    if (!mod || !mod->str_values || id < 0) {
+        ref_fail("fe_sw2_get_roughness: no str_values or a negative string id");
+        return NAN;
+    }
+    const FRICTION *f = &mod->str_values[id].fterms;
+    if (f->sav_flag == REF_YES || f->urv_flag == REF_YES || f->icethick != REF_UNSET_FLT) {
+        ref_fail("fe_sw2_get_roughness: vegetation and ice drag (fr_sav_drag_coef, fr_urv_drag_coef, "
+                 "fr_stationary_ice_coef) are not on the SW2 path this library serves");
+        return NAN;
+    }
+    tl_status = 0;
+    return f->eqrheight > DBL_EPSILON ? fr_bedshstr_drag_coef(depth, f->eqrheight) : 0.0;
+}
+
+/* ------------------------------------------------------------------------------------ sseries.h */
+
+#define REF_TIME_SERIES 1 /* TIME_SERIES, define.h:492 */
+
+/* sseries_get_interval, sseries.c:746-775: the left end of the piece holding t, or -1 outside the range
+ * widened by the relative DBL_EPSILON */
+static int ser_interval(const SSERIES *s, double t)
+{
+    if (!s->entry || s->size < 1) return -1;
+    const int last = s->size - 1;
+    if (t < s->entry[0].time * (1.0 - DBL_EPSILON) || t > s->entry[last].time * (1.0 + DBL_EPSILON)) return -1;
+    int i = 0;
+    while (i < last && t >= s->entry[i].time * (1.0 - DBL_EPSILON)) i++;
+    return i - 1;
+}
+
+/* tc_eval_series, tc_eval_series.c: theta-weighted value on the piece (the last entry's value past it) */
+static double ser_value(const SSERIES *s, int interval, double t, int index)
+{
+    if (interval == s->size - 1) return s->entry[s->size - 1].value[index];
+    const double tl = s->entry[interval].time, tr = s->entry[interval + 1].time;
+    const double theta = (tr - t) / (tr - tl);
+    return theta * s->entry[interval].value[index] + (1.0 - theta) * s->entry[interval + 1].value[index];
+}
+
+/* tc_trap_area, tc_trap_area.c */
+static double ser_trap(double lv, double rv, double lt, double rt)
+{
+    const double d = rt - lt;
+    return d < DBL_EPSILON ? 0.0 : (lv + rv) * 0.5 * d;
+}
+
+/* sseries_set_values_average, sseries.c:622-668 */
+static int ser_set_average(SSERIES *s, double t1, double t2)
+{
+    const int i1 = ser_interval(s, t1), i2 = ser_interval(s, t2);
+    if (i1 < 0 || i2 < 0 || !s->ivalue) return -1;
+    for (int k = 0; k < s->nvalues; k++) {
+        const double v1 = ser_value(s, i1, t1, k), v2 = ser_value(s, i2, t2, k);
+        double area = 0.0;
+        if (i1 == i2) {
+            area = ser_trap(v1, v2, t1, t2);
+        } else {
+            for (int i = i1 + 1; i < i2; i++) area += s->entry[i].area[k];
+            area += ser_trap(v1, s->entry[i1 + 1].value[k], t1, s->entry[i1 + 1].time);
+            area += ser_trap(s->entry[i2].value[k], v2, s->entry[i2].time, t2);
+        }
+        s->ivalue[k] = area / (t2 - t1);
+    }
+    return 0;
+}
+
+/* sseries_setall_ts_valuesAVG, sseries.c:673-682 */
+void sseries_setall_ts_valuesAVG(SSERIES *head, double t1, double t2)
+{
+    for (SSERIES *s = head; s; s = s->next)
+        if (s->type == REF_TIME_SERIES && ser_set_average(s, t1, t2)) {
+            char msg[160];
+            snprintf(msg, sizeof msg, "series %d: time %.6e or %.6e is not in the range of the series", s->id + 1,
+                     t1, t2);
+            ref_fail(msg);
+            return;
+        }
+    tl_status = 0;
+}
+
+/* ---------------------------------------------------------------------------------- time loop */
+
+#define REF_FE_NEWTON 0 /* FE_NEWTON, model_codes.h:77 */
+
+/* smodel_super_prep_sol, smodel_super_prep_sol.c:2-8: sol_older <- sol_old <- sol */
+void smodel_super_prep_sol(SMODEL_SUPER *sm)
+{
+    const size_t nb = (size_t)sm->ndofs * sizeof(double);
+    memcpy(sm->sol_older, sm->sol_old, nb);
+    memcpy(sm->sol_old, sm->sol, nb);
+}
+
+/* smodel_super_update_dirichlet_data, smodel_super_update_dirichlet_data.c:2-28: the reference sets
+ * Dirichlet data only for its HEAT2D test model; an SW2 model's stay as they are */
+void smodel_super_update_dirichlet_data(SMODEL_SUPER *sm) { (void)sm; }
+
+/* tc_init, tc_init.c:4-33: dt from the dt series at t_prev, shortened to reach t_final exactly (or halved
+ * when two steps would overshoot it) */
+void tc_init(SMODEL_DESIGN *dm)
+{
+    dm->ts.dt_old = dm->ts.dt;
+    const int i = dm->series_dt ? ser_interval(dm->series_dt, dm->ts.t_prev) : -1;
+    if (i < 0) {
+        ref_fail("tc_init: no dt series, or t_prev is not in its range");
+        return;
+    }
+    dm->ts.dt = ser_value(dm->series_dt, i, dm->ts.t_prev, 0);
+    if (dm->ts.t_prev + dm->ts.dt >= dm->ts.t_final)
+        dm->ts.dt = dm->ts.t_final - dm->ts.t_prev;
+    else if (dm->ts.t_prev + 2.0 * dm->ts.dt >= dm->ts.t_final)
+        dm->ts.dt = 0.5 * (dm->ts.t_final - dm->ts.t_prev);
+    if (dm->ts.dt < DBL_EPSILON) {
+        ref_fail("tc_init: time step is too small");
+        return;
+    }
+    tl_status = 0;
+}
+
+/* update_dt, time_loop.c:108-120: t_prev advances by dt; FALSE once t_final is reached */
+int update_dt(SMODEL_DESIGN *dm)
+{
+    dm->ts.t_prev += dm->ts.dt;
+    return dm->ts.t_prev + FLT_EPSILON >= dm->ts.t_final ? 0 : 1;
+}
+
+/* advance_time, time_loop.c:56-93 */
+int advance_time(SMODEL_DESIGN *dm, int nsuper)
+{
+    tl_status = 0;
+    for (int k = 0; k < nsuper; k++) {
+        SMODEL_SUPER *sm = &dm->superModel[k];
+        if (sm->forward_step != REF_FE_NEWTON) {
+            ref_fail("advance_time: the super model's forward_step is not FE_NEWTON");
+            return 0;
+        }
+        for (int j = 0; j < sm->nsubsteps; j++) {
+            smodel_super_prep_sol(sm);
+            sseries_setall_ts_valuesAVG(sm->series_head, sm->ts->t_prev - sm->ts->dt, sm->ts->t_prev);
+            if (tl_status) return 0;
+            smodel_super_update_dirichlet_data(sm);
+            if (!fe_newton(sm)) return 0;
+        }
+    }
+    return 1;
+}
+
+/* time_loop, time_loop.c:14-41: YES (1) as the reference returns whether or not every step worked; 0 when
+ * this library failed (adhk_ref_status() -1 and adhk_last_error()) */
+int time_loop(SMODEL_DESIGN *dm)
+{
+    if (!dm || dm->nSuperModels < 0 || (dm->nSuperModels > 0 && !dm->superModel)) {
+        ref_fail("time_loop: no design model");
+        return 0;
+    }
+    tc_init(dm);
+    if (tl_status) return 0;
+    int more, worked;
+    do {
+        more = update_dt(dm);
+        worked = advance_time(dm, dm->nSuperModels);
+        if (tl_status) return 0;
+    } while (more && worked);
+    return REF_YES;
 }

@@ -9,7 +9,8 @@
  * carried as pointers on this path stay incomplete.
  *
  * Part 2 declares the functions with the reference's names and signatures (residual.h, jacobian.h,
- * newton.h, la.h, sw2.h, slin_sys.h, dofmaps.h).  They run on the GPU through the adhk_* context of
+ * newton.h, la.h, sw2.h, slin_sys.h, dofmaps.h, time_loop.h, messg.h and the series, friction and dof-map
+ * helpers the time loop and the residual call).  They run on the GPU through the adhk_* context of
  * include/adhk.h, created for each SMODEL_SUPER on its first call (from sm->grid) and kept in a side table
  * keyed by the SMODEL_SUPER pointer; the caller's host arrays (sol, sol_old, sol_older, bc_mask,
  * dirichlet_data, lin_sys->residual / dsol / vals_diag) stay authoritative: each call copies what it reads
@@ -21,6 +22,7 @@
 #define ADHK_REF_H
 
 #include <stdbool.h>
+#include <stdio.h> /* FILE, carried as a pointer by SMODEL_DESIGN */
 
 #ifdef __cplusplus
 extern "C" {
@@ -37,7 +39,7 @@ typedef struct adhk_ref_smpi SMPI;
 typedef struct adhk_ref_smeteor_file SMETEOR_FILE;
 typedef struct adhk_ref_swind SWIND;
 typedef struct adhk_ref_swave SWAVE;
-typedef struct adhk_ref_str_value STR_VALUE;
+typedef struct adhk_ref_scoverage SCOVERAGE;
 typedef struct adhk_ref_stestcase STESTCASE;
 typedef struct adhk_ref_smeteor_station SMETEOR_STATION;
 
@@ -133,6 +135,27 @@ typedef struct {
     bool TIDE, UNREFINE, CSTORM_WSID;
     bool FLUX_WEIGHTED_NORMALS;
 } SFLAGS;
+
+/* src/structs/sstr_value/sstr_value.h -- per-string values; fterms.eqrheight is the roughness height that
+ * fe_sw2_get_roughness turns into a drag coefficient */
+typedef struct {
+    int bc_flag, src_flag, isigma, iu_0, iq, ici, ivx, ivy, ivz;
+    double ceq_0;
+} INPUT_DATA;
+typedef struct {
+    double manningsn, skfricoef, dragcoef, eqrheight, rheight, hghtstem, diamstem, densstem;
+    double icethick, icedense, bedrhght, icerhght;
+    int mng_flag, erh_flag, sav_flag, urv_flag, icemors;
+} FRICTION;
+typedef struct {
+    INPUT_DATA flow, heat, ol_flow, ch_flow, pressure, displacement, bed;
+    INPUT_DATA *trans, *sed;
+    INPUT_DATA Qs, Td;
+    FRICTION fterms;
+    double roughness, conveyance, height, ref_height, ref_pressure;
+    int ps_flag, string_type, ice_string, phys_flag, flux_flag, sed_div_flag, link, weir_num, flap_num, sluice_num;
+    double total_area;
+} STR_VALUE;
 
 /* src/structs/sivar_position/sivar_position.h:12-31 */
 typedef struct {
@@ -297,6 +320,27 @@ struct SMODEL_SUPER {
     STESTCASE *testcase;
 };
 
+/* src/structs/smodel_design/smodel_design.h:7-52 -- the design model: super models and the shared time */
+typedef struct {
+    int model1, model2, type;
+} SMODEL_COUPLE;
+typedef struct {
+    char filebase[ADHK_REF_MAXLINE], filename_design[ADHK_REF_MAXLINE], filename_grid[ADHK_REF_MAXLINE],
+        filename_params[ADHK_REF_MAXLINE];
+    FILE *xmf;
+    int nSuperModels;
+    SMODEL_SUPER *superModel; /* [nSuperModels]; each one's ts points at this design's ts */
+    SMODEL_COUPLE *superCouple;
+    int nMono, nSimple;
+    SGRID *grid;
+    SDT ts;
+    SSERIES *series_dt, *series_out;
+    SCOVERAGE *params;
+    int nlin_sys;
+    int *lin_sys_id, *unique_id;
+    SLIN_SYS *lin_sys;
+} SMODEL_DESIGN;
+
 /* -------------------------------------------------------------------- Part 2: reference entry points */
 
 /* residual.h:5-7.  residual -> sm->lin_sys->residual (= minus the sum of the elemental residuals). */
@@ -359,6 +403,44 @@ void global_to_local_SVECT2D_ivars(SVECT2D *local_vals, int *nodeIDs, int nnodes
                                    double *global_vals);
 void local_dofs_to_global_dofs(int *global_dofs, int ndofs_on_ele, int *dofs, int *local_range, int *ghosts);
 int global_to_local(int global_col_no, int local_size, int *ghosts, int nghost);
+
+/* time_loop.h:7-9, tc_init.c:4-33 and smodel_super.h:126 -- the caller of fe_newton.  time_loop runs
+ * tc_init, then update_dt and advance_time until t_final or a failed step, and returns YES (1) as the
+ * reference's does; 0 instead marks an error of this library (adhk_last_error).  advance_time takes each
+ * super model through its nsubsteps: smodel_super_prep_sol, the series averaged over (t_prev - dt, t_prev),
+ * smodel_super_update_dirichlet_data and the time stepper forward_step names (FE_NEWTON = 0, fe_newton:
+ * the only one on this path); TRUE, or FALSE after a failed step, which leaves dt cut by fe_newton. */
+int time_loop(SMODEL_DESIGN *dm);
+int advance_time(SMODEL_DESIGN *dm, int nsuper);
+int update_dt(SMODEL_DESIGN *dm);
+void tc_init(SMODEL_DESIGN *dm);
+void smodel_super_prep_sol(SMODEL_SUPER *sm);
+void smodel_super_update_dirichlet_data(SMODEL_SUPER *sm);
+
+/* sseries.h -- every TIME_SERIES (type 1) of the list set to its average over (t1, t2)
+ * (sseries_set_values_average, sseries.c:622-668: the full trapezoids between the two times come from
+ * entry[i].area, as the series reader fills it).  A time outside a series' range is an error (status -1,
+ * values unchanged), where the reference exits. */
+void sseries_setall_ts_valuesAVG(SSERIES *head, double t1, double t2);
+
+/* sw2.h:9 and src/friction/fr_defs.h -- the bed-shear drag coefficient of a string (log law over the
+ * roughness height fterms.eqrheight; 0 without one); vegetation and ice drag are not on this path and give
+ * status -1 and NaN */
+double fe_sw2_get_roughness(SMODEL_SUPER *mod, double depth, double velocity, int id, int which);
+double fr_bedshstr_drag_coef(double depth, double roughness_height);
+double fr_manningsn_to_rheight(double n, double kn, double g);
+
+/* messg.h:5-33 in the single-rank build (no _MESSG): every reduction is the identity */
+double messg_dmax(double x);
+double messg_dmin(double x);
+int messg_imin(int x);
+int messg_imax(int x);
+double messg_dsum(double x);
+
+/* sivar_position.h:40 -- ivars[ip->var[j]][node] = the node's dofs in node-major order over the active
+ * variables j of ipNode (the 25 variable codes of adh_def); allocates *ivars (ip->n rows of nnodes, free
+ * each row, then the array); returns the number of dofs */
+int sivar_position_build_dof_map(SIVAR_POSITION *ip, int nnodes, SIVAR_POSITION *ipNode, int ***ivars);
 
 /* No counterparts in the reference.
  * fe_sw2_bc_flux: the residual_ptr of an explicit normal flux per unit width (fe_sw2_1d_explicit_flow,

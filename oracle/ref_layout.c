@@ -53,6 +53,8 @@
 #include "sseries.h"
 #include "stestcase.h"
 #include "smodel_super.h"
+#include "sadh_def.h"
+#include "smodel_design.h"
 #else
 #include "adhk_ref.h"
 #endif

@@ -17,10 +17,10 @@
  *   FD Jacobian                                             -> directional differences of the residual
  *   block-Jacobi, BiCGStab, Newton                          -> dense inverses, direct sparse solve,
  *                                                              converged-state residual
- * Partially pinned: the SUPG combination (sw2_body_resid.c:1074-1079).  Its
- * integrals are checked by quadrature, but the weights that combine the strong
- * residuals are the reference's algebra, stated the same way in the quadrature
- * check, so a slip common to both would not show.
+ *   SUPG combination (sw2_body_resid.c:921-1079)             -> weights from the flux Jacobians of the
+ *                                                              shallow-water operator taken by central
+ *                                                              differences; rotation / reflection
+ *                                                              equivariance; zero on uniform normal flow
  *
  * Layout conventions follow the reference: dof(node, var) = 3*node + var with
  * var 0 = h (depth), 1 = u, 2 = v (depth-averaged velocity)

@@ -31,6 +31,9 @@ STRUCTS = [
     ("SSERIES_ENTRY", "structs/sseries/sseries.h"), ("SSERIES", "structs/sseries/sseries.h"),
     ("SLIN_SYS", "structs/slin_sys/slin_sys.h"), ("NODAL_ATTRIBUTES", "structs/sgrid/sgrid.h"),
     ("SGRID", "structs/sgrid/sgrid.h"), ("SMODEL_SUPER", "structs/smodel_super/smodel_super.h"),
+    ("INPUT_DATA", "structs/sstr_value/sstr_value.h"), ("FRICTION", "structs/sstr_value/sstr_value.h"),
+    ("STR_VALUE", "structs/sstr_value/sstr_value.h"), ("SMODEL_COUPLE", "structs/smodel_design/smodel_design.h"),
+    ("SMODEL_DESIGN", "structs/smodel_design/smodel_design.h"),
 ]
 
 

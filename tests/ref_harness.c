@@ -19,6 +19,10 @@ typedef struct {
     double ival[2][2];
     int ndofs, local_size, var[6], var_code[3], *ivars[3], *ivar_store, *e2d_mat, *e1d_mat, *wd, *wd_rows[1];
     int *tri_nodes, *seg_nodes;
+    SMODEL_DESIGN dm;       /* h_design: the design model whose one super model is sm */
+    SSERIES ser_dt;         /* its dt series */
+    SSERIES_ENTRY ent[2][64]; /* entries of ser_dt (0) and of the flux series (1) */
+    double ent_vals[2][64][3]; /* value, slope, area of each entry */
 } H;
 
 /* kind: 0 depth (series 0), 1 explicit flux (series 1); values: ser0 = the depth of every depth segment,
@@ -242,6 +246,163 @@ int h_increment(H *h, const double *dsol)
     memcpy(h->ls.dsol, dsol, (size_t)h->ndofs * sizeof(double));
     increment_function(&h->sm);
     return adhk_ref_status();
+}
+
+/* the time loop through SMODEL_DESIGN: series 1 (the flux per unit width) becomes a TIME_SERIES of the n
+ * points (tq, qv), with the entry areas the reference's series reader computes; the dt series is constant
+ * dt over [t0, tf]; sm->ts points at the design's clock as smodel_design_init_no_read sets it */
+static void fill_entries(SSERIES_ENTRY *e, double (*vals)[3], int n, const double *t, const double *v)
+{
+    for (int i = 0; i < n; i++) {
+        e[i].time = t[i];
+        e[i].value = &vals[i][0];
+        e[i].slope = &vals[i][1];
+        e[i].area = &vals[i][2];
+        vals[i][0] = v[i];
+    }
+    for (int i = 0; i + 1 < n; i++) {
+        vals[i][1] = (v[i + 1] - v[i]) / (t[i + 1] - t[i]);
+        vals[i][2] = (v[i] + v[i + 1]) * 0.5 * (t[i + 1] - t[i]);
+    }
+}
+
+int h_design(H *h, double t0, double tf, double dt, int n, const double *tq, const double *qv)
+{
+    if (n < 2 || n > 64) return -1;
+    const double tt[2] = {t0, tf}, dd[2] = {dt, dt};
+    fill_entries(h->ent[0], h->ent_vals[0], 2, tt, dd);
+    fill_entries(h->ent[1], h->ent_vals[1], n, tq, qv);
+    h->ser_dt.size = 2;
+    h->ser_dt.nvalues = 1;
+    h->ser_dt.entry = h->ent[0];
+    h->ser[1].type = 1; /* TIME_SERIES */
+    h->ser[1].size = n;
+    h->ser[1].nvalues = 1;
+    h->ser[1].entry = h->ent[1];
+    memset(&h->dm, 0, sizeof h->dm);
+    h->dm.nSuperModels = 1;
+    h->dm.superModel = &h->sm;
+    h->dm.ts.t_init = h->dm.ts.t_prev = t0;
+    h->dm.ts.t_final = tf;
+    h->dm.series_dt = &h->ser_dt;
+    h->sm.ts = &h->dm.ts;
+    h->sm.forward_step = 0; /* FE_NEWTON */
+    return 0;
+}
+
+/* time_loop(&dm): its return value; the clock and the Newton iterations it took */
+int h_time_loop(H *h, double *t_prev, double *dt, int *newton_total)
+{
+    h->sm.nonlinear_it_total = 0;
+    const int rc = time_loop(&h->dm);
+    *t_prev = h->dm.ts.t_prev;
+    *dt = h->dm.ts.dt;
+    *newton_total = h->sm.nonlinear_it_total;
+    return adhk_ref_status() ? -1 : rc;
+}
+
+/* ---- host-only helpers (no GPU): series averages, the clock, friction, the dof map, prep_sol */
+
+/* a list of two TIME_SERIES over the same n times (values v0 and v1) averaged over (t1, t2) */
+int hs_series_avg(int n, const double *t, const double *v0, const double *v1, double t1, double t2, double *out2)
+{
+    static SSERIES_ENTRY e[2][64];
+    static double vals[2][64][3];
+    if (n < 1 || n > 64) return -2;
+    SSERIES s[3];
+    double iv[3] = {0.0, 0.0, 7.0};
+    memset(s, 0, sizeof s);
+    fill_entries(e[0], vals[0], n, t, v0);
+    fill_entries(e[1], vals[1], n, t, v1);
+    for (int k = 0; k < 3; k++) {
+        s[k].id = k;
+        s[k].type = k < 2 ? 1 : 0; /* the third is not a time series and keeps its value */
+        s[k].size = n;
+        s[k].nvalues = 1;
+        s[k].ivalue = &iv[k];
+        s[k].entry = k < 2 ? e[k] : e[0];
+        s[k].next = k < 2 ? &s[k + 1] : NULL;
+    }
+    sseries_setall_ts_valuesAVG(&s[0], t1, t2);
+    out2[0] = iv[0];
+    out2[1] = iv[1];
+    return adhk_ref_status() ? -1 : (iv[2] == 7.0 ? 0 : -3);
+}
+
+/* tc_init, then update_dt until it reports t_final reached: the dt, the number of steps and the end time */
+int hs_clock(double t0, double tf, double dt, double *dt_out, int *nsteps, double *t_end)
+{
+    SSERIES_ENTRY e[2];
+    double vals[2][3];
+    const double tt[2] = {t0, tf}, dd[2] = {dt, dt};
+    SSERIES s;
+    SMODEL_DESIGN dm;
+    memset(&s, 0, sizeof s);
+    memset(&dm, 0, sizeof dm);
+    fill_entries(e, vals, 2, tt, dd);
+    s.size = 2;
+    s.nvalues = 1;
+    s.entry = e;
+    dm.ts.t_init = dm.ts.t_prev = t0;
+    dm.ts.t_final = tf;
+    dm.series_dt = &s;
+    tc_init(&dm);
+    if (adhk_ref_status()) return -1;
+    *dt_out = dm.ts.dt;
+    int k = 0, more;
+    do {
+        more = update_dt(&dm);
+        k++;
+    } while (more && k < 1000000);
+    *nsteps = k;
+    *t_end = dm.ts.t_prev;
+    return 0;
+}
+
+double hs_roughness(double depth, double eqrheight, int sav)
+{
+    STR_VALUE sv[2];
+    SMODEL_SUPER sm;
+    memset(sv, 0, sizeof sv);
+    memset(&sm, 0, sizeof sm);
+    for (int k = 0; k < 2; k++) sv[k].fterms.icethick = -9999999.9; /* UNSET_FLT */
+    sv[1].fterms.eqrheight = eqrheight;
+    sv[1].fterms.sav_flag = sav ? 1 : 0;
+    sm.str_values = sv;
+    return fe_sw2_get_roughness(&sm, depth, 0.0, 1, 0);
+}
+
+/* the SW2 dof map of nnodes nodes (variables h, uda, vda active: codes 0, 4, 5) into out[3][nnodes] */
+int hs_dof_map(int nnodes, int *out)
+{
+    int var[25], varn[25];
+    for (int j = 0; j < 25; j++) var[j] = varn[j] = -3;
+    var[0] = 0;
+    var[4] = 1;
+    var[5] = 2;
+    varn[0] = varn[4] = varn[5] = 1;
+    SIVAR_POSITION ip = {3, 0, var, NULL}, ipn = {3, 0, varn, NULL};
+    int **iv = NULL;
+    const int nd = sivar_position_build_dof_map(&ip, nnodes, &ipn, &iv);
+    if (nd < 0) return -1;
+    for (int r = 0; r < 3; r++) {
+        memcpy(out + (size_t)r * nnodes, iv[r], (size_t)nnodes * sizeof(int));
+        free(iv[r]);
+    }
+    free(iv);
+    return nd;
+}
+
+/* smodel_super_prep_sol on three host arrays */
+void hs_prep_sol(int n, double *sol, double *sol_old, double *sol_older)
+{
+    SMODEL_SUPER sm;
+    memset(&sm, 0, sizeof sm);
+    sm.ndofs = n;
+    sm.sol = sol;
+    sm.sol_old = sol_old;
+    sm.sol_older = sol_older;
+    smodel_super_prep_sol(&sm);
 }
 
 void h_destroy(H *h)

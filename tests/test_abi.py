@@ -39,7 +39,9 @@ def test_library_exports_the_struct_level_surface():
     for ref in ["assemble_residual", "assemble_jacobian", "fe_newton", "check_diag", "apply_Dirichlet_BC",
                 "initialize_system", "increment_function", "get_residual_norms", "fe_sw2_body_resid", "fe_sw2_bc_h",
                 "fe_sw2_bc_vel", "fe_sw2_init", "slin_sys_init_sparsity_mono", "load_elem_mat", "perturb_var",
-                "get_cell_dofs_ivars", "load_global_resid", "add_replace_elem_rhs"]:
+                "get_cell_dofs_ivars", "load_global_resid", "add_replace_elem_rhs", "time_loop", "advance_time",
+                "update_dt", "tc_init", "smodel_super_prep_sol", "sseries_setall_ts_valuesAVG", "fe_sw2_get_roughness",
+                "fr_bedshstr_drag_coef", "messg_dsum", "messg_dmax", "messg_imin", "sivar_position_build_dof_map"]:
         assert ref in names
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing

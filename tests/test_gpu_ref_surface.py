@@ -4,7 +4,8 @@ tests/ref_harness.c builds SMODEL_SUPER / SGRID / SLIN_SYS for a flume as a refe
 materials carrying fe_sw2_body_resid, fe_sw2_bc_h and fe_sw2_bc_flux; the dof map; boundary values in the
 series) and calls libadhk.so by the reference's function names: slin_sys_init_sparsity_mono,
 assemble_residual, assemble_jacobian, the elemental residual pointers, load_elem_mat, check_diag,
-apply_Dirichlet_BC, get_residual_norms, increment_function, initialize_system and fe_newton in a time loop."""
+apply_Dirichlet_BC, get_residual_norms, increment_function, initialize_system and fe_newton in a time loop,
+and the reference's own time loop (time_loop over an SMODEL_DESIGN) under an inflow hydrograph."""
 import ctypes as C
 import os
 import subprocess
@@ -40,7 +41,9 @@ def harness(tmp_path_factory):
                        ("h_step", [C.c_void_p, ip_t]),
                        ("h_elem_resid", [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, dp]),
                        ("h_elem_mat", [C.c_void_p, C.c_int, dp]), ("h_norms", [C.c_void_p, dp, dp, dp, ip_t]),
-                       ("h_increment", [C.c_void_p, dp])]:
+                       ("h_increment", [C.c_void_p, dp]),
+                       ("h_design", [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp]),
+                       ("h_time_loop", [C.c_void_p, dp, dp, ip_t])]:
         getattr(h, name).restype = C.c_int
         getattr(h, name).argtypes = args
     for name, args in [("h_set_state", [C.c_void_p, dp, dp, dp]), ("h_set_bc", [C.c_void_p, ip_t, dp]),
@@ -197,6 +200,46 @@ def test_struct_surface_time_loop_matches_oracle(gpu, harness):
     so_ = orc.state()[0]
     assert its_g == its_o
     assert dt == 1.0
+    assert rel_err(sol[0::3], so_[0::3]) <= 1e-8
+    assert rel_err(sol[1::3], so_[1::3]) <= 1e-6
+    m.close()
+
+
+def test_struct_surface_design_time_loop_under_hydrograph(gpu, harness):
+    """time_loop(SMODEL_DESIGN *) on the 200 m flume with the inflow flux a time series (a rising
+    hydrograph, averaged over each step by advance_time) against the oracle stepped by hand with the
+    averages from scipy quadrature: 15 steps of 1 s, the same Newton iterations, depth 1e-8"""
+    from scipy.integrate import quad
+    mesh, sol0 = A.flume_mesh(A.FLUME_C1)
+    width = A.FLUME_C1.ymax - A.FLUME_C1.ymin
+    t_tab, q_tab = np.array([0.0, 6.0, 20.0]), np.array([10.0, 25.0, 40.0])
+    qn = np.ascontiguousarray(-q_tab / width)  # flux per unit width, negative into the domain
+    dt, nsteps = 1.0, 15
+    m = Model(harness, mesh, dt)
+    m.state(sol0, sol0)
+    assert harness.h_init_system(m.p) == 0
+    assert harness.h_design(m.p, 0.0, nsteps * dt, dt, len(t_tab), d(np.ascontiguousarray(t_tab)), d(qn)) == 0
+    t_end, dt_end, its = np.zeros(1), np.zeros(1), C.c_int()
+    assert harness.h_time_loop(m.p, d(t_end), d(dt_end), C.byref(its)) == 1
+    assert t_end[0] == pytest.approx(nsteps * dt) and dt_end[0] == dt
+
+    orc = O.Oracle(mesh, dt=dt, friction="const", cf_const=0.002, threads=8)
+    orc.set_initial(sol0)
+    inflow = np.flatnonzero(mesh.seg_kind == O.BC_FLUX)
+    val = mesh.seg_val.copy()
+    its_o = 0
+    for k in range(nsteps):
+        t1, t2 = k * dt, (k + 1) * dt
+        pts = [p for p in t_tab if t1 < p < t2]
+        qavg = quad(lambda s: np.interp(s, t_tab, qn), t1, t2, points=pts or None, epsabs=1e-14)[0] / dt
+        val[inflow, 0] = qavg
+        orc.set_seg_values(val)
+        n, st = orc.time_loop(1)
+        assert n == 1
+        its_o += st["newton_its"]
+    assert its.value == its_o
+    sol = m.get()[0]
+    so_ = orc.state()[0]
     assert rel_err(sol[0::3], so_[0::3]) <= 1e-8
     assert rel_err(sol[1::3], so_[1::3]) <= 1e-6
     m.close()
