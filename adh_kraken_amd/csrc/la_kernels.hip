@@ -1385,18 +1385,18 @@ __device__ __forceinline__ SRange part_range(int i, const SPart P)
 /* N consecutive slots [cs, cs + N) of the row's slice, in straight-line code that issues every column
  * index and matrix value, then every gather, before the first is used (a load under a branch is waited for
  * at the branch's end, and values loaded in the order the sums use them would be waited for one by one) */
-template <int N, typename T, typename G>
+template <int N, typename T, bool NT, typename G>
 __device__ __forceinline__ void part_slots(int lane, const int *__restrict__ colp, int cs, const T *__restrict__ val,
                                            const G &g, double &a0, double &a1, double &a2)
 {
     int m[N];
     T av[N][9];
 #pragma unroll
-    for (int u = 0; u < N; u++) m[u] = colp[(cs + u) * 64 + lane];
+    for (int u = 0; u < N; u++) m[u] = ldm<NT>(colp + (cs + u) * 64 + lane);
 #pragma unroll
     for (int u = 0; u < N; u++)
 #pragma unroll
-        for (int e = 0; e < 9; e++) av[u][e] = val[((size_t)(cs + u) * 9 + e) * 64 + lane];
+        for (int e = 0; e < 9; e++) av[u][e] = ldm<NT>(val + ((size_t)(cs + u) * 9 + e) * 64 + lane);
     double xv[N][3];
 #pragma unroll
     for (int u = 0; u < N; u++) g.get(m[u], xv[u][0], xv[u][1], xv[u][2]);
@@ -1416,17 +1416,17 @@ __device__ __forceinline__ void part_slots(int lane, const int *__restrict__ col
 }
 
 /* a group of min(n, MAXG) slots, the code specialised for each size up to MAXG only */
-template <int MAXG, typename T, typename G>
+template <int MAXG, typename T, bool NT, typename G>
 __device__ __forceinline__ void part_group(int n, int lane, const int *__restrict__ colp, int cs,
                                            const T *__restrict__ val, const G &g, double &a0, double &a1, double &a2)
 {
     if constexpr (MAXG == 1) {
-        part_slots<1, T>(lane, colp, cs, val, g, a0, a1, a2);
+        part_slots<1, T, NT>(lane, colp, cs, val, g, a0, a1, a2);
     } else {
         if (n >= MAXG)
-            part_slots<MAXG, T>(lane, colp, cs, val, g, a0, a1, a2);
+            part_slots<MAXG, T, NT>(lane, colp, cs, val, g, a0, a1, a2);
         else
-            part_group<MAXG - 1, T>(n, lane, colp, cs, val, g, a0, a1, a2);
+            part_group<MAXG - 1, T, NT>(n, lane, colp, cs, val, g, a0, a1, a2);
     }
 }
 
@@ -1438,7 +1438,7 @@ __device__ __forceinline__ void part_group(int n, int lane, const int *__restric
  *   wide: groups of up to eight slots for every input, every size compiled in each kernel: about 230
  *     VGPRs and 2 waves per SIMD, every load of a group in flight at once.  On the 4.1M-dof channel, whose
  *     colour phases take several rounds of waves anyway, this is 4 % faster than narrow. */
-template <bool WIDE, typename T, typename G>
+template <bool WIDE, typename T, bool NT = false, typename G>
 __device__ __forceinline__ void part_sum(int i, const SPart P, SRange rg, const T *__restrict__ val, const G &g,
                                          double &a0, double &a1, double &a2)
 {
@@ -1450,19 +1450,19 @@ __device__ __forceinline__ void part_sum(int i, const SPart P, SRange rg, const 
     if constexpr (WIDE) {
         for (int cs = rg.c0; cs < rg.c1; cs += 8) {
             switch (rg.c1 - cs) {
-            case 1: part_slots<1, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
-            case 2: part_slots<2, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
-            case 3: part_slots<3, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
-            case 4: part_slots<4, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
-            case 5: part_slots<5, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
-            case 6: part_slots<6, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
-            case 7: part_slots<7, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
-            default: part_slots<8, T>(lane, P.col, cs, val, g, a0, a1, a2); break;
+            case 1: part_slots<1, T, NT>(lane, P.col, cs, val, g, a0, a1, a2); break;
+            case 2: part_slots<2, T, NT>(lane, P.col, cs, val, g, a0, a1, a2); break;
+            case 3: part_slots<3, T, NT>(lane, P.col, cs, val, g, a0, a1, a2); break;
+            case 4: part_slots<4, T, NT>(lane, P.col, cs, val, g, a0, a1, a2); break;
+            case 5: part_slots<5, T, NT>(lane, P.col, cs, val, g, a0, a1, a2); break;
+            case 6: part_slots<6, T, NT>(lane, P.col, cs, val, g, a0, a1, a2); break;
+            case 7: part_slots<7, T, NT>(lane, P.col, cs, val, g, a0, a1, a2); break;
+            default: part_slots<8, T, NT>(lane, P.col, cs, val, g, a0, a1, a2); break;
             }
         }
     } else {
         for (int cs = rg.c0; cs < rg.c1; cs += G::kGroup)
-            part_group<G::kGroup, T>(rg.c1 - cs, lane, P.col, cs, val, g, a0, a1, a2);
+            part_group<G::kGroup, T, NT>(rg.c1 - cs, lane, P.col, cs, val, g, a0, a1, a2);
     }
 }
 
@@ -1587,7 +1587,7 @@ __device__ __forceinline__ int sgs_block(int nb, int xcd_map)
     return lb < nb ? lb : -1;
 }
 
-template <typename T, int OP, int PH, bool FIRST, int FIN, bool WIDE>
+template <typename T, int OP, int PH, bool FIRST, int FIN, bool WIDE, bool NT>
 __global__ __launch_bounds__(TPB) void k_sgs_phase(int r0, int nb, int xcd_map, int top0, SPart U, SPart L,
                                                    const T *__restrict__ uval, const T *__restrict__ lval, SgVec<T> V,
                                                    double *partials, int nblk, const adhk_scalars *__restrict__ sc,
@@ -1678,7 +1678,7 @@ __global__ __launch_bounds__(TPB) void k_sgs_phase(int r0, int nb, int xcd_map, 
     /* backward: t = v - sum_U F t */
     if constexpr (PH == PH_B || PH == PH_B0) {
         double a[3];
-        part_sum<WIDE, T>(i, U, ru, uval, GetB<T, typename SgCol<T, OP, FIRST>::type>{V.t, col, top0}, a[0],
+        part_sum<WIDE, T, NT>(i, U, ru, uval, GetB<T, typename SgCol<T, OP, FIRST>::type>{V.t, col, top0}, a[0],
                           a[1], a[2]);
         for (int k = 0; k < 3; k++) {
             tt[k] = (T)(v[k] - a[k]);
@@ -1692,7 +1692,7 @@ __global__ __launch_bounds__(TPB) void k_sgs_phase(int r0, int nb, int xcd_map, 
         for (int k = 0; k < 3; k++) uu[k] = v[k] - t[k];
     } else if constexpr (PH != PH_B) {
         double a[3];
-        part_sum<WIDE, T>(i, L, rl, lval, ColPlain<T>{V.u}, a[0], a[1], a[2]);
+        part_sum<WIDE, T, NT>(i, L, rl, lval, ColPlain<T>{V.u}, a[0], a[1], a[2]);
         for (int k = 0; k < 3; k++) uu[k] = (v[k] - t[k]) - a[k];
     }
     if constexpr (PH == PH_B0 || PH == PH_F)
@@ -1929,12 +1929,20 @@ static void sgs_phase(adhk_ctx *c, const SgM<T> &M, int colour)
     const SPart U{G->U.ptr, G->U.col}, L{G->L.ptr, G->L.col};
     const int r0 = G->cs[colour], nb = (G->cs[colour + 1] - r0) / TPB, xm = c->tune.xcd_map;
     const int grid = xm ? (nb + NXCD - 1) / NXCD * NXCD : nb;
-    if (c->tune.sgs_wide)
-        hipLaunchKernelGGL((k_sgs_phase<T, OP, PH, FIRST, FIN, true>), dim3(grid), dim3(TPB), 0, c->stream, r0, nb,
-                           xm, G->cs[G->nc - 1], U, L, M.uval, M.lval, M.V, G->partials, G->nblk, c->sc, c->sc);
+    /* matrix loads non-temporal when the matrix cannot stay in the MALL (the rule of the block-Jacobi kernels) */
+    const bool nt = stream_matrix(c, sizeof(T));
+    if (c->tune.sgs_wide && nt)
+        hipLaunchKernelGGL((k_sgs_phase<T, OP, PH, FIRST, FIN, true, true>), dim3(grid), dim3(TPB), 0, c->stream, r0,
+                           nb, xm, G->cs[G->nc - 1], U, L, M.uval, M.lval, M.V, G->partials, G->nblk, c->sc, c->sc);
+    else if (c->tune.sgs_wide)
+        hipLaunchKernelGGL((k_sgs_phase<T, OP, PH, FIRST, FIN, true, false>), dim3(grid), dim3(TPB), 0, c->stream, r0,
+                           nb, xm, G->cs[G->nc - 1], U, L, M.uval, M.lval, M.V, G->partials, G->nblk, c->sc, c->sc);
+    else if (nt)
+        hipLaunchKernelGGL((k_sgs_phase<T, OP, PH, FIRST, FIN, false, true>), dim3(grid), dim3(TPB), 0, c->stream, r0,
+                           nb, xm, G->cs[G->nc - 1], U, L, M.uval, M.lval, M.V, G->partials, G->nblk, c->sc, c->sc);
     else
-        hipLaunchKernelGGL((k_sgs_phase<T, OP, PH, FIRST, FIN, false>), dim3(grid), dim3(TPB), 0, c->stream, r0, nb,
-                           xm, G->cs[G->nc - 1], U, L, M.uval, M.lval, M.V, G->partials, G->nblk, c->sc, c->sc);
+        hipLaunchKernelGGL((k_sgs_phase<T, OP, PH, FIRST, FIN, false, false>), dim3(grid), dim3(TPB), 0, c->stream, r0,
+                           nb, xm, G->cs[G->nc - 1], U, L, M.uval, M.lval, M.V, G->partials, G->nblk, c->sc, c->sc);
 }
 
 /* one product in 2 nc - 2 phases; FIN: the first phase also finishes the A step (the S product) */

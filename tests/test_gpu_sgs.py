@@ -120,7 +120,8 @@ def test_sgs_schedules_change_no_bit(gpu, c1, precision):
     runs = []
     for tuning in (None, {"graphs": 1, "chunk": 8}, {"graphs": 0, "chunk": 4, "xcd_map": 0},
                    {"graphs": 1, "chunk": 16, "split": 1}, {"sgs_wide": 1, "chunk": 8},
-                   {"sgs_wide": 1, "split": 1, "graphs": 1, "chunk": 8}):
+                   {"sgs_wide": 1, "split": 1, "graphs": 1, "chunk": 8}, {"stream_matrix": 1, "chunk": 8},
+                   {"stream_matrix": 1, "sgs_wide": 1, "split": 1}):
         sm = sgs_model(mesh, 1.0, sol0, tuning)
         n, st = A.time_loop(sm, 4, precision=precision)
         runs.append((n, st["newton_its"], st["lin_its_total"], sm.sol.copy()))
@@ -131,10 +132,12 @@ def test_sgs_schedules_change_no_bit(gpu, c1, precision):
 @pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_MIXED])
 def test_sgs_register_budgets_agree_on_a_channel(gpu, chan, precision):
     """The unstructured channel's rows hold up to eight slots per part, so the wide phase kernels take their
-    largest groups and the narrow ones split rows into several groups: the same bits either way."""
+    largest groups and the narrow ones split rows into several groups: the same bits either way, and with the
+    matrix loads non-temporal (the policy for a matrix larger than the MALL)."""
     mesh, sol0 = chan
     runs = []
-    for tuning in ({"sgs_wide": 0, "split": 1}, {"sgs_wide": 1, "split": 1}, {"sgs_wide": 1, "split": 0}):
+    for tuning in ({"sgs_wide": 0, "split": 1}, {"sgs_wide": 1, "split": 1}, {"sgs_wide": 1, "split": 0},
+                   {"sgs_wide": 1, "split": 1, "stream_matrix": 1}, {"sgs_wide": 0, "split": 0, "stream_matrix": 1}):
         sm = sgs_model(mesh, 5.0, sol0, tuning)
         n, st = A.time_loop(sm, 2, precision=precision)
         runs.append((n, st["newton_its"], st["lin_its_total"], sm.sol.copy()))

@@ -21,6 +21,8 @@ SCHEDULES = [
     {"graphs": 1, "chunk": 16, "elem_tpb": 128},
     {"split": 1},
     {"split": 1, "graphs": 1, "chunk": 8, "xcd_map": 0},
+    {"stream_matrix": 1},
+    {"stream_matrix": 1, "split": 1, "graphs": 1, "chunk": 8},
 ]
 
 
