@@ -53,11 +53,10 @@ typedef struct {
  * widths inside windows of 1024 rows), each colour's run padded to a multiple of 256 rows, so a 256-row reduction block and a 64-row slice hold one
  * colour; row Np is a zero row that padding slots point at.  The folded matrix's off-diagonal blocks
  * split into U (neighbour of a higher colour) and L (lower), each its own SELL-64 over the permuted rows
- * (blocks of a row in ascending natural column order); src maps a slot to the natural SELL position whose
- * value it takes (-1: padding). */
+ * (blocks of a row in ascending natural column order); its padding slots hold zero blocks, written once. */
 #define ADHK_SGS_MAXC 64
 typedef struct {
-    int *ptr, *col, *src; /* device [nslice + 1], [nslot * 64], [nslot * 64] */
+    int *ptr, *col; /* device [nslice + 1], [nslot * 64] */
     int nslot;
     double *val;
     float *val32;
@@ -68,6 +67,8 @@ typedef struct {
     int *colour;               /* host [N] */
     int *prow;                 /* device [Np]: natural node of a permuted row, -1 for padding */
     adhk_sgs_part U, L;
+    int *dst; /* device [natural nslot * 64]: where a natural SELL slot's block goes, q >= 0 in U, -(q + 2) in
+               * L, -1 nowhere (the diagonal blocks and the natural layout's padding) */
     int *inst_blk;             /* device {0, nblk} */
     double *partials;          /* device [8 * nblk] */
     double *v64;               /* device 8 vectors of 3 (Np + 1) doubles: r (= s) p Ap As q xt t (= u) bt */
@@ -212,7 +213,7 @@ int adhk_launch_krylov_finish(adhk_ctx *c, int mixed);
 int adhk_sgs_setup(adhk_ctx *c);              /* colouring and the permuted L/U layout (sgs_setup.c) */
 int adhk_sgs_ensure(adhk_ctx *c, int mixed);  /* the value arrays and vectors of a precision */
 void adhk_sgs_free(adhk_ctx *c);
-int adhk_launch_sgs_gather(adhk_ctx *c, int mixed); /* folded matrix -> L/U values, diagonal check */
+int adhk_launch_sgs_copy(adhk_ctx *c, int mixed); /* folded matrix -> L/U values, diagonal check */
 int adhk_launch_increment(adhk_ctx *c);
 int adhk_launch_linesearch(adhk_ctx *c);
 int adhk_launch_init_system(adhk_ctx *c);

@@ -231,12 +231,13 @@ static int cmp_u64(const void *a, const void *b)
     return (x > y) - (x < y);
 }
 
-/* one part (upper: colour(j) > colour(i), else lower) as SELL-64 over the permuted rows */
+/* one part (upper: colour(j) > colour(i), else lower) as SELL-64 over the permuted rows; dst (the natural
+ * SELL positions) gets each block's place in the part: q in U, -(q + 2) in L */
 static int build_part(adhk_sgs_part *P, int upper, int Np, const int *prow, const int *colour, const int *brow,
-                      const int *bcol, const int *blk2sell, const int *posof)
+                      const int *bcol, const int *blk2sell, const int *posof, int *dst)
 {
     const int nsl = Np / 64;
-    int *ptr = malloc(((size_t)nsl + 1) * sizeof(int)), *col = NULL, *src = NULL;
+    int *ptr = malloc(((size_t)nsl + 1) * sizeof(int)), *col = NULL;
     if (!ptr) goto oom;
     ptr[0] = 0;
     for (int s = 0; s < nsl; s++) {
@@ -257,12 +258,8 @@ static int build_part(adhk_sgs_part *P, int upper, int Np, const int *prow, cons
     {
         const size_t npos = (size_t)(P->nslot > 0 ? P->nslot : 1) * 64;
         col = malloc(npos * sizeof(int));
-        src = malloc(npos * sizeof(int));
-        if (!col || !src) goto oom;
-        for (size_t q = 0; q < npos; q++) {
-            col[q] = Np; /* the zero row */
-            src[q] = -1;
-        }
+        if (!col) goto oom;
+        for (size_t q = 0; q < npos; q++) col[q] = Np; /* the zero row */
         for (int s = 0; s < nsl; s++)
             for (int l = 0; l < 64; l++) {
                 const int n = prow[s * 64 + l];
@@ -273,27 +270,23 @@ static int build_part(adhk_sgs_part *P, int upper, int Np, const int *prow, cons
                     if (j == n || !(upper ? colour[j] > colour[n] : colour[j] < colour[n])) continue;
                     const size_t q = (size_t)(ptr[s] + k) * 64 + l;
                     col[q] = posof[j];
-                    src[q] = blk2sell[b];
+                    dst[blk2sell[b]] = upper ? (int)q : -((int)q + 2);
                     k++;
                 }
             }
         HC(hipMalloc((void **)&P->ptr, ((size_t)nsl + 1) * sizeof(int)));
         HC(hipMalloc((void **)&P->col, npos * sizeof(int)));
-        HC(hipMalloc((void **)&P->src, npos * sizeof(int)));
         HC(hipMemcpy(P->ptr, ptr, ((size_t)nsl + 1) * sizeof(int), hipMemcpyHostToDevice));
         HC(hipMemcpy(P->col, col, npos * sizeof(int), hipMemcpyHostToDevice));
-        HC(hipMemcpy(P->src, src, npos * sizeof(int), hipMemcpyHostToDevice));
     }
     free(ptr);
     free(col);
-    free(src);
     return 0;
 oom:
     adhk_set_error("adhk_sgs_setup: out of host memory");
 fail:
     free(ptr);
     free(col);
-    free(src);
     return -1;
 }
 
@@ -301,7 +294,8 @@ int adhk_sgs_setup(adhk_ctx *c)
 {
     adhk_sgs *G = &c->sgs;
     const int N = c->N, nnzb = c->nnzb;
-    int *brow = NULL, *bcol = NULL, *b2s = NULL, *prow = NULL, *posof = NULL, *cnt = NULL, *fill = NULL;
+    int *brow = NULL, *bcol = NULL, *b2s = NULL, *prow = NULL, *posof = NULL, *cnt = NULL, *fill = NULL, *dst = NULL;
+    const size_t nnat = (size_t)c->nslot * 64;
     int ib[2];
     if (G->ready) return 0;
     if (c->ninst != 1) {
@@ -315,7 +309,8 @@ int adhk_sgs_setup(adhk_ctx *c)
     posof = malloc((size_t)(N > 0 ? N : 1) * sizeof(int));
     cnt = calloc(ADHK_SGS_MAXC, sizeof(int));
     fill = calloc(ADHK_SGS_MAXC, sizeof(int));
-    if (!brow || !bcol || !b2s || !G->colour || !posof || !cnt || !fill) {
+    dst = malloc((nnat > 0 ? nnat : 1) * sizeof(int));
+    if (!brow || !bcol || !b2s || !G->colour || !posof || !cnt || !fill || !dst) {
         adhk_set_error("adhk_sgs_setup: out of host memory");
         goto fail;
     }
@@ -371,8 +366,11 @@ int adhk_sgs_setup(adhk_ctx *c)
     }
     for (int i = 0; i < G->Np; i++)
         if (prow[i] >= 0) posof[prow[i]] = i;
-    if (build_part(&G->U, 1, G->Np, prow, G->colour, brow, bcol, b2s, posof)) goto fail;
-    if (build_part(&G->L, 0, G->Np, prow, G->colour, brow, bcol, b2s, posof)) goto fail;
+    for (size_t q = 0; q < nnat; q++) dst[q] = -1;
+    if (build_part(&G->U, 1, G->Np, prow, G->colour, brow, bcol, b2s, posof, dst)) goto fail;
+    if (build_part(&G->L, 0, G->Np, prow, G->colour, brow, bcol, b2s, posof, dst)) goto fail;
+    HC(hipMalloc((void **)&G->dst, (nnat > 0 ? nnat : 1) * sizeof(int)));
+    HC(hipMemcpy(G->dst, dst, nnat * sizeof(int), hipMemcpyHostToDevice));
     HC(hipMalloc((void **)&G->prow, (size_t)G->Np * sizeof(int)));
     HC(hipMemcpy(G->prow, prow, (size_t)G->Np * sizeof(int), hipMemcpyHostToDevice));
     ib[0] = 0;
@@ -388,8 +386,10 @@ int adhk_sgs_setup(adhk_ctx *c)
     free(posof);
     free(cnt);
     free(fill);
+    free(dst);
     return 0;
 fail:
+    free(dst);
     free(brow);
     free(bcol);
     free(b2s);
@@ -416,8 +416,17 @@ int adhk_sgs_ensure(adhk_ctx *c, int mixed)
             HC(hipMalloc(vp, SGS_VECS * nvec * vb));
             HC(hipMemset(*vp, 0, SGS_VECS * nvec * vb));
         }
-        if (!*up) HC(hipMalloc(up, (size_t)(G->U.nslot > 0 ? G->U.nslot : 1) * 64 * 9 * vb));
-        if (!*lp) HC(hipMalloc(lp, (size_t)(G->L.nslot > 0 ? G->L.nslot : 1) * 64 * 9 * vb));
+        /* zeroed once: the parts' padding slots are never written again (k_sgs_scatter writes real blocks) */
+        if (!*up) {
+            const size_t nb = (size_t)(G->U.nslot > 0 ? G->U.nslot : 1) * 64 * 9 * vb;
+            HC(hipMalloc(up, nb));
+            HC(hipMemset(*up, 0, nb));
+        }
+        if (!*lp) {
+            const size_t nb = (size_t)(G->L.nslot > 0 ? G->L.nslot : 1) * 64 * 9 * vb;
+            HC(hipMalloc(lp, nb));
+            HC(hipMemset(*lp, 0, nb));
+        }
     }
     return 0;
 fail:
@@ -428,7 +437,6 @@ static void free_part(adhk_sgs_part *P)
 {
     if (P->ptr) (void)hipFree(P->ptr);
     if (P->col) (void)hipFree(P->col);
-    if (P->src) (void)hipFree(P->src);
     if (P->val) (void)hipFree(P->val);
     if (P->val32) (void)hipFree(P->val32);
     memset(P, 0, sizeof *P);
@@ -440,6 +448,7 @@ void adhk_sgs_free(adhk_ctx *c)
     free_part(&G->U);
     free_part(&G->L);
     if (G->prow) (void)hipFree(G->prow);
+    if (G->dst) (void)hipFree(G->dst);
     if (G->inst_blk) (void)hipFree(G->inst_blk);
     if (G->partials) (void)hipFree(G->partials);
     if (G->v64) (void)hipFree(G->v64);
