@@ -399,3 +399,62 @@ def test_fd_form_of_the_residual_agrees_with_the_residual_form(gpu, c1):
                     assert lib.adhk_elem_residual_one(sm.h, 3, e, pe, i, k, -1, L.dptr(rb)) == 0
                     worst = max(worst, np.abs(ra - rb).max() / np.abs(ra).max())
     assert worst <= 1e-13
+
+
+def closed_basin(mesh):
+    """the same triangles without boundary segments: a basin closed on every side"""
+    return A.Mesh(mesh.x, mesh.y, mesh.z, mesh.tri, np.zeros((0, 2), np.int32), np.zeros(0, np.int32),
+                  np.zeros((0, 2)), mesh.mann)
+
+
+def test_closed_basin_without_boundary_segments(gpu):
+    """No 1-D elements at all: pattern, residual and Jacobian against the oracle, then a sloshing basin
+    (a tilted water surface released from rest) time-stepped against it: same Newton counts, depth 1e-8,
+    velocity 1e-6, and the water volume kept (no boundary flux)."""
+    _, (mesh0, sol0) = small_flume(npx=9, npy=5, seed=4)
+    mesh = closed_basin(mesh0)
+    s, so = perturbed_state(sol0, seed=3)
+    sm, orc = make_pair(mesh, s, so)
+    ip_o, c_o = orc.pattern()
+    ip_g, c_g = A.sparsity(sm)
+    assert np.array_equal(ip_g, ip_o) and np.array_equal(c_g, c_o)
+    assert rel_err(A.assemble_residual(sm), orc.residual()) <= 1e-12
+    assert rel_err(A.assemble_jacobian(sm), orc.jacobian()) <= 1e-7
+    tilt = sol0.copy()
+    tilt[0::3] += 0.02 * (mesh.x - mesh.x.mean()) / (mesh.x.max() - mesh.x.min())
+    sm = A.SModelSuper(mesh, dt=0.5)
+    sm.set_initial(tilt)
+    orc = O.Oracle(mesh, dt=0.5, threads=8)
+    orc.set_initial(tilt)
+    m0 = orc.grid_mass(1000.0)
+    ng, sg = A.time_loop(sm, 6)
+    no, so_ = orc.time_loop(6)
+    assert ng == no == 6 and sg["newton_its"] == so_["newton_its"]
+    gs, os_ = sm.sol, orc.state()[0]
+    assert rel_err(gs[0::3], os_[0::3]) <= 1e-8
+    assert rel_err(gs[1::3], os_[1::3]) <= 1e-6 and rel_err(gs[2::3], os_[2::3]) <= 1e-6
+    assert abs(sm.grid_mass(1000.0) - m0) <= 1e-9 * m0
+
+
+def test_single_triangle(gpu):
+    """The smallest mesh: one triangle, no segments; its 9 x 9 system against the oracle and one Newton
+    step from a perturbed state."""
+    x, y = np.array([0.0, 2.0, 0.5]), np.array([0.0, 0.3, 1.7])
+    z = np.array([-0.1, -0.12, -0.05])
+    mesh = A.Mesh(x, y, z, np.array([[0, 1, 2]], np.int32), np.zeros((0, 2), np.int32), np.zeros(0, np.int32),
+                  np.zeros((0, 2)), np.full(1, 0.03))
+    sol0 = np.zeros(9)
+    sol0[0::3] = 1.0 - z
+    s, so = perturbed_state(sol0, seed=8)
+    sm, orc = make_pair(mesh, s, so)
+    ip_o, c_o = orc.pattern()
+    ip_g, c_g = A.sparsity(sm)
+    assert np.array_equal(ip_g, ip_o) and np.array_equal(c_g, c_o) and len(c_g) == 81
+    assert rel_err(A.assemble_residual(sm), orc.residual()) <= 1e-12
+    assert rel_err(A.assemble_jacobian(sm), orc.jacobian()) <= 1e-7
+    ok_g, st_g = A.fe_newton(sm)
+    ok_o, st_o = orc.fe_newton()
+    assert ok_g == ok_o == 1 and st_g["newton_its"] == st_o["newton_its"]
+    gs, os_ = sm.sol, orc.state()[0]
+    assert rel_err(gs[0::3], os_[0::3]) <= 1e-8
+    assert rel_err(gs[1::3], os_[1::3]) <= 1e-6 and rel_err(gs[2::3], os_[2::3]) <= 1e-6
