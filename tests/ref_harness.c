@@ -393,6 +393,34 @@ int hs_dof_map(int nnodes, int *out)
     return nd;
 }
 
+/* time_loop's refusals, before any device work: which 0 = a dt series that starts after t_prev (tc_init),
+ * 1 = a super model whose forward_step is not FE_NEWTON (advance_time); returns time_loop's value */
+int hs_time_loop_refusal(int which)
+{
+    SSERIES_ENTRY e[2];
+    double vals[2][3];
+    const double tt[2] = {which == 0 ? 5.0 : 0.0, 10.0}, dd[2] = {1.0, 1.0};
+    SSERIES s;
+    SMODEL_DESIGN dm;
+    SMODEL_SUPER sm;
+    memset(&s, 0, sizeof s);
+    memset(&dm, 0, sizeof dm);
+    memset(&sm, 0, sizeof sm);
+    fill_entries(e, vals, 2, tt, dd);
+    s.size = 2;
+    s.nvalues = 1;
+    s.entry = e;
+    dm.ts.t_init = dm.ts.t_prev = 0.0;
+    dm.ts.t_final = 10.0;
+    dm.series_dt = &s;
+    dm.nSuperModels = 1;
+    dm.superModel = &sm;
+    sm.ts = &dm.ts;
+    sm.nsubsteps = 1;
+    sm.forward_step = 3;
+    return time_loop(&dm);
+}
+
 /* smodel_super_prep_sol on three host arrays */
 void hs_prep_sol(int n, double *sol, double *sol_old, double *sol_older)
 {

@@ -42,6 +42,7 @@ def hs(tmp_path_factory):
     h.hs_dof_map.argtypes = [C.c_int, ip_t]
     h.hs_prep_sol.argtypes = [C.c_int, dp, dp, dp]
     h.hs_prep_sol.restype = None
+    h.hs_time_loop_refusal.argtypes = [C.c_int]
     return h
 
 
@@ -131,3 +132,12 @@ def test_prep_sol_rotates_levels(hs):
     a, b, c = s.copy(), so.copy(), soo.copy()
     hs.hs_prep_sol(12, d(a), d(b), d(c))
     assert np.array_equal(a, s) and np.array_equal(b, s) and np.array_equal(c, so)
+
+
+def test_time_loop_refusals(hs):
+    """time_loop returns 0 with a message, before any device work, for a dt series that does not cover
+    t_prev (the reference exits in sseries_get_interval) and for a time stepper other than FE_NEWTON"""
+    assert hs.hs_time_loop_refusal(0) == 0
+    assert "t_prev is not in its range" in L.last_error()
+    assert hs.hs_time_loop_refusal(1) == 0
+    assert "FE_NEWTON" in L.last_error()
